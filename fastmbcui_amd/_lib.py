@@ -1,0 +1,268 @@
+"""ctypes binding of libmfs_core.so (the C-ABI in include/mfs_core.h).
+
+The product path has no CPU fallback: if the HIP library is missing this module
+raises on first use, naming the build command.  Tensors cross the boundary as
+raw device pointers plus sizes; streams are torch's current HIP stream.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import threading
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libmfs_core.so")
+
+MFS_OK, MFS_ERR_ARG, MFS_ERR_HIP, MFS_ERR_DUPLICATE, MFS_ERR_NOT_SPD, MFS_ERR_STATE, MFS_ERR_ALLOC = range(7)
+MFS_FP64, MFS_FP32 = 0, 1
+MFS_RHS_BOUNDARY, MFS_RHS_SOURCE = 0, 1
+MFS_K_NONE, MFS_K_MATVEC_ROWS, MFS_K_MATVEC_COLS, MFS_K_GRAM, MFS_K_RESID_ROWS, MFS_K_RESID_COLS = range(6)
+
+_STATUS_NAMES = {1: "MFS_ERR_ARG", 2: "MFS_ERR_HIP", 3: "MFS_ERR_DUPLICATE", 4: "MFS_ERR_NOT_SPD",
+                 5: "MFS_ERR_STATE", 6: "MFS_ERR_ALLOC"}
+
+
+class MfsError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{_STATUS_NAMES.get(code, code)}: {msg}")
+        self.code = code
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [
+        ("n", ctypes.c_int32), ("n_super", ctypes.c_int32), ("max_local", ctypes.c_int32),
+        ("pad0", ctypes.c_int32),
+        ("nnz_pattern", ctypes.c_int64), ("nnz_factor", ctypes.c_int64),
+        ("theta_entries", ctypes.c_int64),
+        ("sum_ns2", ctypes.c_double), ("sum_ns3", ctypes.c_double),
+        ("plan_ms", ctypes.c_double), ("factor_ms", ctypes.c_double), ("solve_ms", ctypes.c_double),
+        ("iters", ctypes.c_int32 * 64), ("relres", ctypes.c_double * 64),
+        ("outer_iters", ctypes.c_int32), ("retried_supernodes", ctypes.c_int32),
+    ]
+
+
+_lib = None
+_lock = threading.Lock()
+
+
+def lib():
+    """Load libmfs_core.so once; raise if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    with _lock:
+        if _lib is not None:
+            return _lib
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing: build it with `python -m fastmbcui_amd.build` "
+                              "(hipcc, gfx950); there is no CPU fallback")
+        L = ctypes.CDLL(LIB_PATH)
+        vp, i32, i64, f64 = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64, ctypes.c_double
+        ci = ctypes.c_int
+        sig = {
+            "mfs_ctx_create": (ci, [ctypes.POINTER(vp), ci, vp]),
+            "mfs_ctx_destroy": (None, [vp]),
+            "mfs_ctx_set_stream": (ci, [vp, vp]),
+            "mfs_ctx_last_error": (ctypes.c_char_p, [vp]),
+            "mfs_ctx_stats": (ci, [vp, ctypes.POINTER(Stats)]),
+            "mfs_plan": (ci, [vp, vp, i64, f64, f64, i32]),
+            "mfs_plan_f64": (ci, [vp, vp, i64, f64, f64, i32]),
+            "mfs_plan_export": (ci, [vp] + [vp] * 10),
+            "mfs_factor_dense": (ci, [vp, vp, i64, i64, ci]),
+            "mfs_factor_export": (ci, [vp, vp]),
+            "mfs_solve_dense": (ci, [vp, vp, i64, i64, vp, i64, ci, vp, i32, f64]),
+            "mfs_solve_mixed": (ci, [vp, vp, i64, i64, vp, i64, ci, vp, vp, i32, f64, f64]),
+            "mfs_normal_residual": (ci, [vp, vp, i64, i64, vp, i64, vp, vp]),
+            "mfs_assemble_dense": (ci, [vp, vp, vp, vp, i64, i64, vp]),
+            "mfs_ctx_track_kernel": (ci, [vp, ci]),
+            "mfs_ctx_tracked": (ci, [vp, ci, ctypes.POINTER(i64), ctypes.POINTER(f64), ctypes.POINTER(f64)]),
+        }
+        for name, (res, args) in sig.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _ptr(t):
+    """Device (or host) address of a tensor / ndarray, or None."""
+    if t is None:
+        return None
+    if isinstance(t, np.ndarray):
+        return t.ctypes.data_as(ctypes.c_void_p)
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def _stream_handle(device):
+    import torch
+    return ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)
+
+
+class Context:
+    """One plan + one factor on one device (mfs_ctx)."""
+
+    def __init__(self, device=0):
+        import torch
+        self.device = torch.device("cuda", device if isinstance(device, int) else device.index or 0)
+        h = ctypes.c_void_p()
+        self._check(lib().mfs_ctx_create(ctypes.byref(h), self.device.index,
+                                         _stream_handle(self.device)), "mfs_ctx_create", raw=True)
+        self._h = h
+        self.n = None
+        self.factor_precision = None
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().mfs_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, code, what, raw=False):
+        if code != MFS_OK:
+            msg = what if raw else lib().mfs_ctx_last_error(self._h).decode()
+            raise MfsError(code, msg)
+
+    def _sync_stream(self):
+        lib().mfs_ctx_set_stream(self._h, _stream_handle(self.device))
+
+    # ---- plan ---------------------------------------------------------------
+    def plan(self, xyz, rho, lam=1.5, max_super=2048):
+        """xyz: host n x 3 (float32 numpy/tensor -> mfs_plan, float64 -> mfs_plan_f64)."""
+        import torch
+        self._sync_stream()
+        if isinstance(xyz, torch.Tensor):
+            xyz = xyz.detach().cpu().numpy()
+        xyz = np.ascontiguousarray(xyz)
+        if xyz.ndim != 2 or xyz.shape[1] != 3:
+            raise ValueError("source points must be n x 3")
+        if xyz.dtype == np.float32:
+            code = lib().mfs_plan(self._h, _ptr(xyz), xyz.shape[0], float(rho), float(lam), int(max_super))
+        else:
+            xyz = xyz.astype(np.float64, copy=False)
+            code = lib().mfs_plan_f64(self._h, _ptr(xyz), xyz.shape[0], float(rho), float(lam), int(max_super))
+        self._check(code, "mfs_plan")
+        self.n = int(xyz.shape[0])
+        self.factor_precision = None
+
+    def plan_export(self):
+        s = self.stats()
+        n, ns = s.n, s.n_super
+        out = dict(P=np.empty(n, np.int32), Pinv=np.empty(n, np.int32), scale=np.empty(n, np.float64),
+                   ptr=np.empty(n + 1, np.int32), ind=np.empty(max(s.nnz_pattern, 1), np.int32),
+                   sup_ptr=np.empty(ns + 1, np.int32), sup_ind=np.empty(n, np.int32),
+                   sup_parent=np.empty(n, np.int32), sptr=np.empty(n + 1, np.int32),
+                   sind=np.empty(max(s.nnz_factor, 1), np.int32))
+        keys = ["P", "Pinv", "scale", "ptr", "ind", "sup_ptr", "sup_ind", "sup_parent", "sptr", "sind"]
+        self._check(lib().mfs_plan_export(self._h, *[_ptr(out[k]) for k in keys]), "mfs_plan_export")
+        out["ind"] = out["ind"][: s.nnz_pattern]
+        out["sind"] = out["sind"][: s.nnz_factor]
+        return out
+
+    # ---- factor -------------------------------------------------------------
+    def factor_dense(self, M, precision=MFS_FP64):
+        self._sync_stream()
+        _check_dense(M, self.n)
+        self._check(lib().mfs_factor_dense(self._h, _ptr(M), M.shape[0], M.shape[1], int(precision)),
+                    "mfs_factor_dense")
+        self.factor_precision = precision
+
+    def factor_export(self):
+        s = self.stats()
+        val = np.empty(max(s.nnz_factor, 1), np.float64)
+        self._check(lib().mfs_factor_export(self._h, _ptr(val)), "mfs_factor_export")
+        return val[: s.nnz_factor]
+
+    # ---- solves -------------------------------------------------------------
+    def solve_dense(self, M, B, rhs_space=MFS_RHS_BOUNDARY, maxit=100, tol=1e-6, out=None):
+        """Plain FP64 PCG per right-hand side.  B: m x k (or n x k), float32 cuda."""
+        import torch
+        self._sync_stream()
+        _check_dense(M, self.n)
+        B = _as_2d_f32(B)
+        if out is None:
+            out = torch.empty((M.shape[1], B.shape[1]), dtype=torch.float32, device=M.device)
+        self._check(lib().mfs_solve_dense(self._h, _ptr(M), M.shape[0], M.shape[1], _ptr(B), B.shape[1],
+                                          int(rhs_space), _ptr(out), int(maxit), float(tol)),
+                    "mfs_solve_dense")
+        return out
+
+    def solve_mixed(self, M, B, maxit=1000, tol=1e-6, inner_tol=0.5, x0=None, out=None):
+        """Batched FP32 PCG with FP64 refinement.  B: m x k float32 cuda."""
+        import torch
+        self._sync_stream()
+        _check_dense(M, self.n)
+        B = _as_2d_f32(B)
+        if out is None:
+            out = torch.empty((M.shape[1], B.shape[1]), dtype=torch.float32, device=M.device)
+        if x0 is not None:
+            x0 = _as_2d_f32(x0)
+        self._check(lib().mfs_solve_mixed(self._h, _ptr(M), M.shape[0], M.shape[1], _ptr(B), B.shape[1],
+                                          MFS_RHS_BOUNDARY, _ptr(x0), _ptr(out), int(maxit), float(tol),
+                                          float(inner_tol)), "mfs_solve_mixed")
+        return out
+
+    def normal_residual(self, M, B, X):
+        self._sync_stream()
+        B = _as_2d_f32(B)
+        X = _as_2d_f32(X)
+        out = np.empty(B.shape[1], np.float64)
+        self._check(lib().mfs_normal_residual(self._h, _ptr(M), M.shape[0], M.shape[1], _ptr(B),
+                                              B.shape[1], _ptr(X), _ptr(out)), "mfs_normal_residual")
+        return out
+
+    def assemble_dense(self, x, nx, y):
+        """M (float32, m x n) from FP64 device points x, nx (m x 3) and y (n x 3)."""
+        import torch
+        self._sync_stream()
+        M = torch.empty((x.shape[0], y.shape[0]), dtype=torch.float32, device=x.device)
+        self._check(lib().mfs_assemble_dense(self._h, _ptr(x.contiguous()), _ptr(nx.contiguous()),
+                                             _ptr(y.contiguous()), x.shape[0], y.shape[0], _ptr(M)),
+                    "mfs_assemble_dense")
+        return M
+
+    # ---- statistics -----------------------------------------------------------
+    def stats(self):
+        s = Stats()
+        self._check(lib().mfs_ctx_stats(self._h, ctypes.byref(s)), "mfs_ctx_stats")
+        return s
+
+    def track(self, classes):
+        """Enable per-launch event timing for the given kernel classes (iterable of ints)."""
+        mask = 0
+        for c in classes:
+            mask |= 1 << int(c)
+        self._check(lib().mfs_ctx_track_kernel(self._h, mask), "mfs_ctx_track_kernel")
+
+    def tracked(self, kernel_class):
+        """(launches, summed event ms, algorithmic bytes) for one kernel class."""
+        n = ctypes.c_int64(0)
+        ms = ctypes.c_double(0)
+        by = ctypes.c_double(0)
+        self._check(lib().mfs_ctx_tracked(self._h, int(kernel_class), ctypes.byref(n), ctypes.byref(ms),
+                                          ctypes.byref(by)), "mfs_ctx_tracked")
+        return n.value, ms.value, by.value
+
+
+def _check_dense(M, n):
+    import torch
+    if not isinstance(M, torch.Tensor) or M.dtype != torch.float32 or not M.is_cuda or M.dim() != 2:
+        raise ValueError("M must be a 2-D float32 device tensor")
+    if not M.is_contiguous():
+        raise ValueError("M must be contiguous (row-major)")
+    if n is None or M.shape[1] != n:
+        raise ValueError("M's column count must equal the planned source count")
+
+
+def _as_2d_f32(B):
+    import torch
+    if B.dim() == 1:
+        B = B.unsqueeze(1)
+    return B.to(torch.float32).contiguous()

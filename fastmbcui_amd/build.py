@@ -1,0 +1,62 @@
+"""Builds libmfs_core.so (HIP for gfx950 + host plan code) in-tree.
+
+    python -m fastmbcui_amd.build
+
+Every HIP translation unit is compiled with packed FP32 instructions disabled
+(``-Xclang -target-feature -Xclang -packed-fp32-ops``); the host plan code is
+built with ``-ffp-contract=off`` so its distance sums round as written.
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+from concurrent.futures import ThreadPoolExecutor
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+OBJ = os.path.join(HERE, "_build")
+LIB_DIR = os.path.join(HERE, "lib")
+LIB = os.path.join(LIB_DIR, "libmfs_core.so")
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = os.environ.get("MFS_OFFLOAD_ARCH", "gfx950")
+
+HIP_SOURCES = ["mfs_core.hip", "k_plan.hip", "k_factor.hip", "k_solve.hip"]
+HOST_SOURCES = ["plan_host.cpp"]
+HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
+             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+             "-Wall", "-Wno-unused-function", "-Wno-unused-variable"]
+HOST_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall"]
+
+
+def _run(cmd):
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(" ".join(cmd) + "\n" + r.stdout + r.stderr)
+        raise RuntimeError(f"build step failed: {os.path.basename(cmd[-1])}")
+    return r
+
+
+def build(verbose: bool = False) -> str:
+    os.makedirs(OBJ, exist_ok=True)
+    os.makedirs(LIB_DIR, exist_ok=True)
+    jobs = []
+    for src in HIP_SOURCES:
+        out = os.path.join(OBJ, src + ".o")
+        jobs.append([HIPCC, *HIP_FLAGS, "-c", "-o", out, os.path.join(CSRC, src)])
+    for src in HOST_SOURCES:
+        out = os.path.join(OBJ, src + ".o")
+        jobs.append(["g++", *HOST_FLAGS, "-c", "-o", out, os.path.join(CSRC, src)])
+    with ThreadPoolExecutor(max_workers=min(8, len(jobs))) as ex:
+        for r in ex.map(_run, jobs):
+            if verbose and (r.stdout or r.stderr):
+                sys.stderr.write(r.stdout + r.stderr)
+    objs = [os.path.join(OBJ, s + ".o") for s in HIP_SOURCES + HOST_SOURCES]
+    tmp = LIB + ".tmp"
+    _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp, *objs])
+    os.replace(tmp, LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    print(build(verbose=True))

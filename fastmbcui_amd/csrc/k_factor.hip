@@ -1,0 +1,363 @@
+// k_factor.hip -- supernodal KL factor on gfx950.
+//
+// The reference builds the factor with three one-thread-per-item kernels
+// (src/kl_chol.cu:166-223 Gram, :347-360 Cholesky, :362-387 back-substitution):
+// each thread walks whole dot products and triangular solves serially.  Here
+// one 256-thread block owns one supernode and does all three steps in LDS:
+//
+//  1. Gram  TH = A^T A, A = the supernode's columns of M (gathered rows of M^T),
+//     on MFMA 16x16x4 tiles (FP64 or FP32), lower tile triangle only, the
+//     f-dimension streamed in 32-row chunks through LDS with the next chunk's
+//     loads in flight; stored reversed, TH[ns-1-p][ns-1-q], as the reference
+//     stores it (kl_chol.cu:216-220).
+//  2. Upper Cholesky TH = U^T U, right-looking, one barrier per column.  A
+//     non-positive pivot marks the supernode (status = 1) instead of producing
+//     NaN (the reference does not check, kl_chol.cu:33).
+//  3. For each member column j with nb rows: x = U[0:nb,0:nb]^{-1} e_{nb-1},
+//     written in ascending row order -- the KL column
+//     Theta_j^{-1} e_1 / sqrt(e_1^T Theta_j^{-1} e_1) (kl_chol.cu:378-386).
+//     Members are solved in batches of 16 with one barrier per row.
+#include "mfs_internal.h"
+
+namespace mfs {
+namespace {
+
+template <typename T>
+struct Mfma;
+
+// A (16x4) and B (4x16) operands: lane l holds A[l%16][l/16] and B[l/16][l%16].
+// The 16x16 result sits in 4 registers per lane with column l%16; the row of
+// register v differs by precision (measured on gfx950): FP64 puts rows
+// l/16 + 4v in a lane, FP32 puts rows 4(l/16) + v.
+template <>
+struct Mfma<double> {
+  typedef double v4 __attribute__((ext_vector_type(4)));
+  __device__ static inline v4 run(double a, double b, v4 c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  __device__ static inline int row(int lane, int v) { return (lane >> 4) + 4 * v; }
+};
+
+template <>
+struct Mfma<float> {
+  typedef float v4 __attribute__((ext_vector_type(4)));
+  __device__ static inline v4 run(float a, float b, v4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  __device__ static inline int row(int lane, int v) { return 4 * (lane >> 4) + v; }
+};
+
+template <int A, int B>
+struct CMax {
+  static constexpr int v = A > B ? A : B;
+};
+
+// ------------------------------------------------------------------ transpose
+// Mt[c][f] = M[f][c] for f < m, 0 for m <= f < ldmt.  64x64 tiles through LDS,
+// float4 in and out when the row length allows it.
+template <bool VEC>
+__global__ __launch_bounds__(kThreads) void k_transpose(const float *__restrict__ M, int64_t m,
+                                                        int64_t n, float *__restrict__ Mt,
+                                                        int64_t ldmt) {
+  __shared__ float tile[64][65];
+  const int t = threadIdx.x;
+  const int64_t c0 = int64_t(blockIdx.x) * 64, f0 = int64_t(blockIdx.y) * 64;
+  const int q = t & 15, r16 = t >> 4;  // 16 float4 per 64-wide row, 16 rows per pass
+  for (int r = r16; r < 64; r += 16) {
+    const int64_t f = f0 + r, c = c0 + 4 * q;
+    float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (f < m) {
+      if (VEC && c + 3 < n) {
+        v = *reinterpret_cast<const float4 *>(M + f * n + c);
+      } else {
+        if (c + 0 < n) v.x = M[f * n + c + 0];
+        if (c + 1 < n) v.y = M[f * n + c + 1];
+        if (c + 2 < n) v.z = M[f * n + c + 2];
+        if (c + 3 < n) v.w = M[f * n + c + 3];
+      }
+    }
+    tile[r][4 * q + 0] = v.x;
+    tile[r][4 * q + 1] = v.y;
+    tile[r][4 * q + 2] = v.z;
+    tile[r][4 * q + 3] = v.w;
+  }
+  __syncthreads();
+  for (int r = r16; r < 64; r += 16) {
+    const int64_t c = c0 + r;
+    if (c >= n) continue;
+    const float4 v = make_float4(tile[4 * q + 0][r], tile[4 * q + 1][r], tile[4 * q + 2][r],
+                                 tile[4 * q + 3][r]);
+    *reinterpret_cast<float4 *>(Mt + c * ldmt + f0 + 4 * q) = v;
+  }
+}
+
+// ------------------------------------------------------------------ fused supernode factor
+template <int T, typename TC, typename TV>
+__global__ __launch_bounds__(kThreads) void k_super_factor(
+    const float *__restrict__ Mt, int64_t ldmt, const SuperDesc *__restrict__ sd,
+    const int32_t *__restrict__ list, const int32_t *__restrict__ sind,
+    const int32_t *__restrict__ Pinv, const int32_t *__restrict__ sup_ind,
+    const int32_t *__restrict__ sptr, TV *__restrict__ val, int32_t *__restrict__ status) {
+  constexpr int NS = 16 * T;                  // padded local size
+  constexpr int KC = kGramChunk;              // f-rows per staged chunk (512 B per row)
+  constexpr int SS = KC + 4;                  // stage row stride in floats (bank spread, 16 B aligned)
+  constexpr int RS = NS + 1;                  // TH row stride
+  constexpr int MB = 16;                      // member columns per batch
+  constexpr int PAIRS = T * (T + 1) / 2;      // lower-triangle tile pairs
+  constexpr int KS = (T <= 4) ? 4 : (T <= 6 ? 2 : 1);  // waves splitting the k-steps
+  constexpr int NG = 4 / KS;                  // groups splitting the tile pairs
+  constexpr int PPW = (PAIRS + NG - 1) / NG;  // tile pairs per wave
+  constexpr int LPT = (NS * (KC / 4) + kThreads - 1) / kThreads;  // float4 loads per thread
+  typedef typename Mfma<TC>::v4 acc_t;
+
+  // The staging chunk (FP32) and, after the Gram, TH (TC) share one buffer.
+  constexpr int BIG_BYTES = CMax<NS * SS * 4, NS * RS * int(sizeof(TC))>::v;
+  __shared__ __align__(16) unsigned char big_raw[BIG_BYTES];
+  __shared__ int32_t src[NS];
+  __shared__ TC dg[NS], dinv[NS];
+  __shared__ TC W[NS * MB];
+  float *stage = reinterpret_cast<float *>(big_raw);
+  __shared__ int32_t mcol[MB], mc[MB], moff[MB];
+
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int32_t s = list[blockIdx.x];
+  const SuperDesc D = sd[s];
+  const int ns = D.ns;
+
+  for (int p = tid; p < NS; p += kThreads) src[p] = (p < ns) ? Pinv[sind[D.rows_off + p]] : -1;
+  __syncthreads();
+
+  // ---- 1. Gram on MFMA ------------------------------------------------------
+  const int grp = wave % NG, kph = wave / NG;
+  int pa[PPW], pb[PPW];
+#pragma unroll
+  for (int u = 0; u < PPW; ++u) {
+    int pi = grp + NG * u, a = 0;
+    if (pi >= PAIRS) pi = -1;
+    if (pi >= 0) {
+      while ((a + 1) * (a + 2) / 2 <= pi) ++a;
+      pa[u] = a;
+      pb[u] = pi - a * (a + 1) / 2;
+    } else {
+      pa[u] = -1;
+      pb[u] = -1;
+    }
+  }
+  acc_t acc[PPW];
+#pragma unroll
+  for (int u = 0; u < PPW; ++u) acc[u] = acc_t{TC(0), TC(0), TC(0), TC(0)};
+
+  const int64_t nch = ldmt / KC;
+  float4 buf[LPT];
+  auto load_chunk = [&](int64_t ch) {
+#pragma unroll
+    for (int u = 0; u < LPT; ++u) {
+      const int idx = tid + u * kThreads;
+      const int p = idx / (KC / 4), c4 = idx % (KC / 4);
+      buf[u] = make_float4(0.f, 0.f, 0.f, 0.f);
+      if (p < NS && src[p] >= 0)
+        buf[u] = *reinterpret_cast<const float4 *>(Mt + int64_t(src[p]) * ldmt + ch * KC + c4 * 4);
+    }
+  };
+  load_chunk(0);
+  for (int64_t ch = 0; ch < nch; ++ch) {
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < LPT; ++u) {
+      const int idx = tid + u * kThreads;
+      const int p = idx / (KC / 4), c4 = idx % (KC / 4);
+      if (p < NS) *reinterpret_cast<float4 *>(stage + p * SS + c4 * 4) = buf[u];
+    }
+    __syncthreads();
+    if (ch + 1 < nch) load_chunk(ch + 1);
+    for (int kk = kph * 4; kk < KC; kk += 4 * KS) {
+      TC fr[T];
+#pragma unroll
+      for (int a = 0; a < T; ++a) fr[a] = TC(stage[(a * 16 + (lane & 15)) * SS + kk + (lane >> 4)]);
+#pragma unroll
+      for (int u = 0; u < PPW; ++u) {
+        if (pa[u] >= 0) {
+          TC fa = fr[0], fb = fr[0];
+#pragma unroll
+          for (int a = 0; a < T; ++a) {
+            if (a == pa[u]) fa = fr[a];
+            if (a == pb[u]) fb = fr[a];
+          }
+          acc[u] = Mfma<TC>::run(fa, fb, acc[u]);
+        }
+      }
+    }
+  }
+  // Accumulators -> TH (reversed order, both triangles); k-split phases add in turn.
+  TC *TH = reinterpret_cast<TC *>(big_raw);
+#pragma unroll
+  for (int ph = 0; ph < KS; ++ph) {
+    __syncthreads();
+    if (kph == ph) {
+#pragma unroll
+      for (int u = 0; u < PPW; ++u) {
+        if (pa[u] < 0) continue;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int p = pa[u] * 16 + Mfma<TC>::row(lane, v);
+          const int q = pb[u] * 16 + (lane & 15);
+          if (p < ns && q < ns) {
+            const int rp = ns - 1 - p, rq = ns - 1 - q;
+            const TC g = acc[u][v];
+            if (ph == 0) {
+              TH[rp * RS + rq] = g;
+              if (pa[u] != pb[u]) TH[rq * RS + rp] = g;
+            } else {
+              TH[rp * RS + rq] += g;
+              if (pa[u] != pb[u]) TH[rq * RS + rp] += g;
+            }
+          }
+        }
+      }
+    }
+  }
+  __syncthreads();
+
+  // ---- 2. Upper Cholesky TH = U^T U ----------------------------------------
+  for (int j = 0; j < ns; ++j) {
+    if (j > 0) {  // finish row j-1 (disjoint from the update of step j)
+      const int jr = j - 1;
+      for (int t = jr + tid; t < ns; t += kThreads)
+        TH[jr * RS + t] = (t == jr) ? dg[jr] : TH[jr * RS + t] * dinv[jr];
+    }
+    const TC d = TH[j * RS + j];
+    if (!(d > TC(0))) {
+      if (tid == 0) status[s] = 1;
+      return;  // uniform: every thread read the same pivot
+    }
+    const TC invd = TC(1) / d;
+    if (tid == 0) {
+      const TC u = sqrt(d);
+      dg[j] = u;
+      dinv[j] = TC(1) / u;
+    }
+    for (int r = j + 1 + (tid >> 4); r < ns; r += 16) {
+      const TC f = TH[j * RS + r] * invd;
+      for (int t = r + (tid & 15); t < ns; t += 16) TH[r * RS + t] -= f * TH[j * RS + t];
+    }
+    __syncthreads();
+  }
+  {
+    const int jr = ns - 1;
+    if (tid == 0) TH[jr * RS + jr] = dg[jr];
+  }
+  __syncthreads();
+
+  // ---- 3. KL columns of the members ------------------------------------------
+  for (int mb0 = D.mem_beg; mb0 < D.mem_end; mb0 += MB) {
+    const int nmb = (D.mem_end - mb0 < MB) ? D.mem_end - mb0 : MB;
+    if (tid < MB) {
+      if (tid < nmb) {
+        const int32_t j = sup_ind[mb0 + tid];
+        mcol[tid] = j;
+        moff[tid] = sptr[j];
+        mc[tid] = sptr[j + 1] - sptr[j] - 1;
+      } else {
+        mcol[tid] = -1;
+        moff[tid] = 0;
+        mc[tid] = -1;
+      }
+    }
+    __syncthreads();
+    int cmax = 0;
+    for (int t = 0; t < nmb; ++t) cmax = (mc[t] > cmax) ? mc[t] : cmax;
+    for (int idx = tid; idx < (cmax + 1) * MB; idx += kThreads) {
+      const int r = idx / MB, t = idx % MB;
+      W[idx] = (t < nmb && r == mc[t]) ? TC(1) : TC(0);
+    }
+    __syncthreads();
+    for (int tt = cmax; tt >= 0; --tt) {
+      for (int idx = tid; idx < (tt + 1) * MB; idx += kThreads) {
+        const int r = idx / MB, t = idx % MB;
+        if (t < nmb && tt <= mc[t]) {
+          const TC w = W[tt * MB + t] * dinv[tt];
+          if (r == tt)
+            val[moff[t] + (mc[t] - tt)] = TV(w);
+          else
+            W[r * MB + t] -= TH[r * RS + tt] * w;
+        }
+      }
+      __syncthreads();
+    }
+  }
+  if (tid == 0) status[s] = 0;
+}
+
+template <typename TV>
+__global__ __launch_bounds__(kThreads) void k_gather_values(int32_t nnz,
+                                                            const int32_t *__restrict__ tmap,
+                                                            const TV *__restrict__ val,
+                                                            TV *__restrict__ tval) {
+  const int32_t t = int32_t(blockIdx.x) * kThreads + threadIdx.x;
+  if (t < nnz) tval[t] = val[tmap[t]];
+}
+
+template <int T, typename TC, typename TV>
+hipError_t launch_t(hipStream_t st, const float *Mt, int64_t ldmt, const SuperDesc *sd,
+                    const int32_t *list, int32_t count, const int32_t *sind, const int32_t *Pinv,
+                    const int32_t *sup_ind, const int32_t *sptr, void *val, int32_t *status) {
+  if (count <= 0) return hipSuccess;
+  k_super_factor<T, TC, TV><<<unsigned(count), kThreads, 0, st>>>(
+      Mt, ldmt, sd, list, sind, Pinv, sup_ind, sptr, static_cast<TV *>(val), status);
+  return hipGetLastError();
+}
+
+template <typename TC, typename TV>
+hipError_t launch_prec(hipStream_t st, int tiles, const float *Mt, int64_t ldmt,
+                       const SuperDesc *sd, const int32_t *list, int32_t count,
+                       const int32_t *sind, const int32_t *Pinv, const int32_t *sup_ind,
+                       const int32_t *sptr, void *val, int32_t *status) {
+  switch (tiles) {
+    case 1: return launch_t<1, TC, TV>(st, Mt, ldmt, sd, list, count, sind, Pinv, sup_ind, sptr, val, status);
+    case 2: return launch_t<2, TC, TV>(st, Mt, ldmt, sd, list, count, sind, Pinv, sup_ind, sptr, val, status);
+    case 3: return launch_t<3, TC, TV>(st, Mt, ldmt, sd, list, count, sind, Pinv, sup_ind, sptr, val, status);
+    case 4: return launch_t<4, TC, TV>(st, Mt, ldmt, sd, list, count, sind, Pinv, sup_ind, sptr, val, status);
+    case 6: return launch_t<6, TC, TV>(st, Mt, ldmt, sd, list, count, sind, Pinv, sup_ind, sptr, val, status);
+    case 8: return launch_t<8, TC, TV>(st, Mt, ldmt, sd, list, count, sind, Pinv, sup_ind, sptr, val, status);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+}  // namespace
+
+hipError_t launch_transpose(hipStream_t st, const float *M, int64_t m, int64_t n, float *Mt,
+                            int64_t ldmt) {
+  dim3 grid(unsigned((n + 63) / 64), unsigned(ldmt / 64));
+  if (n % 4 == 0)
+    k_transpose<true><<<grid, kThreads, 0, st>>>(M, m, n, Mt, ldmt);
+  else
+    k_transpose<false><<<grid, kThreads, 0, st>>>(M, m, n, Mt, ldmt);
+  return hipGetLastError();
+}
+
+hipError_t launch_super_factor(hipStream_t st, int tiles, int precision, const float *Mt,
+                               int64_t ldmt, const SuperDesc *sd, const int32_t *list,
+                               int32_t count, const int32_t *sind, const int32_t *Pinv,
+                               const int32_t *sup_ind, const int32_t *sptr, void *val,
+                               int32_t *status) {
+  if (precision == 0)
+    return launch_prec<double, double>(st, tiles, Mt, ldmt, sd, list, count, sind, Pinv, sup_ind, sptr, val, status);
+  if (precision == 1)
+    return launch_prec<float, float>(st, tiles, Mt, ldmt, sd, list, count, sind, Pinv, sup_ind, sptr, val, status);
+  return launch_prec<double, float>(st, tiles, Mt, ldmt, sd, list, count, sind, Pinv, sup_ind, sptr, val, status);
+}
+
+hipError_t launch_gather_values(hipStream_t st, int precision, int32_t nnz, const int32_t *tmap,
+                                const void *val, void *tval) {
+  const unsigned grid = unsigned((nnz + kThreads - 1) / kThreads);
+  if (nnz <= 0) return hipSuccess;
+  if (precision == 0)
+    k_gather_values<double><<<grid, kThreads, 0, st>>>(nnz, tmap, static_cast<const double *>(val),
+                                                        static_cast<double *>(tval));
+  else
+    k_gather_values<float><<<grid, kThreads, 0, st>>>(nnz, tmap, static_cast<const float *>(val),
+                                                       static_cast<float *>(tval));
+  return hipGetLastError();
+}
+
+}  // namespace mfs

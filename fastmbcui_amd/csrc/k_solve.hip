@@ -1,0 +1,788 @@
+// k_solve.hip -- PCG on M^T M with the supernodal KL preconditioner z = L (L^T r).
+//
+// Replaces pcg_guesss / pcg_VpL_guesss (src/kl_chol.cu:891-1264), whose every
+// step was a separate cuBLAS/cuSPARSE call with a host round trip per dot.
+// Here a batch of K right-hand sides moves through seven kernels per
+// iteration, the dots are FP64 block partials finished by the last block to
+// arrive, and alpha/beta/convergence live in device memory (PcgState), so the
+// host only polls a flag every few iterations.
+//
+//   y = M p          k_mv_rows     one HBM pass over M for the whole batch
+//   part = M^T y     k_mv_cols     second HBM pass, row blocks -> partials
+//   q = sum(part)    k_mv_reduce   + dot(p, q)
+//   x += a p, r -= a q            k_upd_xr   + dot(r, r), convergence
+//   t = L^T r        k_spmv (CSC of L read as CSR of L^T)
+//   z = L t          k_spmv (CSR of L)  + dot(r, z)
+//   p = z + b p      k_upd_p    (+ copy of p in original order for the matvec)
+#include "mfs_internal.h"
+
+namespace mfs {
+namespace {
+
+// ------------------------------------------------------------------ reductions
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+__device__ inline float wave_sum(float v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// Sum K values over the block into red[0..K) (valid after the call, all threads).
+template <int K>
+__device__ inline void block_sum(double (&v)[K], double *red) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < K; ++r) {
+    const double s = wave_sum(v[r]);
+    if (lane == 0) red[wave * K + r] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x < K) {
+    const int r = threadIdx.x;
+    red[4 * K + r] = ((red[r] + red[K + r]) + red[2 * K + r]) + red[3 * K + r];
+  }
+  __syncthreads();
+}
+
+// Publish this block's K partials, then report whether it is the last block to
+// arrive.  Thread 0 stores the partials and releases them with a device-scope
+// fence before the arrival atomic; every thread of the last block fences again
+// (acquire) before reading, so partials written on another XCD (own L2) are
+// seen.  The last block re-arms the counter, so launches replay without a
+// memset.
+template <int K>
+__device__ inline bool publish_and_arrive(const double *blocksum, double *dpart, uint32_t *counter,
+                                          uint32_t nblocks, int *flag) {
+  if (threadIdx.x == 0) {
+    for (int r = 0; r < K; ++r) dpart[size_t(blockIdx.x) * K + r] = blocksum[r];
+    __threadfence();
+    const uint32_t prev = atomicAdd(counter, 1u);
+    *flag = (prev == nblocks - 1);
+    if (*flag) atomicExch(counter, 0u);
+  }
+  __syncthreads();
+  if (*flag) __threadfence();
+  return *flag != 0;
+}
+
+// Sum the published partials with the whole (last) block: thread t takes
+// blocks t, t+256, ..., then a fixed-shape block reduction.  Deterministic;
+// totals land in red[4K .. 5K) for every thread.
+template <int K>
+__device__ inline void gather_partials(const double *dpart, uint32_t nblocks, double *red) {
+  double v[K];
+#pragma unroll
+  for (int r = 0; r < K; ++r) v[r] = 0.0;
+  for (uint32_t b = threadIdx.x; b < nblocks; b += blockDim.x)
+#pragma unroll
+    for (int r = 0; r < K; ++r) v[r] += dpart[size_t(b) * K + r];
+  block_sum<K>(v, red);
+}
+
+__device__ inline bool batch_done(const PcgState *st) { return st != nullptr && st->n_active == 0; }
+
+// ------------------------------------------------------------------ y = M x
+// A block of 8 waves owns 8*RW rows; each wave owns RW rows and its lanes
+// stream float4 of M along them.  x is staged in LDS in chunks of CH columns,
+// right-hand-side major (xs[r][c]) so a lane's four columns for one
+// right-hand side are one conflict-free 16-byte read, double-buffered so the
+// next chunk is staged while this one is consumed: one barrier per chunk.
+// Optional Bsub gives y = b - M x.
+constexpr int kRowsThreads = 512;
+
+template <typename TA, int K>
+struct RowsCfg {
+  static constexpr int RW = (K == 1) ? 8 : ((sizeof(TA) == 8 && K >= 16) ? 2 : 4);
+  // two buffers of K*CH*sizeof(TA) within 64 KB, CH a multiple of 256, at most 2048
+  static constexpr int CHB = 32768 / (K * int(sizeof(TA)));
+  static constexpr int CH = CHB >= 2048 ? 2048 : (CHB >= 1024 ? 1024 : (CHB >= 512 ? 512 : 256));
+};
+
+template <typename TA, int K, int RW, bool VEC>
+__global__ __launch_bounds__(kRowsThreads) void k_mv_rows(const float *__restrict__ M, int64_t m,
+                                                          int64_t n, const TA *__restrict__ X,
+                                                          const float *__restrict__ Bsub,
+                                                          int64_t ldb, int64_t col0, int kval,
+                                                          TA *__restrict__ Y,
+                                                          const PcgState *st) {
+  if (batch_done(st)) return;
+  constexpr int CH = RowsCfg<TA, K>::CH;
+  __shared__ __align__(16) TA xs[2][K * CH];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int64_t row0 = (int64_t(blockIdx.x) * (kRowsThreads / 64) + wave) * RW;
+  TA acc[RW][K];
+#pragma unroll
+  for (int i = 0; i < RW; ++i)
+#pragma unroll
+    for (int r = 0; r < K; ++r) acc[i][r] = TA(0);
+
+  auto stage = [&](int64_t c0, TA *dst) {
+    for (int idx = threadIdx.x; idx < CH * K; idx += kRowsThreads) {
+      const int cl = idx / K, r = idx % K;
+      const int64_t c = c0 + cl;
+      dst[r * CH + cl] = (c < n) ? X[c * K + r] : TA(0);
+    }
+  };
+  stage(0, xs[0]);
+  __syncthreads();
+  int buf = 0;
+  for (int64_t c0 = 0; c0 < n; c0 += CH) {
+    if (c0 + CH < n) stage(c0 + CH, xs[buf ^ 1]);
+    const TA *xb = xs[buf];
+#pragma unroll 2
+    for (int u = 0; u < CH / 256; ++u) {
+      const int cl = u * 256 + lane * 4;
+      const int64_t cc = c0 + cl;
+      float4 mv[RW];
+#pragma unroll
+      for (int i = 0; i < RW; ++i) {
+        const int64_t f = row0 + i;
+        const float *rowp = M + f * n;
+        if (f < m && VEC && cc + 3 < n) {
+          mv[i] = *reinterpret_cast<const float4 *>(rowp + cc);
+        } else if (f < m) {
+          mv[i].x = (cc + 0 < n) ? rowp[cc + 0] : 0.f;
+          mv[i].y = (cc + 1 < n) ? rowp[cc + 1] : 0.f;
+          mv[i].z = (cc + 2 < n) ? rowp[cc + 2] : 0.f;
+          mv[i].w = (cc + 3 < n) ? rowp[cc + 3] : 0.f;
+        } else {
+          mv[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < K; ++r) {
+        const TA *xr = xb + r * CH + cl;
+        const TA x0 = xr[0], x1 = xr[1], x2 = xr[2], x3 = xr[3];
+#pragma unroll
+        for (int i = 0; i < RW; ++i) {
+          TA s = acc[i][r];
+          s += TA(mv[i].x) * x0;
+          s += TA(mv[i].y) * x1;
+          s += TA(mv[i].z) * x2;
+          s += TA(mv[i].w) * x3;
+          acc[i][r] = s;
+        }
+      }
+    }
+    __syncthreads();
+    buf ^= 1;
+  }
+#pragma unroll
+  for (int i = 0; i < RW; ++i) {
+    const int64_t f = row0 + i;
+#pragma unroll
+    for (int r = 0; r < K; ++r) {
+      const TA s = wave_sum(acc[i][r]);
+      if (lane == 0 && f < m) {
+        if (Bsub != nullptr) {
+          const TA b = (r < kval) ? TA(Bsub[f * ldb + col0 + r]) : TA(0);
+          Y[f * K + r] = b - s;
+        } else {
+          Y[f * K + r] = s;
+        }
+      }
+    }
+  }
+}
+
+// ------------------------------------------------------------------ part = M^T y
+// Block (cx, rb): columns [cx*1024, +1024) (4 per thread, float4), rows of row
+// block rb.  FP32 products are summed over 32-row chunks in TA and the chunk
+// sums are carried in FP64, so the FP32 path keeps a compensated running sum.
+constexpr int kRowChunk = 32;
+
+template <typename TA, int K, bool VEC>
+__global__ __launch_bounds__(kThreads) void k_mv_cols(const float *__restrict__ M, int64_t m,
+                                                      int64_t n, const TA *__restrict__ Y,
+                                                      int64_t rows_per_block,
+                                                      double *__restrict__ part,
+                                                      const PcgState *st) {
+  if (batch_done(st)) return;
+  __shared__ TA ys[kRowChunk * K];
+  const int64_t i0 = (int64_t(blockIdx.x) * kThreads + threadIdx.x) * 4;
+  const int64_t fb = int64_t(blockIdx.y) * rows_per_block;
+  const int64_t fe = (fb + rows_per_block < m) ? fb + rows_per_block : m;
+  double acc[4][K];
+#pragma unroll
+  for (int j = 0; j < 4; ++j)
+#pragma unroll
+    for (int r = 0; r < K; ++r) acc[j][r] = 0.0;
+
+  for (int64_t fc = fb; fc < fe; fc += kRowChunk) {
+    const int rows = int((fe - fc < kRowChunk) ? fe - fc : kRowChunk);
+    __syncthreads();
+    for (int idx = threadIdx.x; idx < rows * K; idx += kThreads) ys[idx] = Y[fc * K + idx];
+    __syncthreads();
+    TA c[4][K];
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int r = 0; r < K; ++r) c[j][r] = TA(0);
+    if (i0 < n) {
+      int f = 0;
+      for (; f + 4 <= rows; f += 4) {
+        float4 v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const float *rowp = M + (fc + f + u) * n;
+          if (VEC && i0 + 3 < n) {
+            v[u] = *reinterpret_cast<const float4 *>(rowp + i0);
+          } else {
+            v[u].x = rowp[i0];
+            v[u].y = (i0 + 1 < n) ? rowp[i0 + 1] : 0.f;
+            v[u].z = (i0 + 2 < n) ? rowp[i0 + 2] : 0.f;
+            v[u].w = (i0 + 3 < n) ? rowp[i0 + 3] : 0.f;
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+          for (int r = 0; r < K; ++r) {
+            const TA yv = ys[(f + u) * K + r];
+            c[0][r] += TA(v[u].x) * yv;
+            c[1][r] += TA(v[u].y) * yv;
+            c[2][r] += TA(v[u].z) * yv;
+            c[3][r] += TA(v[u].w) * yv;
+          }
+      }
+      for (; f < rows; ++f) {
+        const float *rowp = M + (fc + f) * n;
+        float4 v;
+        v.x = rowp[i0];
+        v.y = (i0 + 1 < n) ? rowp[i0 + 1] : 0.f;
+        v.z = (i0 + 2 < n) ? rowp[i0 + 2] : 0.f;
+        v.w = (i0 + 3 < n) ? rowp[i0 + 3] : 0.f;
+#pragma unroll
+        for (int r = 0; r < K; ++r) {
+          const TA yv = ys[f * K + r];
+          c[0][r] += TA(v.x) * yv;
+          c[1][r] += TA(v.y) * yv;
+          c[2][r] += TA(v.z) * yv;
+          c[3][r] += TA(v.w) * yv;
+        }
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int r = 0; r < K; ++r) acc[j][r] += double(c[j][r]);
+  }
+  if (i0 < n) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      if (i0 + j < n) {
+        double *dst = part + (int64_t(blockIdx.y) * n + i0 + j) * K;
+#pragma unroll
+        for (int r = 0; r < K; ++r) dst[r] = acc[j][r];
+      }
+    }
+  }
+}
+
+// ------------------------------------------------------------------ reductions of part
+enum ReduceMode { RED_Q = 0, RED_R0 = 1, RED_ORIG = 2 };
+
+// RED_Q:    q_new[P[i]] = s, dot(pv, s) -> st->dot_pq
+// RED_R0:   r_new[P[i]] = s, ||s||^2 -> rhs2/thresh/active (PCG start)
+// RED_ORIG: o[i] = s (double, original order), ||s||^2 -> out2
+template <typename TA, int K, int MODE>
+__global__ __launch_bounds__(kThreads) void k_mv_reduce(
+    const double *__restrict__ part, int32_t row_blocks, int64_t n, const int32_t *__restrict__ P,
+    const TA *__restrict__ pv, TA *__restrict__ out_new, double *__restrict__ out_orig,
+    double *__restrict__ dpart, PcgState *st, double *out2, double tol, int kval, int32_t maxit) {
+  if (MODE == RED_Q && batch_done(st)) return;
+  __shared__ double red[5 * K];
+  __shared__ int last;
+  const int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x;
+  double d[K];
+#pragma unroll
+  for (int r = 0; r < K; ++r) d[r] = 0.0;
+  if (i < n) {
+    double s[K];
+#pragma unroll
+    for (int r = 0; r < K; ++r) s[r] = 0.0;
+    for (int32_t b = 0; b < row_blocks; ++b) {
+      const double *src = part + (int64_t(b) * n + i) * K;
+#pragma unroll
+      for (int r = 0; r < K; ++r) s[r] += src[r];
+    }
+    if (MODE == RED_ORIG) {
+#pragma unroll
+      for (int r = 0; r < K; ++r) {
+        out_orig[i * K + r] = s[r];
+        d[r] = s[r] * s[r];
+      }
+    } else {
+      const int64_t dst = int64_t(P[i]) * K;
+#pragma unroll
+      for (int r = 0; r < K; ++r) {
+        const TA v = TA(s[r]);
+        out_new[dst + r] = v;
+        d[r] = (MODE == RED_Q) ? double(pv[i * K + r]) * double(v) : double(v) * double(v);
+      }
+    }
+  }
+  block_sum<K>(d, red);
+  const uint32_t ctr = (MODE == RED_Q) ? C_PQ : (MODE == RED_R0 ? C_R0 : C_NORM);
+  if (publish_and_arrive<K>(red + 4 * K, dpart, &st->counter[ctr], gridDim.x, &last)) {
+    gather_partials<K>(dpart, gridDim.x, red);
+    if (threadIdx.x < K) {
+      const int r = threadIdx.x;
+      const double tot = red[4 * K + r];
+      if (MODE == RED_Q) {
+        st->dot_pq[r] = tot;
+      } else if (MODE == RED_R0) {
+        st->rhs2[r] = tot;
+        st->thresh[r] = tol * tol * tot;
+        st->active[r] = (r < kval && tot >= 1e-16 && maxit > 0) ? 1 : 0;  // kl_chol.cu:950
+        st->iters[r] = 0;
+        st->dot_rr[r] = tot;
+      } else {
+        out2[r] = tot;
+      }
+    }
+    if (MODE == RED_R0) {
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        int na = 0;
+        for (int r = 0; r < K; ++r) na += st->active[r];
+        st->n_active = na;
+        st->iter = 0;
+        st->maxit = maxit;
+      }
+    }
+  }
+}
+
+// r_new[c] = b[Pinv[c]] for source-space right-hand sides (VpL), with ||r||^2.
+template <typename TA, int K>
+__global__ __launch_bounds__(kThreads) void k_load_source_rhs(
+    int64_t n, const int32_t *__restrict__ Pinv, const float *__restrict__ B, int64_t ldb,
+    int64_t col0, const double *__restrict__ B64, int kval, TA *__restrict__ r_new,
+    double *__restrict__ dpart, PcgState *st, double tol, int32_t maxit,
+    const double *__restrict__ thresh_abs) {
+  __shared__ double red[5 * K];
+  __shared__ int last;
+  const int64_t c = int64_t(blockIdx.x) * kThreads + threadIdx.x;
+  double d[K];
+#pragma unroll
+  for (int r = 0; r < K; ++r) d[r] = 0.0;
+  if (c < n) {
+    const int64_t o = Pinv[c];
+#pragma unroll
+    for (int r = 0; r < K; ++r) {
+      TA v = TA(0);
+      if (r < kval) v = B64 ? TA(B64[o * K + r]) : TA(B[o * ldb + col0 + r]);
+      r_new[c * K + r] = v;
+      d[r] = double(v) * double(v);
+    }
+  }
+  block_sum<K>(d, red);
+  if (publish_and_arrive<K>(red + 4 * K, dpart, &st->counter[C_R0], gridDim.x, &last)) {
+    gather_partials<K>(dpart, gridDim.x, red);
+    if (threadIdx.x < K) {
+      const int r = threadIdx.x;
+      const double tot = red[4 * K + r];
+      const double th = thresh_abs ? thresh_abs[r] : tol * tol * tot;
+      st->rhs2[r] = tot;
+      st->thresh[r] = th;
+      st->active[r] = (r < kval && tot >= 1e-16 && maxit > 0 && !(tot < th)) ? 1 : 0;
+      st->iters[r] = 0;
+      st->dot_rr[r] = tot;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      int na = 0;
+      for (int r = 0; r < K; ++r) na += st->active[r];
+      st->n_active = na;
+      st->iter = 0;
+      st->maxit = maxit;
+    }
+  }
+}
+
+// y = B (FP32 boundary data, strided) as TA, m x K.
+template <typename TA, int K>
+__global__ __launch_bounds__(kThreads) void k_load_boundary(int64_t m, const float *__restrict__ B,
+                                                            int64_t ldb, int64_t col0, int kval,
+                                                            TA *__restrict__ y) {
+  const int64_t f = int64_t(blockIdx.x) * kThreads + threadIdx.x;
+  if (f >= m) return;
+#pragma unroll
+  for (int r = 0; r < K; ++r) y[f * K + r] = (r < kval) ? TA(B[f * ldb + col0 + r]) : TA(0);
+}
+
+template <typename TA>
+__global__ __launch_bounds__(kThreads) void k_zero(int64_t count, TA *__restrict__ a) {
+  const int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x;
+  if (i < count) a[i] = TA(0);
+}
+
+// ------------------------------------------------------------------ x, r update
+template <typename TA, int K>
+__global__ __launch_bounds__(kThreads) void k_upd_xr(int64_t n, TA *__restrict__ x,
+                                                     TA *__restrict__ r, const TA *__restrict__ p,
+                                                     const TA *__restrict__ q,
+                                                     double *__restrict__ dpart, PcgState *st) {
+  if (batch_done(st)) return;
+  __shared__ double red[5 * K];
+  __shared__ int last;
+  const int64_t c = int64_t(blockIdx.x) * kThreads + threadIdx.x;
+  double d[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) d[k] = 0.0;
+  if (c < n) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      if (st->active[k]) {
+        const TA alpha = TA(st->absnew[k] / st->dot_pq[k]);
+        x[c * K + k] += alpha * p[c * K + k];
+        const TA rv = r[c * K + k] + (-alpha) * q[c * K + k];
+        r[c * K + k] = rv;
+        d[k] = double(rv) * double(rv);
+      }
+    }
+  }
+  block_sum<K>(d, red);
+  if (publish_and_arrive<K>(red + 4 * K, dpart, &st->counter[C_RR], gridDim.x, &last)) {
+    gather_partials<K>(dpart, gridDim.x, red);
+    if (threadIdx.x < K) {
+      const int k = threadIdx.x;
+      const double tot = red[4 * K + k];
+      if (st->active[k]) {
+        st->iters[k] += 1;
+        st->dot_rr[k] = tot;
+        if (tot < st->thresh[k]) st->active[k] = 0;   // kl_chol.cu:1038
+      }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      const int it = st->iter + 1;
+      st->iter = it;
+      int na = 0;
+      for (int k = 0; k < K; ++k) {
+        if (it >= st->maxit) st->active[k] = 0;       // while (i < maxits)
+        na += st->active[k];
+      }
+      st->n_active = na;
+    }
+  }
+}
+
+// ------------------------------------------------------------------ sparse L / L^T
+// out[row] = sum_e val[e] * in[idx[e]] for K right-hand sides; 8 lanes per row.
+// With DOT, also dot(out, w) per right-hand side -> st->dot_rz.
+template <typename TA, int K, bool DOT>
+__global__ __launch_bounds__(kThreads) void k_spmv(int64_t n, const int32_t *__restrict__ ptr,
+                                                   const int32_t *__restrict__ idx,
+                                                   const TA *__restrict__ val,
+                                                   const TA *__restrict__ in, TA *__restrict__ out,
+                                                   const TA *__restrict__ w,
+                                                   double *__restrict__ dpart, PcgState *st) {
+  if (batch_done(st)) return;
+  constexpr int G = 8;                         // lanes per row
+  constexpr int SPLIT = (K >= G) ? 1 : G / K;  // lanes sharing one right-hand side
+  constexpr int RPL = (K > G) ? K / G : 1;     // right-hand sides per lane
+  __shared__ double red[5 * K];
+  __shared__ int last;
+  const int g = threadIdx.x & (G - 1);
+  const int64_t row = (int64_t(blockIdx.x) * kThreads + threadIdx.x) / G;
+  const int rbase = (K >= G) ? g : (g % K);
+  const int phase = (K >= G) ? 0 : (g / K);
+  TA acc[RPL];
+#pragma unroll
+  for (int u = 0; u < RPL; ++u) acc[u] = TA(0);
+  if (row < n) {
+    const int32_t b = ptr[row], e = ptr[row + 1];
+    for (int32_t t = b + phase; t < e; t += SPLIT) {
+      const TA v = val[t];
+      const int64_t col = idx[t];
+#pragma unroll
+      for (int u = 0; u < RPL; ++u) acc[u] += v * in[col * K + rbase + u * G];
+    }
+  }
+#pragma unroll
+  for (int off = K; off < G; off <<= 1)
+#pragma unroll
+    for (int u = 0; u < RPL; ++u) acc[u] += __shfl_xor(acc[u], off, 64);
+  double d[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) d[k] = 0.0;
+  if (row < n && phase == 0) {
+#pragma unroll
+    for (int u = 0; u < RPL; ++u) {
+      const int k = rbase + u * G;
+      out[row * K + k] = acc[u];
+      if (DOT) {
+#pragma unroll
+        for (int kk = 0; kk < K; ++kk)
+          if (kk == k) d[kk] = double(acc[u]) * double(w[row * K + k]);
+      }
+    }
+  }
+  if (DOT) {
+    block_sum<K>(d, red);
+    if (publish_and_arrive<K>(red + 4 * K, dpart, &st->counter[C_RZ], gridDim.x, &last)) {
+      gather_partials<K>(dpart, gridDim.x, red);
+      if (threadIdx.x < K) st->dot_rz[threadIdx.x] = red[4 * K + threadIdx.x];
+    }
+  }
+}
+
+// ------------------------------------------------------------------ p update
+// INIT: p = z, absnew = r^T z (kl_chol.cu:957-982).  Else p = z + beta p
+// (kl_chol.cu:1056-1077).  Also scatters p into original order for the matvec.
+template <typename TA, int K, bool INIT>
+__global__ __launch_bounds__(kThreads) void k_upd_p(int64_t n, const int32_t *__restrict__ Pinv,
+                                                    TA *__restrict__ p, const TA *__restrict__ z,
+                                                    TA *__restrict__ pv, PcgState *st) {
+  if (batch_done(st)) return;
+  __shared__ int last;
+  const int64_t c = int64_t(blockIdx.x) * kThreads + threadIdx.x;
+  if (c < n) {
+    const int64_t o = Pinv[c];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      TA v;
+      if (INIT) {
+        v = z[c * K + k];
+      } else if (st->active[k]) {
+        const TA beta = TA(st->dot_rz[k] / st->absnew[k]);
+        v = beta * p[c * K + k] + z[c * K + k];
+      } else {
+        v = p[c * K + k];
+      }
+      p[c * K + k] = v;
+      pv[o * K + k] = v;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence();
+    const uint32_t prev = atomicAdd(&st->counter[C_P], 1u);
+    last = (prev == gridDim.x - 1);
+    if (last) {
+      atomicExch(&st->counter[C_P], 0u);
+      __threadfence();
+      for (int k = 0; k < K; ++k)
+        if (INIT || st->active[k]) st->absnew[k] = st->dot_rz[k];
+    }
+  }
+}
+
+// X_out[i] = x_new[P[i]] (FP32, strided) and optionally X64[i] += x_new[P[i]].
+template <typename TA, int K>
+__global__ __launch_bounds__(kThreads) void k_finish(int64_t n, const int32_t *__restrict__ P,
+                                                     const TA *__restrict__ x, float *__restrict__ X,
+                                                     int64_t ldx, int64_t col0, int kval,
+                                                     double *__restrict__ X64) {
+  const int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x;
+  if (i >= n) return;
+  const int64_t s = int64_t(P[i]) * K;
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    if (k >= kval) continue;
+    if (X64 != nullptr) {
+      const double v = X64[i * K + k] + double(x[s + k]);
+      X64[i * K + k] = v;
+      if (X != nullptr) X[i * ldx + col0 + k] = float(v);
+    } else if (X != nullptr) {
+      X[i * ldx + col0 + k] = float(x[s + k]);
+    }
+  }
+}
+
+// ------------------------------------------------------------------ launch helpers
+inline unsigned blocks_for(int64_t count) { return unsigned((count + kThreads - 1) / kThreads); }
+
+template <typename TA, int K>
+void mv_rows(hipStream_t s, const SolveArgs<TA> &a, const TA *X, const float *Bsub, int64_t ldb,
+             int64_t col0, int kval, TA *Y, const PcgState *st) {
+  constexpr int RW = RowsCfg<TA, K>::RW;
+  constexpr int rows_per_block = (kRowsThreads / 64) * RW;
+  const unsigned grid = unsigned((a.m + rows_per_block - 1) / rows_per_block);
+  if (a.n % 4 == 0)
+    k_mv_rows<TA, K, RW, true><<<grid, kRowsThreads, 0, s>>>(a.M, a.m, a.n, X, Bsub, ldb, col0, kval, Y, st);
+  else
+    k_mv_rows<TA, K, RW, false><<<grid, kRowsThreads, 0, s>>>(a.M, a.m, a.n, X, Bsub, ldb, col0, kval, Y, st);
+}
+
+template <typename TA, int K>
+void mv_cols(hipStream_t s, const SolveArgs<TA> &a, const TA *Y, const PcgState *st) {
+  const int64_t rpb = (a.m + a.row_blocks - 1) / a.row_blocks;
+  dim3 grid(unsigned((a.n + 4 * kThreads - 1) / (4 * kThreads)), unsigned(a.row_blocks));
+  if (a.n % 4 == 0)
+    k_mv_cols<TA, K, true><<<grid, kThreads, 0, s>>>(a.M, a.m, a.n, Y, rpb, a.part, st);
+  else
+    k_mv_cols<TA, K, false><<<grid, kThreads, 0, s>>>(a.M, a.m, a.n, Y, rpb, a.part, st);
+}
+
+typedef void (*TrackFn)(void *, int, int);
+inline void track(TrackFn fn, void *ctx, int cls, int phase) {
+  if (fn) fn(ctx, cls, phase);
+}
+
+template <typename TA, int K>
+hipError_t init_k(hipStream_t s, const SolveArgs<TA> &a, const float *B, int64_t ldb, int64_t col0,
+                  int rhs_space, int kval, double tol, int32_t maxit, const double *B64,
+                  const double *thresh_abs) {
+  const unsigned gn = blocks_for(a.n);
+  k_zero<TA><<<blocks_for(a.n * K), kThreads, 0, s>>>(a.n * K, a.x);
+  if (rhs_space == 0) {
+    k_load_boundary<TA, K><<<blocks_for(a.m), kThreads, 0, s>>>(a.m, B, ldb, col0, kval, a.y);
+    mv_cols<TA, K>(s, a, a.y, nullptr);
+    k_mv_reduce<TA, K, RED_R0><<<gn, kThreads, 0, s>>>(a.part, a.row_blocks, a.n, a.P, nullptr,
+                                                        a.r, nullptr, a.dpart, a.st, nullptr, tol,
+                                                        kval, maxit);
+  } else {
+    k_load_source_rhs<TA, K><<<gn, kThreads, 0, s>>>(a.n, a.Pinv, B, ldb, col0, B64, kval, a.r,
+                                                      a.dpart, a.st, tol, maxit, thresh_abs);
+  }
+  // z = L (L^T r); p = z; absnew = r^T z.
+  const unsigned gs = unsigned((a.n * 8 + kThreads - 1) / kThreads);
+  k_spmv<TA, K, false><<<gs, kThreads, 0, s>>>(a.n, a.sptr, a.sind, a.lval, a.r, a.t, nullptr,
+                                               nullptr, a.st);
+  k_spmv<TA, K, true><<<gs, kThreads, 0, s>>>(a.n, a.tptr, a.tcol, a.ltval, a.t, a.z, a.r, a.dpart,
+                                              a.st);
+  k_upd_p<TA, K, true><<<gn, kThreads, 0, s>>>(a.n, a.Pinv, a.p, a.z, a.pv, a.st);
+  return hipGetLastError();
+}
+
+template <typename TA, int K>
+hipError_t iter_k(hipStream_t s, const SolveArgs<TA> &a, TrackFn fn, void *tctx) {
+  const unsigned gn = blocks_for(a.n);
+  track(fn, tctx, 1, 0);
+  mv_rows<TA, K>(s, a, a.pv, nullptr, 0, 0, K, a.y, a.st);
+  track(fn, tctx, 1, 1);
+  track(fn, tctx, 2, 0);
+  mv_cols<TA, K>(s, a, a.y, a.st);
+  track(fn, tctx, 2, 1);
+  k_mv_reduce<TA, K, RED_Q><<<gn, kThreads, 0, s>>>(a.part, a.row_blocks, a.n, a.P, a.pv, a.q,
+                                                     nullptr, a.dpart, a.st, nullptr, 0.0, K, 0);
+  k_upd_xr<TA, K><<<gn, kThreads, 0, s>>>(a.n, a.x, a.r, a.p, a.q, a.dpart, a.st);
+  const unsigned gs = unsigned((a.n * 8 + kThreads - 1) / kThreads);
+  k_spmv<TA, K, false><<<gs, kThreads, 0, s>>>(a.n, a.sptr, a.sind, a.lval, a.r, a.t, nullptr,
+                                               nullptr, a.st);
+  k_spmv<TA, K, true><<<gs, kThreads, 0, s>>>(a.n, a.tptr, a.tcol, a.ltval, a.t, a.z, a.r, a.dpart,
+                                              a.st);
+  k_upd_p<TA, K, false><<<gn, kThreads, 0, s>>>(a.n, a.Pinv, a.p, a.z, a.pv, a.st);
+  return hipGetLastError();
+}
+
+template <typename TA, int K>
+hipError_t finish_k(hipStream_t s, const SolveArgs<TA> &a, float *X, int64_t ldx, int64_t col0,
+                    int kval, double *X64) {
+  k_finish<TA, K><<<blocks_for(a.n), kThreads, 0, s>>>(a.n, a.P, a.x, X, ldx, col0, kval, X64);
+  return hipGetLastError();
+}
+
+template <int K>
+hipError_t residual_k(hipStream_t s, const SolveArgs<double> &a, const float *B, int64_t ldb,
+                      int64_t col0, int kval, const double *X64, double *R, double *out2,
+                      TrackFn fn, void *tctx) {
+  if (X64 != nullptr) {
+    track(fn, tctx, 4, 0);
+    mv_rows<double, K>(s, a, X64, B, ldb, col0, kval, a.y, nullptr);
+    track(fn, tctx, 4, 1);
+  } else {
+    k_load_boundary<double, K><<<blocks_for(a.m), kThreads, 0, s>>>(a.m, B, ldb, col0, kval, a.y);
+  }
+  track(fn, tctx, 5, 0);
+  mv_cols<double, K>(s, a, a.y, nullptr);
+  track(fn, tctx, 5, 1);
+  k_mv_reduce<double, K, RED_ORIG><<<blocks_for(a.n), kThreads, 0, s>>>(
+      a.part, a.row_blocks, a.n, a.P, nullptr, nullptr, R, a.dpart, a.st, out2, 0.0, kval, 0);
+  return hipGetLastError();
+}
+
+}  // namespace
+
+#define MFS_DISPATCH_K(K, CALL)                  \
+  switch (K) {                                   \
+    case 1: { constexpr int KK = 1; return CALL; } \
+    case 2: { constexpr int KK = 2; return CALL; } \
+    case 4: { constexpr int KK = 4; return CALL; } \
+    case 8: { constexpr int KK = 8; return CALL; } \
+    case 16: { constexpr int KK = 16; return CALL; } \
+    default: return hipErrorInvalidValue;        \
+  }
+
+template <typename TA>
+hipError_t pcg_init(hipStream_t s, const SolveArgs<TA> &a, const float *B, int64_t ldb,
+                    int64_t col0, int rhs_space, int kval, double tol, int32_t maxit,
+                    const double *B64, const double *thresh_abs) {
+  MFS_DISPATCH_K(a.K, (init_k<TA, KK>(s, a, B, ldb, col0, rhs_space, kval, tol, maxit, B64, thresh_abs)));
+}
+
+template <typename TA>
+hipError_t pcg_iteration(hipStream_t s, const SolveArgs<TA> &a, TrackFn fn, void *tctx) {
+  MFS_DISPATCH_K(a.K, (iter_k<TA, KK>(s, a, fn, tctx)));
+}
+
+template <typename TA>
+hipError_t pcg_finish(hipStream_t s, const SolveArgs<TA> &a, float *X, int64_t ldx, int64_t col0,
+                      int kval, double *X64) {
+  MFS_DISPATCH_K(a.K, (finish_k<TA, KK>(s, a, X, ldx, col0, kval, X64)));
+}
+
+hipError_t residual64(hipStream_t s, const SolveArgs<double> &a, const float *B, int64_t ldb,
+                      int64_t col0, int kval, const double *X64, double *R, double *out2,
+                      TrackFn fn, void *tctx) {
+  MFS_DISPATCH_K(a.K, (residual_k<KK>(s, a, B, ldb, col0, kval, X64, R, out2, fn, tctx)));
+}
+
+template hipError_t pcg_init<double>(hipStream_t, const SolveArgs<double> &, const float *, int64_t,
+                                     int64_t, int, int, double, int32_t, const double *,
+                                     const double *);
+template hipError_t pcg_init<float>(hipStream_t, const SolveArgs<float> &, const float *, int64_t,
+                                    int64_t, int, int, double, int32_t, const double *,
+                                    const double *);
+
+// thr[r] = (frac * tol)^2 * nb2[r]: absolute inner thresholds from ||M^T b||^2.
+__global__ void k_inner_thresholds(const double *nb2, double tol_frac, int K, double *thr) {
+  const int r = threadIdx.x;
+  if (r < K) thr[r] = tol_frac * tol_frac * nb2[r];
+}
+
+hipError_t inner_thresholds(hipStream_t s, const double *nb2, double tol_frac, int K, double *thr) {
+  k_inner_thresholds<<<1, 64, 0, s>>>(nb2, tol_frac, K, thr);
+  return hipGetLastError();
+}
+template hipError_t pcg_iteration<double>(hipStream_t, const SolveArgs<double> &, TrackFn, void *);
+template hipError_t pcg_iteration<float>(hipStream_t, const SolveArgs<float> &, TrackFn, void *);
+template hipError_t pcg_finish<double>(hipStream_t, const SolveArgs<double> &, float *, int64_t,
+                                       int64_t, int, double *);
+template hipError_t pcg_finish<float>(hipStream_t, const SolveArgs<float> &, float *, int64_t,
+                                      int64_t, int, double *);
+
+// ------------------------------------------------------------------ kernel assembly
+// M[i,j] = -(x_i - y_j).n_i / (4 pi r^3) in FP64, stored FP32; thread per (i, 4 j).
+__global__ __launch_bounds__(kThreads) void k_assemble(const double *__restrict__ x,
+                                                       const double *__restrict__ nx,
+                                                       const double *__restrict__ y, int64_t m,
+                                                       int64_t n, float *__restrict__ M) {
+  const int64_t i = blockIdx.y;
+  const int64_t j = int64_t(blockIdx.x) * kThreads + threadIdx.x;
+  if (i >= m || j >= n) return;
+  const double dx = x[3 * i] - y[3 * j];
+  const double dy = x[3 * i + 1] - y[3 * j + 1];
+  const double dz = x[3 * i + 2] - y[3 * j + 2];
+  const double r2 = dx * dx + dy * dy + dz * dz;
+  const double dn = dx * nx[3 * i] + dy * nx[3 * i + 1] + dz * nx[3 * i + 2];
+  const double r = sqrt(r2);
+  M[i * n + j] = float(-dn / (4.0 * 3.14159265358979323846 * r2 * r));
+}
+
+hipError_t launch_assemble(hipStream_t s, const double *x, const double *nx, const double *y,
+                           int64_t m, int64_t n, float *M) {
+  dim3 grid(blocks_for(n), unsigned(m));
+  k_assemble<<<grid, kThreads, 0, s>>>(x, nx, y, m, n, M);
+  return hipGetLastError();
+}
+
+}  // namespace mfs

@@ -1,0 +1,893 @@
+// mfs_core.hip -- C-ABI entry points (include/mfs_core.h) and orchestration.
+//
+// One mfs_ctx holds one plan (ordering, patterns, supernodes) and one factor on
+// one device, plus grow-only device workspaces reused across calls.  It owns
+// no process-global state: the reference's static factor cache
+// (mfs_torch/csrc/mfs_torch_ops.cu:17-19, 136-157) becomes an explicit context
+// that the caller keys by geometry.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/mfs_core.h"
+#include "mfs_internal.h"
+#include "plan_host.h"
+
+namespace {
+
+using namespace mfs;
+
+struct DevBuf {
+  void *p = nullptr;
+  size_t cap = 0;
+  hipError_t ensure(size_t bytes) {
+    if (bytes <= cap && p != nullptr) return hipSuccess;
+    if (p != nullptr) {
+      hipError_t e = hipFree(p);
+      p = nullptr;
+      cap = 0;
+      if (e != hipSuccess) return e;
+    }
+    const size_t nb = std::max<size_t>(bytes, 256);
+    hipError_t e = hipMalloc(&p, nb);
+    if (e == hipSuccess) cap = nb;
+    return e;
+  }
+  void release() {
+    if (p != nullptr) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+  }
+  template <class T>
+  T *as() const {
+    return static_cast<T *>(p);
+  }
+};
+
+struct Bucket {
+  int tiles;
+  int32_t offset, count;
+};
+
+}  // namespace
+
+struct mfs_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+
+  // plan (host)
+  bool has_plan = false;
+  int32_t n = 0;
+  double rho32 = 0;
+  std::vector<int32_t> P, Pinv, ptr, ind, sup_ptr, sup_ind, sup_parent, sptr, sind, tptr, tcol, tmap;
+  std::vector<double> scale;
+  std::vector<SuperDesc> sdesc;
+  std::vector<int32_t> list;
+  std::vector<Bucket> buckets;
+  int32_t max_local = 0;
+
+  // plan (device)
+  DevBuf dP, dPinv, dsptr, dsind, dtptr, dtcol, dtmap, dsupind, dsd, dlist, dstatus;
+  DevBuf dpx, dpy, dpz, dscale, dcnt, dptr, dind, dtotal;
+
+  // factor
+  bool has_factor = false;
+  int prec = MFS_FP64;
+  DevBuf dval, dtval, dMt, dval_alt, dtval_alt;
+  int alt_prec = -1;
+
+  // solve workspaces
+  DevBuf dvec, dpart, ddpart, dstate, dx64, dr64, dnorm;
+  PcgState *hstate = nullptr;  // pinned
+  double *hnorm = nullptr;     // pinned, 2 * kMaxRhs
+
+  // statistics and timing
+  mfs_stats stats{};
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  int track_mask = 0;
+  struct Track {
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> pairs;
+    size_t used = 0;
+    bool open = false;
+    int64_t launches = 0;
+    double ms = 0.0, bytes = 0.0;
+  } tr[8];
+  double bytes_per_launch[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+
+  int fail(int code, const std::string &msg) {
+    err = msg;
+    return code;
+  }
+  int hip_fail(hipError_t e, const char *what) {
+    err = std::string(what) + ": " + hipGetErrorString(e);
+    return MFS_ERR_HIP;
+  }
+};
+
+#define HIPCHK(call, what)                                      \
+  do {                                                          \
+    hipError_t e_ = (call);                                     \
+    if (e_ != hipSuccess) return ctx->hip_fail(e_, what);       \
+  } while (0)
+
+namespace {
+
+template <class T>
+hipError_t upload(mfs_ctx *ctx, DevBuf &b, const std::vector<T> &v) {
+  hipError_t e = b.ensure(std::max<size_t>(v.size(), 1) * sizeof(T));
+  if (e != hipSuccess) return e;
+  if (v.empty()) return hipSuccess;
+  return hipMemcpyAsync(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, ctx->stream);
+}
+
+int tiles_for(int ns) {
+  const int t = (ns + 15) / 16;
+  if (t <= 4) return std::max(t, 1);
+  if (t <= 6) return 6;
+  if (t <= 8) return 8;
+  return -1;
+}
+
+// ---- kernel-class timing (events bracket each launch of an enabled class)
+void track_cb(void *p, int cls, int phase) {
+  mfs_ctx *ctx = static_cast<mfs_ctx *>(p);
+  if (cls <= 0 || cls >= 8 || !(ctx->track_mask & (1 << cls))) return;
+  mfs_ctx::Track &t = ctx->tr[cls];
+  if (phase == 0) {
+    if (t.used == t.pairs.size()) {
+      hipEvent_t a, b;
+      if (hipEventCreate(&a) != hipSuccess) return;
+      if (hipEventCreate(&b) != hipSuccess) return;
+      t.pairs.push_back({a, b});
+    }
+    (void)hipEventRecord(t.pairs[t.used].first, ctx->stream);
+    t.open = true;
+  } else if (t.open) {
+    (void)hipEventRecord(t.pairs[t.used].second, ctx->stream);
+    t.used++;
+    t.launches++;
+    t.bytes += ctx->bytes_per_launch[cls];
+    t.open = false;
+  }
+}
+
+void track_fold(mfs_ctx *ctx) {
+  bool any = false;
+  for (auto &t : ctx->tr) any = any || t.used > 0;
+  if (!any) return;
+  (void)hipStreamSynchronize(ctx->stream);
+  for (auto &t : ctx->tr) {
+    for (size_t i = 0; i < t.used; ++i) {
+      float ms = 0.f;
+      if (hipEventElapsedTime(&ms, t.pairs[i].first, t.pairs[i].second) == hipSuccess) t.ms += ms;
+    }
+    t.used = 0;
+  }
+}
+
+void phase_begin(mfs_ctx *ctx) { (void)hipEventRecord(ctx->ev0, ctx->stream); }
+double phase_end(mfs_ctx *ctx) {
+  (void)hipEventRecord(ctx->ev1, ctx->stream);
+  (void)hipEventSynchronize(ctx->ev1);
+  float ms = 0.f;
+  (void)hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
+  return ms;
+}
+
+int plan_impl(mfs_ctx *ctx, std::vector<double> &&xyz, int64_t n64, double rho, double lambda,
+              int32_t max_super) {
+  if (n64 <= 0 || n64 > (int64_t(1) << 30)) return ctx->fail(MFS_ERR_ARG, "plan: bad n");
+  if (!(rho > 0) || !(lambda > 0) || max_super <= 0)
+    return ctx->fail(MFS_ERR_ARG, "plan: rho, lambda and max_super must be positive");
+  HIPCHK(hipSetDevice(ctx->device), "hipSetDevice");
+  const int32_t n = int32_t(n64);
+  ctx->has_plan = false;
+  ctx->has_factor = false;
+  phase_begin(ctx);
+
+  // 1. ordering (host)
+  int rc = order_maximin(n, xyz.data(), ctx->P, ctx->Pinv, ctx->scale);
+  if (rc == PLAN_ERR_DUPLICATE)
+    return ctx->fail(MFS_ERR_DUPLICATE, "plan: coincident source points (zero length scale)");
+  if (rc != PLAN_OK) return ctx->fail(MFS_ERR_ARG, "plan: ordering failed");
+
+  // 2. rho-radius pattern (device); rho rounded through float as the op does.
+  ctx->rho32 = double(float(rho));
+  std::vector<double> px(n), py(n), pz(n);
+  for (int32_t c = 0; c < n; ++c) {
+    const int64_t o = ctx->Pinv[c];
+    px[c] = xyz[3 * o];
+    py[c] = xyz[3 * o + 1];
+    pz[c] = xyz[3 * o + 2];
+  }
+  HIPCHK(upload(ctx, ctx->dpx, px), "upload points");
+  HIPCHK(upload(ctx, ctx->dpy, py), "upload points");
+  HIPCHK(upload(ctx, ctx->dpz, pz), "upload points");
+  HIPCHK(upload(ctx, ctx->dscale, ctx->scale), "upload scale");
+  HIPCHK(ctx->dcnt.ensure(size_t(n) * 4), "alloc");
+  HIPCHK(ctx->dptr.ensure(size_t(n + 1) * 4), "alloc");
+  HIPCHK(ctx->dtotal.ensure(8), "alloc");
+  HIPCHK(launch_pattern(ctx->stream, n, ctx->dpx.as<double>(), ctx->dpy.as<double>(),
+                        ctx->dpz.as<double>(), ctx->dscale.as<double>(), ctx->rho32,
+                        ctx->dcnt.as<int32_t>(), nullptr, nullptr, 0),
+         "pattern count");
+  HIPCHK(launch_scan(ctx->stream, ctx->dcnt.as<int32_t>(), n, ctx->dptr.as<int32_t>(),
+                     ctx->dtotal.as<int64_t>()),
+         "pattern scan");
+  int64_t total = 0;
+  HIPCHK(hipMemcpyAsync(&total, ctx->dtotal.p, 8, hipMemcpyDeviceToHost, ctx->stream), "copy");
+  HIPCHK(hipStreamSynchronize(ctx->stream), "sync");
+  if (total > INT32_MAX) return ctx->fail(MFS_ERR_ARG, "plan: pattern exceeds 2^31 entries");
+  HIPCHK(ctx->dind.ensure(size_t(std::max<int64_t>(total, 1)) * 4), "alloc pattern");
+  HIPCHK(launch_pattern(ctx->stream, n, ctx->dpx.as<double>(), ctx->dpy.as<double>(),
+                        ctx->dpz.as<double>(), ctx->dscale.as<double>(), ctx->rho32, nullptr,
+                        ctx->dptr.as<int32_t>(), ctx->dind.as<int32_t>(), 1),
+         "pattern fill");
+  ctx->ptr.resize(size_t(n) + 1);
+  ctx->ind.resize(size_t(total));
+  HIPCHK(hipMemcpyAsync(ctx->ptr.data(), ctx->dptr.p, size_t(n + 1) * 4, hipMemcpyDeviceToHost,
+                        ctx->stream),
+         "copy pattern");
+  if (total > 0)
+    HIPCHK(hipMemcpyAsync(ctx->ind.data(), ctx->dind.p, size_t(total) * 4, hipMemcpyDeviceToHost,
+                          ctx->stream),
+           "copy pattern");
+  HIPCHK(hipStreamSynchronize(ctx->stream), "sync");
+
+  // 3. supernodes and supernodal pattern (host)
+  rc = aggregate(n, ctx->ptr.data(), ctx->ind.data(), ctx->scale.data(), lambda, max_super,
+                 ctx->sup_ptr, ctx->sup_ind, ctx->sup_parent);
+  if (rc != PLAN_OK)
+    return ctx->fail(MFS_ERR_DUPLICATE, "plan: a column lacks its diagonal (zero length scale)");
+  const int32_t n_super = int32_t(ctx->sup_ptr.size()) - 1;
+  super_pattern(n, ctx->ptr.data(), ctx->ind.data(), ctx->sup_parent, n_super, ctx->sptr, ctx->sind);
+  transpose_pattern(n, ctx->sptr, ctx->sind, ctx->tptr, ctx->tcol, ctx->tmap);
+
+  // 4. supernode descriptors, bucketed by tile count (largest first in each)
+  ctx->sdesc.resize(n_super);
+  ctx->max_local = 0;
+  double s2 = 0, s3 = 0;
+  int64_t theta = 0;
+  std::vector<std::vector<int32_t>> by_t(9);
+  for (int32_t s = 0; s < n_super; ++s) {
+    SuperDesc d{};
+    d.first = ctx->sup_ind[ctx->sup_ptr[s]];
+    d.ns = ctx->sptr[d.first + 1] - ctx->sptr[d.first];
+    d.rows_off = ctx->sptr[d.first];
+    d.mem_beg = ctx->sup_ptr[s];
+    d.mem_end = ctx->sup_ptr[s + 1];
+    ctx->sdesc[s] = d;
+    ctx->max_local = std::max(ctx->max_local, d.ns);
+    s2 += double(d.ns) * d.ns;
+    s3 += double(d.ns) * d.ns * d.ns;
+    theta += int64_t(d.ns) * d.ns;
+    const int t = tiles_for(d.ns);
+    if (t < 0)
+      return ctx->fail(MFS_ERR_ARG, "plan: supernode local size " + std::to_string(d.ns) +
+                                        " exceeds the supported maximum of 128");
+    by_t[t].push_back(s);
+  }
+  // Launch order within a bucket: supernodes sorted along a Morton curve of
+  // their opener's position, then dealt so that block b runs the (b / 8)-th
+  // supernode of the b % 8-th contiguous slice.  Workgroups are dispatched to
+  // the 8 XCDs round-robin, so each XCD walks one spatially compact slice and
+  // blocks running together on it gather overlapping rows of M^T through the
+  // same L2.
+  {
+    double lo[3] = {px[0], py[0], pz[0]}, hi[3] = {px[0], py[0], pz[0]};
+    for (int32_t c = 1; c < n; ++c) {
+      lo[0] = std::min(lo[0], px[c]); hi[0] = std::max(hi[0], px[c]);
+      lo[1] = std::min(lo[1], py[c]); hi[1] = std::max(hi[1], py[c]);
+      lo[2] = std::min(lo[2], pz[c]); hi[2] = std::max(hi[2], pz[c]);
+    }
+    auto spread = [](uint64_t v) {  // 21 bits -> every third bit
+      v &= 0x1fffff;
+      v = (v | v << 32) & 0x1f00000000ffffULL;
+      v = (v | v << 16) & 0x1f0000ff0000ffULL;
+      v = (v | v << 8) & 0x100f00f00f00f00fULL;
+      v = (v | v << 4) & 0x10c30c30c30c30c3ULL;
+      v = (v | v << 2) & 0x1249249249249249ULL;
+      return v;
+    };
+    std::vector<uint64_t> key(n_super);
+    for (int32_t s = 0; s < n_super; ++s) {
+      const int32_t c = ctx->sdesc[s].first;
+      const double v[3] = {px[c], py[c], pz[c]};
+      uint64_t k = 0;
+      for (int d = 0; d < 3; ++d) {
+        const double w = hi[d] > lo[d] ? (v[d] - lo[d]) / (hi[d] - lo[d]) : 0.0;
+        k |= spread(uint64_t(std::min(w, 1.0) * 2097151.0)) << d;
+      }
+      key[s] = k;
+    }
+    ctx->list.clear();
+    ctx->buckets.clear();
+    constexpr int kXcd = 8;
+    for (int t : {8, 6, 4, 3, 2, 1}) {
+      std::vector<int32_t> &v = by_t[t];
+      if (v.empty()) continue;
+      std::stable_sort(v.begin(), v.end(), [&](int32_t a, int32_t b) { return key[a] < key[b]; });
+      const int32_t cnt = int32_t(v.size());
+      const int32_t per = (cnt + kXcd - 1) / kXcd;
+      Bucket b{t, int32_t(ctx->list.size()), cnt};
+      std::vector<int32_t> dealt;
+      dealt.reserve(cnt);
+      for (int32_t i = 0; i < per; ++i)
+        for (int x = 0; x < kXcd; ++x) {
+          const int32_t src = x * per + i;
+          if (src < cnt) dealt.push_back(v[src]);
+        }
+      ctx->list.insert(ctx->list.end(), dealt.begin(), dealt.end());
+      ctx->buckets.push_back(b);
+    }
+  }
+
+  // 5. upload
+  HIPCHK(upload(ctx, ctx->dP, ctx->P), "upload");
+  HIPCHK(upload(ctx, ctx->dPinv, ctx->Pinv), "upload");
+  HIPCHK(upload(ctx, ctx->dsptr, ctx->sptr), "upload");
+  HIPCHK(upload(ctx, ctx->dsind, ctx->sind), "upload");
+  HIPCHK(upload(ctx, ctx->dtptr, ctx->tptr), "upload");
+  HIPCHK(upload(ctx, ctx->dtcol, ctx->tcol), "upload");
+  HIPCHK(upload(ctx, ctx->dtmap, ctx->tmap), "upload");
+  HIPCHK(upload(ctx, ctx->dsupind, ctx->sup_ind), "upload");
+  HIPCHK(upload(ctx, ctx->dsd, ctx->sdesc), "upload");
+  HIPCHK(upload(ctx, ctx->dlist, ctx->list), "upload");
+  HIPCHK(ctx->dstatus.ensure(size_t(n_super) * 4), "alloc");
+
+  ctx->n = n;
+  ctx->has_plan = true;
+  ctx->stats.n = n;
+  ctx->stats.n_super = n_super;
+  ctx->stats.max_local = ctx->max_local;
+  ctx->stats.nnz_pattern = int64_t(ctx->ptr[n]);
+  ctx->stats.nnz_factor = int64_t(ctx->sptr[n]);
+  ctx->stats.theta_entries = theta;
+  ctx->stats.sum_ns2 = s2;
+  ctx->stats.sum_ns3 = s3;
+  ctx->stats.plan_ms = phase_end(ctx);
+  return MFS_OK;
+}
+
+// Factor values in the precision a solver needs (converted copy if required).
+__global__ void k_convert_d2f(int64_t count, const double *a, float *b) {
+  const int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x;
+  if (i < count) b[i] = float(a[i]);
+}
+__global__ void k_convert_f2d(int64_t count, const float *a, double *b) {
+  const int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x;
+  if (i < count) b[i] = double(a[i]);
+}
+
+int values_for(mfs_ctx *ctx, int want, const void **lval, const void **ltval) {
+  if (ctx->prec == want) {
+    *lval = ctx->dval.p;
+    *ltval = ctx->dtval.p;
+    return MFS_OK;
+  }
+  const int64_t nnz = ctx->sptr[ctx->n];
+  const size_t w = (want == MFS_FP64) ? 8 : 4;
+  if (ctx->alt_prec != want) {
+    HIPCHK(ctx->dval_alt.ensure(size_t(std::max<int64_t>(nnz, 1)) * w), "alloc");
+    HIPCHK(ctx->dtval_alt.ensure(size_t(std::max<int64_t>(nnz, 1)) * w), "alloc");
+    const unsigned g = unsigned((nnz + kThreads - 1) / kThreads);
+    if (nnz > 0) {
+      if (want == MFS_FP32) {
+        k_convert_d2f<<<g, kThreads, 0, ctx->stream>>>(nnz, ctx->dval.as<double>(), ctx->dval_alt.as<float>());
+        k_convert_d2f<<<g, kThreads, 0, ctx->stream>>>(nnz, ctx->dtval.as<double>(), ctx->dtval_alt.as<float>());
+      } else {
+        k_convert_f2d<<<g, kThreads, 0, ctx->stream>>>(nnz, ctx->dval.as<float>(), ctx->dval_alt.as<double>());
+        k_convert_f2d<<<g, kThreads, 0, ctx->stream>>>(nnz, ctx->dtval.as<float>(), ctx->dtval_alt.as<double>());
+      }
+      HIPCHK(hipGetLastError(), "convert factor");
+    }
+    ctx->alt_prec = want;
+  }
+  *lval = ctx->dval_alt.p;
+  *ltval = ctx->dtval_alt.p;
+  return MFS_OK;
+}
+
+int32_t row_blocks_for(int64_t m, int64_t n) {
+  const int64_t cols = (n + 1023) / 1024;
+  int64_t rb = (2048 + cols - 1) / cols;
+  rb = std::min<int64_t>(rb, std::max<int64_t>(m / 64, 1));
+  return int32_t(std::max<int64_t>(rb, 1));
+}
+
+int batch_width(int64_t k) {
+  if (k <= 1) return 1;
+  if (k <= 2) return 2;
+  if (k <= 4) return 4;
+  if (k <= 8) return 8;
+  return 16;
+}
+
+// Lay out a SolveArgs workspace for batch width K.
+template <typename TA>
+int workspace(mfs_ctx *ctx, const float *M, int64_t m, int64_t n, int K, SolveArgs<TA> &a) {
+  const int want = sizeof(TA) == 8 ? MFS_FP64 : MFS_FP32;
+  const void *lv = nullptr, *ltv = nullptr;
+  int rc = values_for(ctx, want, &lv, &ltv);
+  if (rc != MFS_OK) return rc;
+  a.M = M;
+  a.m = m;
+  a.n = n;
+  a.K = K;
+  a.P = ctx->dP.as<int32_t>();
+  a.Pinv = ctx->dPinv.as<int32_t>();
+  a.sptr = ctx->dsptr.as<int32_t>();
+  a.sind = ctx->dsind.as<int32_t>();
+  a.tptr = ctx->dtptr.as<int32_t>();
+  a.tcol = ctx->dtcol.as<int32_t>();
+  a.lval = static_cast<const TA *>(lv);
+  a.ltval = static_cast<const TA *>(ltv);
+  a.row_blocks = row_blocks_for(m, n);
+  const size_t nv = size_t(n) * K, mv = size_t(m) * K;
+  HIPCHK(ctx->dvec.ensure((7 * nv + mv) * sizeof(TA) + 64), "alloc vectors");
+  TA *base = ctx->dvec.as<TA>();
+  a.x = base;
+  a.r = base + nv;
+  a.p = base + 2 * nv;
+  a.q = base + 3 * nv;
+  a.z = base + 4 * nv;
+  a.t = base + 5 * nv;
+  a.pv = base + 6 * nv;
+  a.y = base + 7 * nv;
+  HIPCHK(ctx->dpart.ensure(size_t(a.row_blocks) * nv * sizeof(double)), "alloc partials");
+  a.part = ctx->dpart.as<double>();
+  const size_t maxblocks = size_t((n * 8 + kThreads - 1) / kThreads) + size_t((m + kThreads - 1) / kThreads) + 16;
+  HIPCHK(ctx->ddpart.ensure(maxblocks * K * sizeof(double)), "alloc dot partials");
+  a.dpart = ctx->ddpart.as<double>();
+  if (ctx->dstate.p == nullptr) {
+    HIPCHK(ctx->dstate.ensure(sizeof(PcgState)), "alloc state");
+    HIPCHK(hipMemsetAsync(ctx->dstate.p, 0, sizeof(PcgState), ctx->stream), "memset state");
+  }
+  a.st = ctx->dstate.as<PcgState>();
+  return MFS_OK;
+}
+
+// Run PCG iterations until every column is inactive (polls the device flag).
+template <typename TA>
+int iterate(mfs_ctx *ctx, const SolveArgs<TA> &a, int32_t maxit) {
+  int chunk = 2;
+  int32_t launched = 0;
+  for (;;) {
+    HIPCHK(hipMemcpyAsync(ctx->hstate, a.st, sizeof(PcgState), hipMemcpyDeviceToHost, ctx->stream),
+           "poll state");
+    HIPCHK(hipStreamSynchronize(ctx->stream), "sync");
+    if (ctx->hstate->n_active == 0) break;
+    if (launched > maxit + 64) return ctx->fail(MFS_ERR_STATE, "pcg: iteration bookkeeping diverged");
+    for (int i = 0; i < chunk; ++i) {
+      HIPCHK(pcg_iteration<TA>(ctx->stream, a, track_cb, ctx), "pcg iteration");
+      ++launched;
+    }
+    chunk = std::min(chunk * 2, 16);
+  }
+  return MFS_OK;
+}
+
+void record_batch_stats(mfs_ctx *ctx, int64_t col0, int kval) {
+  for (int r = 0; r < kval; ++r) {
+    const int64_t c = col0 + r;
+    if (c >= 64) break;
+    ctx->stats.iters[c] = ctx->hstate->iters[r];
+    const double rhs2 = ctx->hstate->rhs2[r];
+    ctx->stats.relres[c] = rhs2 > 0 ? std::sqrt(ctx->hstate->dot_rr[r] / rhs2) : 0.0;
+  }
+}
+
+__global__ void k_x64_from_f32(int64_t n, const float *X, int64_t ldx, int64_t col0, int kval,
+                               int K, double *X64) {
+  const int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x;
+  if (i >= n) return;
+  for (int r = 0; r < K; ++r) X64[i * K + r] = (r < kval && X) ? double(X[i * ldx + col0 + r]) : 0.0;
+}
+
+__global__ void k_x64_to_f32(int64_t n, const double *X64, int K, float *X, int64_t ldx,
+                             int64_t col0, int kval) {
+  const int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x;
+  if (i >= n) return;
+  for (int r = 0; r < kval; ++r) X[i * ldx + col0 + r] = float(X64[i * K + r]);
+}
+
+int check_dense_args(mfs_ctx *ctx, const float *M, int64_t m, int64_t n) {
+  if (!ctx->has_plan) return ctx->fail(MFS_ERR_STATE, "no plan: call mfs_plan first");
+  if (M == nullptr || m <= 0 || n != ctx->n)
+    return ctx->fail(MFS_ERR_ARG, "M must be m x n with n equal to the planned source count");
+  return MFS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mfs_ctx_create(mfs_ctx **out, int device, void *stream) {
+  if (out == nullptr) return MFS_ERR_ARG;
+  *out = nullptr;
+  mfs_ctx *ctx = new (std::nothrow) mfs_ctx();
+  if (ctx == nullptr) return MFS_ERR_ALLOC;
+  ctx->device = device;
+  ctx->stream = static_cast<hipStream_t>(stream);
+  if (hipSetDevice(device) != hipSuccess || hipEventCreate(&ctx->ev0) != hipSuccess ||
+      hipEventCreate(&ctx->ev1) != hipSuccess ||
+      hipHostMalloc(reinterpret_cast<void **>(&ctx->hstate), sizeof(PcgState)) != hipSuccess ||
+      hipHostMalloc(reinterpret_cast<void **>(&ctx->hnorm), 4 * kMaxRhs * sizeof(double)) != hipSuccess) {
+    mfs_ctx_destroy(ctx);
+    return MFS_ERR_HIP;
+  }
+  *out = ctx;
+  return MFS_OK;
+}
+
+void mfs_ctx_destroy(mfs_ctx *ctx) {
+  if (ctx == nullptr) return;
+  (void)hipSetDevice(ctx->device);
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  else (void)hipDeviceSynchronize();
+  for (DevBuf *b : {&ctx->dP, &ctx->dPinv, &ctx->dsptr, &ctx->dsind, &ctx->dtptr, &ctx->dtcol,
+                    &ctx->dtmap, &ctx->dsupind, &ctx->dsd, &ctx->dlist, &ctx->dstatus, &ctx->dpx,
+                    &ctx->dpy, &ctx->dpz, &ctx->dscale, &ctx->dcnt, &ctx->dptr, &ctx->dind,
+                    &ctx->dtotal, &ctx->dval, &ctx->dtval, &ctx->dMt, &ctx->dval_alt,
+                    &ctx->dtval_alt, &ctx->dvec, &ctx->dpart, &ctx->ddpart, &ctx->dstate,
+                    &ctx->dx64, &ctx->dr64, &ctx->dnorm})
+    b->release();
+  if (ctx->hstate) (void)hipHostFree(ctx->hstate);
+  if (ctx->hnorm) (void)hipHostFree(ctx->hnorm);
+  if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
+  if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+  for (auto &t : ctx->tr)
+    for (auto &pr : t.pairs) {
+      (void)hipEventDestroy(pr.first);
+      (void)hipEventDestroy(pr.second);
+    }
+  delete ctx;
+}
+
+int mfs_ctx_set_stream(mfs_ctx *ctx, void *stream) {
+  if (ctx == nullptr) return MFS_ERR_ARG;
+  ctx->stream = static_cast<hipStream_t>(stream);
+  return MFS_OK;
+}
+
+const char *mfs_ctx_last_error(const mfs_ctx *ctx) {
+  return ctx == nullptr ? "null context" : ctx->err.c_str();
+}
+
+int mfs_ctx_stats(const mfs_ctx *ctx, mfs_stats *out) {
+  if (ctx == nullptr || out == nullptr) return MFS_ERR_ARG;
+  *out = ctx->stats;
+  return MFS_OK;
+}
+
+int mfs_plan(mfs_ctx *ctx, const float *src_xyz_host, int64_t n, double rho, double lambda,
+             int32_t max_super) {
+  if (ctx == nullptr || src_xyz_host == nullptr) return MFS_ERR_ARG;
+  std::vector<double> xyz(size_t(std::max<int64_t>(n, 0)) * 3);
+  for (size_t i = 0; i < xyz.size(); ++i) xyz[i] = double(src_xyz_host[i]);
+  return plan_impl(ctx, std::move(xyz), n, rho, lambda, max_super);
+}
+
+int mfs_plan_f64(mfs_ctx *ctx, const double *src_xyz_host, int64_t n, double rho, double lambda,
+                 int32_t max_super) {
+  if (ctx == nullptr || src_xyz_host == nullptr) return MFS_ERR_ARG;
+  std::vector<double> xyz(src_xyz_host, src_xyz_host + size_t(std::max<int64_t>(n, 0)) * 3);
+  return plan_impl(ctx, std::move(xyz), n, rho, lambda, max_super);
+}
+
+int mfs_plan_export(const mfs_ctx *ctx, int32_t *P, int32_t *Pinv, double *scale, int32_t *ptr,
+                    int32_t *ind, int32_t *sup_ptr, int32_t *sup_ind, int32_t *sup_parent,
+                    int32_t *sptr, int32_t *sind) {
+  if (ctx == nullptr || !ctx->has_plan) return MFS_ERR_STATE;
+  auto cp = [](auto *dst, const auto &v) {
+    if (dst != nullptr && !v.empty()) std::memcpy(dst, v.data(), v.size() * sizeof(v[0]));
+  };
+  cp(P, ctx->P);
+  cp(Pinv, ctx->Pinv);
+  cp(scale, ctx->scale);
+  cp(ptr, ctx->ptr);
+  cp(ind, ctx->ind);
+  cp(sup_ptr, ctx->sup_ptr);
+  cp(sup_ind, ctx->sup_ind);
+  cp(sup_parent, ctx->sup_parent);
+  cp(sptr, ctx->sptr);
+  cp(sind, ctx->sind);
+  return MFS_OK;
+}
+
+int mfs_factor_dense(mfs_ctx *ctx, const float *M, int64_t m, int64_t n, int precision) {
+  if (ctx == nullptr) return MFS_ERR_ARG;
+  int rc = check_dense_args(ctx, M, m, n);
+  if (rc != MFS_OK) return rc;
+  if (precision != MFS_FP64 && precision != MFS_FP32)
+    return ctx->fail(MFS_ERR_ARG, "factor: precision must be MFS_FP64 or MFS_FP32");
+  HIPCHK(hipSetDevice(ctx->device), "hipSetDevice");
+  ctx->has_factor = false;
+  ctx->alt_prec = -1;
+  phase_begin(ctx);
+  const int64_t ldmt = (m + kGramChunk - 1) / kGramChunk * kGramChunk;
+  HIPCHK(ctx->dMt.ensure(size_t(n) * size_t(ldmt) * sizeof(float)), "alloc M^T");
+  HIPCHK(launch_transpose(ctx->stream, M, m, n, ctx->dMt.as<float>(), ldmt), "transpose");
+  const int64_t nnz = ctx->sptr[ctx->n];
+  const size_t w = (precision == MFS_FP64) ? 8 : 4;
+  HIPCHK(ctx->dval.ensure(size_t(std::max<int64_t>(nnz, 1)) * w), "alloc factor");
+  HIPCHK(ctx->dtval.ensure(size_t(std::max<int64_t>(nnz, 1)) * w), "alloc factor");
+  const int32_t n_super = int32_t(ctx->sdesc.size());
+  HIPCHK(hipMemsetAsync(ctx->dstatus.p, 0, size_t(n_super) * 4, ctx->stream), "memset");
+  ctx->bytes_per_launch[MFS_K_GRAM] = double(m) * 4.0 * double([&] { double t = 0; for (auto &d : ctx->sdesc) t += d.ns; return t; }());
+  track_cb(ctx, MFS_K_GRAM, 0);
+  for (const Bucket &b : ctx->buckets)
+    HIPCHK(launch_super_factor(ctx->stream, b.tiles, precision == MFS_FP64 ? 0 : 1,
+                               ctx->dMt.as<float>(), ldmt, ctx->dsd.as<SuperDesc>(),
+                               ctx->dlist.as<int32_t>() + b.offset, b.count, ctx->dsind.as<int32_t>(),
+                               ctx->dPinv.as<int32_t>(), ctx->dsupind.as<int32_t>(),
+                               ctx->dsptr.as<int32_t>(), ctx->dval.p, ctx->dstatus.as<int32_t>()),
+           "supernode factor");
+  track_cb(ctx, MFS_K_GRAM, 1);
+  std::vector<int32_t> status(n_super);
+  HIPCHK(hipMemcpyAsync(status.data(), ctx->dstatus.p, size_t(n_super) * 4, hipMemcpyDeviceToHost,
+                        ctx->stream),
+         "copy status");
+  HIPCHK(hipStreamSynchronize(ctx->stream), "sync");
+  std::vector<int32_t> failed;
+  for (int32_t s = 0; s < n_super; ++s)
+    if (status[s] != 0) failed.push_back(s);
+  ctx->stats.retried_supernodes = 0;
+  if (!failed.empty() && precision == MFS_FP32) {
+    // FP64 retry of the supernodes whose FP32 pivots failed; values stay FP32.
+    ctx->stats.retried_supernodes = int32_t(failed.size());
+    std::vector<std::vector<int32_t>> by_t(9);
+    for (int32_t s : failed) by_t[tiles_for(ctx->sdesc[s].ns)].push_back(s);
+    std::vector<int32_t> flist;
+    std::vector<Bucket> fb;
+    for (int t : {8, 6, 4, 3, 2, 1}) {
+      if (by_t[t].empty()) continue;
+      fb.push_back({t, int32_t(flist.size()), int32_t(by_t[t].size())});
+      flist.insert(flist.end(), by_t[t].begin(), by_t[t].end());
+    }
+    DevBuf dfl;
+    HIPCHK(upload(ctx, dfl, flist), "upload retry list");
+    for (const Bucket &b : fb)
+      HIPCHK(launch_super_factor(ctx->stream, b.tiles, 2, ctx->dMt.as<float>(), ldmt,
+                                 ctx->dsd.as<SuperDesc>(), dfl.as<int32_t>() + b.offset, b.count,
+                                 ctx->dsind.as<int32_t>(), ctx->dPinv.as<int32_t>(),
+                                 ctx->dsupind.as<int32_t>(), ctx->dsptr.as<int32_t>(), ctx->dval.p,
+                                 ctx->dstatus.as<int32_t>()),
+             "supernode factor retry");
+    HIPCHK(hipMemcpyAsync(status.data(), ctx->dstatus.p, size_t(n_super) * 4,
+                          hipMemcpyDeviceToHost, ctx->stream),
+           "copy status");
+    HIPCHK(hipStreamSynchronize(ctx->stream), "sync");
+    dfl.release();
+    failed.clear();
+    for (int32_t s = 0; s < n_super; ++s)
+      if (status[s] != 0) failed.push_back(s);
+  }
+  if (!failed.empty())
+    return ctx->fail(MFS_ERR_NOT_SPD, "factor: supernode " + std::to_string(failed[0]) +
+                                          " has a non-positive pivot (" +
+                                          std::to_string(failed.size()) + " supernodes failed)");
+  HIPCHK(launch_gather_values(ctx->stream, precision == MFS_FP64 ? 0 : 1, int32_t(nnz),
+                              ctx->dtmap.as<int32_t>(), ctx->dval.p, ctx->dtval.p),
+         "gather values");
+  ctx->prec = precision;
+  ctx->has_factor = true;
+  ctx->stats.factor_ms = phase_end(ctx);
+  track_fold(ctx);
+  return MFS_OK;
+}
+
+int mfs_factor_export(const mfs_ctx *ctx_c, double *val) {
+  mfs_ctx *ctx = const_cast<mfs_ctx *>(ctx_c);
+  if (ctx == nullptr || val == nullptr) return MFS_ERR_ARG;
+  if (!ctx->has_factor) return ctx->fail(MFS_ERR_STATE, "no factor");
+  const int64_t nnz = ctx->sptr[ctx->n];
+  if (ctx->prec == MFS_FP64) {
+    HIPCHK(hipMemcpy(val, ctx->dval.p, size_t(nnz) * 8, hipMemcpyDeviceToHost), "copy factor");
+  } else {
+    std::vector<float> tmp(nnz);
+    HIPCHK(hipMemcpy(tmp.data(), ctx->dval.p, size_t(nnz) * 4, hipMemcpyDeviceToHost), "copy factor");
+    for (int64_t i = 0; i < nnz; ++i) val[i] = tmp[i];
+  }
+  return MFS_OK;
+}
+
+int mfs_solve_dense(mfs_ctx *ctx, const float *M, int64_t m, int64_t n, const float *B, int64_t k,
+                    int rhs_space, float *X, int32_t maxit, double tol) {
+  if (ctx == nullptr) return MFS_ERR_ARG;
+  int rc = check_dense_args(ctx, M, m, n);
+  if (rc != MFS_OK) return rc;
+  if (!ctx->has_factor) return ctx->fail(MFS_ERR_STATE, "no factor: call mfs_factor_dense first");
+  if (B == nullptr || X == nullptr || k <= 0 || (rhs_space != 0 && rhs_space != 1))
+    return ctx->fail(MFS_ERR_ARG, "solve: bad right-hand side arguments");
+  HIPCHK(hipSetDevice(ctx->device), "hipSetDevice");
+  SolveArgs<double> a{};
+  rc = workspace<double>(ctx, M, m, n, 1, a);
+  if (rc != MFS_OK) return rc;
+  phase_begin(ctx);
+  for (int c = 1; c < 8; ++c) ctx->bytes_per_launch[c] = double(m) * double(n) * 4.0;
+  for (int64_t c = 0; c < k; ++c) {
+    HIPCHK(pcg_init<double>(ctx->stream, a, B, k, c, rhs_space, 1, tol, maxit, nullptr), "pcg init");
+    rc = iterate<double>(ctx, a, maxit);
+    if (rc != MFS_OK) return rc;
+    HIPCHK(pcg_finish<double>(ctx->stream, a, X, k, c, 1, nullptr), "pcg finish");
+    record_batch_stats(ctx, c, 1);
+  }
+  ctx->stats.outer_iters = 0;
+  ctx->stats.solve_ms = phase_end(ctx);
+  track_fold(ctx);
+  return MFS_OK;
+}
+
+int mfs_solve_mixed(mfs_ctx *ctx, const float *M, int64_t m, int64_t n, const float *B, int64_t k,
+                    int rhs_space, const float *X0, float *X, int32_t maxit, double tol,
+                    double inner_tol) {
+  if (ctx == nullptr) return MFS_ERR_ARG;
+  int rc = check_dense_args(ctx, M, m, n);
+  if (rc != MFS_OK) return rc;
+  if (!ctx->has_factor) return ctx->fail(MFS_ERR_STATE, "no factor: call mfs_factor_dense first");
+  if (B == nullptr || X == nullptr || k <= 0 || rhs_space != 0)
+    return ctx->fail(MFS_ERR_ARG, "solve_mixed: boundary right-hand sides only");
+  if (!(inner_tol > 0) || !(inner_tol <= 1)) inner_tol = 0.5;
+  HIPCHK(hipSetDevice(ctx->device), "hipSetDevice");
+  phase_begin(ctx);
+  for (int c = 1; c < 8; ++c) ctx->bytes_per_launch[c] = double(m) * double(n) * 4.0;
+  int outer_total = 0;
+  for (int64_t col0 = 0; col0 < k; col0 += kMaxRhs) {
+    const int kval = int(std::min<int64_t>(kMaxRhs, k - col0));
+    const int K = batch_width(kval);
+    SolveArgs<double> a64{};
+    rc = workspace<double>(ctx, M, m, n, K, a64);
+    if (rc != MFS_OK) return rc;
+    // The FP32 workspace shares the vector buffers: lay it out after the FP64 one.
+    HIPCHK(ctx->dx64.ensure(size_t(n) * K * 8), "alloc x64");
+    HIPCHK(ctx->dr64.ensure(size_t(n) * K * 8), "alloc r64");
+    HIPCHK(ctx->dnorm.ensure(4 * kMaxRhs * 8), "alloc norms");
+    double *X64 = ctx->dx64.as<double>();
+    double *R64 = ctx->dr64.as<double>();
+    double *nb2 = ctx->dnorm.as<double>();
+    double *res2 = nb2 + kMaxRhs;
+    double *thr = nb2 + 2 * kMaxRhs;
+    const unsigned gn = unsigned((n + kThreads - 1) / kThreads);
+    k_x64_from_f32<<<gn, kThreads, 0, ctx->stream>>>(n, X0, k, col0, kval, K, X64);
+    // ||M^T b||^2 and, for a zero start, the first residual.
+    HIPCHK(residual64(ctx->stream, a64, B, k, col0, kval, nullptr, R64, nb2, track_cb, ctx), "residual");
+    if (X0 != nullptr)
+      HIPCHK(residual64(ctx->stream, a64, B, k, col0, kval, X64, R64, res2, track_cb, ctx), "residual");
+    else
+      HIPCHK(hipMemcpyAsync(res2, nb2, kMaxRhs * 8, hipMemcpyDeviceToDevice, ctx->stream), "copy");
+    int32_t inner_used = 0;
+    int outer = 0;
+    std::vector<int32_t> iters(kval, 0);
+    for (;;) {
+      HIPCHK(hipMemcpyAsync(ctx->hnorm, nb2, 2 * kMaxRhs * 8, hipMemcpyDeviceToHost, ctx->stream), "copy");
+      HIPCHK(hipStreamSynchronize(ctx->stream), "sync");
+      bool done = true;
+      for (int r = 0; r < kval; ++r) {
+        const double b2 = ctx->hnorm[r], e2 = ctx->hnorm[kMaxRhs + r];
+        const double rel = b2 > 0 ? std::sqrt(e2 / b2) : 0.0;
+        if (col0 + r < 64) ctx->stats.relres[col0 + r] = rel;
+        if (b2 >= 1e-16 && !(rel <= tol)) done = false;
+      }
+      if (done || inner_used >= maxit || outer >= 30) break;
+      // Inner FP32 PCG on M^T M d = R64 (source-space right-hand side).
+      SolveArgs<float> a32{};
+      {
+        const void *lv = nullptr, *ltv = nullptr;
+        rc = values_for(ctx, MFS_FP32, &lv, &ltv);
+        if (rc != MFS_OK) return rc;
+        a32.M = M; a32.m = m; a32.n = n; a32.K = K;
+        a32.P = a64.P; a32.Pinv = a64.Pinv; a32.sptr = a64.sptr; a32.sind = a64.sind;
+        a32.tptr = a64.tptr; a32.tcol = a64.tcol;
+        a32.lval = static_cast<const float *>(lv);
+        a32.ltval = static_cast<const float *>(ltv);
+        a32.row_blocks = a64.row_blocks;
+        // FP32 vectors live in the FP64 vector buffer (same byte size suffices).
+        float *base = reinterpret_cast<float *>(a64.x);
+        const size_t nv = size_t(n) * K;
+        a32.x = base; a32.r = base + nv; a32.p = base + 2 * nv; a32.q = base + 3 * nv;
+        a32.z = base + 4 * nv; a32.t = base + 5 * nv; a32.pv = base + 6 * nv; a32.y = base + 7 * nv;
+        a32.part = a64.part; a32.dpart = a64.dpart; a32.st = a64.st;
+      }
+      // Each sweep aims straight at inner_tol * tol * ||M^T b|| (absolute), so
+      // one sweep normally meets the tolerance and the FP64 residual check
+      // only confirms it; FP32 round-off left above it is removed by the next.
+      HIPCHK(inner_thresholds(ctx->stream, nb2, inner_tol * tol, K, thr), "inner thresholds");
+      HIPCHK(pcg_init<float>(ctx->stream, a32, nullptr, 0, 0, 1, K, 0.0, maxit - inner_used, R64, thr),
+             "inner init");
+      rc = iterate<float>(ctx, a32, maxit - inner_used);
+      if (rc != MFS_OK) return rc;
+      HIPCHK(pcg_finish<float>(ctx->stream, a32, nullptr, 0, 0, K, X64), "inner finish");
+      int32_t used = 0;
+      for (int r = 0; r < kval; ++r) {
+        iters[r] += ctx->hstate->iters[r];
+        used = std::max(used, ctx->hstate->iters[r]);
+      }
+      inner_used += std::max(used, 1);
+      ++outer;
+      HIPCHK(residual64(ctx->stream, a64, B, k, col0, kval, X64, R64, res2, track_cb, ctx), "residual");
+    }
+    k_x64_to_f32<<<gn, kThreads, 0, ctx->stream>>>(n, X64, K, X, k, col0, kval);
+    HIPCHK(hipGetLastError(), "output");
+    for (int r = 0; r < kval; ++r)
+      if (col0 + r < 64) ctx->stats.iters[col0 + r] = iters[r];
+    outer_total = std::max(outer_total, outer);
+  }
+  ctx->stats.outer_iters = outer_total;
+  ctx->stats.solve_ms = phase_end(ctx);
+  track_fold(ctx);
+  return MFS_OK;
+}
+
+int mfs_normal_residual(mfs_ctx *ctx, const float *M, int64_t m, int64_t n, const float *B,
+                        int64_t k, const float *X, double *out) {
+  if (ctx == nullptr || out == nullptr) return MFS_ERR_ARG;
+  int rc = check_dense_args(ctx, M, m, n);
+  if (rc != MFS_OK) return rc;
+  if (!ctx->has_factor) return ctx->fail(MFS_ERR_STATE, "no factor: call mfs_factor_dense first");
+  HIPCHK(hipSetDevice(ctx->device), "hipSetDevice");
+  for (int64_t col0 = 0; col0 < k; col0 += kMaxRhs) {
+    const int kval = int(std::min<int64_t>(kMaxRhs, k - col0));
+    const int K = batch_width(kval);
+    SolveArgs<double> a{};
+    rc = workspace<double>(ctx, M, m, n, K, a);
+    if (rc != MFS_OK) return rc;
+    HIPCHK(ctx->dx64.ensure(size_t(n) * K * 8), "alloc");
+    HIPCHK(ctx->dr64.ensure(size_t(n) * K * 8), "alloc");
+    HIPCHK(ctx->dnorm.ensure(4 * kMaxRhs * 8), "alloc");
+    double *nb2 = ctx->dnorm.as<double>();
+    const unsigned gn = unsigned((n + kThreads - 1) / kThreads);
+    k_x64_from_f32<<<gn, kThreads, 0, ctx->stream>>>(n, X, k, col0, kval, K, ctx->dx64.as<double>());
+    HIPCHK(residual64(ctx->stream, a, B, k, col0, kval, nullptr, ctx->dr64.as<double>(), nb2, nullptr, nullptr), "residual");
+    HIPCHK(residual64(ctx->stream, a, B, k, col0, kval, ctx->dx64.as<double>(), ctx->dr64.as<double>(),
+                      nb2 + kMaxRhs, nullptr, nullptr), "residual");
+    HIPCHK(hipMemcpyAsync(ctx->hnorm, nb2, 2 * kMaxRhs * 8, hipMemcpyDeviceToHost, ctx->stream), "copy");
+    HIPCHK(hipStreamSynchronize(ctx->stream), "sync");
+    for (int r = 0; r < kval; ++r) {
+      const double b2 = ctx->hnorm[r];
+      out[col0 + r] = b2 > 0 ? std::sqrt(ctx->hnorm[kMaxRhs + r] / b2) : 0.0;
+    }
+  }
+  return MFS_OK;
+}
+
+int mfs_assemble_dense(mfs_ctx *ctx, const double *x, const double *nx, const double *y, int64_t m,
+                       int64_t n, float *M) {
+  if (ctx == nullptr || x == nullptr || nx == nullptr || y == nullptr || M == nullptr || m <= 0 ||
+      n <= 0 || m > INT32_MAX)
+    return MFS_ERR_ARG;
+  HIPCHK(hipSetDevice(ctx->device), "hipSetDevice");
+  HIPCHK(launch_assemble(ctx->stream, x, nx, y, m, n, M), "assemble");
+  return MFS_OK;
+}
+
+int mfs_ctx_track_kernel(mfs_ctx *ctx, int mask) {
+  if (ctx == nullptr) return MFS_ERR_ARG;
+  track_fold(ctx);
+  ctx->track_mask = mask;
+  for (auto &t : ctx->tr) {
+    t.launches = 0;
+    t.ms = 0.0;
+    t.bytes = 0.0;
+  }
+  return MFS_OK;
+}
+
+int mfs_ctx_tracked(mfs_ctx *ctx, int kernel_class, int64_t *launches, double *total_ms,
+                    double *bytes) {
+  if (ctx == nullptr || kernel_class <= 0 || kernel_class >= 8) return MFS_ERR_ARG;
+  track_fold(ctx);
+  const mfs_ctx::Track &t = ctx->tr[kernel_class];
+  if (launches) *launches = t.launches;
+  if (total_ms) *total_ms = t.ms;
+  if (bytes) *bytes = t.bytes;
+  return MFS_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,108 @@
+// mfs_internal.h -- declarations shared by the HIP translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace mfs {
+
+constexpr int kThreads = 256;     // every kernel here runs 4 wave64 per block
+constexpr int kMaxRhs = 16;       // right-hand sides per PCG batch
+constexpr int kGramChunk = 128;   // f-rows of M^T staged per Gram step
+constexpr int kMaxLocal = 128;    // largest supernode handled in LDS
+
+// Per-supernode descriptor (built on the host from the plan).
+struct SuperDesc {
+  int32_t first;     // opener column (smallest member)
+  int32_t ns;        // local size = rows of the opener's column
+  int32_t rows_off;  // sptr[first]: offset of the local rows in sind
+  int32_t mem_beg;   // members are sup_ind[mem_beg .. mem_end)
+  int32_t mem_end;
+  int32_t pad;
+};
+
+// Device-resident PCG state for a batch of up to kMaxRhs right-hand sides.
+struct PcgState {
+  double rhs2[kMaxRhs];     // ||r0||^2
+  double thresh[kMaxRhs];   // tol^2 ||r0||^2
+  double absnew[kMaxRhs];   // r^T z of the current iteration
+  double dot_pq[kMaxRhs];
+  double dot_rr[kMaxRhs];
+  double dot_rz[kMaxRhs];
+  int32_t active[kMaxRhs];  // 1 while the column iterates
+  int32_t iters[kMaxRhs];
+  int32_t n_active;
+  int32_t iter;
+  int32_t maxit;
+  int32_t pad;
+  uint32_t counter[8];      // arrival counters of the last-block reductions
+};
+
+enum Counter { C_PQ = 0, C_RR = 1, C_RZ = 2, C_P = 3, C_R0 = 4, C_NORM = 5 };
+
+// ----------------------------------------------------------------- plan kernels
+hipError_t launch_pattern(hipStream_t st, int32_t n, const double *px, const double *py,
+                          const double *pz, const double *scale, double rho, int32_t *cnt,
+                          int32_t *ptr, int32_t *ind, int fill);
+hipError_t launch_scan(hipStream_t st, const int32_t *cnt, int32_t n, int32_t *ptr,
+                       int64_t *total);
+
+// --------------------------------------------------------------- factor kernels
+hipError_t launch_transpose(hipStream_t st, const float *M, int64_t m, int64_t n, float *Mt,
+                            int64_t ldmt);
+// precision: 0 = FP64 compute / FP64 values, 1 = FP32 compute / FP32 values,
+// 2 = FP64 compute / FP32 values (retry of FP32 failures).
+hipError_t launch_super_factor(hipStream_t st, int tiles, int precision, const float *Mt,
+                               int64_t ldmt, const SuperDesc *sd, const int32_t *list,
+                               int32_t count, const int32_t *sind, const int32_t *Pinv,
+                               const int32_t *sup_ind, const int32_t *sptr, void *val,
+                               int32_t *status);
+hipError_t launch_gather_values(hipStream_t st, int precision, int32_t nnz, const int32_t *tmap,
+                                const void *val, void *tval);
+
+// ---------------------------------------------------------------- solve kernels
+// TA = double (plain / residual) or float (mixed inner PCG).
+template <typename TA>
+struct SolveArgs {
+  const float *M;
+  int64_t m, n;
+  int K;                       // batch width (1, 2, 4, 8 or 16)
+  const int32_t *P, *Pinv;     // orig -> new, new -> orig
+  const int32_t *sptr, *sind;  // L, CSC (= L^T in CSR)
+  const int32_t *tptr, *tcol;  // L, CSR
+  const TA *lval, *ltval;      // factor values in both layouts
+  TA *x, *r, *p, *q, *z, *t;   // n x K, new order
+  TA *pv;                      // n x K, orig order (copy of p for the matvec)
+  TA *y;                       // m x K
+  double *part;                // partial column sums (FP64 for both precisions)
+  double *dpart;               // per-block dot partials
+  PcgState *st;
+  int32_t row_blocks;          // number of row blocks of the M^T pass
+};
+
+// thresh_abs (device, K values, may be null) replaces tol^2 ||r0||^2 as the
+// stopping threshold of each column (used by the refinement's inner solves).
+template <typename TA>
+hipError_t pcg_init(hipStream_t st, const SolveArgs<TA> &a, const float *B, int64_t ldb,
+                    int64_t col0, int rhs_space, int k_valid, double tol, int32_t maxit,
+                    const double *B64, const double *thresh_abs = nullptr);
+hipError_t inner_thresholds(hipStream_t st, const double *nb2, double tol_frac, int K, double *thr);
+template <typename TA>
+hipError_t pcg_iteration(hipStream_t st, const SolveArgs<TA> &a, void (*track)(void *, int, int),
+                         void *track_ctx);
+template <typename TA>
+hipError_t pcg_finish(hipStream_t st, const SolveArgs<TA> &a, float *X, int64_t ldx, int64_t col0,
+                      int k_valid, double *X64);
+
+// FP64 normal-equation residual R = M^T (B - M X) in original order, plus
+// per-column ||R||^2 into out2[0..K) and (if with_rhs) ||M^T B||^2 into rhs2.
+hipError_t residual64(hipStream_t st, const SolveArgs<double> &a, const float *B, int64_t ldb,
+                      int64_t col0, int k_valid, const double *X64, double *R, double *out2,
+                      void (*track)(void *, int, int), void *track_ctx);
+
+// Kernel assembly of M from FP64 points.
+hipError_t launch_assemble(hipStream_t st, const double *x, const double *nx, const double *y,
+                           int64_t m, int64_t n, float *M);
+
+}  // namespace mfs

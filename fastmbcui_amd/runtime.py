@@ -1,0 +1,134 @@
+"""Per-device solver contexts and the plan/factor cache behind the operators.
+
+The reference recomputes ordering, pattern and supernodes in every forward and
+every backward call, and passes the factor from forward to backward through a
+process-global static cache that is not keyed by geometry
+(mfs_torch/csrc/mfs_torch_ops.cu:17-19, 136-157, 213-215, 316-330).  Here one
+context per (device, stream) keeps the last plan and factor together with the
+keys they were built from:
+
+* plan key   = hash of the source points (float32 bytes) and rho;
+* factor key = plan key + identity of M (address, shape, strides, in-place
+  version counter) + precision.
+
+A call whose keys match reuses the plan and/or the factor -- same inputs, same
+result -- and any mismatch rebuilds, so a stale factor is never applied to a
+different geometry.
+"""
+from __future__ import annotations
+
+import hashlib
+import threading
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib
+
+LAMBDA = 1.5        # supernode aggregation ratio (mfs_torch_ops.cu:115)
+MAX_SUPER = 2048    # supernode member cap (mfs_torch_ops.cu:88)
+
+
+@dataclass
+class SolveInfo:
+    """What the reference computes and drops (mfs_torch_ops.cu:169-170)."""
+    iters: list = field(default_factory=list)
+    relres: list = field(default_factory=list)
+    plan_ms: float = 0.0
+    factor_ms: float = 0.0
+    solve_ms: float = 0.0
+    plan_reused: bool = False
+    factor_reused: bool = False
+    mode: str = "plain"
+
+
+class _Slot:
+    def __init__(self, device):
+        self.ctx = _lib.Context(device)
+        self.plan_key = None
+        self.factor_key = None
+
+
+_slots: dict = {}
+_lock = threading.Lock()
+_forward_seen = False
+_mode = "plain"
+last_info = SolveInfo()
+
+
+def set_solver_mode(mode: str) -> None:
+    """'plain' (default): FP64 factor and one FP64 PCG per column, the drop-in.
+    'mixed': FP32 factor, batched FP32 PCG with FP64 refinement (forward only)."""
+    global _mode
+    if mode not in ("plain", "mixed"):
+        raise ValueError("mode must be 'plain' or 'mixed'")
+    _mode = mode
+
+
+def solver_mode() -> str:
+    return _mode
+
+
+def reset() -> None:
+    """Drop every context and forget that a forward ran (tests)."""
+    global _forward_seen
+    with _lock:
+        for s in _slots.values():
+            s.ctx.close()
+        _slots.clear()
+        _forward_seen = False
+
+
+def geometry_key(Vs_np: np.ndarray, rho: float) -> bytes:
+    h = hashlib.blake2b(digest_size=16)
+    h.update(np.ascontiguousarray(Vs_np, dtype=np.float32).tobytes())
+    h.update(np.float64(rho).tobytes())
+    h.update(np.int64(Vs_np.shape[0]).tobytes())
+    return h.digest()
+
+
+def _slot(device):
+    import torch
+    idx = device.index if device.index is not None else torch.cuda.current_device()
+    key = (idx, torch.cuda.current_stream(idx).cuda_stream)
+    with _lock:
+        s = _slots.get(key)
+        if s is None:
+            s = _Slot(idx)
+            _slots[key] = s
+        return s
+
+
+def prepared(M, Vs_cpu, rho, precision):
+    """Context with a plan for Vs_cpu and a factor of M, reusing what matches."""
+    global last_info
+    s = _slot(M.device)
+    Vs_np = Vs_cpu.detach().contiguous().numpy()
+    gk = geometry_key(Vs_np, rho)
+    info = SolveInfo()
+    if s.plan_key != gk:
+        s.factor_key = None
+        s.plan_key = None
+        s.ctx.plan(Vs_np, rho, LAMBDA, MAX_SUPER)
+        s.plan_key = gk
+        info.plan_ms = s.ctx.stats().plan_ms
+    else:
+        info.plan_reused = True
+    fk = (gk, M.data_ptr(), tuple(M.shape), tuple(M.stride()), M._version, precision)
+    if s.factor_key != fk:
+        s.factor_key = None
+        s.ctx.factor_dense(M, precision)
+        s.factor_key = fk
+        info.factor_ms = s.ctx.stats().factor_ms
+    else:
+        info.factor_reused = True
+    return s.ctx, info
+
+
+def mark_forward():
+    global _forward_seen
+    _forward_seen = True
+
+
+def forward_seen() -> bool:
+    return _forward_seen

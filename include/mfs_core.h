@@ -1,0 +1,141 @@
+/* mfs_core.h -- C-ABI of the MI355X (gfx950) preconditioned MFS solve.
+ *
+ * Plain pointers and sizes only; no torch types.  Device pointers are owned by
+ * the caller; every call is asynchronous on the context's HIP stream unless
+ * it returns host data.  Entry points return an MFS_* status and never exit
+ * the process (the reference exits on any CUDA/cuBLAS/cuSPARSE error,
+ * src/macro.h:6-53).  Contexts are independent: there is no process-global
+ * factor cache (the reference keeps one, mfs_torch/csrc/mfs_torch_ops.cu:17-19).
+ *
+ * Layouts:
+ *   M  : m x n, row-major FP32 (torch's layout of the op's M argument)
+ *   B  : m x k (boundary data) or n x k (source-space data), row-major FP32
+ *   X  : n x k, row-major FP32, original source order
+ */
+#ifndef MFS_CORE_H
+#define MFS_CORE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct mfs_ctx mfs_ctx;
+
+enum mfs_status {
+    MFS_OK = 0,
+    MFS_ERR_ARG = 1,        /* bad shape / pointer / parameter */
+    MFS_ERR_HIP = 2,        /* a HIP runtime call failed */
+    MFS_ERR_DUPLICATE = 3,  /* coincident source points (zero length scale) */
+    MFS_ERR_NOT_SPD = 4,    /* a supernode Gram block lost positive definiteness */
+    MFS_ERR_STATE = 5,      /* call order (e.g. factor before plan) */
+    MFS_ERR_ALLOC = 6       /* device or host allocation failed */
+};
+
+enum mfs_precision {
+    MFS_FP64 = 0,           /* plain drop-in: FP64 factor and FP64 PCG */
+    MFS_FP32 = 1            /* chosen form: FP32 factor (FP64 retry of failed pivots) */
+};
+
+enum mfs_rhs_space {
+    MFS_RHS_BOUNDARY = 0,   /* rhs = M^T b, b of length m (pcg_guesss) */
+    MFS_RHS_SOURCE = 1      /* rhs given directly, length n (pcg_VpL_guesss) */
+};
+
+/* Solve statistics of the last mfs_solve_* call and phase timings (HIP events
+ * on the context stream, milliseconds) of the last call of each phase. */
+typedef struct mfs_stats {
+    int32_t n, n_super, max_local, pad0;
+    int64_t nnz_pattern, nnz_factor, theta_entries;
+    double sum_ns2, sum_ns3;
+    double plan_ms, factor_ms, solve_ms;
+    int32_t iters[64];        /* per right-hand side (first 64) */
+    double relres[64];        /* sqrt(RESD2/RHS2) per right-hand side (first 64) */
+    int32_t outer_iters;      /* refinement sweeps of mfs_solve_mixed */
+    int32_t retried_supernodes; /* FP32 factor: supernodes redone in FP64 */
+} mfs_stats;
+
+/* Context lifecycle.  `stream` is a hipStream_t (NULL = legacy default). */
+int mfs_ctx_create(mfs_ctx **out, int device, void *stream);
+void mfs_ctx_destroy(mfs_ctx *ctx);
+int mfs_ctx_set_stream(mfs_ctx *ctx, void *stream);
+const char *mfs_ctx_last_error(const mfs_ctx *ctx);
+int mfs_ctx_stats(const mfs_ctx *ctx, mfs_stats *out);
+
+/* Ordering, rho-radius pattern and supernodes from host source points
+ * (n x 3 row-major FP32, as the op's Vs_s argument).
+ * Replaces mfs_torch_ops.cu:39-54 (Vs_s -> FP64, fps_sampler, fast_fps) and
+ * :113-126 (simpl_sparsity, aggregate, super_sparsity, set_supernodes,
+ * set_sppatt).  rho is rounded to float for the pattern as at :114. */
+int mfs_plan(mfs_ctx *ctx, const float *src_xyz_host, int64_t n, double rho,
+             double lambda, int32_t max_super);
+/* Same with FP64 host points (used for synthetic scenes). */
+int mfs_plan_f64(mfs_ctx *ctx, const double *src_xyz_host, int64_t n, double rho,
+                 double lambda, int32_t max_super);
+
+/* Host copies of the plan (any pointer may be NULL).  Sizes: P, Pinv, scale,
+ * sup_parent: n; ptr, sptr: n+1; ind: nnz_pattern; sind: nnz_factor;
+ * sup_ptr: n_super+1; sup_ind: n. */
+int mfs_plan_export(const mfs_ctx *ctx, int32_t *P, int32_t *Pinv, double *scale,
+                    int32_t *ptr, int32_t *ind, int32_t *sup_ptr, int32_t *sup_ind,
+                    int32_t *sup_parent, int32_t *sptr, int32_t *sind);
+
+/* Numeric factor from a dense device M.  Replaces build_cov_mat_LS_float +
+ * gpu_super_klchol::compute (mfs_torch_ops.cu:104-134, kl_chol.cu:772-812). */
+int mfs_factor_dense(mfs_ctx *ctx, const float *M, int64_t m, int64_t n, int precision);
+/* Factor values to host as FP64, CSC layout of the supernodal pattern. */
+int mfs_factor_export(const mfs_ctx *ctx, double *val);
+
+/* Plain drop-in solve: one FP64 PCG per right-hand side, in order, with the
+ * reference's stopping rule ||r||^2 < tol^2 ||r0||^2 on the normal-equation
+ * residual and its zero right-hand-side shortcut.  Replaces the column loop of
+ * mfs_torch_ops.cu:159-187 (pcg_guesss) and :332-360 (pcg_VpL_guesss). */
+int mfs_solve_dense(mfs_ctx *ctx, const float *M, int64_t m, int64_t n,
+                    const float *B, int64_t k, int rhs_space, float *X,
+                    int32_t maxit, double tol);
+
+/* Chosen (mixed) form: all k right-hand sides batched through one FP32 PCG
+ * whose matvec reads M once per pass for the whole batch, wrapped in FP64
+ * iterative refinement on the true residual until
+ * ||M^T(b - M x)|| <= tol ||M^T b|| for every column (or maxit total inner
+ * iterations).  Each inner sweep stops at an absolute residual of
+ * inner_tol * tol * ||M^T b|| (inner_tol in (0, 1], default 0.5 when out of
+ * range).  X0 (n x k, may be NULL) is a warm start.  No counterpart in the
+ * reference. */
+int mfs_solve_mixed(mfs_ctx *ctx, const float *M, int64_t m, int64_t n,
+                    const float *B, int64_t k, int rhs_space, const float *X0,
+                    float *X, int32_t maxit, double tol, double inner_tol);
+
+/* FP64 relative normal-equation residual per column,
+ * ||M^T (b - M x)|| / ||M^T b|| (rhs_space 0), written to host out[k]. */
+int mfs_normal_residual(mfs_ctx *ctx, const float *M, int64_t m, int64_t n,
+                        const float *B, int64_t k, const float *X, double *out);
+
+/* Dense point-kernel matrix M[i,j] = -(x_i - y_j).n_i / (4 pi |x_i - y_j|^3),
+ * evaluated in FP64 and stored FP32.  x, nx: m x 3, y: n x 3, device FP64.
+ * Point form of src/kernels.h:24-40 (laplace_nm_3d_sl, up to the factor
+ * -1/(4 pi)) as driven by kl_chol.cu:55-126. */
+int mfs_assemble_dense(mfs_ctx *ctx, const double *x, const double *nx, const double *y,
+                       int64_t m, int64_t n, float *M);
+
+/* Kernel timing for the bench: while a class is enabled (bit (1 << class) of
+ * `mask`), every launch of that class is bracketed by HIP events on the context
+ * stream; mfs_ctx_tracked returns the launch count, the summed event time and
+ * the algorithmic bytes (for the matvec classes: m*n*4 per launch). */
+enum mfs_kernel_class {
+    MFS_K_NONE = 0,
+    MFS_K_MATVEC_ROWS = 1,   /* PCG y = M p     (one HBM pass over M, whole batch) */
+    MFS_K_MATVEC_COLS = 2,   /* PCG q = M^T y   (one HBM pass over M, whole batch) */
+    MFS_K_GRAM = 3,          /* fused supernode Gram + Cholesky + columns of L */
+    MFS_K_RESID_ROWS = 4,    /* FP64 residual y = b - M x */
+    MFS_K_RESID_COLS = 5     /* FP64 residual r = M^T y */
+};
+int mfs_ctx_track_kernel(mfs_ctx *ctx, int mask);
+int mfs_ctx_tracked(mfs_ctx *ctx, int kernel_class, int64_t *launches, double *total_ms,
+                    double *bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,321 @@
+"""GPU parity of the HIP path against the CPU oracle, through the C-ABI.
+
+* plan (ordering, pattern, supernodes, supernodal pattern): bit-exact;
+* FP64 factor values: per-column relative error against the oracle;
+* PCG: fixed-iteration trajectories and converged solutions;
+* operators: MinvF_float_fastGrad / VpL_fast_float against the oracle's FP64
+  restatement of the op, and the autograd formula against a dense FP64 solve.
+"""
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from fastmbcui_amd import _lib, runtime, scenes
+
+pytestmark = pytest.mark.gpu
+
+DEV = torch.device("cuda", 0)
+
+
+def _small(seed=0, k=2):
+    return scenes.coins(seed=seed, n_bodies=3, pts_per_body=160, k=k, box=6.0)
+
+
+def _ellips(seed=0, k=2):
+    return scenes.ellipsoids(seed=seed, n_bodies=4, pts_per_body=256, k=k, box=8.0)
+
+
+def _M32(sc):
+    return scenes.kernel_matrix(sc.x, sc.nx, sc.y).astype(np.float32)
+
+
+def _ctx():
+    return _lib.Context(0)
+
+
+def _plan_pair(points64, rho=3.0, lam=1.5, cap=2048):
+    ctx = _ctx()
+    ctx.plan(points64, rho, lam, cap)
+    got = ctx.plan_export()
+    want = oracle.plan(points64, rho, lam, cap)
+    return ctx, got, want
+
+
+def _assert_plan_equal(got, want):
+    for key in ("P", "Pinv", "scale", "ptr", "ind", "sup_ptr", "sup_ind", "sup_parent", "sptr", "sind"):
+        g = got[key]
+        w = getattr(want, key)
+        assert g.shape == w.shape, key
+        assert np.array_equal(g, w), key
+
+
+# ----------------------------------------------------------------------------- plan
+@pytest.mark.parametrize("maker", [_small, _ellips])
+def test_plan_bit_exact(maker):
+    sc = maker()
+    print(f"plan parity: {sc.name} n={sc.n}", flush=True)
+    _, got, want = _plan_pair(sc.y)
+    _assert_plan_equal(got, want)
+
+
+def test_plan_bit_exact_config1_coins():
+    sc = scenes.coins(seed=0, k=1)   # N = 4096
+    _, got, want = _plan_pair(sc.y)
+    _assert_plan_equal(got, want)
+
+
+def test_plan_from_float32_points_matches_oracle_on_upcast():
+    sc = _small()
+    y32 = sc.y.astype(np.float32)
+    ctx = _ctx()
+    ctx.plan(y32, 3.0)
+    _assert_plan_equal(ctx.plan_export(), oracle.plan(y32.astype(np.float64), 3.0))
+
+
+@pytest.mark.parametrize("rho,cap", [(1.2, 2048), (4.0, 2048), (3.0, 4)])
+def test_plan_bit_exact_other_parameters(rho, cap):
+    sc = _small(seed=2)
+    _, got, want = _plan_pair(sc.y, rho, 1.5, cap)
+    _assert_plan_equal(got, want)
+
+
+def test_plan_random_cloud_and_tiny_sizes():
+    rng = np.random.default_rng(11)
+    for n in (2, 3, 7, 65, 300):
+        pts = rng.uniform(-1, 1, (n, 3))
+        _, got, want = _plan_pair(pts)
+        _assert_plan_equal(got, want)
+
+
+def test_plan_rejects_coincident_points():
+    pts = np.random.default_rng(1).uniform(-1, 1, (40, 3))
+    pts[7] = pts[30]
+    ctx = _ctx()
+    with pytest.raises(_lib.MfsError) as e:
+        ctx.plan(pts, 3.0)
+    assert e.value.code == _lib.MFS_ERR_DUPLICATE
+
+
+# ----------------------------------------------------------------------------- factor
+def _factor_pair(sc, precision=_lib.MFS_FP64, rho=3.0):
+    M = _M32(sc)
+    ctx, got, want = _plan_pair(sc.y, rho)
+    Md = torch.from_numpy(M).to(DEV)
+    ctx.factor_dense(Md, precision)
+    val = ctx.factor_export()
+    Mp = oracle.permuted_colmajor(M.astype(np.float64), want.Pinv)
+    ref = oracle.factor(Mp, M.shape[0], want)
+    return ctx, Md, M, want, val, ref
+
+
+def _col_rel_err(pl, a, b):
+    worst = 0.0
+    for j in range(pl.n):
+        s, e = pl.sptr[j], pl.sptr[j + 1]
+        worst = max(worst, np.linalg.norm(a[s:e] - b[s:e]) / np.linalg.norm(b[s:e]))
+    return worst
+
+
+@pytest.mark.parametrize("maker", [_small, _ellips])
+def test_factor_fp64_matches_oracle(maker):
+    sc = maker()
+    ctx, Md, M, pl, val, ref = _factor_pair(sc)
+    err = _col_rel_err(pl, val, ref)
+    print(f"factor fp64 {sc.name}: max column rel err {err:.2e}", flush=True)
+    assert err < 1e-9
+
+
+def test_factor_fp32_close_to_oracle():
+    sc = _ellips()
+    ctx, Md, M, pl, val, ref = _factor_pair(sc, _lib.MFS_FP32)
+    err = _col_rel_err(pl, val, ref)
+    print(f"factor fp32: max column rel err {err:.2e}, retried {ctx.stats().retried_supernodes}", flush=True)
+    assert err < 5e-2
+
+
+def test_factor_dense_pattern_one_iteration():
+    sc = scenes.coins(seed=1, n_bodies=2, pts_per_body=24, k=1, box=4.0)
+    M = _M32(sc)
+    ctx = _ctx()
+    ctx.plan(sc.y, 1e6)
+    Md = torch.from_numpy(M).to(DEV)
+    ctx.factor_dense(Md)
+    B = torch.from_numpy(sc.b.astype(np.float32)).to(DEV)
+    ctx.solve_dense(Md, B, maxit=10, tol=1e-8)
+    assert ctx.stats().iters[0] == 1
+
+
+# ----------------------------------------------------------------------------- PCG
+def test_pcg_fixed_iterations_match_oracle_trajectory():
+    sc = _small(k=2)
+    ctx, Md, M, pl, val, ref = _factor_pair(sc)
+    Mp = oracle.permuted_colmajor(M.astype(np.float64), pl.Pinv)
+    B = torch.from_numpy(sc.b.astype(np.float32)).to(DEV)
+    for iters in (1, 3, 6):
+        X = ctx.solve_dense(Md, B, maxit=iters, tol=0.0).cpu().numpy()
+        for c in range(sc.k):
+            b = sc.b[:, c].astype(np.float32).astype(np.float64)
+            xo, it, _ = oracle.pcg(Mp, M.shape[0], pl, ref, b, 0, iters, 0.0)
+            assert it == iters and ctx.stats().iters[c] == iters
+            xo = xo[pl.P]
+            rel = np.linalg.norm(X[:, c] - xo) / np.linalg.norm(xo)
+            assert rel < 1e-6, (iters, c, rel)   # FP32 output rounding dominates
+
+
+def test_pcg_converged_matches_direct_solve():
+    sc = _small(k=2)
+    ctx, Md, M, pl, val, ref = _factor_pair(sc)
+    B = torch.from_numpy(sc.b.astype(np.float32)).to(DEV)
+    X = ctx.solve_dense(Md, B, maxit=500, tol=1e-11).cpu().numpy().astype(np.float64)
+    M64 = M.astype(np.float64)
+    xd = np.linalg.solve(M64.T @ M64, M64.T @ sc.b.astype(np.float32).astype(np.float64))
+    assert np.linalg.norm(X - xd) / np.linalg.norm(xd) < 1e-6
+    assert oracle.normal_residual(M64, sc.b.astype(np.float32), X) < 1e-6
+
+
+def test_pcg_iteration_counts_match_oracle_at_reference_tolerance():
+    sc = _ellips(k=3)
+    ctx, Md, M, pl, val, ref = _factor_pair(sc)
+    B = torch.from_numpy(sc.b.astype(np.float32)).to(DEV)
+    X = ctx.solve_dense(Md, B, maxit=200, tol=1e-6).cpu().numpy()
+    r = oracle.minvf(M.astype(np.float64), sc.b.astype(np.float32), sc.y, 3.0, 200, 1e-6, pl=pl, val=ref)
+    got = [ctx.stats().iters[c] for c in range(sc.k)]
+    assert all(abs(a - b) <= 1 for a, b in zip(got, r.iters)), (got, r.iters)
+    for c in range(sc.k):
+        if got[c] == r.iters[c]:
+            rel = np.linalg.norm(X[:, c] - r.x[:, c]) / np.linalg.norm(r.x[:, c])
+            assert rel < 1e-5
+
+
+def test_pcg_zero_rhs_and_zero_iterations():
+    sc = _small(k=2)
+    ctx, Md, M, pl, val, ref = _factor_pair(sc)
+    B = torch.zeros((M.shape[0], 2), dtype=torch.float32, device=DEV)
+    X = ctx.solve_dense(Md, B, maxit=50, tol=1e-6)
+    assert torch.count_nonzero(X).item() == 0 and ctx.stats().iters[0] == 0
+    B = torch.from_numpy(sc.b.astype(np.float32)).to(DEV)
+    X = ctx.solve_dense(Md, B, maxit=0, tol=1e-6)
+    assert torch.count_nonzero(X).item() == 0
+
+
+def test_source_space_solve_matches_oracle_vpl():
+    sc = _small(k=1)
+    ctx, Md, M, pl, val, ref = _factor_pair(sc)
+    rng = np.random.default_rng(5)
+    G = rng.standard_normal((sc.n, 2)).astype(np.float32)
+    X = ctx.solve_dense(Md, torch.from_numpy(G).to(DEV), rhs_space=_lib.MFS_RHS_SOURCE,
+                        maxit=500, tol=1e-11).cpu().numpy()
+    M64 = M.astype(np.float64)
+    xd = np.linalg.solve(M64.T @ M64, G.astype(np.float64))
+    assert np.linalg.norm(X - xd) / np.linalg.norm(xd) < 1e-6
+
+
+# ----------------------------------------------------------------------------- mixed
+@pytest.mark.parametrize("maker,k", [(_small, 2), (_ellips, 5)])
+def test_mixed_solve_reaches_tolerance(maker, k):
+    sc = maker(k=k)
+    ctx, Md, M, pl, val, ref = _factor_pair(sc, _lib.MFS_FP32)
+    B = torch.from_numpy(sc.b.astype(np.float32)).to(DEV)
+    X = ctx.solve_mixed(Md, B, maxit=500, tol=1e-6)
+    res = ctx.normal_residual(Md, B, X)
+    s = ctx.stats()
+    print(f"mixed {sc.name}: residual {res.max():.2e} outer {s.outer_iters} "
+          f"iters {[s.iters[c] for c in range(k)]}", flush=True)
+    assert res.max() <= 1e-6
+    res_o = oracle.normal_residual(M.astype(np.float64), sc.b.astype(np.float32), X.cpu().numpy())
+    assert res_o <= 2e-6
+
+
+# ----------------------------------------------------------------------------- operators
+def test_minvf_op_matches_oracle():
+    import fastmbcui_amd.mfs_torch_py as mfs
+    runtime.reset()
+    sc = _ellips(k=4)
+    M = _M32(sc)
+    Vs = torch.from_numpy(sc.y.astype(np.float32))
+    F = torch.from_numpy(sc.b.astype(np.float32)).to(DEV)
+    out = mfs.ops.MinvF_float_fastGrad(torch.from_numpy(M).to(DEV), F, Vs, 3.0, 75, 1e-6)
+    r = oracle.minvf(M.astype(np.float64), sc.b.astype(np.float32), Vs.numpy().astype(np.float64),
+                     3.0, 75, 1e-6)
+    info = mfs.last_solve_info()
+    assert out.shape == (sc.n, 4) and out.dtype == torch.float32
+    o = out.cpu().numpy()
+    for c in range(4):
+        assert abs(info.iters[c] - r.iters[c]) <= 1
+        if info.iters[c] == r.iters[c]:
+            assert np.linalg.norm(o[:, c] - r.x[:, c]) / np.linalg.norm(r.x[:, c]) < 1e-5
+
+
+def test_vpl_requires_a_forward_first():
+    import fastmbcui_amd.mfs_torch_py as mfs
+    runtime.reset()
+    sc = _small(k=1)
+    M = torch.from_numpy(_M32(sc)).to(DEV)
+    g = torch.ones((sc.n, 1), device=DEV)
+    with pytest.raises(RuntimeError, match="Please run MinvF_float first"):
+        mfs.ops.VpL_fast_float(M, g, torch.from_numpy(sc.y.astype(np.float32)), 3.0, 50, 1e-6)
+
+
+def test_autograd_matches_dense_closed_form():
+    import fastmbcui_amd.mfs_torch_py as mfs
+    runtime.reset()
+    sc = _small(k=2)
+    M32 = _M32(sc)
+    F32 = sc.b.astype(np.float32)
+    Vs = torch.from_numpy(sc.y.astype(np.float32))
+    M = torch.from_numpy(M32).to(DEV).requires_grad_(True)
+    F = torch.from_numpy(F32).to(DEV).requires_grad_(True)
+    rng = np.random.default_rng(9)
+    G = rng.standard_normal((sc.n, 2)).astype(np.float32)
+    A = mfs.ops.MinvF_float_fastGrad(M, F, Vs, 3.0, 500, 1e-11)
+    (A * torch.from_numpy(G).to(DEV)).sum().backward()
+    M64, F64, G64 = M32.astype(np.float64), F32.astype(np.float64), G.astype(np.float64)
+    N = M64.T @ M64
+    A64 = np.linalg.solve(N, M64.T @ F64)
+    V = np.linalg.solve(N, G64)
+    gF = M64 @ V
+    gM = F64 @ V.T - M64 @ (A64 @ V.T + V @ A64.T)
+    assert np.linalg.norm(F.grad.cpu().numpy() - gF) / np.linalg.norm(gF) < 1e-4
+    assert np.linalg.norm(M.grad.cpu().numpy() - gM) / np.linalg.norm(gM) < 1e-4
+
+
+def test_op_reuses_plan_and_factor_for_identical_inputs():
+    import fastmbcui_amd.mfs_torch_py as mfs
+    runtime.reset()
+    sc = _small(k=1)
+    M = torch.from_numpy(_M32(sc)).to(DEV)
+    F = torch.from_numpy(sc.b.astype(np.float32)).to(DEV)
+    Vs = torch.from_numpy(sc.y.astype(np.float32))
+    a = mfs.ops.MinvF_float_fastGrad(M, F, Vs, 3.0, 75, 1e-6)
+    assert not mfs.last_solve_info().plan_reused
+    b = mfs.ops.MinvF_float_fastGrad(M, F, Vs, 3.0, 75, 1e-6)
+    info = mfs.last_solve_info()
+    assert info.plan_reused and info.factor_reused
+    assert torch.equal(a, b)
+    M.mul_(1.0)   # in-place write bumps the version: factor must be rebuilt
+    mfs.ops.MinvF_float_fastGrad(M, F, Vs, 3.0, 75, 1e-6)
+    assert not mfs.last_solve_info().factor_reused
+
+
+def test_op_rejects_coincident_sources():
+    import fastmbcui_amd.mfs_torch_py as mfs
+    runtime.reset()
+    sc = _small(k=1)
+    y = sc.y.astype(np.float32).copy()
+    y[5] = y[17]
+    M = torch.from_numpy(_M32(sc)).to(DEV)
+    F = torch.from_numpy(sc.b.astype(np.float32)).to(DEV)
+    with pytest.raises(RuntimeError):
+        mfs.ops.MinvF_float_fastGrad(M, F, torch.from_numpy(y), 3.0, 75, 1e-6)
+
+
+def test_assemble_matches_numpy_kernel():
+    sc = _small(k=1)
+    ctx = _ctx()
+    x = torch.from_numpy(sc.x).to(DEV)
+    nx = torch.from_numpy(sc.nx).to(DEV)
+    y = torch.from_numpy(sc.y).to(DEV)
+    M = ctx.assemble_dense(x, nx, y).cpu().numpy()
+    ref = scenes.kernel_matrix(sc.x, sc.nx, sc.y)
+    assert np.max(np.abs(M - ref) / np.abs(ref)) < 1e-6
