@@ -47,6 +47,8 @@ def _parse():
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--inner-tol", type=float, default=0.5)
+    ap.add_argument("--copy-gb", type=float, default=0.0,
+                    help="run one plain device copy of this many GB first (counter calibration)")
     return ap.parse_args()
 
 
@@ -109,6 +111,15 @@ def main():
     del x, nx, y
     torch.cuda.synchronize()
     _log(f"bench: M assembled ({m} x {n}, {m * n * 4 / 1e9:.1f} GB FP32)")
+
+    if args.copy_gb > 0:
+        nel = int(args.copy_gb * 1e9 / 4)
+        src = torch.ones(nel, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        _log(f"bench: calibration copy of {nel * 4} bytes done")
+        del src, dst
 
     ctx.plan(sc.y, 3.0)
     st = ctx.stats()

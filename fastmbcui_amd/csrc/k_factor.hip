@@ -52,6 +52,26 @@ struct CMax {
   static constexpr int v = A > B ? A : B;
 };
 
+// Lower-triangle tile pair pi -> (a, b), a >= b, row-major over a.
+__host__ __device__ constexpr int pair_a(int pi) {
+  int a = 0;
+  while ((a + 1) * (a + 2) / 2 <= pi) ++a;
+  return a;
+}
+__host__ __device__ constexpr int pair_b(int pi) { return pi - pair_a(pi) * (pair_a(pi) + 1) / 2; }
+
+// One k-step of the Gram for the tile pairs GRP, GRP+NG, ... (all compile-time,
+// so each MFMA reads its operands from fixed registers).
+template <int T, int NG, int PPW, int GRP, typename TC, typename acc_t>
+__device__ inline void gram_step(const TC (&fr)[T], acc_t (&acc)[PPW]) {
+#pragma unroll
+  for (int u = 0; u < PPW; ++u) {
+    constexpr int kPairs = T * (T + 1) / 2;
+    const int pi = GRP + NG * u;
+    if (pi < kPairs) acc[u] = Mfma<TC>::run(fr[pair_a(pi)], fr[pair_b(pi)], acc[u]);
+  }
+}
+
 // ------------------------------------------------------------------ transpose
 // Mt[c][f] = M[f][c] for f < m, 0 for m <= f < ldmt.  64x64 tiles through LDS,
 // float4 in and out when the row length allows it.
@@ -99,12 +119,15 @@ __global__ __launch_bounds__(kThreads) void k_super_factor(
     const int32_t *__restrict__ Pinv, const int32_t *__restrict__ sup_ind,
     const int32_t *__restrict__ sptr, TV *__restrict__ val, int32_t *__restrict__ status) {
   constexpr int NS = 16 * T;                  // padded local size
-  constexpr int KC = kGramChunk;              // f-rows per staged chunk (512 B per row)
+  constexpr int KC = (T <= 4) ? kGramChunk : kGramChunk / 2;  // f-rows per staged chunk
   constexpr int SS = KC + 4;                  // stage row stride in floats (bank spread, 16 B aligned)
   constexpr int RS = NS + 1;                  // TH row stride
   constexpr int MB = 16;                      // member columns per batch
   constexpr int PAIRS = T * (T + 1) / 2;      // lower-triangle tile pairs
-  constexpr int KS = (T <= 4) ? 4 : (T <= 6 ? 2 : 1);  // waves splitting the k-steps
+  // Waves split the k-steps (KS) and/or the tile pairs (NG groups); FP32 keeps
+  // every pair in every wave, FP64's larger tiles split the pairs to bound
+  // the accumulator registers.
+  constexpr int KS = (sizeof(TC) == 4 || T <= 4) ? 4 : (T <= 6 ? 2 : 1);
   constexpr int NG = 4 / KS;                  // groups splitting the tile pairs
   constexpr int PPW = (PAIRS + NG - 1) / NG;  // tile pairs per wave
   constexpr int LPT = (NS * (KC / 4) + kThreads - 1) / kThreads;  // float4 loads per thread
@@ -174,16 +197,17 @@ __global__ __launch_bounds__(kThreads) void k_super_factor(
       TC fr[T];
 #pragma unroll
       for (int a = 0; a < T; ++a) fr[a] = TC(stage[(a * 16 + (lane & 15)) * SS + kk + (lane >> 4)]);
-#pragma unroll
-      for (int u = 0; u < PPW; ++u) {
-        if (pa[u] >= 0) {
-          TC fa = fr[0], fb = fr[0];
-#pragma unroll
-          for (int a = 0; a < T; ++a) {
-            if (a == pa[u]) fa = fr[a];
-            if (a == pb[u]) fb = fr[a];
-          }
-          acc[u] = Mfma<TC>::run(fa, fb, acc[u]);
+      if (NG == 1) {
+        gram_step<T, NG, PPW, 0>(fr, acc);
+      } else if (NG == 2) {
+        if (grp == 0) gram_step<T, NG, PPW, 0>(fr, acc);
+        else gram_step<T, NG, PPW, 1 % NG>(fr, acc);
+      } else {
+        switch (grp) {
+          case 0: gram_step<T, NG, PPW, 0>(fr, acc); break;
+          case 1: gram_step<T, NG, PPW, 1 % NG>(fr, acc); break;
+          case 2: gram_step<T, NG, PPW, 2 % NG>(fr, acc); break;
+          default: gram_step<T, NG, PPW, 3 % NG>(fr, acc); break;
         }
       }
     }

@@ -97,7 +97,7 @@ constexpr int kRowsThreads = 512;
 
 template <typename TA, int K>
 struct RowsCfg {
-  static constexpr int RW = (K == 1) ? 8 : ((sizeof(TA) == 8 && K >= 16) ? 2 : 4);
+  static constexpr int RW = (K == 1) ? 8 : (K * int(sizeof(TA)) >= 64 ? 2 : 4);
   // two buffers of K*CH*sizeof(TA) within 64 KB, CH a multiple of 256, at most 2048
   static constexpr int CHB = 32768 / (K * int(sizeof(TA)));
   static constexpr int CH = CHB >= 2048 ? 2048 : (CHB >= 1024 ? 1024 : (CHB >= 512 ? 512 : 256));
@@ -121,18 +121,35 @@ __global__ __launch_bounds__(kRowsThreads) void k_mv_rows(const float *__restric
 #pragma unroll
     for (int r = 0; r < K; ++r) acc[i][r] = TA(0);
 
-  auto stage = [&](int64_t c0, TA *dst) {
-    for (int idx = threadIdx.x; idx < CH * K; idx += kRowsThreads) {
+  // x chunk: each thread owns XPT values; the next chunk is loaded into
+  // registers before this chunk's M loads are issued and written to LDS after
+  // the chunk is consumed, so the L2 latency of x hides behind HBM traffic.
+  constexpr int XPT = CH * K / kRowsThreads;
+  TA xr_next[XPT];
+  auto fetch = [&](int64_t c0) {
+#pragma unroll
+    for (int j = 0; j < XPT; ++j) {
+      const int idx = threadIdx.x + j * kRowsThreads;
       const int cl = idx / K, r = idx % K;
       const int64_t c = c0 + cl;
-      dst[r * CH + cl] = (c < n) ? X[c * K + r] : TA(0);
+      xr_next[j] = (c < n) ? X[c * K + r] : TA(0);
     }
   };
-  stage(0, xs[0]);
+  auto store = [&](TA *dst) {
+#pragma unroll
+    for (int j = 0; j < XPT; ++j) {
+      const int idx = threadIdx.x + j * kRowsThreads;
+      const int cl = idx / K, r = idx % K;
+      dst[r * CH + cl] = xr_next[j];
+    }
+  };
+  fetch(0);
+  store(xs[0]);
   __syncthreads();
   int buf = 0;
   for (int64_t c0 = 0; c0 < n; c0 += CH) {
-    if (c0 + CH < n) stage(c0 + CH, xs[buf ^ 1]);
+    const bool more = c0 + CH < n;
+    if (more) fetch(c0 + CH);
     const TA *xb = xs[buf];
 #pragma unroll 2
     for (int u = 0; u < CH / 256; ++u) {
@@ -169,6 +186,7 @@ __global__ __launch_bounds__(kRowsThreads) void k_mv_rows(const float *__restric
         }
       }
     }
+    if (more) store(xs[buf ^ 1]);
     __syncthreads();
     buf ^= 1;
   }
@@ -203,6 +221,9 @@ __global__ __launch_bounds__(kThreads) void k_mv_cols(const float *__restrict__ 
                                                       double *__restrict__ part,
                                                       const PcgState *st) {
   if (batch_done(st)) return;
+  // FP32 products are summed over 32-row chunks in FP32 and the chunk sums are
+  // carried in FP64; FP64 products go straight into the FP64 sums.
+  constexpr bool kTwoLevel = sizeof(TA) == 4;
   __shared__ TA ys[kRowChunk * K];
   const int64_t i0 = (int64_t(blockIdx.x) * kThreads + threadIdx.x) * 4;
   const int64_t fb = int64_t(blockIdx.y) * rows_per_block;
@@ -218,11 +239,19 @@ __global__ __launch_bounds__(kThreads) void k_mv_cols(const float *__restrict__ 
     __syncthreads();
     for (int idx = threadIdx.x; idx < rows * K; idx += kThreads) ys[idx] = Y[fc * K + idx];
     __syncthreads();
-    TA c[4][K];
+    TA c[kTwoLevel ? 4 : 1][kTwoLevel ? K : 1];
+    if (kTwoLevel) {
 #pragma unroll
-    for (int j = 0; j < 4; ++j)
+      for (int j = 0; j < 4; ++j)
 #pragma unroll
-      for (int r = 0; r < K; ++r) c[j][r] = TA(0);
+        for (int r = 0; r < K; ++r) c[kTwoLevel ? j : 0][kTwoLevel ? r : 0] = TA(0);
+    }
+    auto add = [&](int j, int r, TA v) {
+      if (kTwoLevel)
+        c[kTwoLevel ? j : 0][kTwoLevel ? r : 0] += v;
+      else
+        acc[j][r] += double(v);
+    };
     if (i0 < n) {
       int f = 0;
       for (; f + 4 <= rows; f += 4) {
@@ -244,10 +273,10 @@ __global__ __launch_bounds__(kThreads) void k_mv_cols(const float *__restrict__ 
 #pragma unroll
           for (int r = 0; r < K; ++r) {
             const TA yv = ys[(f + u) * K + r];
-            c[0][r] += TA(v[u].x) * yv;
-            c[1][r] += TA(v[u].y) * yv;
-            c[2][r] += TA(v[u].z) * yv;
-            c[3][r] += TA(v[u].w) * yv;
+            add(0, r, TA(v[u].x) * yv);
+            add(1, r, TA(v[u].y) * yv);
+            add(2, r, TA(v[u].z) * yv);
+            add(3, r, TA(v[u].w) * yv);
           }
       }
       for (; f < rows; ++f) {
@@ -260,17 +289,19 @@ __global__ __launch_bounds__(kThreads) void k_mv_cols(const float *__restrict__ 
 #pragma unroll
         for (int r = 0; r < K; ++r) {
           const TA yv = ys[f * K + r];
-          c[0][r] += TA(v.x) * yv;
-          c[1][r] += TA(v.y) * yv;
-          c[2][r] += TA(v.z) * yv;
-          c[3][r] += TA(v.w) * yv;
+          add(0, r, TA(v.x) * yv);
+          add(1, r, TA(v.y) * yv);
+          add(2, r, TA(v.z) * yv);
+          add(3, r, TA(v.w) * yv);
         }
       }
     }
+    if (kTwoLevel) {
 #pragma unroll
-    for (int j = 0; j < 4; ++j)
+      for (int j = 0; j < 4; ++j)
 #pragma unroll
-      for (int r = 0; r < K; ++r) acc[j][r] += double(c[j][r]);
+        for (int r = 0; r < K; ++r) acc[j][r] += double(c[kTwoLevel ? j : 0][kTwoLevel ? r : 0]);
+    }
   }
   if (i0 < n) {
 #pragma unroll
@@ -612,14 +643,20 @@ void mv_rows(hipStream_t s, const SolveArgs<TA> &a, const TA *X, const float *Bs
     k_mv_rows<TA, K, RW, false><<<grid, kRowsThreads, 0, s>>>(a.M, a.m, a.n, X, Bsub, ldb, col0, kval, Y, st);
 }
 
+template <typename TY, int K>
+void mv_cols_raw(hipStream_t s, const float *M, int64_t m, int64_t n, int32_t row_blocks,
+                 const TY *Y, double *part, const PcgState *st) {
+  const int64_t rpb = (m + row_blocks - 1) / row_blocks;
+  dim3 grid(unsigned((n + 4 * kThreads - 1) / (4 * kThreads)), unsigned(row_blocks));
+  if (n % 4 == 0)
+    k_mv_cols<TY, K, true><<<grid, kThreads, 0, s>>>(M, m, n, Y, rpb, part, st);
+  else
+    k_mv_cols<TY, K, false><<<grid, kThreads, 0, s>>>(M, m, n, Y, rpb, part, st);
+}
+
 template <typename TA, int K>
 void mv_cols(hipStream_t s, const SolveArgs<TA> &a, const TA *Y, const PcgState *st) {
-  const int64_t rpb = (a.m + a.row_blocks - 1) / a.row_blocks;
-  dim3 grid(unsigned((a.n + 4 * kThreads - 1) / (4 * kThreads)), unsigned(a.row_blocks));
-  if (a.n % 4 == 0)
-    k_mv_cols<TA, K, true><<<grid, kThreads, 0, s>>>(a.M, a.m, a.n, Y, rpb, a.part, st);
-  else
-    k_mv_cols<TA, K, false><<<grid, kThreads, 0, s>>>(a.M, a.m, a.n, Y, rpb, a.part, st);
+  mv_cols_raw<TA, K>(s, a.M, a.m, a.n, a.row_blocks, Y, a.part, st);
 }
 
 typedef void (*TrackFn)(void *, int, int);
@@ -681,6 +718,10 @@ hipError_t finish_k(hipStream_t s, const SolveArgs<TA> &a, float *X, int64_t ldx
   return hipGetLastError();
 }
 
+// X64 == nullptr: R = M^T B, with FP32 products summed in FP32 over 32-row
+// chunks and FP64 across chunks (sets ||M^T b|| and the first right-hand side;
+// both only need FP32-level accuracy).  Otherwise R = M^T (B - M X64) fully in
+// FP64 -- the residual the stopping test certifies.
 template <int K>
 hipError_t residual_k(hipStream_t s, const SolveArgs<double> &a, const float *B, int64_t ldb,
                       int64_t col0, int kval, const double *X64, double *R, double *out2,
@@ -689,12 +730,16 @@ hipError_t residual_k(hipStream_t s, const SolveArgs<double> &a, const float *B,
     track(fn, tctx, 4, 0);
     mv_rows<double, K>(s, a, X64, B, ldb, col0, kval, a.y, nullptr);
     track(fn, tctx, 4, 1);
+    track(fn, tctx, 5, 0);
+    mv_cols<double, K>(s, a, a.y, nullptr);
+    track(fn, tctx, 5, 1);
   } else {
-    k_load_boundary<double, K><<<blocks_for(a.m), kThreads, 0, s>>>(a.m, B, ldb, col0, kval, a.y);
+    float *yf = reinterpret_cast<float *>(a.y);
+    k_load_boundary<float, K><<<blocks_for(a.m), kThreads, 0, s>>>(a.m, B, ldb, col0, kval, yf);
+    track(fn, tctx, 2, 0);
+    mv_cols_raw<float, K>(s, a.M, a.m, a.n, a.row_blocks, yf, a.part, nullptr);
+    track(fn, tctx, 2, 1);
   }
-  track(fn, tctx, 5, 0);
-  mv_cols<double, K>(s, a, a.y, nullptr);
-  track(fn, tctx, 5, 1);
   k_mv_reduce<double, K, RED_ORIG><<<blocks_for(a.n), kThreads, 0, s>>>(
       a.part, a.row_blocks, a.n, a.P, nullptr, nullptr, R, a.dpart, a.st, out2, 0.0, kval, 0);
   return hipGetLastError();

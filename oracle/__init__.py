@@ -9,7 +9,7 @@ The C restatement lives in ``mfs_oracle.c`` (built by ``make -C oracle`` into
 the steps the way ``mfs_torch/csrc/mfs_torch_ops.cu:21-197`` (MinvF_float) and
 ``:199-370`` (VpL_fast_float) chain them, in FP64.
 
-Parity status per function is recorded in ``oracle/README.md``.
+What pins each function is listed in ``mfs_oracle.h``.
 """
 from __future__ import annotations
 

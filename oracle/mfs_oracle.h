@@ -5,8 +5,26 @@
  * bench.py's cpu_baseline leg load it, and only as the checker.
  *
  * Every function cites the reference file:line it restates (paths relative to
- * the fastMBCUI source tree).  Parity status of each function is listed in
- * oracle/README.md.
+ * the fastMBCUI source tree).  None of the reference's code builds here (its
+ * vendored Eigen has no Core module), so each function is pinned by an
+ * independent check in tests/test_oracle.py:
+ *   mo_fast_fps       == mo_maximin_brute on tie-free inputs (sequence and
+ *                        distances bit-equal); on tie-heavy inputs the maximin
+ *                        invariants hold.  The heap's order among exactly equal
+ *                        keys follows src/maxmin.h line by line but is
+ *                        parity unpinned (no compiled reference to compare).
+ *   mo_maximin_brute  O(n^2) definition of the maximin ordering.
+ *   mo_pattern        == a numpy evaluation of the same predicate, all pairs.
+ *   mo_aggregate      partition / opener / scale-rule / greedy-maximality
+ *                        invariants.
+ *   mo_super_pattern  == union of member columns restricted to rows >= j.
+ *   mo_factor         each column == Theta_j^{-1} e1 / sqrt(e1^T Theta_j^{-1} e1)
+ *                        (numpy solve); dense pattern gives L L^T = (M^T M)^{-1};
+ *                        mutants (dropped reversal / normalisation /
+ *                        permutation) are rejected.
+ *   mo_pcg            converges to the dense solve of M^T M x = M^T b; exact
+ *                        preconditioner converges in one iteration; stops at the
+ *                        first iteration meeting the threshold.
  */
 #ifndef MFS_ORACLE_H
 #define MFS_ORACLE_H
