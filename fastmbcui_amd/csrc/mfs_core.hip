@@ -157,6 +157,17 @@ void track_cb(void *p, int cls, int phase) {
   }
 }
 
+// Forget the last `count` launches of a class (iterations launched after the
+// batch converged: they exit at once and move no data).
+void track_discard(mfs_ctx *ctx, int cls, int count) {
+  if (count <= 0 || cls <= 0 || cls >= 8 || !(ctx->track_mask & (1 << cls))) return;
+  mfs_ctx::Track &t = ctx->tr[cls];
+  const int k = std::min<int>(count, int(t.used));
+  t.used -= size_t(k);
+  t.launches -= k;
+  t.bytes -= k * ctx->bytes_per_launch[cls];
+}
+
 void track_fold(mfs_ctx *ctx) {
   bool any = false;
   for (auto &t : ctx->tr) any = any || t.used > 0;
@@ -470,6 +481,9 @@ int iterate(mfs_ctx *ctx, const SolveArgs<TA> &a, int32_t maxit) {
     }
     chunk = std::min(chunk * 2, 16);
   }
+  const int dead = launched - ctx->hstate->iter;
+  track_discard(ctx, MFS_K_MATVEC_ROWS, dead);
+  track_discard(ctx, MFS_K_MATVEC_COLS, dead);
   return MFS_OK;
 }
 

@@ -139,7 +139,9 @@ def _(M, pL, Vs_s, rho, pcg_max_iter, tol_r, timing_verbose=False, solve_verbose
 # ---------------------------------------------------------------- autograd
 # A = (M^T M)^{-1} M^T F.  With V = (M^T M)^{-1} G for the incoming gradient G:
 #   dL/dF = M V,   dL/dM = F V^T - M (A V^T + V A^T)
-# (core/mfs_torch_py/ops.py:42-58).
+# (core/mfs_torch_py/ops.py:42-58).  The reference forms W = A V^T (n x n) and
+# M (W + W^T) (m n^2 flops); here the same quantity is evaluated as
+# (F - M A) V^T - (M V) A^T, two rank-k products, and M V is shared with dL/dF.
 def _setup_context(ctx, inputs, output):
     M, F, Vs_s, rho, pcg_max_iter, tol_r, timing_verbose, solve_verbose = inputs
     if ctx.needs_input_grad[0] or ctx.needs_input_grad[1]:
@@ -161,12 +163,11 @@ def _backward(ctx, grad):
         VpL = torch.ops.mfs_torch.VpL_fast_float.default(M, grad.contiguous(), Vs_s, ctx.rho,
                                                          ctx.pcg_max_iter, ctx.tol_r,
                                                          ctx.timing_verbose, ctx.solve_verbose)
+        MV = M @ VpL
         if ctx.needs_input_grad[0]:
-            LV = VpL.T
-            W = A @ LV
-            grad_M = -M @ (W + W.T) + F @ LV
+            grad_M = (F - M @ A) @ VpL.T - MV @ A.T
         if ctx.needs_input_grad[1]:
-            grad_F = M @ VpL
+            grad_F = MV
     return grad_M, grad_F, None, None, None, None, None, None
 
 

@@ -505,11 +505,17 @@ static double dotn(int32_t n, const double *a, const double *b)
 static void normal_matvec(int64_t m, int32_t n, const double *Mp, const double *p,
                           double *y, double *q)
 {
-    for (int64_t f = 0; f < m; ++f) y[f] = 0.0;
-    for (int32_t c = 0; c < n; ++c) {
-        const double pc = p[c];
-        const double *col = Mp + (int64_t)c * m;
-        for (int64_t f = 0; f < m; ++f) y[f] += col[f] * pc;
+    /* y = Mp p: row blocks in parallel, columns in order within each block. */
+    const int64_t blk = 256;
+    #pragma omp parallel for schedule(static)
+    for (int64_t f0 = 0; f0 < m; f0 += blk) {
+        const int64_t f1 = (f0 + blk < m) ? f0 + blk : m;
+        for (int64_t f = f0; f < f1; ++f) y[f] = 0.0;
+        for (int32_t c = 0; c < n; ++c) {
+            const double pc = p[c];
+            const double *col = Mp + (int64_t)c * m;
+            for (int64_t f = f0; f < f1; ++f) y[f] += col[f] * pc;
+        }
     }
     #pragma omp parallel for schedule(static)
     for (int32_t c = 0; c < n; ++c) {
