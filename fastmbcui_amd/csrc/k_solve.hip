@@ -253,11 +253,12 @@ __global__ __launch_bounds__(kThreads) void k_mv_cols(const float *__restrict__ 
         acc[j][r] += double(v);
     };
     if (i0 < n) {
+      constexpr int RU = 4;  // rows in flight per thread
       int f = 0;
-      for (; f + 4 <= rows; f += 4) {
-        float4 v[4];
+      for (; f + RU <= rows; f += RU) {
+        float4 v[RU];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
+        for (int u = 0; u < RU; ++u) {
           const float *rowp = M + (fc + f + u) * n;
           if (VEC && i0 + 3 < n) {
             v[u] = *reinterpret_cast<const float4 *>(rowp + i0);
@@ -269,7 +270,7 @@ __global__ __launch_bounds__(kThreads) void k_mv_cols(const float *__restrict__ 
           }
         }
 #pragma unroll
-        for (int u = 0; u < 4; ++u)
+        for (int u = 0; u < RU; ++u)
 #pragma unroll
           for (int r = 0; r < K; ++r) {
             const TA yv = ys[(f + u) * K + r];

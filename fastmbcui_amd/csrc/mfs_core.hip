@@ -405,6 +405,7 @@ int values_for(mfs_ctx *ctx, int want, const void **lval, const void **ltval) {
   return MFS_OK;
 }
 
+// Row blocks of the M^T pass: about 2048 blocks in all, at least 64 rows each.
 int32_t row_blocks_for(int64_t m, int64_t n) {
   const int64_t cols = (n + 1023) / 1024;
   int64_t rb = (2048 + cols - 1) / cols;
