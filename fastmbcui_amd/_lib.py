@@ -76,6 +76,10 @@ def lib():
             "mfs_solve_mixed": (ci, [vp, vp, i64, i64, vp, i64, ci, vp, vp, i32, f64, f64]),
             "mfs_normal_residual": (ci, [vp, vp, i64, i64, vp, i64, vp, vp]),
             "mfs_assemble_dense": (ci, [vp, vp, vp, vp, i64, i64, vp]),
+            "mfs_factor_points": (ci, [vp, vp, vp, i64, vp, i64, ci]),
+            "mfs_solve_points": (ci, [vp, vp, i64, vp, vp, i32, f64, f64]),
+            "mfs_points_residual": (ci, [vp, vp, i64, vp, vp]),
+            "mfs_points_apply": (ci, [vp, ci, ci, vp, i64, vp]),
             "mfs_ctx_track_kernel": (ci, [vp, ci]),
             "mfs_ctx_tracked": (ci, [vp, ci, ctypes.POINTER(i64), ctypes.POINTER(f64), ctypes.POINTER(f64)]),
         }
@@ -112,6 +116,7 @@ class Context:
                                          _stream_handle(self.device)), "mfs_ctx_create", raw=True)
         self._h = h
         self.n = None
+        self.m_points = None
         self.factor_precision = None
 
     def close(self):
@@ -227,6 +232,65 @@ class Context:
                                              _ptr(y.contiguous()), x.shape[0], y.shape[0], _ptr(M)),
                     "mfs_assemble_dense")
         return M
+
+    # ---- points (matrix-free) form ---------------------------------------------
+    def factor_points(self, x, nx, y, precision=MFS_FP32):
+        """Pack FP64 device points (x, nx: m x 3 targets/normals; y: n x 3 planned
+        sources, original order) and factor with the Gram evaluated from them."""
+        import torch
+        self._sync_stream()
+        pts = [t.to(torch.float64).contiguous() for t in (x, nx, y)]
+        for t in pts:
+            if not t.is_cuda or t.dim() != 2 or t.shape[1] != 3:
+                raise ValueError("points must be (count, 3) device tensors")
+        if pts[0].shape != pts[1].shape or self.n is None or pts[2].shape[0] != self.n:
+            raise ValueError("x and nx must match; y must hold the planned sources")
+        self._points = pts  # keep the inputs alive until the packing has run
+        self.m_points = pts[0].shape[0]
+        self._check(lib().mfs_factor_points(self._h, _ptr(pts[0]), _ptr(pts[1]), pts[0].shape[0],
+                                            _ptr(pts[2]), pts[2].shape[0], int(precision)),
+                    "mfs_factor_points")
+        self.factor_precision = precision
+
+    def solve_points(self, B, maxit=1000, tol=1e-6, inner_tol=0.5, x0=None, out=None):
+        """Matrix-free mixed solve.  B: m x k float32 cuda."""
+        import torch
+        self._sync_stream()
+        B = _as_2d_f32(B)
+        if B.shape[0] != self.m_points:
+            raise ValueError("B must have one row per target point")
+        if out is None:
+            out = torch.empty((self.n, B.shape[1]), dtype=torch.float32, device=B.device)
+        if x0 is not None:
+            x0 = _as_2d_f32(x0)
+        self._check(lib().mfs_solve_points(self._h, _ptr(B), B.shape[1], _ptr(x0), _ptr(out), int(maxit),
+                                           float(tol), float(inner_tol)), "mfs_solve_points")
+        return out
+
+    def points_residual(self, B, X):
+        self._sync_stream()
+        B = _as_2d_f32(B)
+        X = _as_2d_f32(X)
+        out = np.empty(B.shape[1], np.float64)
+        self._check(lib().mfs_points_residual(self._h, _ptr(B), B.shape[1], _ptr(X), _ptr(out)),
+                    "mfs_points_residual")
+        return out
+
+    def points_apply(self, v, trans=False, precision=MFS_FP32):
+        """M v (v: n x k) or M^T v (trans, v: m x k) from the packed points."""
+        import torch
+        self._sync_stream()
+        dt = torch.float64 if precision == MFS_FP64 else torch.float32
+        if v.dim() == 1:
+            v = v.unsqueeze(1)
+        v = v.to(dt).contiguous()
+        rows_in, rows_out = (self.m_points, self.n) if trans else (self.n, self.m_points)
+        if v.shape[0] != rows_in:
+            raise ValueError("operand has the wrong number of rows")
+        out = torch.empty((rows_out, v.shape[1]), dtype=dt, device=v.device)
+        self._check(lib().mfs_points_apply(self._h, int(bool(trans)), int(precision), _ptr(v), v.shape[1],
+                                           _ptr(out)), "mfs_points_apply")
+        return out
 
     # ---- statistics -----------------------------------------------------------
     def stats(self):

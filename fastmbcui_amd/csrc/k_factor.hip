@@ -17,6 +17,7 @@
 //     written in ascending row order -- the KL column
 //     Theta_j^{-1} e_1 / sqrt(e_1^T Theta_j^{-1} e_1) (kl_chol.cu:378-386).
 //     Members are solved in batches of 16 with one barrier per row.
+#include "kernel_eval.h"
 #include "mfs_internal.h"
 
 namespace mfs {
@@ -112,9 +113,14 @@ __global__ __launch_bounds__(kThreads) void k_transpose(const float *__restrict_
 }
 
 // ------------------------------------------------------------------ fused supernode factor
-template <int T, typename TC, typename TV>
+// PTS: the points form -- each staged chunk A[f0:f0+KC, cols] is evaluated
+// from the packed point records (tgt: m x 12, srcp: n x 8) instead of read
+// from M^T; a thread owns one row f of the chunk (its target in registers)
+// and walks the local columns, whose source records sit in LDS.
+template <int T, typename TC, typename TV, bool PTS>
 __global__ __launch_bounds__(kThreads) void k_super_factor(
-    const float *__restrict__ Mt, int64_t ldmt, const SuperDesc *__restrict__ sd,
+    const float *__restrict__ Mt, int64_t ldmt, const float *__restrict__ tgt,
+    const float *__restrict__ srcp, int64_t m, const SuperDesc *__restrict__ sd,
     const int32_t *__restrict__ list, const int32_t *__restrict__ sind,
     const int32_t *__restrict__ Pinv, const int32_t *__restrict__ sup_ind,
     const int32_t *__restrict__ sptr, TV *__restrict__ val, int32_t *__restrict__ status) {
@@ -141,6 +147,7 @@ __global__ __launch_bounds__(kThreads) void k_super_factor(
   __shared__ TC W[NS * MB];
   float *stage = reinterpret_cast<float *>(big_raw);
   __shared__ int32_t mcol[MB], mc[MB], moff[MB];
+  __shared__ float sp[PTS ? NS * 8 : 1];       // source records of the local columns
 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int32_t s = list[blockIdx.x];
@@ -148,6 +155,13 @@ __global__ __launch_bounds__(kThreads) void k_super_factor(
   const int ns = D.ns;
 
   for (int p = tid; p < NS; p += kThreads) src[p] = (p < ns) ? Pinv[sind[D.rows_off + p]] : -1;
+  if constexpr (PTS) {
+    __syncthreads();
+    for (int idx = tid; idx < NS * 8; idx += kThreads) {
+      const int p = idx >> 3, d = idx & 7;
+      sp[idx] = (p < ns && d < 6) ? srcp[int64_t(src[p]) * 8 + d] : 0.f;
+    }
+  }
   __syncthreads();
 
   // ---- 1. Gram on MFMA ------------------------------------------------------
@@ -170,26 +184,54 @@ __global__ __launch_bounds__(kThreads) void k_super_factor(
 #pragma unroll
   for (int u = 0; u < PPW; ++u) acc[u] = acc_t{TC(0), TC(0), TC(0), TC(0)};
 
-  const int64_t nch = ldmt / KC;
-  float4 buf[LPT];
+  const int64_t nch = PTS ? (m + KC - 1) / KC : ldmt / KC;
+  float4 buf[PTS ? 3 : LPT];
   auto load_chunk = [&](int64_t ch) {
+    if constexpr (PTS) {
+      const int64_t f = ch * KC + tid % KC;
+      if (f < m) {
+        buf[0] = *reinterpret_cast<const float4 *>(tgt + f * 12);
+        buf[1] = *reinterpret_cast<const float4 *>(tgt + f * 12 + 4);
+        buf[2] = *reinterpret_cast<const float4 *>(tgt + f * 12 + 8);
+      } else {  // padding row: zero normal gives zero kernel values
+        buf[0] = make_float4(1e30f, 0.f, 0.f, 0.f);
+        buf[1] = make_float4(0.f, 0.f, 0.f, 0.f);
+        buf[2] = make_float4(0.f, 0.f, 0.f, 0.f);
+      }
+    } else {
 #pragma unroll
-    for (int u = 0; u < LPT; ++u) {
-      const int idx = tid + u * kThreads;
-      const int p = idx / (KC / 4), c4 = idx % (KC / 4);
-      buf[u] = make_float4(0.f, 0.f, 0.f, 0.f);
-      if (p < NS && src[p] >= 0)
-        buf[u] = *reinterpret_cast<const float4 *>(Mt + int64_t(src[p]) * ldmt + ch * KC + c4 * 4);
+      for (int u = 0; u < LPT; ++u) {
+        const int idx = tid + u * kThreads;
+        const int p = idx / (KC / 4), c4 = idx % (KC / 4);
+        buf[u] = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (p < NS && src[p] >= 0)
+          buf[u] = *reinterpret_cast<const float4 *>(Mt + int64_t(src[p]) * ldmt + ch * KC + c4 * 4);
+      }
     }
   };
   load_chunk(0);
   for (int64_t ch = 0; ch < nch; ++ch) {
     __syncthreads();
+    if constexpr (PTS) {
+      // K(f, y_p) for this thread's row f and columns p = pg, pg + PG, ...
+      constexpr int PG = kThreads / KC;
+      const int fl = tid % KC, pg = tid / KC;
+      const float xh0 = buf[0].x, xh1 = buf[0].y, xh2 = buf[0].z;
+      const float xl0 = buf[0].w, xl1 = buf[1].x, xl2 = buf[1].y;
+      const float n0 = buf[1].z, n1 = buf[1].w, n2 = buf[2].x;
+      for (int p = pg; p < NS; p += PG) {
+        const float *y = sp + p * 8;
+        const float v = dlp32((xh0 - y[0]) + (xl0 - y[3]), (xh1 - y[1]) + (xl1 - y[4]),
+                              (xh2 - y[2]) + (xl2 - y[5]), n0, n1, n2);
+        stage[p * SS + fl] = (p < ns) ? float(-kInv4Pi) * v : 0.f;
+      }
+    } else {
 #pragma unroll
-    for (int u = 0; u < LPT; ++u) {
-      const int idx = tid + u * kThreads;
-      const int p = idx / (KC / 4), c4 = idx % (KC / 4);
-      if (p < NS) *reinterpret_cast<float4 *>(stage + p * SS + c4 * 4) = buf[u];
+      for (int u = 0; u < LPT; ++u) {
+        const int idx = tid + u * kThreads;
+        const int p = idx / (KC / 4), c4 = idx % (KC / 4);
+        if (p < NS) *reinterpret_cast<float4 *>(stage + p * SS + c4 * 4) = buf[u];
+      }
     }
     __syncthreads();
     if (ch + 1 < nch) load_chunk(ch + 1);
@@ -321,28 +363,41 @@ __global__ __launch_bounds__(kThreads) void k_gather_values(int32_t nnz,
   if (t < nnz) tval[t] = val[tmap[t]];
 }
 
+struct FactorLaunch {
+  const float *Mt;
+  int64_t ldmt;
+  const PointsOp *pts;
+  const SuperDesc *sd;
+  const int32_t *list;
+  int32_t count;
+  const int32_t *sind, *Pinv, *sup_ind, *sptr;
+  void *val;
+  int32_t *status;
+};
+
 template <int T, typename TC, typename TV>
-hipError_t launch_t(hipStream_t st, const float *Mt, int64_t ldmt, const SuperDesc *sd,
-                    const int32_t *list, int32_t count, const int32_t *sind, const int32_t *Pinv,
-                    const int32_t *sup_ind, const int32_t *sptr, void *val, int32_t *status) {
-  if (count <= 0) return hipSuccess;
-  k_super_factor<T, TC, TV><<<unsigned(count), kThreads, 0, st>>>(
-      Mt, ldmt, sd, list, sind, Pinv, sup_ind, sptr, static_cast<TV *>(val), status);
+hipError_t launch_t(hipStream_t st, const FactorLaunch &L) {
+  if (L.count <= 0) return hipSuccess;
+  if (L.pts != nullptr)
+    k_super_factor<T, TC, TV, true><<<unsigned(L.count), kThreads, 0, st>>>(
+        nullptr, 0, L.pts->tgt32, L.pts->src32, L.pts->m, L.sd, L.list, L.sind, L.Pinv,
+        L.sup_ind, L.sptr, static_cast<TV *>(L.val), L.status);
+  else
+    k_super_factor<T, TC, TV, false><<<unsigned(L.count), kThreads, 0, st>>>(
+        L.Mt, L.ldmt, nullptr, nullptr, 0, L.sd, L.list, L.sind, L.Pinv, L.sup_ind, L.sptr,
+        static_cast<TV *>(L.val), L.status);
   return hipGetLastError();
 }
 
 template <typename TC, typename TV>
-hipError_t launch_prec(hipStream_t st, int tiles, const float *Mt, int64_t ldmt,
-                       const SuperDesc *sd, const int32_t *list, int32_t count,
-                       const int32_t *sind, const int32_t *Pinv, const int32_t *sup_ind,
-                       const int32_t *sptr, void *val, int32_t *status) {
+hipError_t launch_prec(hipStream_t st, int tiles, const FactorLaunch &L) {
   switch (tiles) {
-    case 1: return launch_t<1, TC, TV>(st, Mt, ldmt, sd, list, count, sind, Pinv, sup_ind, sptr, val, status);
-    case 2: return launch_t<2, TC, TV>(st, Mt, ldmt, sd, list, count, sind, Pinv, sup_ind, sptr, val, status);
-    case 3: return launch_t<3, TC, TV>(st, Mt, ldmt, sd, list, count, sind, Pinv, sup_ind, sptr, val, status);
-    case 4: return launch_t<4, TC, TV>(st, Mt, ldmt, sd, list, count, sind, Pinv, sup_ind, sptr, val, status);
-    case 6: return launch_t<6, TC, TV>(st, Mt, ldmt, sd, list, count, sind, Pinv, sup_ind, sptr, val, status);
-    case 8: return launch_t<8, TC, TV>(st, Mt, ldmt, sd, list, count, sind, Pinv, sup_ind, sptr, val, status);
+    case 1: return launch_t<1, TC, TV>(st, L);
+    case 2: return launch_t<2, TC, TV>(st, L);
+    case 3: return launch_t<3, TC, TV>(st, L);
+    case 4: return launch_t<4, TC, TV>(st, L);
+    case 6: return launch_t<6, TC, TV>(st, L);
+    case 8: return launch_t<8, TC, TV>(st, L);
     default: return hipErrorInvalidValue;
   }
 }
@@ -360,15 +415,14 @@ hipError_t launch_transpose(hipStream_t st, const float *M, int64_t m, int64_t n
 }
 
 hipError_t launch_super_factor(hipStream_t st, int tiles, int precision, const float *Mt,
-                               int64_t ldmt, const SuperDesc *sd, const int32_t *list,
-                               int32_t count, const int32_t *sind, const int32_t *Pinv,
-                               const int32_t *sup_ind, const int32_t *sptr, void *val,
-                               int32_t *status) {
-  if (precision == 0)
-    return launch_prec<double, double>(st, tiles, Mt, ldmt, sd, list, count, sind, Pinv, sup_ind, sptr, val, status);
-  if (precision == 1)
-    return launch_prec<float, float>(st, tiles, Mt, ldmt, sd, list, count, sind, Pinv, sup_ind, sptr, val, status);
-  return launch_prec<double, float>(st, tiles, Mt, ldmt, sd, list, count, sind, Pinv, sup_ind, sptr, val, status);
+                               int64_t ldmt, const PointsOp *pts, const SuperDesc *sd,
+                               const int32_t *list, int32_t count, const int32_t *sind,
+                               const int32_t *Pinv, const int32_t *sup_ind, const int32_t *sptr,
+                               void *val, int32_t *status) {
+  const FactorLaunch L{Mt, ldmt, pts, sd, list, count, sind, Pinv, sup_ind, sptr, val, status};
+  if (precision == 0) return launch_prec<double, double>(st, tiles, L);
+  if (precision == 1) return launch_prec<float, float>(st, tiles, L);
+  return launch_prec<double, float>(st, tiles, L);
 }
 
 hipError_t launch_gather_values(hipStream_t st, int precision, int32_t nnz, const int32_t *tmap,

@@ -14,6 +14,7 @@
 //   t = L^T r        k_spmv (CSC of L read as CSR of L^T)
 //   z = L t          k_spmv (CSR of L)  + dot(r, z)
 //   p = z + b p      k_upd_p    (+ copy of p in original order for the matvec)
+#include "kernel_eval.h"
 #include "mfs_internal.h"
 
 namespace mfs {
@@ -635,6 +636,10 @@ inline unsigned blocks_for(int64_t count) { return unsigned((count + kThreads - 
 template <typename TA, int K>
 void mv_rows(hipStream_t s, const SolveArgs<TA> &a, const TA *X, const float *Bsub, int64_t ldb,
              int64_t col0, int kval, TA *Y, const PcgState *st) {
+  if (a.pts.tgt32 != nullptr) {  // points form (errors surface through hipGetLastError)
+    (void)points_rows<TA>(s, a.pts, K, X, Bsub, ldb, col0, kval, Y, a.part, st);
+    return;
+  }
   constexpr int RW = RowsCfg<TA, K>::RW;
   constexpr int rows_per_block = (kRowsThreads / 64) * RW;
   const unsigned grid = unsigned((a.m + rows_per_block - 1) / rows_per_block);
@@ -655,9 +660,13 @@ void mv_cols_raw(hipStream_t s, const float *M, int64_t m, int64_t n, int32_t ro
     k_mv_cols<TY, K, false><<<grid, kThreads, 0, s>>>(M, m, n, Y, rpb, part, st);
 }
 
-template <typename TA, int K>
-void mv_cols(hipStream_t s, const SolveArgs<TA> &a, const TA *Y, const PcgState *st) {
-  mv_cols_raw<TA, K>(s, a.M, a.m, a.n, a.row_blocks, Y, a.part, st);
+// part = split sums of M^T Y (a.row_blocks splits in both forms).
+template <typename TA, typename TY, int K>
+void mv_cols(hipStream_t s, const SolveArgs<TA> &a, const TY *Y, const PcgState *st) {
+  if (a.pts.tgt32 != nullptr)
+    (void)points_cols<TY>(s, a.pts, K, Y, a.part, st);
+  else
+    mv_cols_raw<TY, K>(s, a.M, a.m, a.n, a.row_blocks, Y, a.part, st);
 }
 
 typedef void (*TrackFn)(void *, int, int);
@@ -673,7 +682,7 @@ hipError_t init_k(hipStream_t s, const SolveArgs<TA> &a, const float *B, int64_t
   k_zero<TA><<<blocks_for(a.n * K), kThreads, 0, s>>>(a.n * K, a.x);
   if (rhs_space == 0) {
     k_load_boundary<TA, K><<<blocks_for(a.m), kThreads, 0, s>>>(a.m, B, ldb, col0, kval, a.y);
-    mv_cols<TA, K>(s, a, a.y, nullptr);
+    mv_cols<TA, TA, K>(s, a, a.y, nullptr);
     k_mv_reduce<TA, K, RED_R0><<<gn, kThreads, 0, s>>>(a.part, a.row_blocks, a.n, a.P, nullptr,
                                                         a.r, nullptr, a.dpart, a.st, nullptr, tol,
                                                         kval, maxit);
@@ -698,7 +707,7 @@ hipError_t iter_k(hipStream_t s, const SolveArgs<TA> &a, TrackFn fn, void *tctx)
   mv_rows<TA, K>(s, a, a.pv, nullptr, 0, 0, K, a.y, a.st);
   track(fn, tctx, 1, 1);
   track(fn, tctx, 2, 0);
-  mv_cols<TA, K>(s, a, a.y, a.st);
+  mv_cols<TA, TA, K>(s, a, a.y, a.st);
   track(fn, tctx, 2, 1);
   k_mv_reduce<TA, K, RED_Q><<<gn, kThreads, 0, s>>>(a.part, a.row_blocks, a.n, a.P, a.pv, a.q,
                                                      nullptr, a.dpart, a.st, nullptr, 0.0, K, 0);
@@ -732,13 +741,13 @@ hipError_t residual_k(hipStream_t s, const SolveArgs<double> &a, const float *B,
     mv_rows<double, K>(s, a, X64, B, ldb, col0, kval, a.y, nullptr);
     track(fn, tctx, 4, 1);
     track(fn, tctx, 5, 0);
-    mv_cols<double, K>(s, a, a.y, nullptr);
+    mv_cols<double, double, K>(s, a, a.y, nullptr);
     track(fn, tctx, 5, 1);
   } else {
     float *yf = reinterpret_cast<float *>(a.y);
     k_load_boundary<float, K><<<blocks_for(a.m), kThreads, 0, s>>>(a.m, B, ldb, col0, kval, yf);
     track(fn, tctx, 2, 0);
-    mv_cols_raw<float, K>(s, a.M, a.m, a.n, a.row_blocks, yf, a.part, nullptr);
+    mv_cols<double, float, K>(s, a, yf, nullptr);
     track(fn, tctx, 2, 1);
   }
   k_mv_reduce<double, K, RED_ORIG><<<blocks_for(a.n), kThreads, 0, s>>>(
@@ -815,13 +824,9 @@ __global__ __launch_bounds__(kThreads) void k_assemble(const double *__restrict_
   const int64_t i = blockIdx.y;
   const int64_t j = int64_t(blockIdx.x) * kThreads + threadIdx.x;
   if (i >= m || j >= n) return;
-  const double dx = x[3 * i] - y[3 * j];
-  const double dy = x[3 * i + 1] - y[3 * j + 1];
-  const double dz = x[3 * i + 2] - y[3 * j + 2];
-  const double r2 = dx * dx + dy * dy + dz * dz;
-  const double dn = dx * nx[3 * i] + dy * nx[3 * i + 1] + dz * nx[3 * i + 2];
-  const double r = sqrt(r2);
-  M[i * n + j] = float(-dn / (4.0 * 3.14159265358979323846 * r2 * r));
+  M[i * n + j] = float(kernel64(x[3 * i] - y[3 * j], x[3 * i + 1] - y[3 * j + 1],
+                                x[3 * i + 2] - y[3 * j + 2], nx[3 * i], nx[3 * i + 1],
+                                nx[3 * i + 2]));
 }
 
 hipError_t launch_assemble(hipStream_t s, const double *x, const double *nx, const double *y,

@@ -15,6 +15,7 @@
 #include <vector>
 
 #include "../../include/mfs_core.h"
+#include "kernel_eval.h"
 #include "mfs_internal.h"
 #include "plan_host.h"
 
@@ -81,6 +82,11 @@ struct mfs_ctx {
   int prec = MFS_FP64;
   DevBuf dval, dtval, dMt, dval_alt, dtval_alt;
   int alt_prec = -1;
+
+  // points form: packed target/source records (kernel_eval.h layouts)
+  bool has_points = false;
+  PointsOp pts{};
+  DevBuf dt32, dt64, ds32, ds64, dapin, dapout;
 
   // solve workspaces
   DevBuf dvec, dpart, ddpart, dstate, dx64, dr64, dnorm;
@@ -200,6 +206,7 @@ int plan_impl(mfs_ctx *ctx, std::vector<double> &&xyz, int64_t n64, double rho, 
   const int32_t n = int32_t(n64);
   ctx->has_plan = false;
   ctx->has_factor = false;
+  ctx->has_points = false;
   phase_begin(ctx);
 
   // 1. ordering (host)
@@ -423,7 +430,8 @@ int batch_width(int64_t k) {
 
 // Lay out a SolveArgs workspace for batch width K.
 template <typename TA>
-int workspace(mfs_ctx *ctx, const float *M, int64_t m, int64_t n, int K, SolveArgs<TA> &a) {
+int workspace(mfs_ctx *ctx, const float *M, const PointsOp *pts, int64_t m, int64_t n, int K,
+              SolveArgs<TA> &a) {
   const int want = sizeof(TA) == 8 ? MFS_FP64 : MFS_FP32;
   const void *lv = nullptr, *ltv = nullptr;
   int rc = values_for(ctx, want, &lv, &ltv);
@@ -440,7 +448,8 @@ int workspace(mfs_ctx *ctx, const float *M, int64_t m, int64_t n, int K, SolveAr
   a.tcol = ctx->dtcol.as<int32_t>();
   a.lval = static_cast<const TA *>(lv);
   a.ltval = static_cast<const TA *>(ltv);
-  a.row_blocks = row_blocks_for(m, n);
+  a.row_blocks = pts ? points_splits(n, m) : row_blocks_for(m, n);
+  a.pts = pts ? *pts : PointsOp{};
   const size_t nv = size_t(n) * K, mv = size_t(m) * K;
   HIPCHK(ctx->dvec.ensure((7 * nv + mv) * sizeof(TA) + 64), "alloc vectors");
   TA *base = ctx->dvec.as<TA>();
@@ -452,7 +461,9 @@ int workspace(mfs_ctx *ctx, const float *M, int64_t m, int64_t n, int K, SolveAr
   a.t = base + 5 * nv;
   a.pv = base + 6 * nv;
   a.y = base + 7 * nv;
-  HIPCHK(ctx->dpart.ensure(size_t(a.row_blocks) * nv * sizeof(double)), "alloc partials");
+  size_t part_elems = size_t(a.row_blocks) * nv;
+  if (pts) part_elems = std::max(part_elems, size_t(points_splits(m, n)) * mv);
+  HIPCHK(ctx->dpart.ensure(part_elems * sizeof(double)), "alloc partials");
   a.part = ctx->dpart.as<double>();
   const size_t maxblocks = size_t((n * 8 + kThreads - 1) / kThreads) + size_t((m + kThreads - 1) / kThreads) + 16;
   HIPCHK(ctx->ddpart.ensure(maxblocks * K * sizeof(double)), "alloc dot partials");
@@ -519,6 +530,249 @@ int check_dense_args(mfs_ctx *ctx, const float *M, int64_t m, int64_t n) {
   return MFS_OK;
 }
 
+constexpr int kEvalFlops = 17;  // one kernel evaluation, rsqrt counted as 1
+
+int check_points(mfs_ctx *ctx) {
+  if (!ctx->has_plan) return ctx->fail(MFS_ERR_STATE, "no plan: call mfs_plan first");
+  if (!ctx->has_points)
+    return ctx->fail(MFS_ERR_STATE, "no points: call mfs_factor_points first");
+  return MFS_OK;
+}
+
+// Algorithmic work per tracked launch of the solve passes: bytes of M read
+// (dense form) or kernel FLOPs, 17 per evaluation plus 2 per right-hand side
+// (points form).
+void set_pass_work(mfs_ctx *ctx, const PointsOp *pts, int64_t m, int64_t n, int K = 1) {
+  for (int c = 1; c < 8; ++c) {
+    if (pts == nullptr) {
+      ctx->bytes_per_launch[c] = double(m) * double(n) * 4.0;
+    } else {
+      ctx->bytes_per_launch[c] = double(m) * double(n) * double(kEvalFlops + 2 * K);
+    }
+  }
+}
+
+// Supernodal factor from M (pts == nullptr) or from the packed points.
+int factor_impl(mfs_ctx *ctx, const float *M, const PointsOp *pts, int64_t m, int64_t n,
+                int precision) {
+  if (precision != MFS_FP64 && precision != MFS_FP32)
+    return ctx->fail(MFS_ERR_ARG, "factor: precision must be MFS_FP64 or MFS_FP32");
+  HIPCHK(hipSetDevice(ctx->device), "hipSetDevice");
+  ctx->has_factor = false;
+  ctx->alt_prec = -1;
+  phase_begin(ctx);
+  const int64_t ldmt = (m + kGramChunk - 1) / kGramChunk * kGramChunk;
+  if (pts == nullptr) {
+    HIPCHK(ctx->dMt.ensure(size_t(n) * size_t(ldmt) * sizeof(float)), "alloc M^T");
+    HIPCHK(launch_transpose(ctx->stream, M, m, n, ctx->dMt.as<float>(), ldmt), "transpose");
+  }
+  const float *Mt = pts ? nullptr : ctx->dMt.as<float>();
+  const int64_t nnz = ctx->sptr[ctx->n];
+  const size_t w = (precision == MFS_FP64) ? 8 : 4;
+  HIPCHK(ctx->dval.ensure(size_t(std::max<int64_t>(nnz, 1)) * w), "alloc factor");
+  HIPCHK(ctx->dtval.ensure(size_t(std::max<int64_t>(nnz, 1)) * w), "alloc factor");
+  const int32_t n_super = int32_t(ctx->sdesc.size());
+  HIPCHK(hipMemsetAsync(ctx->dstatus.p, 0, size_t(n_super) * 4, ctx->stream), "memset");
+  double sum_ns = 0;
+  for (auto &d : ctx->sdesc) sum_ns += d.ns;
+  // Work per launch: bytes of the gathered columns (dense), kernel FLOPs (points).
+  ctx->bytes_per_launch[MFS_K_GRAM] = double(m) * sum_ns * (pts ? double(kEvalFlops) : 4.0);
+  track_cb(ctx, MFS_K_GRAM, 0);
+  for (const Bucket &b : ctx->buckets)
+    HIPCHK(launch_super_factor(ctx->stream, b.tiles, precision == MFS_FP64 ? 0 : 1,
+                               Mt, ldmt, pts, ctx->dsd.as<SuperDesc>(),
+                               ctx->dlist.as<int32_t>() + b.offset, b.count, ctx->dsind.as<int32_t>(),
+                               ctx->dPinv.as<int32_t>(), ctx->dsupind.as<int32_t>(),
+                               ctx->dsptr.as<int32_t>(), ctx->dval.p, ctx->dstatus.as<int32_t>()),
+           "supernode factor");
+  track_cb(ctx, MFS_K_GRAM, 1);
+  std::vector<int32_t> status(n_super);
+  HIPCHK(hipMemcpyAsync(status.data(), ctx->dstatus.p, size_t(n_super) * 4, hipMemcpyDeviceToHost,
+                        ctx->stream),
+         "copy status");
+  HIPCHK(hipStreamSynchronize(ctx->stream), "sync");
+  std::vector<int32_t> failed;
+  for (int32_t s = 0; s < n_super; ++s)
+    if (status[s] != 0) failed.push_back(s);
+  ctx->stats.retried_supernodes = 0;
+  if (!failed.empty() && precision == MFS_FP32) {
+    // FP64 retry of the supernodes whose FP32 pivots failed; values stay FP32.
+    ctx->stats.retried_supernodes = int32_t(failed.size());
+    std::vector<std::vector<int32_t>> by_t(9);
+    for (int32_t s : failed) by_t[tiles_for(ctx->sdesc[s].ns)].push_back(s);
+    std::vector<int32_t> flist;
+    std::vector<Bucket> fb;
+    for (int t : {8, 6, 4, 3, 2, 1}) {
+      if (by_t[t].empty()) continue;
+      fb.push_back({t, int32_t(flist.size()), int32_t(by_t[t].size())});
+      flist.insert(flist.end(), by_t[t].begin(), by_t[t].end());
+    }
+    DevBuf dfl;
+    HIPCHK(upload(ctx, dfl, flist), "upload retry list");
+    for (const Bucket &b : fb)
+      HIPCHK(launch_super_factor(ctx->stream, b.tiles, 2, Mt, ldmt, pts,
+                                 ctx->dsd.as<SuperDesc>(), dfl.as<int32_t>() + b.offset, b.count,
+                                 ctx->dsind.as<int32_t>(), ctx->dPinv.as<int32_t>(),
+                                 ctx->dsupind.as<int32_t>(), ctx->dsptr.as<int32_t>(), ctx->dval.p,
+                                 ctx->dstatus.as<int32_t>()),
+             "supernode factor retry");
+    HIPCHK(hipMemcpyAsync(status.data(), ctx->dstatus.p, size_t(n_super) * 4,
+                          hipMemcpyDeviceToHost, ctx->stream),
+           "copy status");
+    HIPCHK(hipStreamSynchronize(ctx->stream), "sync");
+    dfl.release();
+    failed.clear();
+    for (int32_t s = 0; s < n_super; ++s)
+      if (status[s] != 0) failed.push_back(s);
+  }
+  if (!failed.empty())
+    return ctx->fail(MFS_ERR_NOT_SPD, "factor: supernode " + std::to_string(failed[0]) +
+                                          " has a non-positive pivot (" +
+                                          std::to_string(failed.size()) + " supernodes failed)");
+  HIPCHK(launch_gather_values(ctx->stream, precision == MFS_FP64 ? 0 : 1, int32_t(nnz),
+                              ctx->dtmap.as<int32_t>(), ctx->dval.p, ctx->dtval.p),
+         "gather values");
+  ctx->prec = precision;
+  ctx->has_factor = true;
+  ctx->stats.factor_ms = phase_end(ctx);
+  track_fold(ctx);
+  return MFS_OK;
+}
+
+// Mixed-precision solve with M (pts == nullptr) or matrix-free from the points.
+int solve_mixed_impl(mfs_ctx *ctx, const float *M, const PointsOp *pts, int64_t m, int64_t n,
+                     const float *B, int64_t k, int rhs_space, const float *X0, float *X,
+                     int32_t maxit, double tol, double inner_tol) {
+  int rc = MFS_OK;
+  if (!ctx->has_factor) return ctx->fail(MFS_ERR_STATE, "no factor: factor before solving");
+  if (B == nullptr || X == nullptr || k <= 0 || rhs_space != 0)
+    return ctx->fail(MFS_ERR_ARG, "solve_mixed: boundary right-hand sides only");
+  if (!(inner_tol > 0) || !(inner_tol <= 1)) inner_tol = 0.5;
+  HIPCHK(hipSetDevice(ctx->device), "hipSetDevice");
+  phase_begin(ctx);
+  set_pass_work(ctx, pts, m, n);
+  int outer_total = 0;
+  for (int64_t col0 = 0; col0 < k; col0 += kMaxRhs) {
+    const int kval = int(std::min<int64_t>(kMaxRhs, k - col0));
+    const int K = batch_width(kval);
+    SolveArgs<double> a64{};
+    rc = workspace<double>(ctx, M, pts, m, n, K, a64);
+    if (rc != MFS_OK) return rc;
+    if (pts) set_pass_work(ctx, pts, m, n, K);
+    if (rc != MFS_OK) return rc;
+    // The FP32 workspace shares the vector buffers: lay it out after the FP64 one.
+    HIPCHK(ctx->dx64.ensure(size_t(n) * K * 8), "alloc x64");
+    HIPCHK(ctx->dr64.ensure(size_t(n) * K * 8), "alloc r64");
+    HIPCHK(ctx->dnorm.ensure(4 * kMaxRhs * 8), "alloc norms");
+    double *X64 = ctx->dx64.as<double>();
+    double *R64 = ctx->dr64.as<double>();
+    double *nb2 = ctx->dnorm.as<double>();
+    double *res2 = nb2 + kMaxRhs;
+    double *thr = nb2 + 2 * kMaxRhs;
+    const unsigned gn = unsigned((n + kThreads - 1) / kThreads);
+    k_x64_from_f32<<<gn, kThreads, 0, ctx->stream>>>(n, X0, k, col0, kval, K, X64);
+    // ||M^T b||^2 and, for a zero start, the first residual.
+    HIPCHK(residual64(ctx->stream, a64, B, k, col0, kval, nullptr, R64, nb2, track_cb, ctx), "residual");
+    if (X0 != nullptr)
+      HIPCHK(residual64(ctx->stream, a64, B, k, col0, kval, X64, R64, res2, track_cb, ctx), "residual");
+    else
+      HIPCHK(hipMemcpyAsync(res2, nb2, kMaxRhs * 8, hipMemcpyDeviceToDevice, ctx->stream), "copy");
+    int32_t inner_used = 0;
+    int outer = 0;
+    std::vector<int32_t> iters(kval, 0);
+    for (;;) {
+      HIPCHK(hipMemcpyAsync(ctx->hnorm, nb2, 2 * kMaxRhs * 8, hipMemcpyDeviceToHost, ctx->stream), "copy");
+      HIPCHK(hipStreamSynchronize(ctx->stream), "sync");
+      bool done = true;
+      for (int r = 0; r < kval; ++r) {
+        const double b2 = ctx->hnorm[r], e2 = ctx->hnorm[kMaxRhs + r];
+        const double rel = b2 > 0 ? std::sqrt(e2 / b2) : 0.0;
+        if (col0 + r < 64) ctx->stats.relres[col0 + r] = rel;
+        if (b2 >= 1e-16 && !(rel <= tol)) done = false;
+      }
+      if (done || inner_used >= maxit || outer >= 30) break;
+      // Inner FP32 PCG on M^T M d = R64 (source-space right-hand side).
+      SolveArgs<float> a32{};
+      {
+        const void *lv = nullptr, *ltv = nullptr;
+        rc = values_for(ctx, MFS_FP32, &lv, &ltv);
+        if (rc != MFS_OK) return rc;
+        a32.M = M; a32.m = m; a32.n = n; a32.K = K;
+        a32.P = a64.P; a32.Pinv = a64.Pinv; a32.sptr = a64.sptr; a32.sind = a64.sind;
+        a32.tptr = a64.tptr; a32.tcol = a64.tcol;
+        a32.lval = static_cast<const float *>(lv);
+        a32.ltval = static_cast<const float *>(ltv);
+        a32.row_blocks = a64.row_blocks;
+        a32.pts = a64.pts;
+        // FP32 vectors live in the FP64 vector buffer (same byte size suffices).
+        float *base = reinterpret_cast<float *>(a64.x);
+        const size_t nv = size_t(n) * K;
+        a32.x = base; a32.r = base + nv; a32.p = base + 2 * nv; a32.q = base + 3 * nv;
+        a32.z = base + 4 * nv; a32.t = base + 5 * nv; a32.pv = base + 6 * nv; a32.y = base + 7 * nv;
+        a32.part = a64.part; a32.dpart = a64.dpart; a32.st = a64.st;
+      }
+      // Each sweep aims straight at inner_tol * tol * ||M^T b|| (absolute), so
+      // one sweep normally meets the tolerance and the FP64 residual check
+      // only confirms it; FP32 round-off left above it is removed by the next.
+      HIPCHK(inner_thresholds(ctx->stream, nb2, inner_tol * tol, K, thr), "inner thresholds");
+      HIPCHK(pcg_init<float>(ctx->stream, a32, nullptr, 0, 0, 1, K, 0.0, maxit - inner_used, R64, thr),
+             "inner init");
+      rc = iterate<float>(ctx, a32, maxit - inner_used);
+      if (rc != MFS_OK) return rc;
+      HIPCHK(pcg_finish<float>(ctx->stream, a32, nullptr, 0, 0, K, X64), "inner finish");
+      int32_t used = 0;
+      for (int r = 0; r < kval; ++r) {
+        iters[r] += ctx->hstate->iters[r];
+        used = std::max(used, ctx->hstate->iters[r]);
+      }
+      inner_used += std::max(used, 1);
+      ++outer;
+      HIPCHK(residual64(ctx->stream, a64, B, k, col0, kval, X64, R64, res2, track_cb, ctx), "residual");
+    }
+    k_x64_to_f32<<<gn, kThreads, 0, ctx->stream>>>(n, X64, K, X, k, col0, kval);
+    HIPCHK(hipGetLastError(), "output");
+    for (int r = 0; r < kval; ++r)
+      if (col0 + r < 64) ctx->stats.iters[col0 + r] = iters[r];
+    outer_total = std::max(outer_total, outer);
+  }
+  ctx->stats.outer_iters = outer_total;
+  ctx->stats.solve_ms = phase_end(ctx);
+  track_fold(ctx);
+  return MFS_OK;
+}
+
+// FP64 normal-equation residual per column, with M or from the points.
+int residual_impl(mfs_ctx *ctx, const float *M, const PointsOp *pts, int64_t m, int64_t n,
+                  const float *B, int64_t k, const float *X, double *out) {
+  int rc = MFS_OK;
+  if (!ctx->has_factor) return ctx->fail(MFS_ERR_STATE, "no factor: factor before solving");
+  if (B == nullptr || X == nullptr || k <= 0) return ctx->fail(MFS_ERR_ARG, "residual: bad arguments");
+  HIPCHK(hipSetDevice(ctx->device), "hipSetDevice");
+  for (int64_t col0 = 0; col0 < k; col0 += kMaxRhs) {
+    const int kval = int(std::min<int64_t>(kMaxRhs, k - col0));
+    const int K = batch_width(kval);
+    SolveArgs<double> a{};
+    rc = workspace<double>(ctx, M, pts, m, n, K, a);
+    if (rc != MFS_OK) return rc;
+    HIPCHK(ctx->dx64.ensure(size_t(n) * K * 8), "alloc");
+    HIPCHK(ctx->dr64.ensure(size_t(n) * K * 8), "alloc");
+    HIPCHK(ctx->dnorm.ensure(4 * kMaxRhs * 8), "alloc");
+    double *nb2 = ctx->dnorm.as<double>();
+    const unsigned gn = unsigned((n + kThreads - 1) / kThreads);
+    k_x64_from_f32<<<gn, kThreads, 0, ctx->stream>>>(n, X, k, col0, kval, K, ctx->dx64.as<double>());
+    HIPCHK(residual64(ctx->stream, a, B, k, col0, kval, nullptr, ctx->dr64.as<double>(), nb2, nullptr, nullptr), "residual");
+    HIPCHK(residual64(ctx->stream, a, B, k, col0, kval, ctx->dx64.as<double>(), ctx->dr64.as<double>(),
+                      nb2 + kMaxRhs, nullptr, nullptr), "residual");
+    HIPCHK(hipMemcpyAsync(ctx->hnorm, nb2, 2 * kMaxRhs * 8, hipMemcpyDeviceToHost, ctx->stream), "copy");
+    HIPCHK(hipStreamSynchronize(ctx->stream), "sync");
+    for (int r = 0; r < kval; ++r) {
+      const double b2 = ctx->hnorm[r];
+      out[col0 + r] = b2 > 0 ? std::sqrt(ctx->hnorm[kMaxRhs + r] / b2) : 0.0;
+    }
+  }
+  return MFS_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -551,7 +805,8 @@ void mfs_ctx_destroy(mfs_ctx *ctx) {
                     &ctx->dpy, &ctx->dpz, &ctx->dscale, &ctx->dcnt, &ctx->dptr, &ctx->dind,
                     &ctx->dtotal, &ctx->dval, &ctx->dtval, &ctx->dMt, &ctx->dval_alt,
                     &ctx->dtval_alt, &ctx->dvec, &ctx->dpart, &ctx->ddpart, &ctx->dstate,
-                    &ctx->dx64, &ctx->dr64, &ctx->dnorm})
+                    &ctx->dx64, &ctx->dr64, &ctx->dnorm, &ctx->dt32, &ctx->dt64, &ctx->ds32,
+                    &ctx->ds64, &ctx->dapin, &ctx->dapout})
     b->release();
   if (ctx->hstate) (void)hipHostFree(ctx->hstate);
   if (ctx->hnorm) (void)hipHostFree(ctx->hnorm);
@@ -616,88 +871,6 @@ int mfs_plan_export(const mfs_ctx *ctx, int32_t *P, int32_t *Pinv, double *scale
   return MFS_OK;
 }
 
-int mfs_factor_dense(mfs_ctx *ctx, const float *M, int64_t m, int64_t n, int precision) {
-  if (ctx == nullptr) return MFS_ERR_ARG;
-  int rc = check_dense_args(ctx, M, m, n);
-  if (rc != MFS_OK) return rc;
-  if (precision != MFS_FP64 && precision != MFS_FP32)
-    return ctx->fail(MFS_ERR_ARG, "factor: precision must be MFS_FP64 or MFS_FP32");
-  HIPCHK(hipSetDevice(ctx->device), "hipSetDevice");
-  ctx->has_factor = false;
-  ctx->alt_prec = -1;
-  phase_begin(ctx);
-  const int64_t ldmt = (m + kGramChunk - 1) / kGramChunk * kGramChunk;
-  HIPCHK(ctx->dMt.ensure(size_t(n) * size_t(ldmt) * sizeof(float)), "alloc M^T");
-  HIPCHK(launch_transpose(ctx->stream, M, m, n, ctx->dMt.as<float>(), ldmt), "transpose");
-  const int64_t nnz = ctx->sptr[ctx->n];
-  const size_t w = (precision == MFS_FP64) ? 8 : 4;
-  HIPCHK(ctx->dval.ensure(size_t(std::max<int64_t>(nnz, 1)) * w), "alloc factor");
-  HIPCHK(ctx->dtval.ensure(size_t(std::max<int64_t>(nnz, 1)) * w), "alloc factor");
-  const int32_t n_super = int32_t(ctx->sdesc.size());
-  HIPCHK(hipMemsetAsync(ctx->dstatus.p, 0, size_t(n_super) * 4, ctx->stream), "memset");
-  ctx->bytes_per_launch[MFS_K_GRAM] = double(m) * 4.0 * double([&] { double t = 0; for (auto &d : ctx->sdesc) t += d.ns; return t; }());
-  track_cb(ctx, MFS_K_GRAM, 0);
-  for (const Bucket &b : ctx->buckets)
-    HIPCHK(launch_super_factor(ctx->stream, b.tiles, precision == MFS_FP64 ? 0 : 1,
-                               ctx->dMt.as<float>(), ldmt, ctx->dsd.as<SuperDesc>(),
-                               ctx->dlist.as<int32_t>() + b.offset, b.count, ctx->dsind.as<int32_t>(),
-                               ctx->dPinv.as<int32_t>(), ctx->dsupind.as<int32_t>(),
-                               ctx->dsptr.as<int32_t>(), ctx->dval.p, ctx->dstatus.as<int32_t>()),
-           "supernode factor");
-  track_cb(ctx, MFS_K_GRAM, 1);
-  std::vector<int32_t> status(n_super);
-  HIPCHK(hipMemcpyAsync(status.data(), ctx->dstatus.p, size_t(n_super) * 4, hipMemcpyDeviceToHost,
-                        ctx->stream),
-         "copy status");
-  HIPCHK(hipStreamSynchronize(ctx->stream), "sync");
-  std::vector<int32_t> failed;
-  for (int32_t s = 0; s < n_super; ++s)
-    if (status[s] != 0) failed.push_back(s);
-  ctx->stats.retried_supernodes = 0;
-  if (!failed.empty() && precision == MFS_FP32) {
-    // FP64 retry of the supernodes whose FP32 pivots failed; values stay FP32.
-    ctx->stats.retried_supernodes = int32_t(failed.size());
-    std::vector<std::vector<int32_t>> by_t(9);
-    for (int32_t s : failed) by_t[tiles_for(ctx->sdesc[s].ns)].push_back(s);
-    std::vector<int32_t> flist;
-    std::vector<Bucket> fb;
-    for (int t : {8, 6, 4, 3, 2, 1}) {
-      if (by_t[t].empty()) continue;
-      fb.push_back({t, int32_t(flist.size()), int32_t(by_t[t].size())});
-      flist.insert(flist.end(), by_t[t].begin(), by_t[t].end());
-    }
-    DevBuf dfl;
-    HIPCHK(upload(ctx, dfl, flist), "upload retry list");
-    for (const Bucket &b : fb)
-      HIPCHK(launch_super_factor(ctx->stream, b.tiles, 2, ctx->dMt.as<float>(), ldmt,
-                                 ctx->dsd.as<SuperDesc>(), dfl.as<int32_t>() + b.offset, b.count,
-                                 ctx->dsind.as<int32_t>(), ctx->dPinv.as<int32_t>(),
-                                 ctx->dsupind.as<int32_t>(), ctx->dsptr.as<int32_t>(), ctx->dval.p,
-                                 ctx->dstatus.as<int32_t>()),
-             "supernode factor retry");
-    HIPCHK(hipMemcpyAsync(status.data(), ctx->dstatus.p, size_t(n_super) * 4,
-                          hipMemcpyDeviceToHost, ctx->stream),
-           "copy status");
-    HIPCHK(hipStreamSynchronize(ctx->stream), "sync");
-    dfl.release();
-    failed.clear();
-    for (int32_t s = 0; s < n_super; ++s)
-      if (status[s] != 0) failed.push_back(s);
-  }
-  if (!failed.empty())
-    return ctx->fail(MFS_ERR_NOT_SPD, "factor: supernode " + std::to_string(failed[0]) +
-                                          " has a non-positive pivot (" +
-                                          std::to_string(failed.size()) + " supernodes failed)");
-  HIPCHK(launch_gather_values(ctx->stream, precision == MFS_FP64 ? 0 : 1, int32_t(nnz),
-                              ctx->dtmap.as<int32_t>(), ctx->dval.p, ctx->dtval.p),
-         "gather values");
-  ctx->prec = precision;
-  ctx->has_factor = true;
-  ctx->stats.factor_ms = phase_end(ctx);
-  track_fold(ctx);
-  return MFS_OK;
-}
-
 int mfs_factor_export(const mfs_ctx *ctx_c, double *val) {
   mfs_ctx *ctx = const_cast<mfs_ctx *>(ctx_c);
   if (ctx == nullptr || val == nullptr) return MFS_ERR_ARG;
@@ -723,7 +896,7 @@ int mfs_solve_dense(mfs_ctx *ctx, const float *M, int64_t m, int64_t n, const fl
     return ctx->fail(MFS_ERR_ARG, "solve: bad right-hand side arguments");
   HIPCHK(hipSetDevice(ctx->device), "hipSetDevice");
   SolveArgs<double> a{};
-  rc = workspace<double>(ctx, M, m, n, 1, a);
+  rc = workspace<double>(ctx, M, nullptr, m, n, 1, a);
   if (rc != MFS_OK) return rc;
   phase_begin(ctx);
   for (int c = 1; c < 8; ++c) ctx->bytes_per_launch[c] = double(m) * double(n) * 4.0;
@@ -740,104 +913,20 @@ int mfs_solve_dense(mfs_ctx *ctx, const float *M, int64_t m, int64_t n, const fl
   return MFS_OK;
 }
 
+int mfs_factor_dense(mfs_ctx *ctx, const float *M, int64_t m, int64_t n, int precision) {
+  if (ctx == nullptr) return MFS_ERR_ARG;
+  int rc = check_dense_args(ctx, M, m, n);
+  if (rc != MFS_OK) return rc;
+  return factor_impl(ctx, M, nullptr, m, n, precision);
+}
+
 int mfs_solve_mixed(mfs_ctx *ctx, const float *M, int64_t m, int64_t n, const float *B, int64_t k,
                     int rhs_space, const float *X0, float *X, int32_t maxit, double tol,
                     double inner_tol) {
   if (ctx == nullptr) return MFS_ERR_ARG;
   int rc = check_dense_args(ctx, M, m, n);
   if (rc != MFS_OK) return rc;
-  if (!ctx->has_factor) return ctx->fail(MFS_ERR_STATE, "no factor: call mfs_factor_dense first");
-  if (B == nullptr || X == nullptr || k <= 0 || rhs_space != 0)
-    return ctx->fail(MFS_ERR_ARG, "solve_mixed: boundary right-hand sides only");
-  if (!(inner_tol > 0) || !(inner_tol <= 1)) inner_tol = 0.5;
-  HIPCHK(hipSetDevice(ctx->device), "hipSetDevice");
-  phase_begin(ctx);
-  for (int c = 1; c < 8; ++c) ctx->bytes_per_launch[c] = double(m) * double(n) * 4.0;
-  int outer_total = 0;
-  for (int64_t col0 = 0; col0 < k; col0 += kMaxRhs) {
-    const int kval = int(std::min<int64_t>(kMaxRhs, k - col0));
-    const int K = batch_width(kval);
-    SolveArgs<double> a64{};
-    rc = workspace<double>(ctx, M, m, n, K, a64);
-    if (rc != MFS_OK) return rc;
-    // The FP32 workspace shares the vector buffers: lay it out after the FP64 one.
-    HIPCHK(ctx->dx64.ensure(size_t(n) * K * 8), "alloc x64");
-    HIPCHK(ctx->dr64.ensure(size_t(n) * K * 8), "alloc r64");
-    HIPCHK(ctx->dnorm.ensure(4 * kMaxRhs * 8), "alloc norms");
-    double *X64 = ctx->dx64.as<double>();
-    double *R64 = ctx->dr64.as<double>();
-    double *nb2 = ctx->dnorm.as<double>();
-    double *res2 = nb2 + kMaxRhs;
-    double *thr = nb2 + 2 * kMaxRhs;
-    const unsigned gn = unsigned((n + kThreads - 1) / kThreads);
-    k_x64_from_f32<<<gn, kThreads, 0, ctx->stream>>>(n, X0, k, col0, kval, K, X64);
-    // ||M^T b||^2 and, for a zero start, the first residual.
-    HIPCHK(residual64(ctx->stream, a64, B, k, col0, kval, nullptr, R64, nb2, track_cb, ctx), "residual");
-    if (X0 != nullptr)
-      HIPCHK(residual64(ctx->stream, a64, B, k, col0, kval, X64, R64, res2, track_cb, ctx), "residual");
-    else
-      HIPCHK(hipMemcpyAsync(res2, nb2, kMaxRhs * 8, hipMemcpyDeviceToDevice, ctx->stream), "copy");
-    int32_t inner_used = 0;
-    int outer = 0;
-    std::vector<int32_t> iters(kval, 0);
-    for (;;) {
-      HIPCHK(hipMemcpyAsync(ctx->hnorm, nb2, 2 * kMaxRhs * 8, hipMemcpyDeviceToHost, ctx->stream), "copy");
-      HIPCHK(hipStreamSynchronize(ctx->stream), "sync");
-      bool done = true;
-      for (int r = 0; r < kval; ++r) {
-        const double b2 = ctx->hnorm[r], e2 = ctx->hnorm[kMaxRhs + r];
-        const double rel = b2 > 0 ? std::sqrt(e2 / b2) : 0.0;
-        if (col0 + r < 64) ctx->stats.relres[col0 + r] = rel;
-        if (b2 >= 1e-16 && !(rel <= tol)) done = false;
-      }
-      if (done || inner_used >= maxit || outer >= 30) break;
-      // Inner FP32 PCG on M^T M d = R64 (source-space right-hand side).
-      SolveArgs<float> a32{};
-      {
-        const void *lv = nullptr, *ltv = nullptr;
-        rc = values_for(ctx, MFS_FP32, &lv, &ltv);
-        if (rc != MFS_OK) return rc;
-        a32.M = M; a32.m = m; a32.n = n; a32.K = K;
-        a32.P = a64.P; a32.Pinv = a64.Pinv; a32.sptr = a64.sptr; a32.sind = a64.sind;
-        a32.tptr = a64.tptr; a32.tcol = a64.tcol;
-        a32.lval = static_cast<const float *>(lv);
-        a32.ltval = static_cast<const float *>(ltv);
-        a32.row_blocks = a64.row_blocks;
-        // FP32 vectors live in the FP64 vector buffer (same byte size suffices).
-        float *base = reinterpret_cast<float *>(a64.x);
-        const size_t nv = size_t(n) * K;
-        a32.x = base; a32.r = base + nv; a32.p = base + 2 * nv; a32.q = base + 3 * nv;
-        a32.z = base + 4 * nv; a32.t = base + 5 * nv; a32.pv = base + 6 * nv; a32.y = base + 7 * nv;
-        a32.part = a64.part; a32.dpart = a64.dpart; a32.st = a64.st;
-      }
-      // Each sweep aims straight at inner_tol * tol * ||M^T b|| (absolute), so
-      // one sweep normally meets the tolerance and the FP64 residual check
-      // only confirms it; FP32 round-off left above it is removed by the next.
-      HIPCHK(inner_thresholds(ctx->stream, nb2, inner_tol * tol, K, thr), "inner thresholds");
-      HIPCHK(pcg_init<float>(ctx->stream, a32, nullptr, 0, 0, 1, K, 0.0, maxit - inner_used, R64, thr),
-             "inner init");
-      rc = iterate<float>(ctx, a32, maxit - inner_used);
-      if (rc != MFS_OK) return rc;
-      HIPCHK(pcg_finish<float>(ctx->stream, a32, nullptr, 0, 0, K, X64), "inner finish");
-      int32_t used = 0;
-      for (int r = 0; r < kval; ++r) {
-        iters[r] += ctx->hstate->iters[r];
-        used = std::max(used, ctx->hstate->iters[r]);
-      }
-      inner_used += std::max(used, 1);
-      ++outer;
-      HIPCHK(residual64(ctx->stream, a64, B, k, col0, kval, X64, R64, res2, track_cb, ctx), "residual");
-    }
-    k_x64_to_f32<<<gn, kThreads, 0, ctx->stream>>>(n, X64, K, X, k, col0, kval);
-    HIPCHK(hipGetLastError(), "output");
-    for (int r = 0; r < kval; ++r)
-      if (col0 + r < 64) ctx->stats.iters[col0 + r] = iters[r];
-    outer_total = std::max(outer_total, outer);
-  }
-  ctx->stats.outer_iters = outer_total;
-  ctx->stats.solve_ms = phase_end(ctx);
-  track_fold(ctx);
-  return MFS_OK;
+  return solve_mixed_impl(ctx, M, nullptr, m, n, B, k, rhs_space, X0, X, maxit, tol, inner_tol);
 }
 
 int mfs_normal_residual(mfs_ctx *ctx, const float *M, int64_t m, int64_t n, const float *B,
@@ -845,29 +934,97 @@ int mfs_normal_residual(mfs_ctx *ctx, const float *M, int64_t m, int64_t n, cons
   if (ctx == nullptr || out == nullptr) return MFS_ERR_ARG;
   int rc = check_dense_args(ctx, M, m, n);
   if (rc != MFS_OK) return rc;
-  if (!ctx->has_factor) return ctx->fail(MFS_ERR_STATE, "no factor: call mfs_factor_dense first");
+  return residual_impl(ctx, M, nullptr, m, n, B, k, X, out);
+}
+
+int mfs_factor_points(mfs_ctx *ctx, const double *x, const double *nx, int64_t m, const double *y,
+                      int64_t n, int precision) {
+  if (ctx == nullptr) return MFS_ERR_ARG;
+  if (!ctx->has_plan) return ctx->fail(MFS_ERR_STATE, "no plan: call mfs_plan first");
+  if (x == nullptr || nx == nullptr || y == nullptr || m <= 0 || m > INT32_MAX || n != ctx->n)
+    return ctx->fail(MFS_ERR_ARG, "points: need m > 0 targets and the n planned sources");
   HIPCHK(hipSetDevice(ctx->device), "hipSetDevice");
+  ctx->has_points = false;
+  HIPCHK(ctx->dt32.ensure(size_t(m) * kTgt32 * 4), "alloc points");
+  HIPCHK(ctx->dt64.ensure(size_t(m) * kTgt64 * 8), "alloc points");
+  HIPCHK(ctx->ds32.ensure(size_t(n) * kSrc32 * 4), "alloc points");
+  HIPCHK(ctx->ds64.ensure(size_t(n) * kSrc64 * 8), "alloc points");
+  HIPCHK(pack_points(ctx->stream, x, nx, y, m, n, ctx->dt32.as<float>(), ctx->dt64.as<double>(),
+                     ctx->ds32.as<float>(), ctx->ds64.as<double>()),
+         "pack points");
+  ctx->pts.tgt32 = ctx->dt32.as<float>();
+  ctx->pts.src32 = ctx->ds32.as<float>();
+  ctx->pts.tgt64 = ctx->dt64.as<double>();
+  ctx->pts.src64 = ctx->ds64.as<double>();
+  ctx->pts.m = m;
+  ctx->pts.n = n;
+  ctx->has_points = true;
+  return factor_impl(ctx, nullptr, &ctx->pts, m, n, precision);
+}
+
+int mfs_solve_points(mfs_ctx *ctx, const float *B, int64_t k, const float *X0, float *X,
+                     int32_t maxit, double tol, double inner_tol) {
+  if (ctx == nullptr) return MFS_ERR_ARG;
+  int rc = check_points(ctx);
+  if (rc != MFS_OK) return rc;
+  return solve_mixed_impl(ctx, nullptr, &ctx->pts, ctx->pts.m, ctx->n, B, k, 0, X0, X, maxit, tol,
+                          inner_tol);
+}
+
+int mfs_points_residual(mfs_ctx *ctx, const float *B, int64_t k, const float *X, double *out) {
+  if (ctx == nullptr || out == nullptr) return MFS_ERR_ARG;
+  int rc = check_points(ctx);
+  if (rc != MFS_OK) return rc;
+  return residual_impl(ctx, nullptr, &ctx->pts, ctx->pts.m, ctx->n, B, k, X, out);
+}
+
+int mfs_points_apply(mfs_ctx *ctx, int trans, int precision, const void *in, int64_t k,
+                     void *out) {
+  if (ctx == nullptr) return MFS_ERR_ARG;
+  int rc = check_points(ctx);
+  if (rc != MFS_OK) return rc;
+  if (in == nullptr || out == nullptr || k <= 0 || (trans != 0 && trans != 1) ||
+      (precision != MFS_FP32 && precision != MFS_FP64))
+    return ctx->fail(MFS_ERR_ARG, "points_apply: bad arguments");
+  HIPCHK(hipSetDevice(ctx->device), "hipSetDevice");
+  const PointsOp &op = ctx->pts;
+  const size_t w = precision == MFS_FP64 ? 8 : 4;
+  const int64_t rows_in = trans ? op.m : op.n, rows_out = trans ? op.n : op.m;
+  const size_t part_rows = std::max(size_t(points_splits(op.m, op.n)) * size_t(op.m),
+                                    size_t(points_splits(op.n, op.m)) * size_t(op.n));
   for (int64_t col0 = 0; col0 < k; col0 += kMaxRhs) {
     const int kval = int(std::min<int64_t>(kMaxRhs, k - col0));
     const int K = batch_width(kval);
-    SolveArgs<double> a{};
-    rc = workspace<double>(ctx, M, m, n, K, a);
-    if (rc != MFS_OK) return rc;
-    HIPCHK(ctx->dx64.ensure(size_t(n) * K * 8), "alloc");
-    HIPCHK(ctx->dr64.ensure(size_t(n) * K * 8), "alloc");
-    HIPCHK(ctx->dnorm.ensure(4 * kMaxRhs * 8), "alloc");
-    double *nb2 = ctx->dnorm.as<double>();
-    const unsigned gn = unsigned((n + kThreads - 1) / kThreads);
-    k_x64_from_f32<<<gn, kThreads, 0, ctx->stream>>>(n, X, k, col0, kval, K, ctx->dx64.as<double>());
-    HIPCHK(residual64(ctx->stream, a, B, k, col0, kval, nullptr, ctx->dr64.as<double>(), nb2, nullptr, nullptr), "residual");
-    HIPCHK(residual64(ctx->stream, a, B, k, col0, kval, ctx->dx64.as<double>(), ctx->dr64.as<double>(),
-                      nb2 + kMaxRhs, nullptr, nullptr), "residual");
-    HIPCHK(hipMemcpyAsync(ctx->hnorm, nb2, 2 * kMaxRhs * 8, hipMemcpyDeviceToHost, ctx->stream), "copy");
-    HIPCHK(hipStreamSynchronize(ctx->stream), "sync");
-    for (int r = 0; r < kval; ++r) {
-      const double b2 = ctx->hnorm[r];
-      out[col0 + r] = b2 > 0 ? std::sqrt(ctx->hnorm[kMaxRhs + r] / b2) : 0.0;
+    HIPCHK(ctx->dapin.ensure(size_t(rows_in) * K * w), "alloc");
+    HIPCHK(ctx->dapout.ensure(size_t(rows_out) * K * w), "alloc");
+    HIPCHK(ctx->dpart.ensure(part_rows * K * sizeof(double)), "alloc partials");
+    if (kval < K) HIPCHK(hipMemsetAsync(ctx->dapin.p, 0, size_t(rows_in) * K * w, ctx->stream), "memset");
+    HIPCHK(hipMemcpy2DAsync(ctx->dapin.p, K * w, static_cast<const char *>(in) + col0 * w, k * w,
+                            kval * w, rows_in, hipMemcpyDeviceToDevice, ctx->stream),
+           "pack input");
+    double *part = ctx->dpart.as<double>();
+    if (precision == MFS_FP32) {
+      const float *xi = ctx->dapin.as<float>();
+      float *yo = ctx->dapout.as<float>();
+      if (trans == 0) {
+        HIPCHK(points_rows<float>(ctx->stream, op, K, xi, nullptr, 0, 0, K, yo, part, nullptr), "rows");
+      } else {
+        HIPCHK(points_cols<float>(ctx->stream, op, K, xi, part, nullptr), "cols");
+        HIPCHK(points_cols_reduce<float>(ctx->stream, op, K, part, yo), "cols reduce");
+      }
+    } else {
+      const double *xi = ctx->dapin.as<double>();
+      double *yo = ctx->dapout.as<double>();
+      if (trans == 0) {
+        HIPCHK(points_rows<double>(ctx->stream, op, K, xi, nullptr, 0, 0, K, yo, part, nullptr), "rows");
+      } else {
+        HIPCHK(points_cols<double>(ctx->stream, op, K, xi, part, nullptr), "cols");
+        HIPCHK(points_cols_reduce<double>(ctx->stream, op, K, part, yo), "cols reduce");
+      }
     }
+    HIPCHK(hipMemcpy2DAsync(static_cast<char *>(out) + col0 * w, k * w, ctx->dapout.p, K * w,
+                            kval * w, rows_out, hipMemcpyDeviceToDevice, ctx->stream),
+           "unpack output");
   }
   return MFS_OK;
 }

@@ -41,6 +41,16 @@ struct PcgState {
 
 enum Counter { C_PQ = 0, C_RR = 1, C_RZ = 2, C_P = 3, C_R0 = 4, C_NORM = 5 };
 
+// Packed point records of the matrix-free form (layouts in kernel_eval.h).
+// tgt32 == nullptr means the dense form (M is stored).
+struct PointsOp {
+  const float *tgt32 = nullptr;   // m x 12
+  const float *src32 = nullptr;   // n x 8
+  const double *tgt64 = nullptr;  // m x 8
+  const double *src64 = nullptr;  // n x 4
+  int64_t m = 0, n = 0;
+};
+
 // ----------------------------------------------------------------- plan kernels
 hipError_t launch_pattern(hipStream_t st, int32_t n, const double *px, const double *py,
                           const double *pz, const double *scale, double rho, int32_t *cnt,
@@ -53,11 +63,13 @@ hipError_t launch_transpose(hipStream_t st, const float *M, int64_t m, int64_t n
                             int64_t ldmt);
 // precision: 0 = FP64 compute / FP64 values, 1 = FP32 compute / FP32 values,
 // 2 = FP64 compute / FP32 values (retry of FP32 failures).
+// With pts != nullptr (points form) the Gram's staged chunks are evaluated from
+// the point records instead of read from Mt (Mt and ldmt are then unused).
 hipError_t launch_super_factor(hipStream_t st, int tiles, int precision, const float *Mt,
-                               int64_t ldmt, const SuperDesc *sd, const int32_t *list,
-                               int32_t count, const int32_t *sind, const int32_t *Pinv,
-                               const int32_t *sup_ind, const int32_t *sptr, void *val,
-                               int32_t *status);
+                               int64_t ldmt, const PointsOp *pts, const SuperDesc *sd,
+                               const int32_t *list, int32_t count, const int32_t *sind,
+                               const int32_t *Pinv, const int32_t *sup_ind, const int32_t *sptr,
+                               void *val, int32_t *status);
 hipError_t launch_gather_values(hipStream_t st, int precision, int32_t nnz, const int32_t *tmap,
                                 const void *val, void *tval);
 
@@ -78,7 +90,8 @@ struct SolveArgs {
   double *part;                // partial column sums (FP64 for both precisions)
   double *dpart;               // per-block dot partials
   PcgState *st;
-  int32_t row_blocks;          // number of row blocks of the M^T pass
+  int32_t row_blocks;          // number of row blocks (splits) of the M^T pass
+  PointsOp pts;                // points form when pts.tgt32 != nullptr (M unused)
 };
 
 // thresh_abs (device, K values, may be null) replaces tol^2 ||r0||^2 as the
@@ -100,6 +113,25 @@ hipError_t pcg_finish(hipStream_t st, const SolveArgs<TA> &a, float *X, int64_t 
 hipError_t residual64(hipStream_t st, const SolveArgs<double> &a, const float *B, int64_t ldb,
                       int64_t col0, int k_valid, const double *X64, double *R, double *out2,
                       void (*track)(void *, int, int), void *track_ctx);
+
+// ---------------------------------------------------------------- points form
+// Splits of the other side so that owners x splits fills the chip.
+int32_t points_splits(int64_t owners, int64_t others);
+hipError_t pack_points(hipStream_t st, const double *x, const double *nx, const double *y,
+                       int64_t m, int64_t n, float *t32, double *t64, float *s32, double *s64);
+// Y (m x K) = M X, or B - M X when Bsub is given; part holds
+// points_splits(m, n) * m * K doubles.
+template <typename TA>
+hipError_t points_rows(hipStream_t st, const PointsOp &op, int K, const TA *X, const float *Bsub,
+                       int64_t ldb, int64_t col0, int kval, TA *Y, double *part,
+                       const PcgState *state);
+// part[points_splits(n, m)][n][K] = split sums of M^T Y.
+template <typename TA>
+hipError_t points_cols(hipStream_t st, const PointsOp &op, int K, const TA *Y, double *part,
+                       const PcgState *state);
+template <typename TA>
+hipError_t points_cols_reduce(hipStream_t st, const PointsOp &op, int K, const double *part,
+                              TA *out);
 
 // Kernel assembly of M from FP64 points.
 hipError_t launch_assemble(hipStream_t st, const double *x, const double *nx, const double *y,

@@ -112,6 +112,36 @@ int mfs_solve_mixed(mfs_ctx *ctx, const float *M, int64_t m, int64_t n,
 int mfs_normal_residual(mfs_ctx *ctx, const float *M, int64_t m, int64_t n,
                         const float *B, int64_t k, const float *X, double *out);
 
+/* ---- Matrix-free (points) form.  The reference op never receives target
+ * points and normals (it takes M), so these stand beside the drop-in as
+ * siblings; the kernel is the one src/kernels.h:24-40 evaluates (up to the
+ * factor -1/(4 pi)), applied without storing M as the point-kernel driver
+ * kl_chol.cu:55-126 evaluates it.
+ *
+ * Packs the targets x, normals nx (m x 3) and the planned sources y (n x 3,
+ * original order; all device FP64) and builds the supernodal factor with the
+ * Gram evaluated from the points (no m x n matrix).  precision as in
+ * mfs_factor_dense; MFS_FP32 retries failed supernodes in FP64.  Replaces
+ * build_cov_mat_LS_float + compute (kl_chol.cu:772-812, 867-889) for callers
+ * that hold the geometry instead of M. */
+int mfs_factor_points(mfs_ctx *ctx, const double *x, const double *nx, int64_t m,
+                      const double *y, int64_t n, int precision);
+/* mfs_solve_mixed with matrix-free passes: FP32 kernel evaluation (hi/lo
+ * split coordinates, hardware rsqrt) with cascaded FP32/FP64 sums inside
+ * PCG, FP64 evaluation for the refinement residual.  B: m x k FP32 device
+ * boundary data; X0 (may be NULL), X: n x k.  Replaces pcg_guesss
+ * (kl_chol.cu:891-1088) for the points form. */
+int mfs_solve_points(mfs_ctx *ctx, const float *B, int64_t k, const float *X0, float *X,
+                     int32_t maxit, double tol, double inner_tol);
+/* mfs_normal_residual for the points form (FP64 kernel evaluation). */
+int mfs_points_residual(mfs_ctx *ctx, const float *B, int64_t k, const float *X, double *out);
+/* out = M in (trans 0: in n x k, out m x k) or M^T in (trans 1: in m x k,
+ * out n x k), row-major device arrays of float (MFS_FP32) or double
+ * (MFS_FP64), kernel evaluated in that precision.  The matvec of
+ * kl_chol.cu:928-936 / 992-1002 without a stored M. */
+int mfs_points_apply(mfs_ctx *ctx, int trans, int precision, const void *in, int64_t k,
+                     void *out);
+
 /* Dense point-kernel matrix M[i,j] = -(x_i - y_j).n_i / (4 pi |x_i - y_j|^3),
  * evaluated in FP64 and stored FP32.  x, nx: m x 3, y: n x 3, device FP64.
  * Point form of src/kernels.h:24-40 (laplace_nm_3d_sl, up to the factor
@@ -122,7 +152,9 @@ int mfs_assemble_dense(mfs_ctx *ctx, const double *x, const double *nx, const do
 /* Kernel timing for the bench: while a class is enabled (bit (1 << class) of
  * `mask`), every launch of that class is bracketed by HIP events on the context
  * stream; mfs_ctx_tracked returns the launch count, the summed event time and
- * the algorithmic bytes (for the matvec classes: m*n*4 per launch). */
+ * the algorithmic work: bytes in the dense form (matvec classes: m*n*4 per
+ * launch; Gram: m*4 per local column), FLOPs in the points form (m*n*(17+2k)
+ * per matvec launch, 17 per kernel evaluation of the Gram). */
 enum mfs_kernel_class {
     MFS_K_NONE = 0,
     MFS_K_MATVEC_ROWS = 1,   /* PCG y = M p     (one HBM pass over M, whole batch) */

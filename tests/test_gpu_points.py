@@ -1,0 +1,198 @@
+"""GPU parity of the matrix-free (points) form, through the C-ABI.
+
+* mfs_points_apply: M v and M^T v evaluated from the packed points against the
+  FP64 kernel matrix (FP64 evaluation to 1e-12, FP32 hi/lo evaluation to 2e-6);
+* mfs_factor_points: the KL factor with the Gram evaluated from the points,
+  against the oracle's factor of the same scene;
+* mfs_solve_points: relative normal-equation residual <= tol, recomputed by the
+  oracle on the FP64 kernel matrix; warm start; error paths.
+"""
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from fastmbcui_amd import _lib, scenes
+
+pytestmark = pytest.mark.gpu
+
+DEV = torch.device("cuda", 0)
+
+
+def _small(seed=0, k=2):
+    return scenes.coins(seed=seed, n_bodies=3, pts_per_body=160, k=k, box=6.0)
+
+
+def _ellips(seed=0, k=2):
+    return scenes.ellipsoids(seed=seed, n_bodies=4, pts_per_body=256, k=k, box=8.0)
+
+
+def _dev(a):
+    return torch.from_numpy(np.ascontiguousarray(a)).to(DEV)
+
+
+def _points_ctx(sc, precision=_lib.MFS_FP32, rho=3.0):
+    ctx = _lib.Context(0)
+    ctx.plan(sc.y, rho)
+    ctx.factor_points(_dev(sc.x), _dev(sc.nx), _dev(sc.y), precision)
+    return ctx
+
+
+@pytest.mark.parametrize("trans", [False, True])
+@pytest.mark.parametrize("k", [1, 3, 20])
+def test_points_apply_fp64(trans, k):
+    sc = _ellips(k=1)
+    print(f"points apply fp64 trans={trans} k={k} n={sc.n}", flush=True)
+    ctx = _points_ctx(sc)
+    M = scenes.kernel_matrix(sc.x, sc.nx, sc.y)
+    rng = np.random.default_rng(5)
+    v = rng.standard_normal(((sc.m if trans else sc.n), k))
+    got = ctx.points_apply(_dev(v), trans=trans, precision=_lib.MFS_FP64).cpu().numpy()
+    ref = (M.T @ v) if trans else (M @ v)
+    err = np.max(np.abs(got - ref)) / np.max(np.abs(ref))
+    print(f"  max err / max |ref| = {err:.2e}", flush=True)
+    assert err < 1e-12
+
+
+@pytest.mark.parametrize("trans", [False, True])
+@pytest.mark.parametrize("maker", [_small, _ellips])
+def test_points_apply_fp32(trans, maker):
+    sc = maker(k=1)
+    print(f"points apply fp32 {sc.name} trans={trans}", flush=True)
+    ctx = _points_ctx(sc)
+    M = scenes.kernel_matrix(sc.x, sc.nx, sc.y)
+    rng = np.random.default_rng(6)
+    v = rng.standard_normal(((sc.m if trans else sc.n), 8)).astype(np.float32)
+    got = ctx.points_apply(_dev(v), trans=trans).cpu().numpy().astype(np.float64)
+    A = M.T if trans else M
+    ref = A @ v.astype(np.float64)
+    # error relative to the absolute sum of the terms of each entry
+    scale = np.abs(A) @ np.abs(v.astype(np.float64))
+    err = np.max(np.abs(got - ref) / scale)
+    print(f"  max entry err / sum|terms| = {err:.2e}", flush=True)
+    assert err < 2e-6
+
+
+def test_points_apply_near_pairs_keep_fp32_accuracy():
+    # Sources 1e-3 inside targets far from the origin: a plain FP32 difference
+    # of the coordinates (~1e3) would lose ~3 digits; the hi/lo split does not.
+    rng = np.random.default_rng(2)
+    n = 512
+    base = rng.uniform(-1.0, 1.0, (n, 3)) + 1000.0
+    nx = rng.standard_normal((n, 3))
+    nx /= np.linalg.norm(nx, axis=1, keepdims=True)
+    y = base - 1e-3 * nx
+    sc = scenes.Scene("near", base, nx, y, np.zeros((n, 1)), np.zeros(n, np.int32), np.zeros((1, 3)))
+    ctx = _points_ctx(sc)
+    M = scenes.kernel_matrix(sc.x, sc.nx, sc.y)
+    v = rng.standard_normal((n, 1)).astype(np.float32)
+    got = ctx.points_apply(_dev(v)).cpu().numpy()[:, 0].astype(np.float64)
+    ref = (M @ v.astype(np.float64))[:, 0]
+    scale = np.abs(M) @ np.abs(v[:, 0].astype(np.float64))
+    err = np.max(np.abs(got - ref) / scale)
+    print(f"near pairs: max entry err / sum|terms| = {err:.2e}", flush=True)
+    assert err < 2e-6
+
+
+def _col_rel_err(pl, a, b):
+    worst = 0.0
+    for j in range(pl.n):
+        s, e = pl.sptr[j], pl.sptr[j + 1]
+        worst = max(worst, np.linalg.norm(a[s:e] - b[s:e]) / np.linalg.norm(b[s:e]))
+    return worst
+
+
+@pytest.mark.parametrize("precision", [_lib.MFS_FP64, _lib.MFS_FP32])
+def test_factor_points_close_to_oracle(precision):
+    sc = _ellips()
+    print(f"factor points precision={precision} n={sc.n}", flush=True)
+    ctx = _points_ctx(sc, precision)
+    val = ctx.factor_export()
+    M = scenes.kernel_matrix(sc.x, sc.nx, sc.y)
+    pl = oracle.plan(sc.y, 3.0)
+    ref = oracle.factor(oracle.permuted_colmajor(M, pl.Pinv), M.shape[0], pl)
+    err = _col_rel_err(pl, val, ref)
+    print(f"  max column rel err {err:.2e}, retried {ctx.stats().retried_supernodes}", flush=True)
+    assert err < 1e-5
+
+
+def test_factor_points_matches_dense_factor_on_same_plan():
+    sc = _small()
+    ctx = _points_ctx(sc, _lib.MFS_FP32)
+    vp = ctx.factor_export()
+    Md = ctx.assemble_dense(_dev(sc.x), _dev(sc.nx), _dev(sc.y))
+    ctx.factor_dense(Md, _lib.MFS_FP32)
+    vd = ctx.factor_export()
+    pl = oracle.plan(sc.y, 3.0)
+    err = _col_rel_err(pl, vp, vd)
+    print(f"points vs dense fp32 factor: max column rel err {err:.2e}", flush=True)
+    assert err < 5e-2
+
+
+@pytest.mark.parametrize("maker,k", [(_small, 1), (_ellips, 3), (_ellips, 8)])
+def test_solve_points_reaches_tolerance(maker, k):
+    sc = maker(k=k)
+    print(f"solve points {sc.name} k={k}", flush=True)
+    ctx = _points_ctx(sc)
+    B = _dev(sc.b.astype(np.float32))
+    X = ctx.solve_points(B, maxit=500, tol=1e-6)
+    res = ctx.points_residual(B, X)
+    s = ctx.stats()
+    print(f"  residual {res.max():.2e} outer {s.outer_iters} iters {[s.iters[c] for c in range(k)]}",
+          flush=True)
+    assert res.max() <= 1e-6
+    M = scenes.kernel_matrix(sc.x, sc.nx, sc.y)
+    res_o = oracle.normal_residual(M, sc.b.astype(np.float32), X.cpu().numpy())
+    assert res_o <= 2e-6
+
+
+def test_solve_points_config1_coins():
+    sc = scenes.coins(seed=0, k=1)   # N = 4096
+    ctx = _points_ctx(sc)
+    B = _dev(sc.b.astype(np.float32))
+    X = ctx.solve_points(B, maxit=1000, tol=1e-6)
+    M = scenes.kernel_matrix(sc.x, sc.nx, sc.y)
+    res_o = oracle.normal_residual(M, sc.b.astype(np.float32), X.cpu().numpy())
+    print(f"coins N=4096 points: oracle residual {res_o:.2e} iters {ctx.stats().iters[0]}", flush=True)
+    assert res_o <= 2e-6
+
+
+def test_points_residual_matches_oracle():
+    sc = _ellips(k=3)
+    ctx = _points_ctx(sc)
+    rng = np.random.default_rng(9)
+    X = rng.standard_normal((sc.n, 3)).astype(np.float32) * 1e-2
+    B = sc.b.astype(np.float32)
+    got = ctx.points_residual(_dev(B), _dev(X))
+    M = scenes.kernel_matrix(sc.x, sc.nx, sc.y)
+    for c in range(3):
+        want = oracle.normal_residual(M, B[:, c:c + 1], X[:, c:c + 1])
+        # numerator M^T(b - M x) in FP64; the denominator ||M^T b|| is summed
+        # from FP32 kernel values (it only scales the stopping test)
+        assert abs(got[c] - want) <= 1e-6 * want
+
+
+def test_solve_points_warm_start_from_solution():
+    sc = _ellips(k=2)
+    ctx = _points_ctx(sc)
+    B = _dev(sc.b.astype(np.float32))
+    X = ctx.solve_points(B, tol=1e-6)
+    cold = sum(ctx.stats().iters[c] for c in range(2))
+    X2 = ctx.solve_points(B, tol=1e-6, x0=X)
+    s = ctx.stats()
+    warm = sum(s.iters[c] for c in range(2))
+    print(f"warm start: cold {cold} inner iterations, warm {warm}, outer {s.outer_iters}", flush=True)
+    assert s.outer_iters <= 1 and 3 * warm <= cold
+    assert ctx.points_residual(B, X2).max() <= 1e-6
+
+
+def test_points_entry_points_check_state():
+    sc = _small(k=1)
+    ctx = _lib.Context(0)
+    ctx.plan(sc.y, 3.0)
+    ctx.m_points = sc.m
+    with pytest.raises(_lib.MfsError) as e:
+        ctx.solve_points(_dev(sc.b.astype(np.float32)))
+    assert e.value.code == _lib.MFS_ERR_STATE
+    with pytest.raises(ValueError):
+        ctx.factor_points(_dev(sc.x), _dev(sc.nx), _dev(sc.y[:-1]))

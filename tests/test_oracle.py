@@ -316,3 +316,32 @@ def test_vpl_solves_normal_equations_in_source_space():
     r = oracle.vpl(M, G, sc.y, 3.0, 500, 1e-12, pl=pl, val=val)
     xd = np.linalg.solve(M.T @ M, G)
     assert np.linalg.norm(r.x - xd) / np.linalg.norm(xd) < 1e-8
+
+
+# ----------------------------------------------------------------------------- kernel
+def test_kernel_is_normal_derivative_of_laplace_fundamental_solution():
+    # M[i, j] = d/d(eps) G(x_i + eps n_i, y_j), G = 1 / (4 pi |x - y|), by central differences.
+    rng = np.random.default_rng(3)
+    x = rng.uniform(-1, 1, (6, 3))
+    nx = rng.standard_normal((6, 3))
+    nx /= np.linalg.norm(nx, axis=1, keepdims=True)
+    y = rng.uniform(-1, 1, (5, 3)) + 3.0
+    M = scenes.kernel_matrix(x, nx, y)
+    h = 1e-5
+    G = lambda p: 1.0 / (4 * np.pi * np.linalg.norm(p[:, None, :] - y[None, :, :], axis=2))
+    fd = (G(x + h * nx) - G(x - h * nx)) / (2 * h)
+    assert np.allclose(M, fd, rtol=1e-7, atol=0)
+
+
+def test_kernel_flux_through_closed_surface_is_minus_one_inside_zero_outside():
+    # Gauss: the flux of grad G(., y) through a sphere is -1 for y inside, 0 outside.
+    N = 20000
+    k = np.arange(N)
+    z = 1 - (2 * k + 1) / N
+    phi = k * np.pi * (3 - np.sqrt(5))
+    r = np.sqrt(1 - z * z)
+    x = np.stack([r * np.cos(phi), r * np.sin(phi), z], 1)
+    y = np.array([[0.1, -0.2, 0.15], [2.5, 0.3, -0.4]])
+    flux = scenes.kernel_matrix(x, x, y).sum(axis=0) * (4 * np.pi / N)
+    assert abs(flux[0] + 1) < 1e-6
+    assert abs(flux[1]) < 1e-6
