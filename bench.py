@@ -47,6 +47,8 @@ def _parse():
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--inner-tol", type=float, default=0.5)
+    ap.add_argument("--points-steps", type=int, default=1,
+                    help="timed solves of the matrix-free points form (0 skips it)")
     ap.add_argument("--copy-gb", type=float, default=0.0,
                     help="run one plain device copy of this many GB first (counter calibration)")
     return ap.parse_args()
@@ -108,7 +110,6 @@ def main():
     y = torch.from_numpy(sc.y).to(dev)
     M = ctx.assemble_dense(x, nx, y)
     B = torch.from_numpy(sc.b.astype(np.float32)).to(dev)
-    del x, nx, y
     torch.cuda.synchronize()
     _log(f"bench: M assembled ({m} x {n}, {m * n * 4 / 1e9:.1f} GB FP32)")
 
@@ -207,6 +208,49 @@ def main():
         _log(f"bench: plain {np.mean(tp):.1f} ms/step, factor {pf:.1f} ms, iters {plain['iters']}, "
              f"max residual {pres.max():.2e}")
 
+    # ---------------------------------------------------------------- points form
+    points = None
+    if args.points_steps > 0:
+        pclasses = [_lib.MFS_K_MATVEC_ROWS, _lib.MFS_K_MATVEC_COLS, _lib.MFS_K_GRAM,
+                    _lib.MFS_K_RESID_ROWS, _lib.MFS_K_RESID_COLS]
+
+        def points_step():
+            ctx.factor_points(x, nx, y, _lib.MFS_FP32)
+            f_ms = ctx.stats().factor_ms
+            return ctx.solve_points(B, maxit=1000, tol=args.tol, inner_tol=args.inner_tol), f_ms
+
+        points_step()   # warm-up: buffers
+        torch.cuda.synchronize()
+        ctx.track(pclasses)
+        tp, fp = [], []
+        for i in range(args.points_steps):
+            t2 = time.perf_counter()
+            Xq, f_ms = points_step()
+            torch.cuda.synchronize()
+            tp.append((time.perf_counter() - t2) * 1e3)
+            fp.append(f_ms)
+        sq = ctx.stats()
+        qtr = {c: ctx.tracked(c) for c in pclasses}
+        ctx.track([])
+        qres = ctx.points_residual(B, Xq)
+        qnames = {_lib.MFS_K_MATVEC_ROWS: "k_pts_rows32 (y = M p, FP32 kernel, 8 RHS)",
+                  _lib.MFS_K_MATVEC_COLS: "k_pts_cols32 (M^T y, FP32 kernel, 8 RHS)",
+                  _lib.MFS_K_GRAM: "supernode factor, Gram from points",
+                  _lib.MFS_K_RESID_ROWS: "k_pts_rows64 (b - M x, FP64 kernel)",
+                  _lib.MFS_K_RESID_COLS: "k_pts_cols64 (M^T y, FP64 kernel)"}
+        kavg = {}
+        for c in pclasses:
+            l, ms, fl = qtr[c]
+            if l:
+                kavg[qnames[c]] = {"avg_ms": round(ms / l, 4),
+                                   "tflops": round(fl / l / (ms / l * 1e-3) / 1e12, 2)}
+        points = dict(ms_per_solve=round(float(np.mean(tp)) / k, 4), factor_ms=round(float(np.mean(fp)), 3),
+                      iters=[int(sq.iters[c]) for c in range(k)], refinement_sweeps=int(sq.outer_iters),
+                      residual_max=float(qres.max()), kernels=kavg,
+                      note="FP32 VALU peak 157.3 TFLOP/s (packed); these kernels issue unpacked FP32")
+        _log(f"bench: points {np.mean(tp):.1f} ms/step, factor {np.mean(fp):.1f} ms, iters {points['iters']}, "
+             f"max residual {qres.max():.2e}, kernels {kavg}")
+
     cpu = None
     if not args.skip_cpu:
         _log("bench: CPU oracle baseline (bounded sample)")
@@ -245,6 +289,7 @@ def main():
         "gram_gb_gathered_per_step": round(g_bytes / max(args.steps, 1) / 1e9, 2),
         "plain": plain,
         "speedup_vs_plain": (round(plain["ms_per_solve"] / value, 2) if plain else None),
+        "points": points,
     }
     print(json.dumps(line), flush=True)
     return 0
