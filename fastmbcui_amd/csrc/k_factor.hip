@@ -5,11 +5,13 @@
 // each thread walks whole dot products and triangular solves serially.  Here
 // one 256-thread block owns one supernode and does all three steps in LDS:
 //
-//  1. Gram  TH = A^T A, A = the supernode's columns of M (gathered rows of M^T),
-//     on MFMA 16x16x4 tiles (FP64 or FP32), lower tile triangle only, the
-//     f-dimension streamed in 32-row chunks through LDS with the next chunk's
-//     loads in flight; stored reversed, TH[ns-1-p][ns-1-q], as the reference
-//     stores it (kl_chol.cu:216-220).
+//  1. Gram  TH = A^T A, A = the supernode's columns of M (gathered rows of M^T)
+//     or, in the points form, kernel values evaluated on the fly; on MFMA
+//     16x16x4 tiles (FP64 or FP32), lower tile triangle only.  Every lane
+//     holds exactly the operand entries it feeds to the MFMAs (loaded as
+//     float4 rows of M^T, or evaluated), the four waves split the f-range and
+//     add their accumulators at the end; stored reversed, TH[ns-1-p][ns-1-q],
+//     as the reference stores it (kl_chol.cu:216-220).
 //  2. Upper Cholesky TH = U^T U, right-looking, one barrier per column.  A
 //     non-positive pivot marks the supernode (status = 1) instead of producing
 //     NaN (the reference does not check, kl_chol.cu:33).
@@ -113,10 +115,11 @@ __global__ __launch_bounds__(kThreads) void k_transpose(const float *__restrict_
 }
 
 // ------------------------------------------------------------------ fused supernode factor
-// PTS: the points form -- each staged chunk A[f0:f0+KC, cols] is evaluated
-// from the packed point records (tgt: m x 12, srcp: n x 8) instead of read
-// from M^T; a thread owns one row f of the chunk (its target in registers)
-// and walks the local columns, whose source records sit in LDS.
+// PTS: the points form -- the Gram's operands are evaluated from the packed
+// point records (tgt: m x 12, srcp: n x 8) instead of read from M^T.  Each
+// lane evaluates exactly the MFMA operand entries it supplies, A[f][p] with
+// p = 16 a + (lane & 15) for every tile a and f = f0 + (lane >> 4), keeping
+// its T source points in registers: no LDS staging and no barriers.
 template <int T, typename TC, typename TV, bool PTS>
 __global__ __launch_bounds__(kThreads) void k_super_factor(
     const float *__restrict__ Mt, int64_t ldmt, const float *__restrict__ tgt,
@@ -125,8 +128,6 @@ __global__ __launch_bounds__(kThreads) void k_super_factor(
     const int32_t *__restrict__ Pinv, const int32_t *__restrict__ sup_ind,
     const int32_t *__restrict__ sptr, TV *__restrict__ val, int32_t *__restrict__ status) {
   constexpr int NS = 16 * T;                  // padded local size
-  constexpr int KC = (T <= 4) ? kGramChunk : kGramChunk / 2;  // f-rows per staged chunk
-  constexpr int SS = KC + 4;                  // stage row stride in floats (bank spread, 16 B aligned)
   constexpr int RS = NS + 1;                  // TH row stride
   constexpr int MB = 16;                      // member columns per batch
   constexpr int PAIRS = T * (T + 1) / 2;      // lower-triangle tile pairs
@@ -136,18 +137,13 @@ __global__ __launch_bounds__(kThreads) void k_super_factor(
   constexpr int KS = (sizeof(TC) == 4 || T <= 4) ? 4 : (T <= 6 ? 2 : 1);
   constexpr int NG = 4 / KS;                  // groups splitting the tile pairs
   constexpr int PPW = (PAIRS + NG - 1) / NG;  // tile pairs per wave
-  constexpr int LPT = (NS * (KC / 4) + kThreads - 1) / kThreads;  // float4 loads per thread
   typedef typename Mfma<TC>::v4 acc_t;
 
-  // The staging chunk (FP32) and, after the Gram, TH (TC) share one buffer.
-  constexpr int BIG_BYTES = CMax<NS * SS * 4, NS * RS * int(sizeof(TC))>::v;
-  __shared__ __align__(16) unsigned char big_raw[BIG_BYTES];
+  __shared__ __align__(16) unsigned char big_raw[NS * RS * sizeof(TC)];
   __shared__ int32_t src[NS];
   __shared__ TC dg[NS], dinv[NS];
   __shared__ TC W[NS * MB];
-  float *stage = reinterpret_cast<float *>(big_raw);
   __shared__ int32_t mcol[MB], mc[MB], moff[MB];
-  __shared__ float sp[PTS ? NS * 8 : 1];       // source records of the local columns
 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int32_t s = list[blockIdx.x];
@@ -155,13 +151,6 @@ __global__ __launch_bounds__(kThreads) void k_super_factor(
   const int ns = D.ns;
 
   for (int p = tid; p < NS; p += kThreads) src[p] = (p < ns) ? Pinv[sind[D.rows_off + p]] : -1;
-  if constexpr (PTS) {
-    __syncthreads();
-    for (int idx = tid; idx < NS * 8; idx += kThreads) {
-      const int p = idx >> 3, d = idx & 7;
-      sp[idx] = (p < ns && d < 6) ? srcp[int64_t(src[p]) * 8 + d] : 0.f;
-    }
-  }
   __syncthreads();
 
   // ---- 1. Gram on MFMA ------------------------------------------------------
@@ -184,74 +173,107 @@ __global__ __launch_bounds__(kThreads) void k_super_factor(
 #pragma unroll
   for (int u = 0; u < PPW; ++u) acc[u] = acc_t{TC(0), TC(0), TC(0), TC(0)};
 
-  const int64_t nch = PTS ? (m + KC - 1) / KC : ldmt / KC;
-  float4 buf[PTS ? 3 : LPT];
-  auto load_chunk = [&](int64_t ch) {
-    if constexpr (PTS) {
-      const int64_t f = ch * KC + tid % KC;
-      if (f < m) {
-        buf[0] = *reinterpret_cast<const float4 *>(tgt + f * 12);
-        buf[1] = *reinterpret_cast<const float4 *>(tgt + f * 12 + 4);
-        buf[2] = *reinterpret_cast<const float4 *>(tgt + f * 12 + 8);
-      } else {  // padding row: zero normal gives zero kernel values
-        buf[0] = make_float4(1e30f, 0.f, 0.f, 0.f);
-        buf[1] = make_float4(0.f, 0.f, 0.f, 0.f);
-        buf[2] = make_float4(0.f, 0.f, 0.f, 0.f);
-      }
+  auto step = [&](const TC(&fr)[T]) {
+    if (NG == 1) {
+      gram_step<T, NG, PPW, 0>(fr, acc);
+    } else if (NG == 2) {
+      if (grp == 0) gram_step<T, NG, PPW, 0>(fr, acc);
+      else gram_step<T, NG, PPW, 1 % NG>(fr, acc);
     } else {
-#pragma unroll
-      for (int u = 0; u < LPT; ++u) {
-        const int idx = tid + u * kThreads;
-        const int p = idx / (KC / 4), c4 = idx % (KC / 4);
-        buf[u] = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (p < NS && src[p] >= 0)
-          buf[u] = *reinterpret_cast<const float4 *>(Mt + int64_t(src[p]) * ldmt + ch * KC + c4 * 4);
+      switch (grp) {
+        case 0: gram_step<T, NG, PPW, 0>(fr, acc); break;
+        case 1: gram_step<T, NG, PPW, 1 % NG>(fr, acc); break;
+        case 2: gram_step<T, NG, PPW, 2 % NG>(fr, acc); break;
+        default: gram_step<T, NG, PPW, 3 % NG>(fr, acc); break;
       }
     }
   };
-  load_chunk(0);
-  for (int64_t ch = 0; ch < nch; ++ch) {
-    __syncthreads();
-    if constexpr (PTS) {
-      // K(f, y_p) for this thread's row f and columns p = pg, pg + PG, ...
-      constexpr int PG = kThreads / KC;
-      const int fl = tid % KC, pg = tid / KC;
-      const float xh0 = buf[0].x, xh1 = buf[0].y, xh2 = buf[0].z;
-      const float xl0 = buf[0].w, xl1 = buf[1].x, xl2 = buf[1].y;
-      const float n0 = buf[1].z, n1 = buf[1].w, n2 = buf[2].x;
-      for (int p = pg; p < NS; p += PG) {
-        const float *y = sp + p * 8;
-        const float v = dlp32((xh0 - y[0]) + (xl0 - y[3]), (xh1 - y[1]) + (xl1 - y[4]),
-                              (xh2 - y[2]) + (xl2 - y[5]), n0, n1, n2);
-        stage[p * SS + fl] = (p < ns) ? float(-kInv4Pi) * v : 0.f;
-      }
-    } else {
+
+  if constexpr (PTS) {
+    float yh[T][3], yl[T][3];
+    bool ok[T];
 #pragma unroll
-      for (int u = 0; u < LPT; ++u) {
-        const int idx = tid + u * kThreads;
-        const int p = idx / (KC / 4), c4 = idx % (KC / 4);
-        if (p < NS) *reinterpret_cast<float4 *>(stage + p * SS + c4 * 4) = buf[u];
+    for (int a = 0; a < T; ++a) {
+      const int p = a * 16 + (lane & 15);
+      ok[a] = p < ns;
+      const float *y = srcp + int64_t(ok[a] ? src[p] : src[0]) * 8;
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        yh[a][d] = y[d];
+        yl[a][d] = y[3 + d];
       }
     }
-    __syncthreads();
-    if (ch + 1 < nch) load_chunk(ch + 1);
-    for (int kk = kph * 4; kk < KC; kk += 4 * KS) {
+    auto load_target = [&](int64_t f, float4 (&t)[3]) {
+      if (f < m) {
+        t[0] = *reinterpret_cast<const float4 *>(tgt + f * 12);
+        t[1] = *reinterpret_cast<const float4 *>(tgt + f * 12 + 4);
+        t[2] = *reinterpret_cast<const float4 *>(tgt + f * 12 + 8);
+      } else {  // padding row: zero normal gives zero values
+        t[0] = make_float4(1e15f, 0.f, 0.f, 0.f);
+        t[1] = make_float4(0.f, 0.f, 0.f, 0.f);
+        t[2] = make_float4(0.f, 0.f, 0.f, 0.f);
+      }
+    };
+    constexpr int FSTEP = 4 * KS;  // rows per k-step over the waves sharing a pair group
+    const float scale = float(-kInv4Pi);
+    float4 cur[3], nxt[3];
+    int64_t f = kph * 4 + (lane >> 4);
+    load_target(f, cur);
+    for (int64_t f0 = kph * 4; f0 < m; f0 += FSTEP, f += FSTEP) {
+      load_target(f + FSTEP, nxt);
+      const float sn0 = scale * cur[1].z, sn1 = scale * cur[1].w, sn2 = scale * cur[2].x;
       TC fr[T];
 #pragma unroll
-      for (int a = 0; a < T; ++a) fr[a] = TC(stage[(a * 16 + (lane & 15)) * SS + kk + (lane >> 4)]);
-      if (NG == 1) {
-        gram_step<T, NG, PPW, 0>(fr, acc);
-      } else if (NG == 2) {
-        if (grp == 0) gram_step<T, NG, PPW, 0>(fr, acc);
-        else gram_step<T, NG, PPW, 1 % NG>(fr, acc);
-      } else {
-        switch (grp) {
-          case 0: gram_step<T, NG, PPW, 0>(fr, acc); break;
-          case 1: gram_step<T, NG, PPW, 1 % NG>(fr, acc); break;
-          case 2: gram_step<T, NG, PPW, 2 % NG>(fr, acc); break;
-          default: gram_step<T, NG, PPW, 3 % NG>(fr, acc); break;
-        }
+      for (int a = 0; a < T; ++a) {
+        const float v = dlp32((cur[0].x - yh[a][0]) + (cur[0].w - yl[a][0]),
+                              (cur[0].y - yh[a][1]) + (cur[1].x - yl[a][1]),
+                              (cur[0].z - yh[a][2]) + (cur[1].y - yl[a][2]), sn0, sn1, sn2);
+        fr[a] = ok[a] ? TC(v) : TC(0);
       }
+      step(fr);
+#pragma unroll
+      for (int d = 0; d < 3; ++d) cur[d] = nxt[d];
+    }
+  } else {
+    // Lane l loads, for every tile a, the float4 of rows f0 + 4 (l >> 4) .. +3
+    // of column p = 16 a + (l & 15) of A (a row of M^T); MFMA i of the group
+    // takes component i.  Slot (l >> 4) of MFMA i then holds row
+    // f0 + 4 (l >> 4) + i in both operands, so the four MFMAs sum rows
+    // f0 .. f0 + 15 exactly once.  Straight from global memory into registers,
+    // with the next group's loads in flight: no LDS staging, no barriers.
+    const float *col[T];
+    bool ok[T];
+#pragma unroll
+    for (int a = 0; a < T; ++a) {
+      const int p = a * 16 + (lane & 15);
+      ok[a] = p < ns;
+      col[a] = Mt + int64_t(ok[a] ? src[p] : src[0]) * ldmt + 4 * (lane >> 4);
+    }
+    constexpr int FSTEP = 16 * KS;  // ldmt is a multiple of 128, hence of FSTEP
+    auto load = [&](int64_t f0, float4 (&v)[T]) {
+#pragma unroll
+      for (int a = 0; a < T; ++a)
+        v[a] = ok[a] ? *reinterpret_cast<const float4 *>(col[a] + f0) : make_float4(0.f, 0.f, 0.f, 0.f);
+    };
+    float4 cur[T], nxt[T];
+    load(kph * 16, cur);
+    for (int64_t f0 = kph * 16; f0 < ldmt; f0 += FSTEP) {
+      if (f0 + FSTEP < ldmt) load(f0 + FSTEP, nxt);
+      TC fr[T];
+#pragma unroll
+      for (int a = 0; a < T; ++a) fr[a] = TC(cur[a].x);
+      step(fr);
+#pragma unroll
+      for (int a = 0; a < T; ++a) fr[a] = TC(cur[a].y);
+      step(fr);
+#pragma unroll
+      for (int a = 0; a < T; ++a) fr[a] = TC(cur[a].z);
+      step(fr);
+#pragma unroll
+      for (int a = 0; a < T; ++a) fr[a] = TC(cur[a].w);
+      step(fr);
+#pragma unroll
+      for (int a = 0; a < T; ++a) cur[a] = nxt[a];
     }
   }
   // Accumulators -> TH (reversed order, both triangles); k-split phases add in turn.

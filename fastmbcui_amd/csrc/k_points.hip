@@ -17,7 +17,8 @@
 //
 // Sums run in FP32 over 64-term chunks and in FP64 across chunks (cascaded
 // summation: the FP32 part never adds more than 64 terms).  The FP64 passes,
-// used for the certified residual of the refinement, evaluate and sum in FP64.
+// used for the certified residual of the refinement, evaluate and sum in FP64
+// (division-free evaluation, kernel_eval.h dlp64).
 #include "kernel_eval.h"
 #include "mfs_internal.h"
 
@@ -38,7 +39,7 @@ __global__ __launch_bounds__(kThreads) void k_pts_rows32(
   const int64_t f = int64_t(blockIdx.x) * kThreads + threadIdx.x;
   const int64_t j0 = int64_t(blockIdx.y) * per_split;
   const int64_t j1 = (j0 + per_split < n) ? j0 + per_split : n;
-  float xh0 = 1e30f, xh1 = 0.f, xh2 = 0.f, xl0 = 0.f, xl1 = 0.f, xl2 = 0.f;
+  float xh0 = 1e15f, xh1 = 0.f, xh2 = 0.f, xl0 = 0.f, xl1 = 0.f, xl2 = 0.f;
   float n0 = 0.f, n1 = 0.f, n2 = 0.f;  // padding lanes: far away, zero normal
   if (f < m) {
     const float4 a = *reinterpret_cast<const float4 *>(tgt + f * kTgt32);
@@ -55,13 +56,19 @@ __global__ __launch_bounds__(kThreads) void k_pts_rows32(
     float c[K];
 #pragma unroll
     for (int r = 0; r < K; ++r) c[r] = 0.f;
-    for (int64_t j = jc; j < je; ++j) {
+    auto term = [&](int64_t j) {
       const float *s = src + j * kSrc32;
       const float kv = dlp32((xh0 - s[0]) + (xl0 - s[3]), (xh1 - s[1]) + (xl1 - s[4]),
                              (xh2 - s[2]) + (xl2 - s[5]), n0, n1, n2);
       const float *pj = pv + j * K;
 #pragma unroll
       for (int r = 0; r < K; ++r) c[r] += kv * pj[r];
+    };
+    if (je - jc == kChunkTerms) {
+#pragma unroll 4
+      for (int t = 0; t < kChunkTerms; ++t) term(jc + t);
+    } else {
+      for (int64_t j = jc; j < je; ++j) term(j);
     }
 #pragma unroll
     for (int r = 0; r < K; ++r) acc[r] += double(c[r]);
@@ -83,7 +90,7 @@ __global__ __launch_bounds__(kThreads) void k_pts_cols32(
   const int64_t j = int64_t(blockIdx.x) * kThreads + threadIdx.x;
   const int64_t f0 = int64_t(blockIdx.y) * per_split;
   const int64_t f1 = (f0 + per_split < m) ? f0 + per_split : m;
-  float yh0 = 1e30f, yh1 = 0.f, yh2 = 0.f, yl0 = 0.f, yl1 = 0.f, yl2 = 0.f;
+  float yh0 = 1e15f, yh1 = 0.f, yh2 = 0.f, yl0 = 0.f, yl1 = 0.f, yl2 = 0.f;
   if (j < n) {
     const float4 a = *reinterpret_cast<const float4 *>(src + j * kSrc32);
     const float4 b = *reinterpret_cast<const float4 *>(src + j * kSrc32 + 4);
@@ -97,13 +104,19 @@ __global__ __launch_bounds__(kThreads) void k_pts_cols32(
     float c[K];
 #pragma unroll
     for (int r = 0; r < K; ++r) c[r] = 0.f;
-    for (int64_t f = fc; f < fe; ++f) {
+    auto term = [&](int64_t f) {
       const float *t = tgt + f * kTgt32;
       const float kv = dlp32((t[0] - yh0) + (t[3] - yl0), (t[1] - yh1) + (t[4] - yl1),
                              (t[2] - yh2) + (t[5] - yl2), t[6], t[7], t[8]);
       const float *wf = w + f * K;
 #pragma unroll
       for (int r = 0; r < K; ++r) c[r] += kv * wf[r];
+    };
+    if (fe - fc == kChunkTerms) {
+#pragma unroll 4
+      for (int t = 0; t < kChunkTerms; ++t) term(fc + t);
+    } else {
+      for (int64_t f = fc; f < fe; ++f) term(f);
     }
 #pragma unroll
     for (int r = 0; r < K; ++r) acc[r] += double(c[r]);
@@ -123,7 +136,7 @@ __global__ __launch_bounds__(kThreads) void k_pts_rows64(
   const int64_t f = int64_t(blockIdx.x) * kThreads + threadIdx.x;
   const int64_t j0 = int64_t(blockIdx.y) * per_split;
   const int64_t j1 = (j0 + per_split < n) ? j0 + per_split : n;
-  double x0 = 1e300, x1 = 0, x2 = 0, n0 = 0, n1 = 0, n2 = 0;
+  double x0 = 1e100, x1 = 0, x2 = 0, n0 = 0, n1 = 0, n2 = 0;
   if (f < m) {
     const double *t = tgt + f * kTgt64;
     x0 = t[0]; x1 = t[1]; x2 = t[2]; n0 = t[3]; n1 = t[4]; n2 = t[5];
@@ -131,16 +144,17 @@ __global__ __launch_bounds__(kThreads) void k_pts_rows64(
   double acc[K];
 #pragma unroll
   for (int r = 0; r < K; ++r) acc[r] = 0.0;
+#pragma unroll 2
   for (int64_t j = j0; j < j1; ++j) {
     const double *s = src + j * kSrc64;
-    const double kv = kernel64(x0 - s[0], x1 - s[1], x2 - s[2], n0, n1, n2);
+    const double kv = dlp64(x0 - s[0], x1 - s[1], x2 - s[2], n0, n1, n2);
 #pragma unroll
     for (int r = 0; r < K; ++r) acc[r] += kv * X[j * K + r];
   }
   if (f < m) {
     double *dst = part + (int64_t(blockIdx.y) * m + f) * K;
 #pragma unroll
-    for (int r = 0; r < K; ++r) dst[r] = acc[r];
+    for (int r = 0; r < K; ++r) dst[r] = -kInv4Pi * acc[r];
   }
 }
 
@@ -152,7 +166,7 @@ __global__ __launch_bounds__(kThreads) void k_pts_cols64(
   const int64_t j = int64_t(blockIdx.x) * kThreads + threadIdx.x;
   const int64_t f0 = int64_t(blockIdx.y) * per_split;
   const int64_t f1 = (f0 + per_split < m) ? f0 + per_split : m;
-  double y0 = 1e300, y1 = 0, y2 = 0;
+  double y0 = 1e100, y1 = 0, y2 = 0;
   if (j < n) {
     const double *s = src + j * kSrc64;
     y0 = s[0]; y1 = s[1]; y2 = s[2];
@@ -160,16 +174,17 @@ __global__ __launch_bounds__(kThreads) void k_pts_cols64(
   double acc[K];
 #pragma unroll
   for (int r = 0; r < K; ++r) acc[r] = 0.0;
+#pragma unroll 2
   for (int64_t f = f0; f < f1; ++f) {
     const double *t = tgt + f * kTgt64;
-    const double kv = kernel64(t[0] - y0, t[1] - y1, t[2] - y2, t[3], t[4], t[5]);
+    const double kv = dlp64(t[0] - y0, t[1] - y1, t[2] - y2, t[3], t[4], t[5]);
 #pragma unroll
     for (int r = 0; r < K; ++r) acc[r] += kv * Y[f * K + r];
   }
   if (j < n) {
     double *dst = part + (int64_t(blockIdx.y) * n + j) * K;
 #pragma unroll
-    for (int r = 0; r < K; ++r) dst[r] = acc[r];
+    for (int r = 0; r < K; ++r) dst[r] = -kInv4Pi * acc[r];
   }
 }
 

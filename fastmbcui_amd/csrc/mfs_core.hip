@@ -96,6 +96,11 @@ struct mfs_ctx {
   // statistics and timing
   mfs_stats stats{};
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  // Side streams (default priority) that run the factor's size buckets side by
+  // side, forked from and joined back into `stream` with events.
+  static constexpr int kSide = 4;
+  hipStream_t side[kSide] = {};
+  hipEvent_t fork_ev = nullptr, join_ev[kSide] = {};
   int track_mask = 0;
   struct Track {
     std::vector<std::pair<hipEvent_t, hipEvent_t>> pairs;
@@ -532,6 +537,16 @@ int check_dense_args(mfs_ctx *ctx, const float *M, int64_t m, int64_t n) {
 
 constexpr int kEvalFlops = 17;  // one kernel evaluation, rsqrt counted as 1
 
+int ensure_side_streams(mfs_ctx *ctx) {
+  if (ctx->fork_ev != nullptr) return MFS_OK;
+  for (int i = 0; i < mfs_ctx::kSide; ++i) {
+    HIPCHK(hipStreamCreateWithFlags(&ctx->side[i], hipStreamNonBlocking), "create stream");
+    HIPCHK(hipEventCreateWithFlags(&ctx->join_ev[i], hipEventDisableTiming), "create event");
+  }
+  HIPCHK(hipEventCreateWithFlags(&ctx->fork_ev, hipEventDisableTiming), "create event");
+  return MFS_OK;
+}
+
 int check_points(mfs_ctx *ctx) {
   if (!ctx->has_plan) return ctx->fail(MFS_ERR_STATE, "no plan: call mfs_plan first");
   if (!ctx->has_points)
@@ -578,13 +593,27 @@ int factor_impl(mfs_ctx *ctx, const float *M, const PointsOp *pts, int64_t m, in
   // Work per launch: bytes of the gathered columns (dense), kernel FLOPs (points).
   ctx->bytes_per_launch[MFS_K_GRAM] = double(m) * sum_ns * (pts ? double(kEvalFlops) : 4.0);
   track_cb(ctx, MFS_K_GRAM, 0);
-  for (const Bucket &b : ctx->buckets)
-    HIPCHK(launch_super_factor(ctx->stream, b.tiles, precision == MFS_FP64 ? 0 : 1,
+  // Buckets (largest tiles first) run on side streams at once, so that the
+  // few large supernodes and each launch's tail overlap the bulk of the
+  // smaller ones instead of leaving CUs idle between launches.
+  int rc = ensure_side_streams(ctx);
+  if (rc != MFS_OK) return rc;
+  HIPCHK(hipEventRecord(ctx->fork_ev, ctx->stream), "fork");
+  const int nside = std::min<int>(mfs_ctx::kSide, int(ctx->buckets.size()));
+  for (int i = 0; i < nside; ++i) HIPCHK(hipStreamWaitEvent(ctx->side[i], ctx->fork_ev, 0), "fork");
+  for (size_t bi = 0; bi < ctx->buckets.size(); ++bi) {
+    const Bucket &b = ctx->buckets[bi];
+    HIPCHK(launch_super_factor(ctx->side[bi % mfs_ctx::kSide], b.tiles, precision == MFS_FP64 ? 0 : 1,
                                Mt, ldmt, pts, ctx->dsd.as<SuperDesc>(),
                                ctx->dlist.as<int32_t>() + b.offset, b.count, ctx->dsind.as<int32_t>(),
                                ctx->dPinv.as<int32_t>(), ctx->dsupind.as<int32_t>(),
                                ctx->dsptr.as<int32_t>(), ctx->dval.p, ctx->dstatus.as<int32_t>()),
            "supernode factor");
+  }
+  for (int i = 0; i < nside; ++i) {
+    HIPCHK(hipEventRecord(ctx->join_ev[i], ctx->side[i]), "join");
+    HIPCHK(hipStreamWaitEvent(ctx->stream, ctx->join_ev[i], 0), "join");
+  }
   track_cb(ctx, MFS_K_GRAM, 1);
   std::vector<int32_t> status(n_super);
   HIPCHK(hipMemcpyAsync(status.data(), ctx->dstatus.p, size_t(n_super) * 4, hipMemcpyDeviceToHost,
@@ -812,6 +841,11 @@ void mfs_ctx_destroy(mfs_ctx *ctx) {
   if (ctx->hnorm) (void)hipHostFree(ctx->hnorm);
   if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
   if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+  for (int i = 0; i < mfs_ctx::kSide; ++i) {
+    if (ctx->side[i]) (void)hipStreamDestroy(ctx->side[i]);
+    if (ctx->join_ev[i]) (void)hipEventDestroy(ctx->join_ev[i]);
+  }
+  if (ctx->fork_ev) (void)hipEventDestroy(ctx->fork_ev);
   for (auto &t : ctx->tr)
     for (auto &pr : t.pairs) {
       (void)hipEventDestroy(pr.first);

@@ -9,7 +9,7 @@ namespace mfs {
 
 constexpr int kThreads = 256;     // every kernel here runs 4 wave64 per block
 constexpr int kMaxRhs = 16;       // right-hand sides per PCG batch
-constexpr int kGramChunk = 128;   // f-rows of M^T staged per Gram step
+constexpr int kGramChunk = 128;   // M^T rows are padded with zeros to a multiple of this
 constexpr int kMaxLocal = 128;    // largest supernode handled in LDS
 
 // Per-supernode descriptor (built on the host from the plan).
