@@ -91,7 +91,7 @@ def main():
     args = _parse()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              "(the path would shard target rows across GPUs, see DESIGN.md)", file=sys.stderr)
+              "(the path would shard target rows across GPUs, one all-reduce per pass)", file=sys.stderr)
         return 2
 
     import numpy as np

@@ -40,6 +40,8 @@ class Stats(ctypes.Structure):
         ("plan_ms", ctypes.c_double), ("factor_ms", ctypes.c_double), ("solve_ms", ctypes.c_double),
         ("iters", ctypes.c_int32 * 64), ("relres", ctypes.c_double * 64),
         ("outer_iters", ctypes.c_int32), ("retried_supernodes", ctypes.c_int32),
+        ("plan_order_ms", ctypes.c_double), ("plan_pattern_ms", ctypes.c_double),
+        ("plan_grid_columns", ctypes.c_int32), ("pad1", ctypes.c_int32),
     ]
 
 

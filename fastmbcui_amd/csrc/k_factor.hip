@@ -63,14 +63,27 @@ __host__ __device__ constexpr int pair_a(int pi) {
 }
 __host__ __device__ constexpr int pair_b(int pi) { return pi - pair_a(pi) * (pair_a(pi) + 1) / 2; }
 
-// One k-step of the Gram for the tile pairs GRP, GRP+NG, ... (all compile-time,
-// so each MFMA reads its operands from fixed registers).
-template <int T, int NG, int PPW, int GRP, typename TC, typename acc_t>
+template <int I>
+struct IntC {
+  static constexpr int value = I;
+};
+// f(IntC<0>{}), ..., f(IntC<N-1>{}) in order.
+template <int N, int I = 0, typename F>
+__device__ inline void static_for(F &&f) {
+  if constexpr (I < N) {
+    f(IntC<I>{});
+    static_for<N, I + 1>(f);
+  }
+}
+
+// One k-step of the Gram for the tile pairs GRP + NG (U0 + u) (all
+// compile-time, so each MFMA reads its operands from fixed registers).
+template <int T, int NG, int PPW, int GRP, int U0, typename TC, typename acc_t>
 __device__ inline void gram_step(const TC (&fr)[T], acc_t (&acc)[PPW]) {
 #pragma unroll
   for (int u = 0; u < PPW; ++u) {
     constexpr int kPairs = T * (T + 1) / 2;
-    const int pi = GRP + NG * u;
+    const int pi = GRP + NG * (U0 + u);
     if (pi < kPairs) acc[u] = Mfma<TC>::run(fr[pair_a(pi)], fr[pair_b(pi)], acc[u]);
   }
 }
@@ -126,20 +139,30 @@ __global__ __launch_bounds__(kThreads) void k_super_factor(
     const float *__restrict__ srcp, int64_t m, const SuperDesc *__restrict__ sd,
     const int32_t *__restrict__ list, const int32_t *__restrict__ sind,
     const int32_t *__restrict__ Pinv, const int32_t *__restrict__ sup_ind,
-    const int32_t *__restrict__ sptr, TV *__restrict__ val, int32_t *__restrict__ status) {
+    const int32_t *__restrict__ sptr, TV *__restrict__ val, int32_t *__restrict__ status,
+    TC *__restrict__ th_scratch) {
   constexpr int NS = 16 * T;                  // padded local size
+  // Supernodes of more than 128 local rows (T = 12, 16) keep TH in global
+  // scratch (NS x RS per block; it stays in L2) since it would not fit in LDS.
+  constexpr bool LARGE = T > 8;
   constexpr int RS = NS + 1;                  // TH row stride
   constexpr int MB = 16;                      // member columns per batch
   constexpr int PAIRS = T * (T + 1) / 2;      // lower-triangle tile pairs
-  // Waves split the k-steps (KS) and/or the tile pairs (NG groups); FP32 keeps
-  // every pair in every wave, FP64's larger tiles split the pairs to bound
+  // Waves split the rows (KS) and/or the tile pairs (NG groups); the larger
+  // the tiles (and FP64 accumulators), the more the pairs are split, to bound
   // the accumulator registers.
-  constexpr int KS = (sizeof(TC) == 4 || T <= 4) ? 4 : (T <= 6 ? 2 : 1);
+  constexpr int KS = LARGE ? 1
+                     : T <= 4 ? 4
+                     : sizeof(TC) == 4 ? (T <= 6 ? 4 : 2)
+                     : (T <= 6 ? 2 : 1);
   constexpr int NG = 4 / KS;                  // groups splitting the tile pairs
-  constexpr int PPW = (PAIRS + NG - 1) / NG;  // tile pairs per wave
+  // The large tiles also split the pairs over sequential passes over the rows
+  // (operands re-read or re-evaluated) to bound the accumulator registers.
+  constexpr int NPASS = LARGE ? (sizeof(TC) == 8 ? 3 : 2) : 1;
+  constexpr int PPW = (PAIRS + NG * NPASS - 1) / (NG * NPASS);  // tile pairs per wave and pass
   typedef typename Mfma<TC>::v4 acc_t;
 
-  __shared__ __align__(16) unsigned char big_raw[NS * RS * sizeof(TC)];
+  __shared__ __align__(16) unsigned char big_raw[LARGE ? 16 : NS * RS * sizeof(TC)];
   __shared__ int32_t src[NS];
   __shared__ TC dg[NS], dinv[NS];
   __shared__ TC W[NS * MB];
@@ -155,10 +178,14 @@ __global__ __launch_bounds__(kThreads) void k_super_factor(
 
   // ---- 1. Gram on MFMA ------------------------------------------------------
   const int grp = wave % NG, kph = wave / NG;
+  TC *TH = LARGE ? th_scratch + size_t(blockIdx.x) * NS * RS : reinterpret_cast<TC *>(big_raw);
+  // Pass P accumulates the pairs grp + NG (P PPW + u), u < PPW, over all rows.
+  auto gram_pass = [&](auto pass_c) {
+  constexpr int U0 = decltype(pass_c)::value * PPW;
   int pa[PPW], pb[PPW];
 #pragma unroll
   for (int u = 0; u < PPW; ++u) {
-    int pi = grp + NG * u, a = 0;
+    int pi = grp + NG * (U0 + u), a = 0;
     if (pi >= PAIRS) pi = -1;
     if (pi >= 0) {
       while ((a + 1) * (a + 2) / 2 <= pi) ++a;
@@ -175,16 +202,16 @@ __global__ __launch_bounds__(kThreads) void k_super_factor(
 
   auto step = [&](const TC(&fr)[T]) {
     if (NG == 1) {
-      gram_step<T, NG, PPW, 0>(fr, acc);
+      gram_step<T, NG, PPW, 0, U0>(fr, acc);
     } else if (NG == 2) {
-      if (grp == 0) gram_step<T, NG, PPW, 0>(fr, acc);
-      else gram_step<T, NG, PPW, 1 % NG>(fr, acc);
+      if (grp == 0) gram_step<T, NG, PPW, 0, U0>(fr, acc);
+      else gram_step<T, NG, PPW, 1 % NG, U0>(fr, acc);
     } else {
       switch (grp) {
-        case 0: gram_step<T, NG, PPW, 0>(fr, acc); break;
-        case 1: gram_step<T, NG, PPW, 1 % NG>(fr, acc); break;
-        case 2: gram_step<T, NG, PPW, 2 % NG>(fr, acc); break;
-        default: gram_step<T, NG, PPW, 3 % NG>(fr, acc); break;
+        case 0: gram_step<T, NG, PPW, 0, U0>(fr, acc); break;
+        case 1: gram_step<T, NG, PPW, 1 % NG, U0>(fr, acc); break;
+        case 2: gram_step<T, NG, PPW, 2 % NG, U0>(fr, acc); break;
+        default: gram_step<T, NG, PPW, 3 % NG, U0>(fr, acc); break;
       }
     }
   };
@@ -255,10 +282,15 @@ __global__ __launch_bounds__(kThreads) void k_super_factor(
       for (int a = 0; a < T; ++a)
         v[a] = ok[a] ? *reinterpret_cast<const float4 *>(col[a] + f0) : make_float4(0.f, 0.f, 0.f, 0.f);
     };
-    float4 cur[T], nxt[T];
-    load(kph * 16, cur);
+    constexpr bool PREFETCH = !LARGE;  // the large tiles need the registers
+    float4 cur[T], nxt[PREFETCH ? T : 1];
+    if (PREFETCH) load(kph * 16, cur);
     for (int64_t f0 = kph * 16; f0 < ldmt; f0 += FSTEP) {
-      if (f0 + FSTEP < ldmt) load(f0 + FSTEP, nxt);
+      if constexpr (PREFETCH) {
+        if (f0 + FSTEP < ldmt) load(f0 + FSTEP, nxt);
+      } else {
+        load(f0, cur);
+      }
       TC fr[T];
 #pragma unroll
       for (int a = 0; a < T; ++a) fr[a] = TC(cur[a].x);
@@ -272,12 +304,13 @@ __global__ __launch_bounds__(kThreads) void k_super_factor(
 #pragma unroll
       for (int a = 0; a < T; ++a) fr[a] = TC(cur[a].w);
       step(fr);
+      if constexpr (PREFETCH) {
 #pragma unroll
-      for (int a = 0; a < T; ++a) cur[a] = nxt[a];
+        for (int a = 0; a < T; ++a) cur[a] = nxt[a];
+      }
     }
   }
   // Accumulators -> TH (reversed order, both triangles); k-split phases add in turn.
-  TC *TH = reinterpret_cast<TC *>(big_raw);
 #pragma unroll
   for (int ph = 0; ph < KS; ++ph) {
     __syncthreads();
@@ -305,6 +338,8 @@ __global__ __launch_bounds__(kThreads) void k_super_factor(
     }
   }
   __syncthreads();
+  };
+  static_for<NPASS>(gram_pass);
 
   // ---- 2. Upper Cholesky TH = U^T U ----------------------------------------
   for (int j = 0; j < ns; ++j) {
@@ -395,6 +430,7 @@ struct FactorLaunch {
   const int32_t *sind, *Pinv, *sup_ind, *sptr;
   void *val;
   int32_t *status;
+  void *th_scratch;
 };
 
 template <int T, typename TC, typename TV>
@@ -403,11 +439,11 @@ hipError_t launch_t(hipStream_t st, const FactorLaunch &L) {
   if (L.pts != nullptr)
     k_super_factor<T, TC, TV, true><<<unsigned(L.count), kThreads, 0, st>>>(
         nullptr, 0, L.pts->tgt32, L.pts->src32, L.pts->m, L.sd, L.list, L.sind, L.Pinv,
-        L.sup_ind, L.sptr, static_cast<TV *>(L.val), L.status);
+        L.sup_ind, L.sptr, static_cast<TV *>(L.val), L.status, static_cast<TC *>(L.th_scratch));
   else
     k_super_factor<T, TC, TV, false><<<unsigned(L.count), kThreads, 0, st>>>(
         L.Mt, L.ldmt, nullptr, nullptr, 0, L.sd, L.list, L.sind, L.Pinv, L.sup_ind, L.sptr,
-        static_cast<TV *>(L.val), L.status);
+        static_cast<TV *>(L.val), L.status, static_cast<TC *>(L.th_scratch));
   return hipGetLastError();
 }
 
@@ -420,6 +456,8 @@ hipError_t launch_prec(hipStream_t st, int tiles, const FactorLaunch &L) {
     case 4: return launch_t<4, TC, TV>(st, L);
     case 6: return launch_t<6, TC, TV>(st, L);
     case 8: return launch_t<8, TC, TV>(st, L);
+    case 12: return launch_t<12, TC, TV>(st, L);
+    case 16: return launch_t<16, TC, TV>(st, L);
     default: return hipErrorInvalidValue;
   }
 }
@@ -440,8 +478,9 @@ hipError_t launch_super_factor(hipStream_t st, int tiles, int precision, const f
                                int64_t ldmt, const PointsOp *pts, const SuperDesc *sd,
                                const int32_t *list, int32_t count, const int32_t *sind,
                                const int32_t *Pinv, const int32_t *sup_ind, const int32_t *sptr,
-                               void *val, int32_t *status) {
-  const FactorLaunch L{Mt, ldmt, pts, sd, list, count, sind, Pinv, sup_ind, sptr, val, status};
+                               void *val, int32_t *status, void *th_scratch) {
+  const FactorLaunch L{Mt, ldmt, pts, sd, list, count, sind, Pinv, sup_ind, sptr, val, status,
+                       th_scratch};
   if (precision == 0) return launch_prec<double, double>(st, tiles, L);
   if (precision == 1) return launch_prec<float, float>(st, tiles, L);
   return launch_prec<double, float>(st, tiles, L);

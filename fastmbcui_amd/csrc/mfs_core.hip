@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -75,12 +76,12 @@ struct mfs_ctx {
 
   // plan (device)
   DevBuf dP, dPinv, dsptr, dsind, dtptr, dtcol, dtmap, dsupind, dsd, dlist, dstatus;
-  DevBuf dpx, dpy, dpz, dscale, dcnt, dptr, dind, dtotal;
+  DevBuf dpx, dpy, dpz, dscale, dcnt, dptr, dind, dtotal, dbands, dcells, dcpts, dlong;
 
   // factor
   bool has_factor = false;
   int prec = MFS_FP64;
-  DevBuf dval, dtval, dMt, dval_alt, dtval_alt;
+  DevBuf dval, dtval, dMt, dval_alt, dtval_alt, dthg;
   int alt_prec = -1;
 
   // points form: packed target/source records (kernel_eval.h layouts)
@@ -142,8 +143,12 @@ int tiles_for(int ns) {
   if (t <= 4) return std::max(t, 1);
   if (t <= 6) return 6;
   if (t <= 8) return 8;
+  if (t <= 12) return 12;
+  if (t <= 16) return 16;
   return -1;
 }
+
+constexpr int kTileSizes[] = {16, 12, 8, 6, 4, 3, 2, 1};  // launch order: largest first
 
 // ---- kernel-class timing (events bracket each launch of an enabled class)
 void track_cb(void *p, int cls, int phase) {
@@ -193,6 +198,10 @@ void track_fold(mfs_ctx *ctx) {
   }
 }
 
+double ms_since(std::chrono::steady_clock::time_point t0) {
+  return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
 void phase_begin(mfs_ctx *ctx) { (void)hipEventRecord(ctx->ev0, ctx->stream); }
 double phase_end(mfs_ctx *ctx) {
   (void)hipEventRecord(ctx->ev1, ctx->stream);
@@ -215,7 +224,10 @@ int plan_impl(mfs_ctx *ctx, std::vector<double> &&xyz, int64_t n64, double rho, 
   phase_begin(ctx);
 
   // 1. ordering (host)
+  const auto t_order = std::chrono::steady_clock::now();
   int rc = order_maximin(n, xyz.data(), ctx->P, ctx->Pinv, ctx->scale);
+  ctx->stats.plan_order_ms = ms_since(t_order);
+  const auto t_pattern = std::chrono::steady_clock::now();
   if (rc == PLAN_ERR_DUPLICATE)
     return ctx->fail(MFS_ERR_DUPLICATE, "plan: coincident source points (zero length scale)");
   if (rc != PLAN_OK) return ctx->fail(MFS_ERR_ARG, "plan: ordering failed");
@@ -236,32 +248,60 @@ int plan_impl(mfs_ctx *ctx, std::vector<double> &&xyz, int64_t n64, double rho, 
   HIPCHK(ctx->dcnt.ensure(size_t(n) * 4), "alloc");
   HIPCHK(ctx->dptr.ensure(size_t(n + 1) * 4), "alloc");
   HIPCHK(ctx->dtotal.ensure(8), "alloc");
-  HIPCHK(launch_pattern(ctx->stream, n, ctx->dpx.as<double>(), ctx->dpy.as<double>(),
-                        ctx->dpz.as<double>(), ctx->dscale.as<double>(), ctx->rho32,
+  // grids for the neighbour search (host), then count / scan / fill (device)
+  std::vector<GridBand> bands;
+  std::vector<int32_t> cell_start, cell_pts;
+  int32_t j_brute = n;
+  build_grid_bands(n, px.data(), py.data(), pz.data(), ctx->scale.data(), ctx->rho32, bands,
+                   cell_start, cell_pts, &j_brute);
+  HIPCHK(upload(ctx, ctx->dbands, bands), "upload grid");
+  HIPCHK(upload(ctx, ctx->dcells, cell_start), "upload grid");
+  HIPCHK(upload(ctx, ctx->dcpts, cell_pts), "upload grid");
+  const double *dx_ = ctx->dpx.as<double>(), *dy_ = ctx->dpy.as<double>(), *dz_ = ctx->dpz.as<double>();
+  const double *dsc = ctx->dscale.as<double>();
+  HIPCHK(launch_grid_pattern(ctx->stream, j_brute, ctx->dbands.as<GridBand>(), int32_t(bands.size()),
+                             ctx->dcells.as<int32_t>(), ctx->dcpts.as<int32_t>(), dx_, dy_, dz_, dsc,
+                             ctx->rho32, ctx->dcnt.as<int32_t>(), nullptr, nullptr, 0),
+         "pattern count (grid)");
+  HIPCHK(launch_pattern(ctx->stream, n, j_brute, nullptr, n - j_brute, dx_, dy_, dz_, dsc, ctx->rho32,
                         ctx->dcnt.as<int32_t>(), nullptr, nullptr, 0),
-         "pattern count");
+         "pattern count (scan)");
   HIPCHK(launch_scan(ctx->stream, ctx->dcnt.as<int32_t>(), n, ctx->dptr.as<int32_t>(),
                      ctx->dtotal.as<int64_t>()),
          "pattern scan");
   int64_t total = 0;
-  HIPCHK(hipMemcpyAsync(&total, ctx->dtotal.p, 8, hipMemcpyDeviceToHost, ctx->stream), "copy");
-  HIPCHK(hipStreamSynchronize(ctx->stream), "sync");
-  if (total > INT32_MAX) return ctx->fail(MFS_ERR_ARG, "plan: pattern exceeds 2^31 entries");
-  HIPCHK(ctx->dind.ensure(size_t(std::max<int64_t>(total, 1)) * 4), "alloc pattern");
-  HIPCHK(launch_pattern(ctx->stream, n, ctx->dpx.as<double>(), ctx->dpy.as<double>(),
-                        ctx->dpz.as<double>(), ctx->dscale.as<double>(), ctx->rho32, nullptr,
-                        ctx->dptr.as<int32_t>(), ctx->dind.as<int32_t>(), 1),
-         "pattern fill");
   ctx->ptr.resize(size_t(n) + 1);
-  ctx->ind.resize(size_t(total));
+  HIPCHK(hipMemcpyAsync(&total, ctx->dtotal.p, 8, hipMemcpyDeviceToHost, ctx->stream), "copy");
   HIPCHK(hipMemcpyAsync(ctx->ptr.data(), ctx->dptr.p, size_t(n + 1) * 4, hipMemcpyDeviceToHost,
                         ctx->stream),
          "copy pattern");
+  HIPCHK(hipStreamSynchronize(ctx->stream), "sync");
+  if (total > INT32_MAX) return ctx->fail(MFS_ERR_ARG, "plan: pattern exceeds 2^31 entries");
+  // grid columns too long for the in-LDS sort are refilled, in order, by the scan
+  std::vector<int32_t> long_cols;
+  for (int32_t j = 0; j < j_brute; ++j)
+    if (ctx->ptr[j + 1] - ctx->ptr[j] > pattern_sort_max()) long_cols.push_back(j);
+  HIPCHK(upload(ctx, ctx->dlong, long_cols), "upload");
+  HIPCHK(ctx->dind.ensure(size_t(std::max<int64_t>(total, 1)) * 4), "alloc pattern");
+  HIPCHK(launch_grid_pattern(ctx->stream, j_brute, ctx->dbands.as<GridBand>(), int32_t(bands.size()),
+                             ctx->dcells.as<int32_t>(), ctx->dcpts.as<int32_t>(), dx_, dy_, dz_, dsc,
+                             ctx->rho32, nullptr, ctx->dptr.as<int32_t>(), ctx->dind.as<int32_t>(), 1),
+         "pattern fill (grid)");
+  HIPCHK(launch_pattern(ctx->stream, n, j_brute, nullptr, n - j_brute, dx_, dy_, dz_, dsc, ctx->rho32,
+                        nullptr, ctx->dptr.as<int32_t>(), ctx->dind.as<int32_t>(), 1),
+         "pattern fill (scan)");
+  HIPCHK(launch_pattern(ctx->stream, n, 0, ctx->dlong.as<int32_t>(), int32_t(long_cols.size()), dx_,
+                        dy_, dz_, dsc, ctx->rho32, nullptr, ctx->dptr.as<int32_t>(),
+                        ctx->dind.as<int32_t>(), 1),
+         "pattern fill (long columns)");
+  ctx->ind.resize(size_t(total));
   if (total > 0)
     HIPCHK(hipMemcpyAsync(ctx->ind.data(), ctx->dind.p, size_t(total) * 4, hipMemcpyDeviceToHost,
                           ctx->stream),
            "copy pattern");
   HIPCHK(hipStreamSynchronize(ctx->stream), "sync");
+  ctx->stats.plan_grid_columns = j_brute;
+  ctx->stats.plan_pattern_ms = ms_since(t_pattern);
 
   // 3. supernodes and supernodal pattern (host)
   rc = aggregate(n, ctx->ptr.data(), ctx->ind.data(), ctx->scale.data(), lambda, max_super,
@@ -277,7 +317,7 @@ int plan_impl(mfs_ctx *ctx, std::vector<double> &&xyz, int64_t n64, double rho, 
   ctx->max_local = 0;
   double s2 = 0, s3 = 0;
   int64_t theta = 0;
-  std::vector<std::vector<int32_t>> by_t(9);
+  std::vector<std::vector<int32_t>> by_t(17);
   for (int32_t s = 0; s < n_super; ++s) {
     SuperDesc d{};
     d.first = ctx->sup_ind[ctx->sup_ptr[s]];
@@ -293,7 +333,8 @@ int plan_impl(mfs_ctx *ctx, std::vector<double> &&xyz, int64_t n64, double rho, 
     const int t = tiles_for(d.ns);
     if (t < 0)
       return ctx->fail(MFS_ERR_ARG, "plan: supernode local size " + std::to_string(d.ns) +
-                                        " exceeds the supported maximum of 128");
+                                        " exceeds the supported maximum of " +
+                                        std::to_string(kMaxLocal));
     by_t[t].push_back(s);
   }
   // Launch order within a bucket: supernodes sorted along a Morton curve of
@@ -332,7 +373,7 @@ int plan_impl(mfs_ctx *ctx, std::vector<double> &&xyz, int64_t n64, double rho, 
     ctx->list.clear();
     ctx->buckets.clear();
     constexpr int kXcd = 8;
-    for (int t : {8, 6, 4, 3, 2, 1}) {
+    for (int t : kTileSizes) {
       std::vector<int32_t> &v = by_t[t];
       if (v.empty()) continue;
       std::stable_sort(v.begin(), v.end(), [&](int32_t a, int32_t b) { return key[a] < key[b]; });
@@ -598,6 +639,14 @@ int factor_impl(mfs_ctx *ctx, const float *M, const PointsOp *pts, int64_t m, in
   // smaller ones instead of leaving CUs idle between launches.
   int rc = ensure_side_streams(ctx);
   if (rc != MFS_OK) return rc;
+  // global TH scratch of the large buckets (sized for FP64, the widest compute type)
+  std::vector<size_t> scratch_off(ctx->buckets.size(), 0);
+  size_t scratch_bytes = 0;
+  for (size_t bi = 0; bi < ctx->buckets.size(); ++bi) {
+    scratch_off[bi] = scratch_bytes;
+    scratch_bytes += size_t(ctx->buckets[bi].count) * size_t(factor_scratch_elems(ctx->buckets[bi].tiles)) * 8;
+  }
+  HIPCHK(ctx->dthg.ensure(std::max<size_t>(scratch_bytes, 8)), "alloc factor scratch");
   HIPCHK(hipEventRecord(ctx->fork_ev, ctx->stream), "fork");
   const int nside = std::min<int>(mfs_ctx::kSide, int(ctx->buckets.size()));
   for (int i = 0; i < nside; ++i) HIPCHK(hipStreamWaitEvent(ctx->side[i], ctx->fork_ev, 0), "fork");
@@ -607,7 +656,8 @@ int factor_impl(mfs_ctx *ctx, const float *M, const PointsOp *pts, int64_t m, in
                                Mt, ldmt, pts, ctx->dsd.as<SuperDesc>(),
                                ctx->dlist.as<int32_t>() + b.offset, b.count, ctx->dsind.as<int32_t>(),
                                ctx->dPinv.as<int32_t>(), ctx->dsupind.as<int32_t>(),
-                               ctx->dsptr.as<int32_t>(), ctx->dval.p, ctx->dstatus.as<int32_t>()),
+                               ctx->dsptr.as<int32_t>(), ctx->dval.p, ctx->dstatus.as<int32_t>(),
+                               ctx->dthg.as<char>() + scratch_off[bi]),
            "supernode factor");
   }
   for (int i = 0; i < nside; ++i) {
@@ -627,11 +677,11 @@ int factor_impl(mfs_ctx *ctx, const float *M, const PointsOp *pts, int64_t m, in
   if (!failed.empty() && precision == MFS_FP32) {
     // FP64 retry of the supernodes whose FP32 pivots failed; values stay FP32.
     ctx->stats.retried_supernodes = int32_t(failed.size());
-    std::vector<std::vector<int32_t>> by_t(9);
+    std::vector<std::vector<int32_t>> by_t(17);
     for (int32_t s : failed) by_t[tiles_for(ctx->sdesc[s].ns)].push_back(s);
     std::vector<int32_t> flist;
     std::vector<Bucket> fb;
-    for (int t : {8, 6, 4, 3, 2, 1}) {
+    for (int t : kTileSizes) {
       if (by_t[t].empty()) continue;
       fb.push_back({t, int32_t(flist.size()), int32_t(by_t[t].size())});
       flist.insert(flist.end(), by_t[t].begin(), by_t[t].end());
@@ -643,7 +693,7 @@ int factor_impl(mfs_ctx *ctx, const float *M, const PointsOp *pts, int64_t m, in
                                  ctx->dsd.as<SuperDesc>(), dfl.as<int32_t>() + b.offset, b.count,
                                  ctx->dsind.as<int32_t>(), ctx->dPinv.as<int32_t>(),
                                  ctx->dsupind.as<int32_t>(), ctx->dsptr.as<int32_t>(), ctx->dval.p,
-                                 ctx->dstatus.as<int32_t>()),
+                                 ctx->dstatus.as<int32_t>(), ctx->dthg.p),
              "supernode factor retry");
     HIPCHK(hipMemcpyAsync(status.data(), ctx->dstatus.p, size_t(n_super) * 4,
                           hipMemcpyDeviceToHost, ctx->stream),
@@ -832,8 +882,8 @@ void mfs_ctx_destroy(mfs_ctx *ctx) {
   for (DevBuf *b : {&ctx->dP, &ctx->dPinv, &ctx->dsptr, &ctx->dsind, &ctx->dtptr, &ctx->dtcol,
                     &ctx->dtmap, &ctx->dsupind, &ctx->dsd, &ctx->dlist, &ctx->dstatus, &ctx->dpx,
                     &ctx->dpy, &ctx->dpz, &ctx->dscale, &ctx->dcnt, &ctx->dptr, &ctx->dind,
-                    &ctx->dtotal, &ctx->dval, &ctx->dtval, &ctx->dMt, &ctx->dval_alt,
-                    &ctx->dtval_alt, &ctx->dvec, &ctx->dpart, &ctx->ddpart, &ctx->dstate,
+                    &ctx->dtotal, &ctx->dbands, &ctx->dcells, &ctx->dcpts, &ctx->dlong, &ctx->dval, &ctx->dtval, &ctx->dMt, &ctx->dval_alt,
+                    &ctx->dtval_alt, &ctx->dthg, &ctx->dvec, &ctx->dpart, &ctx->ddpart, &ctx->dstate,
                     &ctx->dx64, &ctx->dr64, &ctx->dnorm, &ctx->dt32, &ctx->dt64, &ctx->ds32,
                     &ctx->ds64, &ctx->dapin, &ctx->dapout})
     b->release();

@@ -10,7 +10,7 @@ namespace mfs {
 constexpr int kThreads = 256;     // every kernel here runs 4 wave64 per block
 constexpr int kMaxRhs = 16;       // right-hand sides per PCG batch
 constexpr int kGramChunk = 128;   // M^T rows are padded with zeros to a multiple of this
-constexpr int kMaxLocal = 128;    // largest supernode handled in LDS
+constexpr int kMaxLocal = 256;    // largest supernode (local rows); above 128, TH in global scratch
 
 // Per-supernode descriptor (built on the host from the plan).
 struct SuperDesc {
@@ -52,9 +52,20 @@ struct PointsOp {
 };
 
 // ----------------------------------------------------------------- plan kernels
-hipError_t launch_pattern(hipStream_t st, int32_t n, const double *px, const double *py,
-                          const double *pz, const double *scale, double rho, int32_t *cnt,
-                          int32_t *ptr, int32_t *ind, int fill);
+// Exhaustive scan for columns j_first .. j_first + ncols - 1, or cols[0..ncols).
+hipError_t launch_pattern(hipStream_t st, int32_t n, int32_t j_first, const int32_t *cols,
+                          int32_t ncols, const double *px, const double *py, const double *pz,
+                          const double *scale, double rho, int32_t *cnt, int32_t *ptr,
+                          int32_t *ind, int fill);
+// Grid search for the columns j < j_end; the fill also sorts each column with
+// at most pattern_sort_max() rows (longer ones must be refilled by the scan).
+struct GridBand;
+hipError_t launch_grid_pattern(hipStream_t st, int32_t j_end, const GridBand *bands,
+                               int32_t nbands, const int32_t *cell_start, const int32_t *cell_pts,
+                               const double *px, const double *py, const double *pz,
+                               const double *scale, double rho, int32_t *cnt, int32_t *ptr,
+                               int32_t *ind, int fill);
+int32_t pattern_sort_max();
 hipError_t launch_scan(hipStream_t st, const int32_t *cnt, int32_t n, int32_t *ptr,
                        int64_t *total);
 
@@ -63,13 +74,19 @@ hipError_t launch_transpose(hipStream_t st, const float *M, int64_t m, int64_t n
                             int64_t ldmt);
 // precision: 0 = FP64 compute / FP64 values, 1 = FP32 compute / FP32 values,
 // 2 = FP64 compute / FP32 values (retry of FP32 failures).
-// With pts != nullptr (points form) the Gram's staged chunks are evaluated from
+// With pts != nullptr (points form) the Gram's operands are evaluated from
 // the point records instead of read from Mt (Mt and ldmt are then unused).
+// tiles in {1, 2, 3, 4, 6, 8, 12, 16} (local size <= 16 tiles); for 12 and 16
+// th_scratch holds count * factor_scratch_elems(tiles) elements of the compute
+// type (8 bytes for precision 0 and 2, 4 for precision 1).
 hipError_t launch_super_factor(hipStream_t st, int tiles, int precision, const float *Mt,
                                int64_t ldmt, const PointsOp *pts, const SuperDesc *sd,
                                const int32_t *list, int32_t count, const int32_t *sind,
                                const int32_t *Pinv, const int32_t *sup_ind, const int32_t *sptr,
-                               void *val, int32_t *status);
+                               void *val, int32_t *status, void *th_scratch);
+inline int64_t factor_scratch_elems(int tiles) {
+  return tiles > 8 ? int64_t(16 * tiles) * (16 * tiles + 1) : 0;
+}
 hipError_t launch_gather_values(hipStream_t st, int precision, int32_t nnz, const int32_t *tmap,
                                 const void *val, void *tval);
 

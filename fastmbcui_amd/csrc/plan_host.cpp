@@ -243,4 +243,80 @@ void transpose_pattern(int32_t n, const std::vector<int32_t> &sptr,
     }
 }
 
+void build_grid_bands(int32_t n, const double *px, const double *py, const double *pz,
+                      const double *scale, double rho, std::vector<GridBand> &bands,
+                      std::vector<int32_t> &cell_start, std::vector<int32_t> &cell_pts,
+                      int32_t *j_brute) {
+  // Cells per band at most twice the band's points (the clouds are surfaces,
+  // most cells of a box are empty), larger cells beyond; a band whose grid
+  // would have fewer than kMinCells cells cannot prune and is left to the scan.
+  constexpr int64_t kMinCells = 64;
+  bands.clear();
+  cell_start.clear();
+  cell_pts.clear();
+  *j_brute = n;
+  const double *p[3] = {px, py, pz};
+  int32_t j0 = 0;
+  while (j0 < n) {
+    // columns whose length scale stays within twice the band's first one
+    int32_t j1 = j0 + 1;
+    while (j1 < n && scale[j1] <= 2.0 * scale[j0]) ++j1;
+    // every row of column j lies within rho * l_j (the predicate is strict);
+    // the margin keeps |cell(p) - cell(q)| <= 1 clear of rounding
+    const double radius = rho * scale[j1 - 1];
+    double lo[3], hi[3];
+    for (int d = 0; d < 3; ++d) lo[d] = hi[d] = p[d][j0];
+    for (int32_t i = j0 + 1; i < n; ++i)
+      for (int d = 0; d < 3; ++d) {
+        lo[d] = std::min(lo[d], p[d][i]);
+        hi[d] = std::max(hi[d], p[d][i]);
+      }
+    const int64_t kMaxCells = std::max<int64_t>(2 * int64_t(n - j0), 4096);
+    double h = radius * (1.0 + 1e-6);
+    if (!(h > 0)) break;  // degenerate scale: leave the rest to the scan
+    int64_t dims[3], cells = 1;
+    for (int pass = 0; pass < 16; ++pass) {
+      cells = 1;
+      for (int d = 0; d < 3; ++d) {
+        dims[d] = std::max<int64_t>(1, int64_t((hi[d] - lo[d]) / h) + 1);
+        cells *= dims[d];
+      }
+      if (cells <= kMaxCells) break;
+      h *= std::max(std::cbrt(double(cells) / double(kMaxCells)), 1.05);
+    }
+    if (cells < kMinCells || cells > kMaxCells) break;
+    GridBand b{};
+    b.j0 = j0;
+    b.j1 = j1;
+    for (int d = 0; d < 3; ++d) {
+      b.dims[d] = int32_t(dims[d]);
+      b.lo[d] = lo[d];
+    }
+    b.inv_h = 1.0 / h;
+    b.cell_off = int64_t(cell_start.size());
+    b.pt_off = int64_t(cell_pts.size());
+    // counting sort of the points i >= j0 by cell (same arithmetic as the query)
+    std::vector<int32_t> cell_of(size_t(n - j0));
+    std::vector<int32_t> cnt(size_t(cells) + 1, 0);
+    for (int32_t i = j0; i < n; ++i) {
+      int64_t c[3];
+      for (int d = 0; d < 3; ++d) {
+        c[d] = int64_t((p[d][i] - lo[d]) * b.inv_h);
+        c[d] = std::min<int64_t>(std::max<int64_t>(c[d], 0), dims[d] - 1);
+      }
+      const int32_t id = int32_t((c[2] * dims[1] + c[1]) * dims[0] + c[0]);
+      cell_of[size_t(i - j0)] = id;
+      ++cnt[size_t(id) + 1];
+    }
+    for (int64_t c = 0; c < cells; ++c) cnt[size_t(c) + 1] += cnt[size_t(c)];
+    cell_start.insert(cell_start.end(), cnt.begin(), cnt.end());
+    cell_pts.resize(cell_pts.size() + size_t(n - j0));
+    for (int32_t i = j0; i < n; ++i)
+      cell_pts[size_t(b.pt_off) + size_t(cnt[size_t(cell_of[size_t(i - j0)])]++)] = i;
+    bands.push_back(b);
+    j0 = j1;
+  }
+  *j_brute = j0;
+}
+
 }  // namespace mfs

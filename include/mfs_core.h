@@ -54,6 +54,10 @@ typedef struct mfs_stats {
     double relres[64];        /* sqrt(RESD2/RHS2) per right-hand side (first 64) */
     int32_t outer_iters;      /* refinement sweeps of mfs_solve_mixed */
     int32_t retried_supernodes; /* FP32 factor: supernodes redone in FP64 */
+    double plan_order_ms;       /* host maximin ordering */
+    double plan_pattern_ms;     /* neighbour grids + device pattern search, to host */
+    int32_t plan_grid_columns;  /* columns found by the grid search (rest: scan) */
+    int32_t pad1;
 } mfs_stats;
 
 /* Context lifecycle.  `stream` is a hipStream_t (NULL = legacy default). */

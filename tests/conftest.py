@@ -9,5 +9,5 @@ if ROOT not in sys.path:
 
 
 def pytest_configure(config):
-    config.addinivalue_line("markers", "gpu: needs an MI355X (run with -m gpu on the GPU box)")
+    config.addinivalue_line("markers", "gpu: needs an MI355X GPU (select with -m gpu)")
     config.addinivalue_line("markers", "slow: longer CPU checks")

@@ -319,3 +319,63 @@ def test_assemble_matches_numpy_kernel():
     M = ctx.assemble_dense(x, nx, y).cpu().numpy()
     ref = scenes.kernel_matrix(sc.x, sc.nx, sc.y)
     assert np.max(np.abs(M - ref) / np.abs(ref)) < 1e-6
+
+
+@pytest.mark.parametrize("rho", [3.0, 6.0])
+def test_plan_grid_search_bit_exact_larger_scene(rho):
+    # 32 coins (N = 16,384): the grid search covers most columns, the coarsest
+    # bands use the exhaustive scan.
+    sc = scenes.coins(seed=3, n_bodies=32, pts_per_body=512, k=1, box=16.0)
+    ctx, got, want = _plan_pair(sc.y, rho)
+    s = ctx.stats()
+    print(f"rho {rho}: grid columns {s.plan_grid_columns}/{sc.n}, "
+          f"max column {np.diff(got['ptr']).max()}, order {s.plan_order_ms:.1f} ms, "
+          f"pattern {s.plan_pattern_ms:.1f} ms", flush=True)
+    assert s.plan_grid_columns > 0
+    _assert_plan_equal(got, want)
+
+
+def _large_sc(k=2):
+    # rho = 7 on 4 coins gives supernodes of 129..236 local rows (TH in global scratch)
+    return scenes.coins(seed=3, n_bodies=4, pts_per_body=512, k=k, box=6.0)
+
+
+def test_factor_large_supernodes_matches_oracle():
+    sc = _large_sc()
+    ctx, Md, M, pl, val, ref = _factor_pair(sc, _lib.MFS_FP64, rho=7.0)
+    s = ctx.stats()
+    err = _col_rel_err(pl, val, ref)
+    print(f"large supernodes (max local {s.max_local}) FP64: max column rel err {err:.2e}", flush=True)
+    assert s.max_local > 192
+    # rho = 7 blocks of ~236 rows are far worse conditioned than the rho = 3
+    # ones, so two FP64 factorizations summing in different orders agree to
+    # ~1e-8 here rather than ~1e-13
+    assert err < 1e-7
+
+
+def test_large_supernodes_fp32_factor_still_preconditions():
+    # At rho = 7 the FP32 Cholesky of the ~236-row blocks carries O(1e-1)
+    # relative error (condition numbers ~1e6), so what is checked is its use:
+    # the mixed solve reaches the tolerance in about the FP64 factor's count.
+    sc = _large_sc()
+    ctx, Md, M, pl, val, ref = _factor_pair(sc, _lib.MFS_FP64, rho=7.0)
+    B = torch.from_numpy(sc.b.astype(np.float32)).to(DEV)
+    ctx.solve_dense(Md, B, maxit=500, tol=1e-6)
+    it64 = max(ctx.stats().iters[c] for c in range(sc.k))
+    ctx.factor_dense(Md, _lib.MFS_FP32)
+    X = ctx.solve_mixed(Md, B, maxit=500, tol=1e-6)
+    s = ctx.stats()
+    it32 = max(s.iters[c] for c in range(sc.k))
+    res = oracle.normal_residual(M.astype(np.float64), sc.b.astype(np.float32), X.cpu().numpy())
+    print(f"large supernodes: FP64 factor {it64} iterations, FP32 factor {it32} inner iterations "
+          f"({s.outer_iters} sweeps), residual {res:.2e}", flush=True)
+    assert res <= 2e-6
+    assert it32 <= 2 * it64 + 2
+
+
+def test_supernode_above_256_rows_is_rejected():
+    sc = scenes.ellipsoids(seed=0, n_bodies=4, pts_per_body=512, k=1, box=8.0)
+    ctx = _ctx()
+    with pytest.raises(_lib.MfsError) as e:
+        ctx.plan(sc.y, 8.0)
+    assert e.value.code == _lib.MFS_ERR_ARG and "256" in str(e.value)

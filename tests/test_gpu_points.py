@@ -196,3 +196,21 @@ def test_points_entry_points_check_state():
     assert e.value.code == _lib.MFS_ERR_STATE
     with pytest.raises(ValueError):
         ctx.factor_points(_dev(sc.x), _dev(sc.nx), _dev(sc.y[:-1]))
+
+
+@pytest.mark.parametrize("precision", [_lib.MFS_FP64, _lib.MFS_FP32])
+def test_points_large_supernodes(precision):
+    sc = scenes.coins(seed=3, n_bodies=4, pts_per_body=512, k=2, box=6.0)
+    ctx = _points_ctx(sc, precision, rho=7.0)
+    val = ctx.factor_export()
+    M = scenes.kernel_matrix(sc.x, sc.nx, sc.y)
+    pl = oracle.plan(sc.y, 7.0)
+    ref = oracle.factor(oracle.permuted_colmajor(M, pl.Pinv), M.shape[0], pl)
+    err = _col_rel_err(pl, val, ref)
+    B = _dev(sc.b.astype(np.float32))
+    X = ctx.solve_points(B, maxit=500, tol=1e-6)
+    res = oracle.normal_residual(M, sc.b.astype(np.float32), X.cpu().numpy())
+    print(f"points large supernodes (max local {ctx.stats().max_local}): factor err {err:.2e}, "
+          f"residual {res:.2e}", flush=True)
+    assert ctx.stats().max_local > 192
+    assert res <= 2e-6
