@@ -81,6 +81,8 @@ def lib():
             "mfs_factor_points": (ci, [vp, vp, vp, i64, vp, i64, ci]),
             "mfs_solve_points": (ci, [vp, vp, i64, vp, vp, i32, f64, f64]),
             "mfs_points_residual": (ci, [vp, vp, i64, vp, vp]),
+            "mfs_solve_points64": (ci, [vp, vp, i64, vp, vp, i32, f64, f64]),
+            "mfs_points_residual64": (ci, [vp, vp, i64, vp, vp]),
             "mfs_points_apply": (ci, [vp, ci, ci, vp, i64, vp]),
             "mfs_ctx_track_kernel": (ci, [vp, ci]),
             "mfs_ctx_tracked": (ci, [vp, ci, ctypes.POINTER(i64), ctypes.POINTER(f64), ctypes.POINTER(f64)]),
@@ -269,13 +271,35 @@ class Context:
                                            float(tol), float(inner_tol)), "mfs_solve_points")
         return out
 
-    def points_residual(self, B, X):
+    def solve_points64(self, B, maxit=1000, tol=1e-6, inner_tol=0.5, x0=None, out=None):
+        """solve_points with FP64 warm start and FP64 strengths (n x k float64 cuda)."""
+        import torch
         self._sync_stream()
         B = _as_2d_f32(B)
-        X = _as_2d_f32(X)
+        if B.shape[0] != self.m_points:
+            raise ValueError("B must have one row per target point")
+        if out is None:
+            out = torch.empty((self.n, B.shape[1]), dtype=torch.float64, device=B.device)
+        if x0 is not None:
+            x0 = (x0.unsqueeze(1) if x0.dim() == 1 else x0).to(torch.float64).contiguous()
+        self._check(lib().mfs_solve_points64(self._h, _ptr(B), B.shape[1], _ptr(x0), _ptr(out), int(maxit),
+                                             float(tol), float(inner_tol)), "mfs_solve_points64")
+        return out
+
+    def points_residual(self, B, X):
+        import torch
+        self._sync_stream()
+        B = _as_2d_f32(B)
+        X = X.unsqueeze(1) if X.dim() == 1 else X
         out = np.empty(B.shape[1], np.float64)
-        self._check(lib().mfs_points_residual(self._h, _ptr(B), B.shape[1], _ptr(X), _ptr(out)),
-                    "mfs_points_residual")
+        if X.dtype == torch.float64:
+            X = X.contiguous()
+            self._check(lib().mfs_points_residual64(self._h, _ptr(B), B.shape[1], _ptr(X), _ptr(out)),
+                        "mfs_points_residual64")
+        else:
+            X = _as_2d_f32(X)
+            self._check(lib().mfs_points_residual(self._h, _ptr(B), B.shape[1], _ptr(X), _ptr(out)),
+                        "mfs_points_residual")
         return out
 
     def points_apply(self, v, trans=False, precision=MFS_FP32):

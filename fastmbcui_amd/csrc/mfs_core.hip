@@ -562,6 +562,21 @@ __global__ void k_x64_from_f32(int64_t n, const float *X, int64_t ldx, int64_t c
   for (int r = 0; r < K; ++r) X64[i * K + r] = (r < kval && X) ? double(X[i * ldx + col0 + r]) : 0.0;
 }
 
+// FP64 strided columns <-> the K-padded FP64 iterate (FP64 warm start / output).
+__global__ void k_x64_from_f64(int64_t n, const double *X, int64_t ldx, int64_t col0, int kval,
+                               int K, double *X64) {
+  const int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x;
+  if (i >= n) return;
+  for (int r = 0; r < K; ++r) X64[i * K + r] = (r < kval && X) ? X[i * ldx + col0 + r] : 0.0;
+}
+
+__global__ void k_x64_to_f64(int64_t n, const double *X64, int K, double *X, int64_t ldx,
+                             int64_t col0, int kval) {
+  const int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x;
+  if (i >= n) return;
+  for (int r = 0; r < kval; ++r) X[i * ldx + col0 + r] = X64[i * K + r];
+}
+
 __global__ void k_x64_to_f32(int64_t n, const double *X64, int K, float *X, int64_t ldx,
                              int64_t col0, int kval) {
   const int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x;
@@ -719,12 +734,15 @@ int factor_impl(mfs_ctx *ctx, const float *M, const PointsOp *pts, int64_t m, in
 }
 
 // Mixed-precision solve with M (pts == nullptr) or matrix-free from the points.
+// X0 / X are FP32; X0d / Xd (used instead when given) are FP64 -- for
+// tolerances below what an FP32 copy of the strengths can hold.
 int solve_mixed_impl(mfs_ctx *ctx, const float *M, const PointsOp *pts, int64_t m, int64_t n,
                      const float *B, int64_t k, int rhs_space, const float *X0, float *X,
-                     int32_t maxit, double tol, double inner_tol) {
+                     int32_t maxit, double tol, double inner_tol, const double *X0d = nullptr,
+                     double *Xd = nullptr) {
   int rc = MFS_OK;
   if (!ctx->has_factor) return ctx->fail(MFS_ERR_STATE, "no factor: factor before solving");
-  if (B == nullptr || X == nullptr || k <= 0 || rhs_space != 0)
+  if (B == nullptr || (X == nullptr && Xd == nullptr) || k <= 0 || rhs_space != 0)
     return ctx->fail(MFS_ERR_ARG, "solve_mixed: boundary right-hand sides only");
   if (!(inner_tol > 0) || !(inner_tol <= 1)) inner_tol = 0.5;
   HIPCHK(hipSetDevice(ctx->device), "hipSetDevice");
@@ -749,10 +767,14 @@ int solve_mixed_impl(mfs_ctx *ctx, const float *M, const PointsOp *pts, int64_t 
     double *res2 = nb2 + kMaxRhs;
     double *thr = nb2 + 2 * kMaxRhs;
     const unsigned gn = unsigned((n + kThreads - 1) / kThreads);
-    k_x64_from_f32<<<gn, kThreads, 0, ctx->stream>>>(n, X0, k, col0, kval, K, X64);
+    if (X0d != nullptr)
+      k_x64_from_f64<<<gn, kThreads, 0, ctx->stream>>>(n, X0d, k, col0, kval, K, X64);
+    else
+      k_x64_from_f32<<<gn, kThreads, 0, ctx->stream>>>(n, X0, k, col0, kval, K, X64);
+    const bool warm = X0 != nullptr || X0d != nullptr;
     // ||M^T b||^2 and, for a zero start, the first residual.
     HIPCHK(residual64(ctx->stream, a64, B, k, col0, kval, nullptr, R64, nb2, track_cb, ctx), "residual");
-    if (X0 != nullptr)
+    if (warm)
       HIPCHK(residual64(ctx->stream, a64, B, k, col0, kval, X64, R64, res2, track_cb, ctx), "residual");
     else
       HIPCHK(hipMemcpyAsync(res2, nb2, kMaxRhs * 8, hipMemcpyDeviceToDevice, ctx->stream), "copy");
@@ -808,7 +830,10 @@ int solve_mixed_impl(mfs_ctx *ctx, const float *M, const PointsOp *pts, int64_t 
       ++outer;
       HIPCHK(residual64(ctx->stream, a64, B, k, col0, kval, X64, R64, res2, track_cb, ctx), "residual");
     }
-    k_x64_to_f32<<<gn, kThreads, 0, ctx->stream>>>(n, X64, K, X, k, col0, kval);
+    if (Xd != nullptr)
+      k_x64_to_f64<<<gn, kThreads, 0, ctx->stream>>>(n, X64, K, Xd, k, col0, kval);
+    if (X != nullptr)
+      k_x64_to_f32<<<gn, kThreads, 0, ctx->stream>>>(n, X64, K, X, k, col0, kval);
     HIPCHK(hipGetLastError(), "output");
     for (int r = 0; r < kval; ++r)
       if (col0 + r < 64) ctx->stats.iters[col0 + r] = iters[r];
@@ -822,10 +847,12 @@ int solve_mixed_impl(mfs_ctx *ctx, const float *M, const PointsOp *pts, int64_t 
 
 // FP64 normal-equation residual per column, with M or from the points.
 int residual_impl(mfs_ctx *ctx, const float *M, const PointsOp *pts, int64_t m, int64_t n,
-                  const float *B, int64_t k, const float *X, double *out) {
+                  const float *B, int64_t k, const float *X, double *out,
+                  const double *Xd = nullptr) {
   int rc = MFS_OK;
   if (!ctx->has_factor) return ctx->fail(MFS_ERR_STATE, "no factor: factor before solving");
-  if (B == nullptr || X == nullptr || k <= 0) return ctx->fail(MFS_ERR_ARG, "residual: bad arguments");
+  if (B == nullptr || (X == nullptr && Xd == nullptr) || k <= 0)
+    return ctx->fail(MFS_ERR_ARG, "residual: bad arguments");
   HIPCHK(hipSetDevice(ctx->device), "hipSetDevice");
   for (int64_t col0 = 0; col0 < k; col0 += kMaxRhs) {
     const int kval = int(std::min<int64_t>(kMaxRhs, k - col0));
@@ -838,7 +865,10 @@ int residual_impl(mfs_ctx *ctx, const float *M, const PointsOp *pts, int64_t m, 
     HIPCHK(ctx->dnorm.ensure(4 * kMaxRhs * 8), "alloc");
     double *nb2 = ctx->dnorm.as<double>();
     const unsigned gn = unsigned((n + kThreads - 1) / kThreads);
-    k_x64_from_f32<<<gn, kThreads, 0, ctx->stream>>>(n, X, k, col0, kval, K, ctx->dx64.as<double>());
+    if (Xd != nullptr)
+      k_x64_from_f64<<<gn, kThreads, 0, ctx->stream>>>(n, Xd, k, col0, kval, K, ctx->dx64.as<double>());
+    else
+      k_x64_from_f32<<<gn, kThreads, 0, ctx->stream>>>(n, X, k, col0, kval, K, ctx->dx64.as<double>());
     HIPCHK(residual64(ctx->stream, a, B, k, col0, kval, nullptr, ctx->dr64.as<double>(), nb2, nullptr, nullptr), "residual");
     HIPCHK(residual64(ctx->stream, a, B, k, col0, kval, ctx->dx64.as<double>(), ctx->dr64.as<double>(),
                       nb2 + kMaxRhs, nullptr, nullptr), "residual");
@@ -1053,6 +1083,22 @@ int mfs_solve_points(mfs_ctx *ctx, const float *B, int64_t k, const float *X0, f
   if (rc != MFS_OK) return rc;
   return solve_mixed_impl(ctx, nullptr, &ctx->pts, ctx->pts.m, ctx->n, B, k, 0, X0, X, maxit, tol,
                           inner_tol);
+}
+
+int mfs_solve_points64(mfs_ctx *ctx, const float *B, int64_t k, const double *X0, double *X,
+                       int32_t maxit, double tol, double inner_tol) {
+  if (ctx == nullptr || X == nullptr) return MFS_ERR_ARG;
+  int rc = check_points(ctx);
+  if (rc != MFS_OK) return rc;
+  return solve_mixed_impl(ctx, nullptr, &ctx->pts, ctx->pts.m, ctx->n, B, k, 0, nullptr, nullptr,
+                          maxit, tol, inner_tol, X0, X);
+}
+
+int mfs_points_residual64(mfs_ctx *ctx, const float *B, int64_t k, const double *X, double *out) {
+  if (ctx == nullptr || out == nullptr || X == nullptr) return MFS_ERR_ARG;
+  int rc = check_points(ctx);
+  if (rc != MFS_OK) return rc;
+  return residual_impl(ctx, nullptr, &ctx->pts, ctx->pts.m, ctx->n, B, k, nullptr, out, X);
 }
 
 int mfs_points_residual(mfs_ctx *ctx, const float *B, int64_t k, const float *X, double *out) {

@@ -8,6 +8,9 @@ scenes instead:
   surface points each (N = 4096), sources at x - 0.02 n.
 * ``ellipsoids(seed)``: 64 ellipsoids with semi-axes ~ U(0.5, 1.5), 1024
   Fibonacci-mapped points each (N = 65,536), sources at x - 0.1 min(a,b,c) n.
+* ``mixed_bodies(seed)``: 128 bodies (coins, ellipsoids, rounded boxes with
+  edge radius 0.1), 2048 points each (N = 262,144), in rigid motion at
+  constant v, w ~ N(0, 0.5^2); ``RigidScene.advance(dt)`` steps it.
 
 Bodies get a uniform random rotation (normalised 4-Gaussian quaternion) and a
 centre drawn uniformly in a box, rejected while its bounding sphere overlaps an
@@ -177,6 +180,135 @@ def ellipsoids(seed: int = 0, n_bodies: int = 64, pts_per_body: int = 1024, k: i
                       offset_frac=offset_frac, axes=axes))
 
 
+def _rounded_box_surface(rng, npts, half, r):
+    """Area-uniform samples on a box with half-extents `half` whose edges and
+    corners are rounded with radius r (the inner box grown by a ball of radius
+    r), body frame.  Returns points and outward unit normals."""
+    inner = np.asarray(half, dtype=np.float64) - r
+    ex, ey, ez = 2 * inner
+    faces = [ey * ez, ey * ez, ex * ez, ex * ez, ex * ey, ex * ey]
+    edges = [2 * inner[d] * np.pi * r / 2 for d in (0, 1, 2) for _ in range(4)]
+    corner = 4 * np.pi * r * r
+    areas = np.array(faces + edges + [corner])
+    part = rng.choice(len(areas), size=npts, p=areas / areas.sum())
+    nrm = np.zeros((npts, 3))
+    base = np.zeros((npts, 3))   # closest point on the inner box
+    for f in range(6):           # faces: +x, -x, +y, -y, +z, -z
+        sel = part == f
+        d, sgn = f // 2, (1.0 if f % 2 == 0 else -1.0)
+        o1, o2 = [a for a in (0, 1, 2) if a != d]
+        base[sel, d] = sgn * inner[d]
+        base[sel, o1] = rng.uniform(-inner[o1], inner[o1], sel.sum())
+        base[sel, o2] = rng.uniform(-inner[o2], inner[o2], sel.sum())
+        nrm[sel, d] = sgn
+    for e in range(12):          # edges along axis d, quadrant q of the other two
+        sel = part == 6 + e
+        d, q = e // 4, e % 4
+        o1, o2 = [a for a in (0, 1, 2) if a != d]
+        s1, s2 = (1.0 if q & 1 else -1.0), (1.0 if q & 2 else -1.0)
+        th = rng.uniform(0.0, np.pi / 2, sel.sum())
+        base[sel, d] = rng.uniform(-inner[d], inner[d], sel.sum())
+        base[sel, o1] = s1 * inner[o1]
+        base[sel, o2] = s2 * inner[o2]
+        nrm[sel, o1] = s1 * np.cos(th)
+        nrm[sel, o2] = s2 * np.sin(th)
+    sel = part == 18             # corners: a uniform direction picks the octant
+    g = rng.standard_normal((sel.sum(), 3))
+    g /= np.linalg.norm(g, axis=1, keepdims=True)
+    base[sel] = np.sign(g) * inner
+    nrm[sel] = g
+    return base + r * nrm, nrm
+
+
+def _rotation_step(w, dt):
+    """exp(dt [w]_x) by Rodrigues' formula, one 3x3 per row of w."""
+    th = np.linalg.norm(w, axis=1) * dt
+    out = np.repeat(np.eye(3)[None], w.shape[0], axis=0)
+    nz = th > 0
+    a = w[nz] / np.linalg.norm(w[nz], axis=1, keepdims=True)
+    K = np.zeros((a.shape[0], 3, 3))
+    K[:, 0, 1], K[:, 0, 2], K[:, 1, 2] = -a[:, 2], a[:, 1], -a[:, 0]
+    K[:, 1, 0], K[:, 2, 0], K[:, 2, 1] = a[:, 2], -a[:, 1], a[:, 0]
+    s, c = np.sin(th[nz])[:, None, None], np.cos(th[nz])[:, None, None]
+    out[nz] = np.eye(3)[None] + s * K + (1 - c) * (K @ K)
+    return out
+
+
+@dataclass
+class RigidScene:
+    """Bodies in rigid motion at constant velocities (v, w per body).
+
+    Body-frame points/normals/sources are fixed; `advance(dt)` moves the centres
+    by v dt and turns the frames by exp(dt [w]_x); `scene()` gives the current
+    world-frame Scene whose single right-hand side is the Neumann data of the
+    motion, n . (v + w x (x - c))."""
+
+    name: str
+    local_x: np.ndarray
+    local_n: np.ndarray
+    local_y: np.ndarray
+    body_of_point: np.ndarray
+    rot: np.ndarray       # n_bodies x 3 x 3
+    centres: np.ndarray   # n_bodies x 3
+    v: np.ndarray
+    w: np.ndarray
+    meta: dict = field(default_factory=dict)
+
+    def advance(self, dt: float) -> None:
+        self.centres = self.centres + dt * self.v
+        self.rot = _rotation_step(self.w, dt) @ self.rot
+
+    def scene(self) -> Scene:
+        R = self.rot[self.body_of_point]
+        c = self.centres[self.body_of_point]
+        x = np.einsum("nij,nj->ni", R, self.local_x) + c
+        nx = np.einsum("nij,nj->ni", R, self.local_n)
+        y = np.einsum("nij,nj->ni", R, self.local_y) + c
+        vel = self.v[self.body_of_point] + np.cross(self.w[self.body_of_point], x - c)
+        b = np.einsum("ij,ij->i", nx, vel)[:, None]
+        return Scene(self.name, x, nx, y, b, self.body_of_point, self.centres.copy(), dict(self.meta))
+
+
+def mixed_bodies(seed: int = 0, n_bodies: int = 128, pts_per_body: int = 2048, box: float = 32.0,
+                 speed: float = 0.5, edge_radius: float = 0.1) -> RigidScene:
+    """A third each of coins (radius 1, thickness 0.1, sources 0.02 inside),
+    ellipsoids (semi-axes ~ U(0.5, 1.5), Fibonacci points, sources 0.1 min axis
+    inside) and rounded boxes (half-extents ~ U(0.5, 1.5), edge radius 0.1,
+    sources 0.05 inside), in that rotation by body index; v, w ~ N(0, speed^2)."""
+    rng = np.random.default_rng(seed)
+    shape = np.arange(n_bodies) % 3
+    dims = rng.uniform(0.5, 1.5, (n_bodies, 3))
+    radii = np.where(shape == 0, np.hypot(1.0, 0.05),
+                     np.where(shape == 1, dims.max(axis=1), np.linalg.norm(dims, axis=1)))
+    centres = _place(rng, radii, box)
+    unit = _fibonacci_sphere(pts_per_body)
+    lx, ln, ly, body, rots = [], [], [], [], []
+    for bi in range(n_bodies):
+        rots.append(_random_rotation(rng))
+        if shape[bi] == 0:
+            p, nrm = _cylinder_surface(rng, pts_per_body, 1.0, 0.05)
+            src = p - 0.02 * nrm
+        elif shape[bi] == 1:
+            a = dims[bi]
+            p = unit * a
+            nrm = p / (a * a)
+            nrm /= np.linalg.norm(nrm, axis=1, keepdims=True)
+            src = p - 0.1 * a.min() * nrm
+        else:
+            p, nrm = _rounded_box_surface(rng, pts_per_body, dims[bi], edge_radius)
+            src = p - 0.5 * edge_radius * nrm
+        lx.append(p)
+        ln.append(nrm)
+        ly.append(src)
+        body.append(np.full(pts_per_body, bi))
+    v = speed * rng.standard_normal((n_bodies, 3))
+    w = speed * rng.standard_normal((n_bodies, 3))
+    return RigidScene("mixed", np.concatenate(lx), np.concatenate(ln), np.concatenate(ly),
+                      np.concatenate(body), np.array(rots), centres, v, w,
+                      dict(seed=seed, n_bodies=n_bodies, pts_per_body=pts_per_body, box=box,
+                           speed=speed, shapes=shape))
+
+
 def kernel_matrix(x: np.ndarray, nx: np.ndarray, y: np.ndarray) -> np.ndarray:
     """Dense FP64 M[i, j] = -(x_i - y_j) . n_i / (4 pi |x_i - y_j|^3) (small scenes only)."""
     d = x[:, None, :] - y[None, :, :]
@@ -185,4 +317,4 @@ def kernel_matrix(x: np.ndarray, nx: np.ndarray, y: np.ndarray) -> np.ndarray:
     return -dn / (FOUR_PI * r2 * np.sqrt(r2))
 
 
-SCENES = {"coins": coins, "ellipsoids": ellipsoids}
+SCENES = {"coins": coins, "ellipsoids": ellipsoids, "mixed": mixed_bodies}

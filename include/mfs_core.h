@@ -137,8 +137,15 @@ int mfs_factor_points(mfs_ctx *ctx, const double *x, const double *nx, int64_t m
  * (kl_chol.cu:891-1088) for the points form. */
 int mfs_solve_points(mfs_ctx *ctx, const float *B, int64_t k, const float *X0, float *X,
                      int32_t maxit, double tol, double inner_tol);
+/* mfs_solve_points with FP64 warm start X0 (may be NULL) and FP64 output X,
+ * for tolerances below what FP32 strengths can hold (an FP32 copy of X
+ * perturbs the residual of an ill-conditioned fit by ~1e-7). */
+int mfs_solve_points64(mfs_ctx *ctx, const float *B, int64_t k, const double *X0, double *X,
+                       int32_t maxit, double tol, double inner_tol);
 /* mfs_normal_residual for the points form (FP64 kernel evaluation). */
 int mfs_points_residual(mfs_ctx *ctx, const float *B, int64_t k, const float *X, double *out);
+/* Same for FP64 strengths X (n x k). */
+int mfs_points_residual64(mfs_ctx *ctx, const float *B, int64_t k, const double *X, double *out);
 /* out = M in (trans 0: in n x k, out m x k) or M^T in (trans 1: in m x k,
  * out n x k), row-major device arrays of float (MFS_FP32) or double
  * (MFS_FP64), kernel evaluated in that precision.  The matvec of

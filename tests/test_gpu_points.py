@@ -214,3 +214,16 @@ def test_points_large_supernodes(precision):
           f"residual {res:.2e}", flush=True)
     assert ctx.stats().max_local > 192
     assert res <= 2e-6
+
+
+def test_time_stepper_warm_start_meets_tolerance_each_step():
+    from fastmbcui_amd.timestep import TimeStepper
+    rs = scenes.mixed_bodies(seed=1, n_bodies=6, pts_per_body=256, box=10.0)
+    ts = TimeStepper(rs, rho=4.0, tol=1e-8, check_residual=True)
+    infos = [ts.step()] + [ts.step(1e-2) for _ in range(3)]
+    for s in infos:
+        print(f"step {s.step}: iters {s.inner_iters} sweeps {s.sweeps} residual {s.residual:.2e} "
+              f"warm {s.warm}", flush=True)
+        assert s.residual <= 1e-8
+    assert all(s.warm for s in infos[1:])
+    assert max(s.inner_iters for s in infos[1:]) < infos[0].inner_iters
