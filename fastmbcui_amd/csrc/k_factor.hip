@@ -134,13 +134,24 @@ __global__ __launch_bounds__(kThreads) void k_transpose(const float *__restrict_
 // p = 16 a + (lane & 15) for every tile a and f = f0 + (lane >> 4), keeping
 // its T source points in registers: no LDS staging and no barriers.
 template <int T, typename TC, typename TV, bool PTS>
-__global__ __launch_bounds__(kThreads) void k_super_factor(
-    const float *__restrict__ Mt, int64_t ldmt, const float *__restrict__ tgt,
-    const float *__restrict__ srcp, int64_t m, const SuperDesc *__restrict__ sd,
-    const int32_t *__restrict__ list, const int32_t *__restrict__ sind,
-    const int32_t *__restrict__ Pinv, const int32_t *__restrict__ sup_ind,
-    const int32_t *__restrict__ sptr, TV *__restrict__ val, int32_t *__restrict__ status,
-    TC *__restrict__ th_scratch) {
+__global__ __launch_bounds__(kThreads) void k_super_factor(const FactorScene *__restrict__ scenes,
+                                                           const FactorItem *__restrict__ items,
+                                                           TC *__restrict__ th_scratch) {
+  // This block's supernode and its scene (one launch may mix scenes).
+  const FactorItem item = items[blockIdx.x];
+  const FactorScene &FS = scenes[item.scene];
+  const float *__restrict__ Mt = FS.Mt;
+  const int64_t ldmt = FS.ldmt;
+  const float *__restrict__ tgt = FS.tgt;
+  const float *__restrict__ srcp = FS.src;
+  const int64_t m = FS.m;
+  const SuperDesc *__restrict__ sd = FS.sd;
+  const int32_t *__restrict__ sind = FS.sind;
+  const int32_t *__restrict__ Pinv = FS.Pinv;
+  const int32_t *__restrict__ sup_ind = FS.sup_ind;
+  const int32_t *__restrict__ sptr = FS.sptr;
+  TV *__restrict__ val = static_cast<TV *>(FS.val);
+  int32_t *__restrict__ status = FS.status;
   constexpr int NS = 16 * T;                  // padded local size
   // Supernodes of more than 128 local rows (T = 12, 16) keep TH in global
   // scratch (NS x RS per block; it stays in L2) since it would not fit in LDS.
@@ -169,7 +180,7 @@ __global__ __launch_bounds__(kThreads) void k_super_factor(
   __shared__ int32_t mcol[MB], mc[MB], moff[MB];
 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int32_t s = list[blockIdx.x];
+  const int32_t s = item.s;
   const SuperDesc D = sd[s];
   const int ns = D.ns;
 
@@ -421,29 +432,22 @@ __global__ __launch_bounds__(kThreads) void k_gather_values(int32_t nnz,
 }
 
 struct FactorLaunch {
-  const float *Mt;
-  int64_t ldmt;
-  const PointsOp *pts;
-  const SuperDesc *sd;
-  const int32_t *list;
+  const FactorScene *scenes;
+  const FactorItem *items;
   int32_t count;
-  const int32_t *sind, *Pinv, *sup_ind, *sptr;
-  void *val;
-  int32_t *status;
+  bool points;
   void *th_scratch;
 };
 
 template <int T, typename TC, typename TV>
 hipError_t launch_t(hipStream_t st, const FactorLaunch &L) {
   if (L.count <= 0) return hipSuccess;
-  if (L.pts != nullptr)
+  if (L.points)
     k_super_factor<T, TC, TV, true><<<unsigned(L.count), kThreads, 0, st>>>(
-        nullptr, 0, L.pts->tgt32, L.pts->src32, L.pts->m, L.sd, L.list, L.sind, L.Pinv,
-        L.sup_ind, L.sptr, static_cast<TV *>(L.val), L.status, static_cast<TC *>(L.th_scratch));
+        L.scenes, L.items, static_cast<TC *>(L.th_scratch));
   else
     k_super_factor<T, TC, TV, false><<<unsigned(L.count), kThreads, 0, st>>>(
-        L.Mt, L.ldmt, nullptr, nullptr, 0, L.sd, L.list, L.sind, L.Pinv, L.sup_ind, L.sptr,
-        static_cast<TV *>(L.val), L.status, static_cast<TC *>(L.th_scratch));
+        L.scenes, L.items, static_cast<TC *>(L.th_scratch));
   return hipGetLastError();
 }
 
@@ -474,13 +478,10 @@ hipError_t launch_transpose(hipStream_t st, const float *M, int64_t m, int64_t n
   return hipGetLastError();
 }
 
-hipError_t launch_super_factor(hipStream_t st, int tiles, int precision, const float *Mt,
-                               int64_t ldmt, const PointsOp *pts, const SuperDesc *sd,
-                               const int32_t *list, int32_t count, const int32_t *sind,
-                               const int32_t *Pinv, const int32_t *sup_ind, const int32_t *sptr,
-                               void *val, int32_t *status, void *th_scratch) {
-  const FactorLaunch L{Mt, ldmt, pts, sd, list, count, sind, Pinv, sup_ind, sptr, val, status,
-                       th_scratch};
+hipError_t launch_super_factor(hipStream_t st, int tiles, int precision, bool points,
+                               const FactorScene *scenes, const FactorItem *items, int32_t count,
+                               void *th_scratch) {
+  const FactorLaunch L{scenes, items, count, points, th_scratch};
   if (precision == 0) return launch_prec<double, double>(st, tiles, L);
   if (precision == 1) return launch_prec<float, float>(st, tiles, L);
   return launch_prec<double, float>(st, tiles, L);

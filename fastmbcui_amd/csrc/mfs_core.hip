@@ -70,7 +70,8 @@ struct mfs_ctx {
   std::vector<int32_t> P, Pinv, ptr, ind, sup_ptr, sup_ind, sup_parent, sptr, sind, tptr, tcol, tmap;
   std::vector<double> scale;
   std::vector<SuperDesc> sdesc;
-  std::vector<int32_t> list;
+  std::vector<int32_t> list;         // supernodes in launch order, bucket by bucket
+  std::vector<FactorItem> items;     // the same as (scene 0, s) items
   std::vector<Bucket> buckets;
   int32_t max_local = 0;
 
@@ -81,7 +82,8 @@ struct mfs_ctx {
   // factor
   bool has_factor = false;
   int prec = MFS_FP64;
-  DevBuf dval, dtval, dMt, dval_alt, dtval_alt, dthg;
+  DevBuf dval, dtval, dMt, dval_alt, dtval_alt, dthg, dfscene;
+  FactorScene fscene{};  // what the factor kernels read for this context
   int alt_prec = -1;
 
   // points form: packed target/source records (kernel_eval.h layouts)
@@ -402,7 +404,9 @@ int plan_impl(mfs_ctx *ctx, std::vector<double> &&xyz, int64_t n64, double rho, 
   HIPCHK(upload(ctx, ctx->dtmap, ctx->tmap), "upload");
   HIPCHK(upload(ctx, ctx->dsupind, ctx->sup_ind), "upload");
   HIPCHK(upload(ctx, ctx->dsd, ctx->sdesc), "upload");
-  HIPCHK(upload(ctx, ctx->dlist, ctx->list), "upload");
+  ctx->items.resize(ctx->list.size());
+  for (size_t i = 0; i < ctx->items.size(); ++i) ctx->items[i] = {0, ctx->list[i]};
+  HIPCHK(upload(ctx, ctx->dlist, ctx->items), "upload");
   HIPCHK(ctx->dstatus.ensure(size_t(n_super) * 4), "alloc");
 
   ctx->n = n;
@@ -623,63 +627,55 @@ void set_pass_work(mfs_ctx *ctx, const PointsOp *pts, int64_t m, int64_t n, int 
   }
 }
 
-// Supernodal factor from M (pts == nullptr) or from the packed points.
-int factor_impl(mfs_ctx *ctx, const float *M, const PointsOp *pts, int64_t m, int64_t n,
-                int precision) {
+// Factor phase 1: M^T (dense form), value buffers, cleared statuses, and the
+// scene record the factor kernels read (ctx->fscene, ctx->dfscene).
+int factor_prepare(mfs_ctx *ctx, const float *M, const PointsOp *pts, int64_t m, int64_t n,
+                   int precision) {
   if (precision != MFS_FP64 && precision != MFS_FP32)
     return ctx->fail(MFS_ERR_ARG, "factor: precision must be MFS_FP64 or MFS_FP32");
   HIPCHK(hipSetDevice(ctx->device), "hipSetDevice");
   ctx->has_factor = false;
   ctx->alt_prec = -1;
-  phase_begin(ctx);
   const int64_t ldmt = (m + kGramChunk - 1) / kGramChunk * kGramChunk;
   if (pts == nullptr) {
     HIPCHK(ctx->dMt.ensure(size_t(n) * size_t(ldmt) * sizeof(float)), "alloc M^T");
     HIPCHK(launch_transpose(ctx->stream, M, m, n, ctx->dMt.as<float>(), ldmt), "transpose");
   }
-  const float *Mt = pts ? nullptr : ctx->dMt.as<float>();
   const int64_t nnz = ctx->sptr[ctx->n];
   const size_t w = (precision == MFS_FP64) ? 8 : 4;
   HIPCHK(ctx->dval.ensure(size_t(std::max<int64_t>(nnz, 1)) * w), "alloc factor");
   HIPCHK(ctx->dtval.ensure(size_t(std::max<int64_t>(nnz, 1)) * w), "alloc factor");
   const int32_t n_super = int32_t(ctx->sdesc.size());
   HIPCHK(hipMemsetAsync(ctx->dstatus.p, 0, size_t(n_super) * 4, ctx->stream), "memset");
+  FactorScene &F = ctx->fscene;
+  F.Mt = pts ? nullptr : ctx->dMt.as<float>();
+  F.ldmt = ldmt;
+  F.tgt = pts ? pts->tgt32 : nullptr;
+  F.src = pts ? pts->src32 : nullptr;
+  F.m = m;
+  F.sd = ctx->dsd.as<SuperDesc>();
+  F.sind = ctx->dsind.as<int32_t>();
+  F.Pinv = ctx->dPinv.as<int32_t>();
+  F.sup_ind = ctx->dsupind.as<int32_t>();
+  F.sptr = ctx->dsptr.as<int32_t>();
+  F.val = ctx->dval.p;
+  F.status = ctx->dstatus.as<int32_t>();
+  HIPCHK(ctx->dfscene.ensure(sizeof(FactorScene)), "alloc");
+  HIPCHK(hipMemcpyAsync(ctx->dfscene.p, &ctx->fscene, sizeof(FactorScene), hipMemcpyHostToDevice,
+                        ctx->stream),
+         "upload scene");
   double sum_ns = 0;
   for (auto &d : ctx->sdesc) sum_ns += d.ns;
   // Work per launch: bytes of the gathered columns (dense), kernel FLOPs (points).
   ctx->bytes_per_launch[MFS_K_GRAM] = double(m) * sum_ns * (pts ? double(kEvalFlops) : 4.0);
-  track_cb(ctx, MFS_K_GRAM, 0);
-  // Buckets (largest tiles first) run on side streams at once, so that the
-  // few large supernodes and each launch's tail overlap the bulk of the
-  // smaller ones instead of leaving CUs idle between launches.
-  int rc = ensure_side_streams(ctx);
-  if (rc != MFS_OK) return rc;
-  // global TH scratch of the large buckets (sized for FP64, the widest compute type)
-  std::vector<size_t> scratch_off(ctx->buckets.size(), 0);
-  size_t scratch_bytes = 0;
-  for (size_t bi = 0; bi < ctx->buckets.size(); ++bi) {
-    scratch_off[bi] = scratch_bytes;
-    scratch_bytes += size_t(ctx->buckets[bi].count) * size_t(factor_scratch_elems(ctx->buckets[bi].tiles)) * 8;
-  }
-  HIPCHK(ctx->dthg.ensure(std::max<size_t>(scratch_bytes, 8)), "alloc factor scratch");
-  HIPCHK(hipEventRecord(ctx->fork_ev, ctx->stream), "fork");
-  const int nside = std::min<int>(mfs_ctx::kSide, int(ctx->buckets.size()));
-  for (int i = 0; i < nside; ++i) HIPCHK(hipStreamWaitEvent(ctx->side[i], ctx->fork_ev, 0), "fork");
-  for (size_t bi = 0; bi < ctx->buckets.size(); ++bi) {
-    const Bucket &b = ctx->buckets[bi];
-    HIPCHK(launch_super_factor(ctx->side[bi % mfs_ctx::kSide], b.tiles, precision == MFS_FP64 ? 0 : 1,
-                               Mt, ldmt, pts, ctx->dsd.as<SuperDesc>(),
-                               ctx->dlist.as<int32_t>() + b.offset, b.count, ctx->dsind.as<int32_t>(),
-                               ctx->dPinv.as<int32_t>(), ctx->dsupind.as<int32_t>(),
-                               ctx->dsptr.as<int32_t>(), ctx->dval.p, ctx->dstatus.as<int32_t>(),
-                               ctx->dthg.as<char>() + scratch_off[bi]),
-           "supernode factor");
-  }
-  for (int i = 0; i < nside; ++i) {
-    HIPCHK(hipEventRecord(ctx->join_ev[i], ctx->side[i]), "join");
-    HIPCHK(hipStreamWaitEvent(ctx->stream, ctx->join_ev[i], 0), "join");
-  }
-  track_cb(ctx, MFS_K_GRAM, 1);
+  return MFS_OK;
+}
+
+// Factor phase 3 (after the bucket launches): statuses, FP64 retry of failed
+// FP32 pivots (values stay FP32), transposed copy of the values.
+int factor_finish(mfs_ctx *ctx, bool points, int precision) {
+  const int32_t n_super = int32_t(ctx->sdesc.size());
+  const int64_t nnz = ctx->sptr[ctx->n];
   std::vector<int32_t> status(n_super);
   HIPCHK(hipMemcpyAsync(status.data(), ctx->dstatus.p, size_t(n_super) * 4, hipMemcpyDeviceToHost,
                         ctx->stream),
@@ -690,25 +686,24 @@ int factor_impl(mfs_ctx *ctx, const float *M, const PointsOp *pts, int64_t m, in
     if (status[s] != 0) failed.push_back(s);
   ctx->stats.retried_supernodes = 0;
   if (!failed.empty() && precision == MFS_FP32) {
-    // FP64 retry of the supernodes whose FP32 pivots failed; values stay FP32.
     ctx->stats.retried_supernodes = int32_t(failed.size());
-    std::vector<std::vector<int32_t>> by_t(17);
-    for (int32_t s : failed) by_t[tiles_for(ctx->sdesc[s].ns)].push_back(s);
-    std::vector<int32_t> flist;
+    std::vector<std::vector<FactorItem>> by_t(17);
+    for (int32_t s : failed) by_t[tiles_for(ctx->sdesc[s].ns)].push_back({0, s});
+    std::vector<FactorItem> flist;
     std::vector<Bucket> fb;
+    size_t scratch_bytes = 8;
     for (int t : kTileSizes) {
       if (by_t[t].empty()) continue;
       fb.push_back({t, int32_t(flist.size()), int32_t(by_t[t].size())});
       flist.insert(flist.end(), by_t[t].begin(), by_t[t].end());
+      scratch_bytes = std::max(scratch_bytes, by_t[t].size() * size_t(factor_scratch_elems(t)) * 8);
     }
     DevBuf dfl;
     HIPCHK(upload(ctx, dfl, flist), "upload retry list");
+    HIPCHK(ctx->dthg.ensure(scratch_bytes), "alloc factor scratch");
     for (const Bucket &b : fb)
-      HIPCHK(launch_super_factor(ctx->stream, b.tiles, 2, Mt, ldmt, pts,
-                                 ctx->dsd.as<SuperDesc>(), dfl.as<int32_t>() + b.offset, b.count,
-                                 ctx->dsind.as<int32_t>(), ctx->dPinv.as<int32_t>(),
-                                 ctx->dsupind.as<int32_t>(), ctx->dsptr.as<int32_t>(), ctx->dval.p,
-                                 ctx->dstatus.as<int32_t>(), ctx->dthg.p),
+      HIPCHK(launch_super_factor(ctx->stream, b.tiles, 2, points, ctx->dfscene.as<FactorScene>(),
+                                 dfl.as<FactorItem>() + b.offset, b.count, ctx->dthg.p),
              "supernode factor retry");
     HIPCHK(hipMemcpyAsync(status.data(), ctx->dstatus.p, size_t(n_super) * 4,
                           hipMemcpyDeviceToHost, ctx->stream),
@@ -728,6 +723,50 @@ int factor_impl(mfs_ctx *ctx, const float *M, const PointsOp *pts, int64_t m, in
          "gather values");
   ctx->prec = precision;
   ctx->has_factor = true;
+  return MFS_OK;
+}
+
+// Supernodal factor from M (pts == nullptr) or from the packed points.
+int factor_impl(mfs_ctx *ctx, const float *M, const PointsOp *pts, int64_t m, int64_t n,
+                int precision) {
+  if (precision != MFS_FP64 && precision != MFS_FP32)
+    return ctx->fail(MFS_ERR_ARG, "factor: precision must be MFS_FP64 or MFS_FP32");
+  HIPCHK(hipSetDevice(ctx->device), "hipSetDevice");
+  phase_begin(ctx);
+  int rc = factor_prepare(ctx, M, pts, m, n, precision);
+  if (rc != MFS_OK) return rc;
+  track_cb(ctx, MFS_K_GRAM, 0);
+  // Buckets (largest tiles first) run on side streams at once, so that the
+  // few large supernodes and each launch's tail overlap the bulk of the
+  // smaller ones instead of leaving CUs idle between launches.
+  rc = ensure_side_streams(ctx);
+  if (rc != MFS_OK) return rc;
+  // global TH scratch of the large buckets (sized for FP64, the widest compute type)
+  std::vector<size_t> scratch_off(ctx->buckets.size(), 0);
+  size_t scratch_bytes = 0;
+  for (size_t bi = 0; bi < ctx->buckets.size(); ++bi) {
+    scratch_off[bi] = scratch_bytes;
+    scratch_bytes += size_t(ctx->buckets[bi].count) * size_t(factor_scratch_elems(ctx->buckets[bi].tiles)) * 8;
+  }
+  HIPCHK(ctx->dthg.ensure(std::max<size_t>(scratch_bytes, 8)), "alloc factor scratch");
+  HIPCHK(hipEventRecord(ctx->fork_ev, ctx->stream), "fork");
+  const int nside = std::min<int>(mfs_ctx::kSide, int(ctx->buckets.size()));
+  for (int i = 0; i < nside; ++i) HIPCHK(hipStreamWaitEvent(ctx->side[i], ctx->fork_ev, 0), "fork");
+  for (size_t bi = 0; bi < ctx->buckets.size(); ++bi) {
+    const Bucket &b = ctx->buckets[bi];
+    HIPCHK(launch_super_factor(ctx->side[bi % mfs_ctx::kSide], b.tiles, precision == MFS_FP64 ? 0 : 1,
+                               pts != nullptr, ctx->dfscene.as<FactorScene>(),
+                               ctx->dlist.as<FactorItem>() + b.offset, b.count,
+                               ctx->dthg.as<char>() + scratch_off[bi]),
+           "supernode factor");
+  }
+  for (int i = 0; i < nside; ++i) {
+    HIPCHK(hipEventRecord(ctx->join_ev[i], ctx->side[i]), "join");
+    HIPCHK(hipStreamWaitEvent(ctx->stream, ctx->join_ev[i], 0), "join");
+  }
+  track_cb(ctx, MFS_K_GRAM, 1);
+  rc = factor_finish(ctx, pts != nullptr, precision);
+  if (rc != MFS_OK) return rc;
   ctx->stats.factor_ms = phase_end(ctx);
   track_fold(ctx);
   return MFS_OK;
@@ -913,7 +952,7 @@ void mfs_ctx_destroy(mfs_ctx *ctx) {
                     &ctx->dtmap, &ctx->dsupind, &ctx->dsd, &ctx->dlist, &ctx->dstatus, &ctx->dpx,
                     &ctx->dpy, &ctx->dpz, &ctx->dscale, &ctx->dcnt, &ctx->dptr, &ctx->dind,
                     &ctx->dtotal, &ctx->dbands, &ctx->dcells, &ctx->dcpts, &ctx->dlong, &ctx->dval, &ctx->dtval, &ctx->dMt, &ctx->dval_alt,
-                    &ctx->dtval_alt, &ctx->dthg, &ctx->dvec, &ctx->dpart, &ctx->ddpart, &ctx->dstate,
+                    &ctx->dtval_alt, &ctx->dthg, &ctx->dfscene, &ctx->dvec, &ctx->dpart, &ctx->ddpart, &ctx->dstate,
                     &ctx->dx64, &ctx->dr64, &ctx->dnorm, &ctx->dt32, &ctx->dt64, &ctx->ds32,
                     &ctx->ds64, &ctx->dapin, &ctx->dapout})
     b->release();

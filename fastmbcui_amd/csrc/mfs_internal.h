@@ -74,16 +74,32 @@ hipError_t launch_transpose(hipStream_t st, const float *M, int64_t m, int64_t n
                             int64_t ldmt);
 // precision: 0 = FP64 compute / FP64 values, 1 = FP32 compute / FP32 values,
 // 2 = FP64 compute / FP32 values (retry of FP32 failures).
-// With pts != nullptr (points form) the Gram's operands are evaluated from
-// the point records instead of read from Mt (Mt and ldmt are then unused).
-// tiles in {1, 2, 3, 4, 6, 8, 12, 16} (local size <= 16 tiles); for 12 and 16
-// th_scratch holds count * factor_scratch_elems(tiles) elements of the compute
-// type (8 bytes for precision 0 and 2, 4 for precision 1).
-hipError_t launch_super_factor(hipStream_t st, int tiles, int precision, const float *Mt,
-                               int64_t ldmt, const PointsOp *pts, const SuperDesc *sd,
-                               const int32_t *list, int32_t count, const int32_t *sind,
-                               const int32_t *Pinv, const int32_t *sup_ind, const int32_t *sptr,
-                               void *val, int32_t *status, void *th_scratch);
+// One scene's inputs and outputs of the factor (device pointers).  Dense form:
+// Mt (n x ldmt); points form: tgt (m x 12) and src (n x 8) records.
+struct FactorScene {
+  const float *Mt;
+  int64_t ldmt;
+  const float *tgt, *src;
+  int64_t m;
+  const SuperDesc *sd;
+  const int32_t *sind, *Pinv, *sup_ind, *sptr;
+  void *val;        // factor values, TV
+  int32_t *status;  // per supernode: 0 done, 1 non-positive pivot
+};
+
+// A supernode of a scene: one block of a factor launch.
+struct FactorItem {
+  int32_t scene, s;
+};
+
+// Factors the supernodes items[0..count) (all of `tiles` tiles) of the scenes
+// in `scenes` (device table); points = the scenes are in the points form (the
+// Gram's operands are evaluated from the records).  tiles in {1, 2, 3, 4, 6,
+// 8, 12, 16}; for 12 and 16 th_scratch holds count * factor_scratch_elems(tiles)
+// elements of the compute type (8 bytes for precision 0 and 2, 4 for 1).
+hipError_t launch_super_factor(hipStream_t st, int tiles, int precision, bool points,
+                               const FactorScene *scenes, const FactorItem *items, int32_t count,
+                               void *th_scratch);
 inline int64_t factor_scratch_elems(int tiles) {
   return tiles > 8 ? int64_t(16 * tiles) * (16 * tiles + 1) : 0;
 }
