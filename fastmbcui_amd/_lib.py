@@ -45,6 +45,15 @@ class Stats(ctypes.Structure):
     ]
 
 
+class BatchStats(ctypes.Structure):
+    _fields_ = [
+        ("plan_ms", ctypes.c_double), ("factor_ms", ctypes.c_double), ("solve_ms", ctypes.c_double),
+        ("factor_launches", ctypes.c_int32), ("factor_items", ctypes.c_int32),
+        ("graph_nodes", ctypes.c_int32), ("graph_replays", ctypes.c_int32),
+        ("sweeps", ctypes.c_int32), ("pad", ctypes.c_int32),
+    ]
+
+
 _lib = None
 _lock = threading.Lock()
 
@@ -84,6 +93,14 @@ def lib():
             "mfs_solve_points64": (ci, [vp, vp, i64, vp, vp, i32, f64, f64]),
             "mfs_points_residual64": (ci, [vp, vp, i64, vp, vp]),
             "mfs_points_apply": (ci, [vp, ci, ci, vp, i64, vp]),
+            "mfs_batch_create": (ci, [ctypes.POINTER(vp), ci, vp, i32]),
+            "mfs_batch_destroy": (None, [vp]),
+            "mfs_batch_scene": (vp, [vp, i32]),
+            "mfs_batch_last_error": (ctypes.c_char_p, [vp]),
+            "mfs_batch_stats_get": (ci, [vp, ctypes.POINTER(BatchStats)]),
+            "mfs_batch_plan": (ci, [vp, vp, vp, f64, f64, i32, i32]),
+            "mfs_batch_factor_points": (ci, [vp, vp, vp, vp, vp, ci]),
+            "mfs_batch_solve_points": (ci, [vp, vp, vp, vp, i32, f64, f64, i32, ci]),
             "mfs_ctx_track_kernel": (ci, [vp, ci]),
             "mfs_ctx_tracked": (ci, [vp, ci, ctypes.POINTER(i64), ctypes.POINTER(f64), ctypes.POINTER(f64)]),
         }
@@ -356,3 +373,95 @@ def _as_2d_f32(B):
     if B.dim() == 1:
         B = B.unsqueeze(1)
     return B.to(torch.float32).contiguous()
+
+
+def _ptr_array(items):
+    arr = (ctypes.c_void_p * len(items))()
+    for i, t in enumerate(items):
+        arr[i] = None if t is None else (t.ctypes.data if isinstance(t, np.ndarray) else t.data_ptr())
+    return arr
+
+
+class Batch:
+    """Independent scenes solved together (mfs_batch): parallel host plans, one
+    merged factor launch per tile size, graph-replayed PCG iterations."""
+
+    def __init__(self, count, device=0):
+        import torch
+        self.device = torch.device("cuda", device)
+        self.count = int(count)
+        h = ctypes.c_void_p()
+        code = lib().mfs_batch_create(ctypes.byref(h), self.device.index, _stream_handle(self.device),
+                                      self.count)
+        if code != MFS_OK:
+            raise MfsError(code, "mfs_batch_create")
+        self._h = h
+        self.n = [None] * self.count
+        self.m = [None] * self.count
+        self._keep = None
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().mfs_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, code, what):
+        if code != MFS_OK:
+            raise MfsError(code, f"{what}: {lib().mfs_batch_last_error(self._h).decode()}")
+
+    @staticmethod
+    def _sync():
+        import torch
+        torch.cuda.synchronize()
+
+    def plan(self, sources, rho, lam=1.5, max_super=2048, threads=8):
+        """sources: list of (n_i, 3) host arrays (FP64)."""
+        pts = [np.ascontiguousarray(p, dtype=np.float64) for p in sources]
+        n = np.array([p.shape[0] for p in pts], dtype=np.int64)
+        self._check(lib().mfs_batch_plan(self._h, _ptr_array(pts), n.ctypes.data, float(rho), float(lam),
+                                         int(max_super), int(threads)), "mfs_batch_plan")
+        self.n = [int(v) for v in n]
+
+    def factor_points(self, xs, nxs, ys, precision=MFS_FP32):
+        """Device FP64 (m_i, 3) targets/normals and (n_i, 3) planned sources per scene."""
+        import torch
+        xs = [t.to(torch.float64).contiguous() for t in xs]
+        nxs = [t.to(torch.float64).contiguous() for t in nxs]
+        ys = [t.to(torch.float64).contiguous() for t in ys]
+        m = np.array([t.shape[0] for t in xs], dtype=np.int64)
+        self._keep = (xs, nxs, ys)
+        self._sync()
+        self._check(lib().mfs_batch_factor_points(self._h, _ptr_array(xs), _ptr_array(nxs), m.ctypes.data,
+                                                  _ptr_array(ys), int(precision)), "mfs_batch_factor_points")
+        self.m = [int(v) for v in m]
+
+    def solve_points(self, Bs, maxit=1000, tol=1e-6, inner_tol=0.5, chunk=4, use_graph=True):
+        """Bs: list of (m_i, k_i) float32 device tensors, k_i <= 16.  Returns the X_i."""
+        import torch
+        Bs = [_as_2d_f32(b) for b in Bs]
+        Xs = [torch.empty((self.n[i], b.shape[1]), dtype=torch.float32, device=b.device)
+              for i, b in enumerate(Bs)]
+        k = np.array([b.shape[1] for b in Bs], dtype=np.int64)
+        self._sync()
+        self._check(lib().mfs_batch_solve_points(self._h, _ptr_array(Bs), k.ctypes.data, _ptr_array(Xs),
+                                                 int(maxit), float(tol), float(inner_tol), int(chunk),
+                                                 int(bool(use_graph))), "mfs_batch_solve_points")
+        return Xs
+
+    def stats(self):
+        s = BatchStats()
+        self._check(lib().mfs_batch_stats_get(self._h, ctypes.byref(s)), "mfs_batch_stats_get")
+        return s
+
+    def scene_stats(self, i):
+        s = Stats()
+        code = lib().mfs_ctx_stats(lib().mfs_batch_scene(self._h, int(i)), ctypes.byref(s))
+        if code != MFS_OK:
+            raise MfsError(code, "mfs_ctx_stats")
+        return s

@@ -8,11 +8,13 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/mfs_core.h"
@@ -773,13 +775,134 @@ int factor_impl(mfs_ctx *ctx, const float *M, const PointsOp *pts, int64_t m, in
 }
 
 // Mixed-precision solve with M (pts == nullptr) or matrix-free from the points.
+// One column batch (col0 .. col0 + kval) of the mixed-precision solve, as a
+// sequence of phases so that many scenes can run them side by side:
+//   begin   -- X64 from X0, ||M^T b||^2 and the first residual, norms to host
+//   decide  -- (host, after a sync) relative residuals; is another sweep due?
+//   start   -- inner FP32 PCG on M^T M d = R64 with absolute thresholds
+//   (PCG iterations: polled by `iterate`, or replayed from a graph)
+//   end     -- X64 += d, FP64 residual, norms to host
+//   finish  -- X64 to the FP64 and/or FP32 outputs
+struct MixedRun {
+  mfs_ctx *ctx = nullptr;
+  const float *B = nullptr;
+  int64_t ldb = 0, col0 = 0, n = 0;
+  int kval = 0, K = 1;
+  int32_t maxit = 0;
+  double tol = 0, inner_tol = 0.5;
+  SolveArgs<double> a64{};
+  SolveArgs<float> a32{};
+  double *X64 = nullptr, *R64 = nullptr, *nb2 = nullptr, *res2 = nullptr, *thr = nullptr;
+  int32_t inner_used = 0;
+  int outer = 0;
+  std::vector<int32_t> iters;
+
+  int begin(mfs_ctx *c, const float *M, const PointsOp *pts, int64_t m, int64_t n_, const float *B_,
+            int64_t ldb_, int64_t col0_, int kval_, int32_t maxit_, double tol_, double inner_tol_,
+            const float *X0, const double *X0d) {
+    ctx = c; B = B_; ldb = ldb_; col0 = col0_; n = n_; kval = kval_; K = batch_width(kval);
+    maxit = maxit_; tol = tol_; inner_tol = inner_tol_;
+    iters.assign(kval, 0);
+    int rc = workspace<double>(ctx, M, pts, m, n, K, a64);
+    if (rc != MFS_OK) return rc;
+    if (pts) set_pass_work(ctx, pts, m, n, K);
+    HIPCHK(ctx->dx64.ensure(size_t(n) * K * 8), "alloc x64");
+    HIPCHK(ctx->dr64.ensure(size_t(n) * K * 8), "alloc r64");
+    HIPCHK(ctx->dnorm.ensure(4 * kMaxRhs * 8), "alloc norms");
+    X64 = ctx->dx64.as<double>();
+    R64 = ctx->dr64.as<double>();
+    nb2 = ctx->dnorm.as<double>();
+    res2 = nb2 + kMaxRhs;
+    thr = nb2 + 2 * kMaxRhs;
+    // FP32 inner workspace; its vectors live in the FP64 vector buffer
+    const void *lv = nullptr, *ltv = nullptr;
+    rc = values_for(ctx, MFS_FP32, &lv, &ltv);
+    if (rc != MFS_OK) return rc;
+    a32 = SolveArgs<float>{};
+    a32.M = M; a32.m = m; a32.n = n; a32.K = K;
+    a32.P = a64.P; a32.Pinv = a64.Pinv; a32.sptr = a64.sptr; a32.sind = a64.sind;
+    a32.tptr = a64.tptr; a32.tcol = a64.tcol;
+    a32.lval = static_cast<const float *>(lv);
+    a32.ltval = static_cast<const float *>(ltv);
+    a32.row_blocks = a64.row_blocks;
+    a32.pts = a64.pts;
+    float *base = reinterpret_cast<float *>(a64.x);
+    const size_t nv = size_t(n) * K;
+    a32.x = base; a32.r = base + nv; a32.p = base + 2 * nv; a32.q = base + 3 * nv;
+    a32.z = base + 4 * nv; a32.t = base + 5 * nv; a32.pv = base + 6 * nv; a32.y = base + 7 * nv;
+    a32.part = a64.part; a32.dpart = a64.dpart; a32.st = a64.st;
+
+    const unsigned gn = unsigned((n + kThreads - 1) / kThreads);
+    if (X0d != nullptr)
+      k_x64_from_f64<<<gn, kThreads, 0, ctx->stream>>>(n, X0d, ldb, col0, kval, K, X64);
+    else
+      k_x64_from_f32<<<gn, kThreads, 0, ctx->stream>>>(n, X0, ldb, col0, kval, K, X64);
+    // ||M^T b||^2 and, for a zero start, the first residual.
+    HIPCHK(residual64(ctx->stream, a64, B, ldb, col0, kval, nullptr, R64, nb2, track_cb, ctx), "residual");
+    if (X0 != nullptr || X0d != nullptr)
+      HIPCHK(residual64(ctx->stream, a64, B, ldb, col0, kval, X64, R64, res2, track_cb, ctx), "residual");
+    else
+      HIPCHK(hipMemcpyAsync(res2, nb2, kMaxRhs * 8, hipMemcpyDeviceToDevice, ctx->stream), "copy");
+    HIPCHK(hipMemcpyAsync(ctx->hnorm, nb2, 2 * kMaxRhs * 8, hipMemcpyDeviceToHost, ctx->stream), "copy");
+    return MFS_OK;
+  }
+
+  // After ctx->stream has been synchronized.
+  bool decide() {
+    bool done = true;
+    for (int r = 0; r < kval; ++r) {
+      const double b2 = ctx->hnorm[r], e2 = ctx->hnorm[kMaxRhs + r];
+      const double rel = b2 > 0 ? std::sqrt(e2 / b2) : 0.0;
+      if (col0 + r < 64) ctx->stats.relres[col0 + r] = rel;
+      if (b2 >= 1e-16 && !(rel <= tol)) done = false;
+    }
+    return !(done || inner_used >= maxit || outer >= 30);
+  }
+
+  int start() {
+    // Each sweep aims straight at inner_tol * tol * ||M^T b|| (absolute), so
+    // one sweep normally meets the tolerance and the FP64 residual check
+    // only confirms it; FP32 round-off left above it is removed by the next.
+    HIPCHK(inner_thresholds(ctx->stream, nb2, inner_tol * tol, K, thr), "inner thresholds");
+    HIPCHK(pcg_init<float>(ctx->stream, a32, nullptr, 0, 0, 1, K, 0.0, maxit - inner_used, R64, thr),
+           "inner init");
+    return MFS_OK;
+  }
+
+  // After ctx->hstate holds the finished inner PCG's state.
+  int end() {
+    HIPCHK(pcg_finish<float>(ctx->stream, a32, nullptr, 0, 0, K, X64), "inner finish");
+    int32_t used = 0;
+    for (int r = 0; r < kval; ++r) {
+      iters[r] += ctx->hstate->iters[r];
+      used = std::max(used, ctx->hstate->iters[r]);
+    }
+    inner_used += std::max(used, 1);
+    ++outer;
+    HIPCHK(residual64(ctx->stream, a64, B, ldb, col0, kval, X64, R64, res2, track_cb, ctx), "residual");
+    HIPCHK(hipMemcpyAsync(ctx->hnorm, nb2, 2 * kMaxRhs * 8, hipMemcpyDeviceToHost, ctx->stream), "copy");
+    return MFS_OK;
+  }
+
+  int finish(float *X, double *Xd) {
+    const unsigned gn = unsigned((n + kThreads - 1) / kThreads);
+    if (Xd != nullptr)
+      k_x64_to_f64<<<gn, kThreads, 0, ctx->stream>>>(n, X64, K, Xd, ldb, col0, kval);
+    if (X != nullptr)
+      k_x64_to_f32<<<gn, kThreads, 0, ctx->stream>>>(n, X64, K, X, ldb, col0, kval);
+    HIPCHK(hipGetLastError(), "output");
+    for (int r = 0; r < kval; ++r)
+      if (col0 + r < 64) ctx->stats.iters[col0 + r] = iters[r];
+    return MFS_OK;
+  }
+};
+
 // X0 / X are FP32; X0d / Xd (used instead when given) are FP64 -- for
 // tolerances below what an FP32 copy of the strengths can hold.
 int solve_mixed_impl(mfs_ctx *ctx, const float *M, const PointsOp *pts, int64_t m, int64_t n,
                      const float *B, int64_t k, int rhs_space, const float *X0, float *X,
                      int32_t maxit, double tol, double inner_tol, const double *X0d = nullptr,
                      double *Xd = nullptr) {
-  int rc = MFS_OK;
   if (!ctx->has_factor) return ctx->fail(MFS_ERR_STATE, "no factor: factor before solving");
   if (B == nullptr || (X == nullptr && Xd == nullptr) || k <= 0 || rhs_space != 0)
     return ctx->fail(MFS_ERR_ARG, "solve_mixed: boundary right-hand sides only");
@@ -790,93 +913,18 @@ int solve_mixed_impl(mfs_ctx *ctx, const float *M, const PointsOp *pts, int64_t 
   int outer_total = 0;
   for (int64_t col0 = 0; col0 < k; col0 += kMaxRhs) {
     const int kval = int(std::min<int64_t>(kMaxRhs, k - col0));
-    const int K = batch_width(kval);
-    SolveArgs<double> a64{};
-    rc = workspace<double>(ctx, M, pts, m, n, K, a64);
+    MixedRun run;
+    int rc = run.begin(ctx, M, pts, m, n, B, k, col0, kval, maxit, tol, inner_tol, X0, X0d);
     if (rc != MFS_OK) return rc;
-    if (pts) set_pass_work(ctx, pts, m, n, K);
-    if (rc != MFS_OK) return rc;
-    // The FP32 workspace shares the vector buffers: lay it out after the FP64 one.
-    HIPCHK(ctx->dx64.ensure(size_t(n) * K * 8), "alloc x64");
-    HIPCHK(ctx->dr64.ensure(size_t(n) * K * 8), "alloc r64");
-    HIPCHK(ctx->dnorm.ensure(4 * kMaxRhs * 8), "alloc norms");
-    double *X64 = ctx->dx64.as<double>();
-    double *R64 = ctx->dr64.as<double>();
-    double *nb2 = ctx->dnorm.as<double>();
-    double *res2 = nb2 + kMaxRhs;
-    double *thr = nb2 + 2 * kMaxRhs;
-    const unsigned gn = unsigned((n + kThreads - 1) / kThreads);
-    if (X0d != nullptr)
-      k_x64_from_f64<<<gn, kThreads, 0, ctx->stream>>>(n, X0d, k, col0, kval, K, X64);
-    else
-      k_x64_from_f32<<<gn, kThreads, 0, ctx->stream>>>(n, X0, k, col0, kval, K, X64);
-    const bool warm = X0 != nullptr || X0d != nullptr;
-    // ||M^T b||^2 and, for a zero start, the first residual.
-    HIPCHK(residual64(ctx->stream, a64, B, k, col0, kval, nullptr, R64, nb2, track_cb, ctx), "residual");
-    if (warm)
-      HIPCHK(residual64(ctx->stream, a64, B, k, col0, kval, X64, R64, res2, track_cb, ctx), "residual");
-    else
-      HIPCHK(hipMemcpyAsync(res2, nb2, kMaxRhs * 8, hipMemcpyDeviceToDevice, ctx->stream), "copy");
-    int32_t inner_used = 0;
-    int outer = 0;
-    std::vector<int32_t> iters(kval, 0);
     for (;;) {
-      HIPCHK(hipMemcpyAsync(ctx->hnorm, nb2, 2 * kMaxRhs * 8, hipMemcpyDeviceToHost, ctx->stream), "copy");
       HIPCHK(hipStreamSynchronize(ctx->stream), "sync");
-      bool done = true;
-      for (int r = 0; r < kval; ++r) {
-        const double b2 = ctx->hnorm[r], e2 = ctx->hnorm[kMaxRhs + r];
-        const double rel = b2 > 0 ? std::sqrt(e2 / b2) : 0.0;
-        if (col0 + r < 64) ctx->stats.relres[col0 + r] = rel;
-        if (b2 >= 1e-16 && !(rel <= tol)) done = false;
-      }
-      if (done || inner_used >= maxit || outer >= 30) break;
-      // Inner FP32 PCG on M^T M d = R64 (source-space right-hand side).
-      SolveArgs<float> a32{};
-      {
-        const void *lv = nullptr, *ltv = nullptr;
-        rc = values_for(ctx, MFS_FP32, &lv, &ltv);
-        if (rc != MFS_OK) return rc;
-        a32.M = M; a32.m = m; a32.n = n; a32.K = K;
-        a32.P = a64.P; a32.Pinv = a64.Pinv; a32.sptr = a64.sptr; a32.sind = a64.sind;
-        a32.tptr = a64.tptr; a32.tcol = a64.tcol;
-        a32.lval = static_cast<const float *>(lv);
-        a32.ltval = static_cast<const float *>(ltv);
-        a32.row_blocks = a64.row_blocks;
-        a32.pts = a64.pts;
-        // FP32 vectors live in the FP64 vector buffer (same byte size suffices).
-        float *base = reinterpret_cast<float *>(a64.x);
-        const size_t nv = size_t(n) * K;
-        a32.x = base; a32.r = base + nv; a32.p = base + 2 * nv; a32.q = base + 3 * nv;
-        a32.z = base + 4 * nv; a32.t = base + 5 * nv; a32.pv = base + 6 * nv; a32.y = base + 7 * nv;
-        a32.part = a64.part; a32.dpart = a64.dpart; a32.st = a64.st;
-      }
-      // Each sweep aims straight at inner_tol * tol * ||M^T b|| (absolute), so
-      // one sweep normally meets the tolerance and the FP64 residual check
-      // only confirms it; FP32 round-off left above it is removed by the next.
-      HIPCHK(inner_thresholds(ctx->stream, nb2, inner_tol * tol, K, thr), "inner thresholds");
-      HIPCHK(pcg_init<float>(ctx->stream, a32, nullptr, 0, 0, 1, K, 0.0, maxit - inner_used, R64, thr),
-             "inner init");
-      rc = iterate<float>(ctx, a32, maxit - inner_used);
-      if (rc != MFS_OK) return rc;
-      HIPCHK(pcg_finish<float>(ctx->stream, a32, nullptr, 0, 0, K, X64), "inner finish");
-      int32_t used = 0;
-      for (int r = 0; r < kval; ++r) {
-        iters[r] += ctx->hstate->iters[r];
-        used = std::max(used, ctx->hstate->iters[r]);
-      }
-      inner_used += std::max(used, 1);
-      ++outer;
-      HIPCHK(residual64(ctx->stream, a64, B, k, col0, kval, X64, R64, res2, track_cb, ctx), "residual");
+      if (!run.decide()) break;
+      if ((rc = run.start()) != MFS_OK) return rc;
+      if ((rc = iterate<float>(ctx, run.a32, maxit - run.inner_used)) != MFS_OK) return rc;
+      if ((rc = run.end()) != MFS_OK) return rc;
     }
-    if (Xd != nullptr)
-      k_x64_to_f64<<<gn, kThreads, 0, ctx->stream>>>(n, X64, K, Xd, k, col0, kval);
-    if (X != nullptr)
-      k_x64_to_f32<<<gn, kThreads, 0, ctx->stream>>>(n, X64, K, X, k, col0, kval);
-    HIPCHK(hipGetLastError(), "output");
-    for (int r = 0; r < kval; ++r)
-      if (col0 + r < 64) ctx->stats.iters[col0 + r] = iters[r];
-    outer_total = std::max(outer_total, outer);
+    if ((rc = run.finish(X, Xd)) != MFS_OK) return rc;
+    outer_total = std::max(outer_total, run.outer);
   }
   ctx->stats.outer_iters = outer_total;
   ctx->stats.solve_ms = phase_end(ctx);
@@ -1228,6 +1276,389 @@ int mfs_ctx_tracked(mfs_ctx *ctx, int kernel_class, int64_t *launches, double *t
   if (launches) *launches = t.launches;
   if (total_ms) *total_ms = t.ms;
   if (bytes) *bytes = t.bytes;
+  return MFS_OK;
+}
+
+}  // extern "C"
+
+// ============================================================================
+// Batches of independent scenes (points form).
+//
+// The reference solves one scene per call.  A batch keeps one context per
+// scene, each on a stream of its own, and
+//   * plans the scenes on parallel host threads (the ordering is serial per
+//     scene, the scenes are independent);
+//   * factors the supernodes of all scenes together: one launch per tile-size
+//     bucket over the (scene, supernode) items of every scene;
+//   * runs the PCG iterations of all scenes from one captured hipGraph (a
+//     branch per scene, `chunk` iterations each, ending in a gather of every
+//     scene's active count), replayed until no scene iterates; converged
+//     scenes' kernels exit at once, so one graph serves every sweep.
+// ============================================================================
+
+namespace {
+
+__global__ void k_gather_active(PcgState *const *states, int32_t count, int32_t *out) {
+  const int32_t i = int32_t(blockIdx.x) * kThreads + threadIdx.x;
+  if (i < count) out[i] = states[i]->n_active;
+}
+
+}  // namespace
+
+struct mfs_batch {
+  int device = 0;
+  hipStream_t caller = nullptr;  // work queued there before a call is waited for
+  hipStream_t root = nullptr;    // the batch's own stream (graphs are captured on it)
+  std::string err;
+  std::vector<mfs_ctx *> scenes;
+  std::vector<hipStream_t> streams;
+  std::vector<hipEvent_t> ev;     // one per scene (fork/join)
+  hipEvent_t root_ev = nullptr;
+  hipStream_t side[mfs_ctx::kSide] = {};
+  hipEvent_t side_ev[mfs_ctx::kSide] = {};
+  DevBuf dtable, ditems, dthg, dstates, dactive;
+  std::vector<FactorScene> table;
+  std::vector<FactorItem> items;
+  std::vector<PcgState *> states;
+  int32_t *hactive = nullptr;  // pinned
+  mfs_batch_stats stats{};
+
+  int fail(int code, const std::string &msg) {
+    err = msg;
+    return code;
+  }
+};
+
+namespace {
+
+#define BHIPCHK(call, what)                                                  \
+  do {                                                                       \
+    hipError_t e_ = (call);                                                  \
+    if (e_ != hipSuccess) return b->fail(MFS_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+// root waits for every scene stream
+int join_scenes(mfs_batch *b) {
+  for (size_t i = 0; i < b->scenes.size(); ++i) {
+    BHIPCHK(hipEventRecord(b->ev[i], b->streams[i]), "record");
+    BHIPCHK(hipStreamWaitEvent(b->root, b->ev[i], 0), "wait");
+  }
+  return MFS_OK;
+}
+
+// every scene stream waits for root
+int fork_scenes(mfs_batch *b) {
+  BHIPCHK(hipEventRecord(b->root_ev, b->root), "record");
+  for (size_t i = 0; i < b->scenes.size(); ++i)
+    BHIPCHK(hipStreamWaitEvent(b->streams[i], b->root_ev, 0), "wait");
+  return MFS_OK;
+}
+
+int scene_error(mfs_batch *b, size_t i, int rc) {
+  return b->fail(rc, "scene " + std::to_string(i) + ": " + b->scenes[i]->err);
+}
+
+}  // namespace
+
+extern "C" {
+
+int mfs_batch_create(mfs_batch **out, int device, void *stream, int32_t count) {
+  if (out == nullptr || count <= 0) return MFS_ERR_ARG;
+  *out = nullptr;
+  mfs_batch *b = new (std::nothrow) mfs_batch();
+  if (b == nullptr) return MFS_ERR_ALLOC;
+  b->device = device;
+  b->caller = static_cast<hipStream_t>(stream);
+  bool ok = hipSetDevice(device) == hipSuccess &&
+            hipStreamCreateWithFlags(&b->root, hipStreamNonBlocking) == hipSuccess &&
+            hipEventCreateWithFlags(&b->root_ev, hipEventDisableTiming) == hipSuccess &&
+            hipHostMalloc(reinterpret_cast<void **>(&b->hactive), size_t(count) * 4) == hipSuccess;
+  for (int i = 0; ok && i < mfs_ctx::kSide; ++i)
+    ok = hipStreamCreateWithFlags(&b->side[i], hipStreamNonBlocking) == hipSuccess &&
+         hipEventCreateWithFlags(&b->side_ev[i], hipEventDisableTiming) == hipSuccess;
+  for (int32_t i = 0; ok && i < count; ++i) {
+    hipStream_t s = nullptr;
+    hipEvent_t e = nullptr;
+    mfs_ctx *c = nullptr;
+    ok = hipStreamCreateWithFlags(&s, hipStreamNonBlocking) == hipSuccess;
+    if (ok) b->streams.push_back(s);
+    ok = ok && hipEventCreateWithFlags(&e, hipEventDisableTiming) == hipSuccess;
+    if (ok) b->ev.push_back(e);
+    ok = ok && mfs_ctx_create(&c, device, s) == MFS_OK;
+    if (ok) b->scenes.push_back(c);
+  }
+  if (!ok) {
+    mfs_batch_destroy(b);
+    return MFS_ERR_HIP;
+  }
+  *out = b;
+  return MFS_OK;
+}
+
+void mfs_batch_destroy(mfs_batch *b) {
+  if (b == nullptr) return;
+  (void)hipSetDevice(b->device);
+  for (hipStream_t s : b->streams) (void)hipStreamSynchronize(s);
+  for (mfs_ctx *c : b->scenes) mfs_ctx_destroy(c);
+  for (hipStream_t s : b->streams) (void)hipStreamDestroy(s);
+  for (hipEvent_t e : b->ev) (void)hipEventDestroy(e);
+  for (int i = 0; i < mfs_ctx::kSide; ++i) {
+    if (b->side[i]) (void)hipStreamDestroy(b->side[i]);
+    if (b->side_ev[i]) (void)hipEventDestroy(b->side_ev[i]);
+  }
+  if (b->root_ev) (void)hipEventDestroy(b->root_ev);
+  if (b->root) (void)hipStreamDestroy(b->root);
+  if (b->hactive) (void)hipHostFree(b->hactive);
+  for (DevBuf *d : {&b->dtable, &b->ditems, &b->dthg, &b->dstates, &b->dactive}) d->release();
+  delete b;
+}
+
+mfs_ctx *mfs_batch_scene(mfs_batch *b, int32_t i) {
+  if (b == nullptr || i < 0 || size_t(i) >= b->scenes.size()) return nullptr;
+  return b->scenes[size_t(i)];
+}
+
+const char *mfs_batch_last_error(const mfs_batch *b) {
+  return b == nullptr ? "null batch" : b->err.c_str();
+}
+
+int mfs_batch_stats_get(const mfs_batch *b, mfs_batch_stats *out) {
+  if (b == nullptr || out == nullptr) return MFS_ERR_ARG;
+  *out = b->stats;
+  return MFS_OK;
+}
+
+int mfs_batch_plan(mfs_batch *b, const double *const *xyz, const int64_t *n, double rho,
+                   double lambda, int32_t max_super, int32_t threads) {
+  if (b == nullptr || xyz == nullptr || n == nullptr) return MFS_ERR_ARG;
+  BHIPCHK(hipStreamSynchronize(b->caller), "sync caller");
+  const auto t0 = std::chrono::steady_clock::now();
+  const size_t count = b->scenes.size();
+  const int nt = std::max<int>(1, std::min<int>(threads > 0 ? threads : 1, int(count)));
+  std::atomic<size_t> next{0};
+  std::vector<int> rcs(count, MFS_OK);
+  auto work = [&]() {
+    for (;;) {
+      const size_t i = next.fetch_add(1);
+      if (i >= count) return;
+      mfs_ctx *c = b->scenes[i];
+      if (xyz[i] == nullptr) {
+        rcs[i] = c->fail(MFS_ERR_ARG, "plan: null points");
+        continue;
+      }
+      std::vector<double> pts(xyz[i], xyz[i] + size_t(std::max<int64_t>(n[i], 0)) * 3);
+      rcs[i] = plan_impl(c, std::move(pts), n[i], rho, lambda, max_super);
+    }
+  };
+  std::vector<std::thread> pool;
+  for (int t = 1; t < nt; ++t) pool.emplace_back(work);
+  work();
+  for (auto &t : pool) t.join();
+  for (size_t i = 0; i < count; ++i)
+    if (rcs[i] != MFS_OK) return scene_error(b, i, rcs[i]);
+  b->stats.plan_ms = ms_since(t0);
+  return MFS_OK;
+}
+
+int mfs_batch_factor_points(mfs_batch *b, const double *const *x, const double *const *nx,
+                            const int64_t *m, const double *const *y, int precision) {
+  if (b == nullptr || x == nullptr || nx == nullptr || m == nullptr || y == nullptr)
+    return MFS_ERR_ARG;
+  if (precision != MFS_FP64 && precision != MFS_FP32)
+    return b->fail(MFS_ERR_ARG, "factor: precision must be MFS_FP64 or MFS_FP32");
+  BHIPCHK(hipSetDevice(b->device), "hipSetDevice");
+  BHIPCHK(hipStreamSynchronize(b->caller), "sync caller");
+  const auto t0 = std::chrono::steady_clock::now();
+  const size_t count = b->scenes.size();
+  // 1. per scene: pack the points, clear statuses, scene record
+  for (size_t i = 0; i < count; ++i) {
+    mfs_ctx *c = b->scenes[i];
+    if (!c->has_plan) return b->fail(MFS_ERR_STATE, "scene " + std::to_string(i) + ": no plan");
+    if (x[i] == nullptr || nx[i] == nullptr || y[i] == nullptr || m[i] <= 0 || m[i] > INT32_MAX)
+      return b->fail(MFS_ERR_ARG, "scene " + std::to_string(i) + ": bad points");
+    const int64_t ni = c->n;
+    c->has_points = false;
+    BHIPCHK(c->dt32.ensure(size_t(m[i]) * kTgt32 * 4), "alloc points");
+    BHIPCHK(c->dt64.ensure(size_t(m[i]) * kTgt64 * 8), "alloc points");
+    BHIPCHK(c->ds32.ensure(size_t(ni) * kSrc32 * 4), "alloc points");
+    BHIPCHK(c->ds64.ensure(size_t(ni) * kSrc64 * 8), "alloc points");
+    BHIPCHK(pack_points(c->stream, x[i], nx[i], y[i], m[i], ni, c->dt32.as<float>(),
+                        c->dt64.as<double>(), c->ds32.as<float>(), c->ds64.as<double>()),
+            "pack points");
+    c->pts = PointsOp{c->dt32.as<float>(), c->ds32.as<float>(), c->dt64.as<double>(),
+                      c->ds64.as<double>(), m[i], ni};
+    c->has_points = true;
+    const int rc = factor_prepare(c, nullptr, &c->pts, m[i], ni, precision);
+    if (rc != MFS_OK) return scene_error(b, i, rc);
+  }
+  // 2. merged buckets: every scene's supernodes of one tile size in one launch
+  b->table.resize(count);
+  for (size_t i = 0; i < count; ++i) b->table[i] = b->scenes[i]->fscene;
+  b->items.clear();
+  std::vector<Bucket> buckets;
+  size_t scratch_bytes = 8;
+  std::vector<size_t> scratch_off;
+  for (int t : kTileSizes) {
+    const int32_t off = int32_t(b->items.size());
+    for (size_t i = 0; i < count; ++i)
+      for (const Bucket &bk : b->scenes[i]->buckets)
+        if (bk.tiles == t)
+          for (int32_t q = 0; q < bk.count; ++q)
+            b->items.push_back({int32_t(i), b->scenes[i]->list[size_t(bk.offset + q)]});
+    const int32_t cnt = int32_t(b->items.size()) - off;
+    if (cnt == 0) continue;
+    buckets.push_back({t, off, cnt});
+    scratch_off.push_back(scratch_bytes);
+    scratch_bytes += size_t(cnt) * size_t(factor_scratch_elems(t)) * 8;
+  }
+  BHIPCHK(b->dtable.ensure(count * sizeof(FactorScene)), "alloc");
+  BHIPCHK(b->ditems.ensure(std::max<size_t>(b->items.size(), 1) * sizeof(FactorItem)), "alloc");
+  BHIPCHK(b->dthg.ensure(scratch_bytes), "alloc factor scratch");
+  if (join_scenes(b) != MFS_OK) return MFS_ERR_HIP;
+  BHIPCHK(hipMemcpyAsync(b->dtable.p, b->table.data(), count * sizeof(FactorScene),
+                         hipMemcpyHostToDevice, b->root),
+          "upload");
+  if (!b->items.empty())
+    BHIPCHK(hipMemcpyAsync(b->ditems.p, b->items.data(), b->items.size() * sizeof(FactorItem),
+                           hipMemcpyHostToDevice, b->root),
+            "upload");
+  BHIPCHK(hipEventRecord(b->root_ev, b->root), "fork");
+  const int nside = std::min<int>(mfs_ctx::kSide, int(buckets.size()));
+  for (int i = 0; i < nside; ++i) BHIPCHK(hipStreamWaitEvent(b->side[i], b->root_ev, 0), "fork");
+  for (size_t bi = 0; bi < buckets.size(); ++bi)
+    BHIPCHK(launch_super_factor(b->side[bi % mfs_ctx::kSide], buckets[bi].tiles,
+                                precision == MFS_FP64 ? 0 : 1, true, b->dtable.as<FactorScene>(),
+                                b->ditems.as<FactorItem>() + buckets[bi].offset, buckets[bi].count,
+                                b->dthg.as<char>() + scratch_off[bi]),
+            "supernode factor");
+  for (int i = 0; i < nside; ++i) {
+    BHIPCHK(hipEventRecord(b->side_ev[i], b->side[i]), "join");
+    BHIPCHK(hipStreamWaitEvent(b->root, b->side_ev[i], 0), "join");
+  }
+  if (fork_scenes(b) != MFS_OK) return MFS_ERR_HIP;
+  // 3. per scene: statuses, FP64 retries, transposed values
+  for (size_t i = 0; i < count; ++i) {
+    const int rc = factor_finish(b->scenes[i], true, precision);
+    if (rc != MFS_OK) return scene_error(b, i, rc);
+  }
+  b->stats.factor_ms = ms_since(t0);
+  b->stats.factor_launches = int32_t(buckets.size());
+  b->stats.factor_items = int32_t(b->items.size());
+  return MFS_OK;
+}
+
+int mfs_batch_solve_points(mfs_batch *b, const float *const *B, const int64_t *k, float *const *X,
+                           int32_t maxit, double tol, double inner_tol, int32_t chunk,
+                           int use_graph) {
+  if (b == nullptr || B == nullptr || k == nullptr || X == nullptr || chunk <= 0) return MFS_ERR_ARG;
+  if (!(inner_tol > 0) || !(inner_tol <= 1)) inner_tol = 0.5;
+  BHIPCHK(hipSetDevice(b->device), "hipSetDevice");
+  BHIPCHK(hipStreamSynchronize(b->caller), "sync caller");
+  const auto t0 = std::chrono::steady_clock::now();
+  const size_t count = b->scenes.size();
+  std::vector<MixedRun> runs(count);
+  b->states.resize(count);
+  for (size_t i = 0; i < count; ++i) {
+    mfs_ctx *c = b->scenes[i];
+    if (!c->has_points || !c->has_factor)
+      return b->fail(MFS_ERR_STATE, "scene " + std::to_string(i) + ": factor the points first");
+    if (B[i] == nullptr || X[i] == nullptr || k[i] <= 0 || k[i] > kMaxRhs)
+      return b->fail(MFS_ERR_ARG, "scene " + std::to_string(i) + ": 1..16 right-hand sides");
+    c->track_mask = 0;
+    const int rc = runs[i].begin(c, nullptr, &c->pts, c->pts.m, c->n, B[i], k[i], 0, int(k[i]),
+                                 maxit, tol, inner_tol, nullptr, nullptr);
+    if (rc != MFS_OK) return scene_error(b, i, rc);
+    b->states[i] = runs[i].a32.st;
+  }
+  BHIPCHK(b->dstates.ensure(count * sizeof(PcgState *)), "alloc");
+  BHIPCHK(b->dactive.ensure(count * 4), "alloc");
+  BHIPCHK(hipMemcpyAsync(b->dstates.p, b->states.data(), count * sizeof(PcgState *),
+                         hipMemcpyHostToDevice, b->root),
+          "upload");
+  // The iteration graph: a branch per scene on the scene's stream.
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  auto launch_chunk = [&]() -> int {
+    if (fork_scenes(b) != MFS_OK) return MFS_ERR_HIP;
+    for (size_t i = 0; i < count; ++i)
+      for (int32_t c = 0; c < chunk; ++c)
+        BHIPCHK(pcg_iteration<float>(b->streams[i], runs[i].a32, nullptr, nullptr), "pcg iteration");
+    if (join_scenes(b) != MFS_OK) return MFS_ERR_HIP;
+    k_gather_active<<<unsigned((count + kThreads - 1) / kThreads), kThreads, 0, b->root>>>(
+        b->dstates.as<PcgState *>(), int32_t(count), b->dactive.as<int32_t>());
+    BHIPCHK(hipGetLastError(), "gather");
+    return MFS_OK;
+  };
+  if (use_graph) {
+    if (join_scenes(b) != MFS_OK) return MFS_ERR_HIP;  // capture starts after all begin() work
+    BHIPCHK(hipStreamSynchronize(b->root), "sync");
+    BHIPCHK(hipStreamBeginCapture(b->root, hipStreamCaptureModeThreadLocal), "begin capture");
+    const int rc = launch_chunk();
+    hipError_t e = hipStreamEndCapture(b->root, &graph);
+    if (rc != MFS_OK) return rc;
+    BHIPCHK(e, "end capture");
+    size_t nodes = 0;
+    BHIPCHK(hipGraphGetNodes(graph, nullptr, &nodes), "graph nodes");
+    b->stats.graph_nodes = int32_t(nodes);
+    BHIPCHK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0), "instantiate");
+  }
+  int32_t replays = 0, sweeps = 0;
+  int rc = MFS_OK;
+  for (;;) {
+    for (size_t i = 0; i < count; ++i) BHIPCHK(hipStreamSynchronize(b->streams[i]), "sync");
+    std::vector<size_t> active;
+    for (size_t i = 0; i < count; ++i)
+      if (runs[i].decide()) {
+        if ((rc = runs[i].start()) != MFS_OK) break;
+        active.push_back(i);
+      }
+    if (rc != MFS_OK || active.empty()) break;
+    ++sweeps;
+    int32_t budget = 0;
+    for (size_t i : active) budget = std::max(budget, maxit - runs[i].inner_used);
+    for (int32_t it = 0;; it += chunk) {
+      if (it > budget + 64) {
+        rc = b->fail(MFS_ERR_STATE, "batch pcg: iteration bookkeeping diverged");
+        break;
+      }
+      if (use_graph) {
+        if (join_scenes(b) != MFS_OK) { rc = MFS_ERR_HIP; break; }
+        hipError_t e = hipGraphLaunch(exec, b->root);
+        if (e != hipSuccess) { rc = b->fail(MFS_ERR_HIP, std::string("graph launch: ") + hipGetErrorString(e)); break; }
+      } else if ((rc = launch_chunk()) != MFS_OK) {
+        break;
+      }
+      ++replays;
+      hipError_t e = hipMemcpyAsync(b->hactive, b->dactive.p, count * 4, hipMemcpyDeviceToHost, b->root);
+      if (e == hipSuccess) e = hipStreamSynchronize(b->root);
+      if (e != hipSuccess) { rc = b->fail(MFS_ERR_HIP, std::string("poll: ") + hipGetErrorString(e)); break; }
+      bool any = false;
+      for (size_t i : active) any = any || b->hactive[i] != 0;
+      if (!any) break;
+    }
+    if (rc != MFS_OK) break;
+    if (fork_scenes(b) != MFS_OK) { rc = MFS_ERR_HIP; break; }
+    for (size_t i : active) {
+      mfs_ctx *c = b->scenes[i];
+      hipError_t e = hipMemcpyAsync(c->hstate, runs[i].a32.st, sizeof(PcgState), hipMemcpyDeviceToHost,
+                                    c->stream);
+      if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+      if (e != hipSuccess) { rc = b->fail(MFS_ERR_HIP, std::string("state: ") + hipGetErrorString(e)); break; }
+      if ((rc = runs[i].end()) != MFS_OK) { rc = scene_error(b, i, rc); break; }
+    }
+    if (rc != MFS_OK) break;
+  }
+  if (exec) (void)hipGraphExecDestroy(exec);
+  if (graph) (void)hipGraphDestroy(graph);
+  if (rc != MFS_OK) return rc;
+  for (size_t i = 0; i < count; ++i) {
+    if ((rc = runs[i].finish(X[i], nullptr)) != MFS_OK) return scene_error(b, i, rc);
+    b->scenes[i]->stats.outer_iters = runs[i].outer;
+  }
+  for (size_t i = 0; i < count; ++i) BHIPCHK(hipStreamSynchronize(b->streams[i]), "sync");
+  b->stats.solve_ms = ms_since(t0);
+  b->stats.graph_replays = replays;
+  b->stats.sweeps = sweeps;
   return MFS_OK;
 }
 

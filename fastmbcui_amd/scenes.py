@@ -270,13 +270,14 @@ class RigidScene:
 
 
 def mixed_bodies(seed: int = 0, n_bodies: int = 128, pts_per_body: int = 2048, box: float = 32.0,
-                 speed: float = 0.5, edge_radius: float = 0.1) -> RigidScene:
-    """A third each of coins (radius 1, thickness 0.1, sources 0.02 inside),
-    ellipsoids (semi-axes ~ U(0.5, 1.5), Fibonacci points, sources 0.1 min axis
-    inside) and rounded boxes (half-extents ~ U(0.5, 1.5), edge radius 0.1,
-    sources 0.05 inside), in that rotation by body index; v, w ~ N(0, speed^2)."""
+                 speed: float = 0.5, edge_radius: float = 0.1, kinds=(0, 1, 2)) -> RigidScene:
+    """Coins (radius 1, thickness 0.1, sources 0.02 inside), ellipsoids
+    (semi-axes ~ U(0.5, 1.5), Fibonacci points, sources 0.1 min axis inside) and
+    rounded boxes (half-extents ~ U(0.5, 1.5), edge radius 0.1, sources 0.05
+    inside) -- kinds 0, 1, 2, taken in turn from `kinds` by body index (a third
+    of each by default); v, w ~ N(0, speed^2)."""
     rng = np.random.default_rng(seed)
-    shape = np.arange(n_bodies) % 3
+    shape = np.asarray(kinds)[np.arange(n_bodies) % len(kinds)]
     dims = rng.uniform(0.5, 1.5, (n_bodies, 3))
     radii = np.where(shape == 0, np.hypot(1.0, 0.05),
                      np.where(shape == 1, dims.max(axis=1), np.linalg.norm(dims, axis=1)))
@@ -307,6 +308,22 @@ def mixed_bodies(seed: int = 0, n_bodies: int = 128, pts_per_body: int = 2048, b
                       np.concatenate(body), np.array(rots), centres, v, w,
                       dict(seed=seed, n_bodies=n_bodies, pts_per_body=pts_per_body, box=box,
                            speed=speed, shapes=shape))
+
+
+def batch_scene(seed: int, bodies=(2, 32), points=(256, 2048)) -> Scene:
+    """One scene of the uneven batch: body count and points per body drawn
+    log-uniform from the given ranges (seeded), coins and ellipsoids in turn,
+    one right-hand side of a rigid motion with v, w ~ N(0, 1) per body."""
+    rng = np.random.default_rng(seed)
+    nb = int(round(np.exp(rng.uniform(np.log(bodies[0]), np.log(bodies[1])))))
+    ppb = int(round(np.exp(rng.uniform(np.log(points[0]), np.log(points[1])))))
+    box = 3.5 * nb ** (1.0 / 3.0) + 3.0
+    rs = mixed_bodies(seed=seed + 1_000_003, n_bodies=nb, pts_per_body=ppb, box=box, speed=1.0,
+                      kinds=(0, 1))
+    sc = rs.scene()
+    sc.name = f"batch{seed}"
+    sc.meta.update(n_bodies=nb, pts_per_body=ppb)
+    return sc
 
 
 def kernel_matrix(x: np.ndarray, nx: np.ndarray, y: np.ndarray) -> np.ndarray:

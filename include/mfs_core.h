@@ -178,6 +178,44 @@ int mfs_ctx_track_kernel(mfs_ctx *ctx, int mask);
 int mfs_ctx_tracked(mfs_ctx *ctx, int kernel_class, int64_t *launches, double *total_ms,
                     double *bytes);
 
+/* ---- Batches of independent scenes (points form).  No counterpart in the
+ * reference, which solves one scene per call (mfs_torch_ops.cu:21-197); a
+ * batch runs the same per-scene algorithm for many scenes at once.  Each
+ * scene has its own context and stream (mfs_batch_scene gives the context, for
+ * mfs_ctx_stats / mfs_plan_export / mfs_factor_export); the batch's work runs
+ * on streams of its own.  Every batch call first waits for the work queued on
+ * `stream` (given at creation) and returns when its own work is done. */
+typedef struct mfs_batch mfs_batch;
+typedef struct mfs_batch_stats {
+    double plan_ms, factor_ms, solve_ms;
+    int32_t factor_launches;    /* merged bucket launches */
+    int32_t factor_items;       /* supernodes of all scenes */
+    int32_t graph_nodes;        /* kernels + dependencies captured per replay */
+    int32_t graph_replays;
+    int32_t sweeps;             /* refinement sweeps (max over scenes) */
+    int32_t pad;
+} mfs_batch_stats;
+int mfs_batch_create(mfs_batch **out, int device, void *stream, int32_t count);
+void mfs_batch_destroy(mfs_batch *b);
+mfs_ctx *mfs_batch_scene(mfs_batch *b, int32_t i);
+const char *mfs_batch_last_error(const mfs_batch *b);
+int mfs_batch_stats_get(const mfs_batch *b, mfs_batch_stats *out);
+/* mfs_plan_f64 for every scene (host FP64 sources xyz[i], n[i] x 3) on up to
+ * `threads` host threads. */
+int mfs_batch_plan(mfs_batch *b, const double *const *xyz, const int64_t *n, double rho,
+                   double lambda, int32_t max_super, int32_t threads);
+/* mfs_factor_points for every scene (device FP64 x[i], nx[i]: m[i] x 3, y[i]:
+ * the planned sources), all scenes' supernodes in one launch per tile size. */
+int mfs_batch_factor_points(mfs_batch *b, const double *const *x, const double *const *nx,
+                            const int64_t *m, const double *const *y, int precision);
+/* mfs_solve_points for every scene (B[i]: m[i] x k[i], X[i]: n[i] x k[i], FP32
+ * device, k[i] <= 16), the PCG iterations of all scenes replayed from one
+ * captured hipGraph of `chunk` iterations per scene (use_graph 0: the same
+ * launches issued directly, for comparison). */
+int mfs_batch_solve_points(mfs_batch *b, const float *const *B, const int64_t *k, float *const *X,
+                           int32_t maxit, double tol, double inner_tol, int32_t chunk,
+                           int use_graph);
+
 #ifdef __cplusplus
 }
 #endif

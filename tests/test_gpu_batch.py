@@ -1,0 +1,67 @@
+"""GPU: batches of independent scenes (mfs_batch) against one-scene solves.
+
+The batch runs the same per-scene arithmetic as a single context -- the same
+factor kernels (merged launches only regroup the blocks), the same PCG kernels
+(replayed from a graph or launched directly) -- so each scene's strengths must
+equal the single-scene result bit for bit."""
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from fastmbcui_amd import _lib, scenes
+
+pytestmark = pytest.mark.gpu
+
+DEV = torch.device("cuda", 0)
+
+
+def _scenes(count=6):
+    return [scenes.batch_scene(s, bodies=(2, 6), points=(96, 384)) for s in range(count)]
+
+
+def _dev(a):
+    return torch.from_numpy(np.ascontiguousarray(a)).to(DEV)
+
+
+def _run_batch(scs, use_graph, chunk=3, tol=1e-6):
+    b = _lib.Batch(len(scs))
+    b.plan([sc.y for sc in scs], 3.0)
+    b.factor_points([_dev(sc.x) for sc in scs], [_dev(sc.nx) for sc in scs], [_dev(sc.y) for sc in scs])
+    Xs = b.solve_points([_dev(sc.b.astype(np.float32)) for sc in scs], tol=tol, chunk=chunk,
+                        use_graph=use_graph)
+    return b, Xs
+
+
+@pytest.mark.parametrize("use_graph", [True, False])
+def test_batch_equals_single_scene_solves(use_graph):
+    scs = _scenes()
+    print("scene sizes", [sc.n for sc in scs], flush=True)
+    b, Xs = _run_batch(scs, use_graph)
+    st = b.stats()
+    print(f"batch: factor launches {st.factor_launches} items {st.factor_items}, graph nodes "
+          f"{st.graph_nodes}, replays {st.graph_replays}, sweeps {st.sweeps}", flush=True)
+    for i, sc in enumerate(scs):
+        ctx = _lib.Context(0)
+        ctx.plan(sc.y, 3.0)
+        ctx.factor_points(_dev(sc.x), _dev(sc.nx), _dev(sc.y), _lib.MFS_FP32)
+        X = ctx.solve_points(_dev(sc.b.astype(np.float32)), tol=1e-6)
+        assert torch.equal(X, Xs[i]), f"scene {i}"
+        assert ctx.stats().iters[0] == b.scene_stats(i).iters[0]
+
+
+def test_batch_solutions_meet_tolerance():
+    scs = _scenes(8)
+    b, Xs = _run_batch(scs, True, chunk=2)
+    for i, sc in enumerate(scs):
+        M = scenes.kernel_matrix(sc.x, sc.nx, sc.y)
+        res = oracle.normal_residual(M, sc.b.astype(np.float32), Xs[i].cpu().numpy())
+        assert res <= 2e-6, f"scene {i}: {res}"
+
+
+def test_batch_rejects_unplanned_scene():
+    scs = _scenes(2)
+    b = _lib.Batch(2)
+    with pytest.raises(_lib.MfsError) as e:
+        b.factor_points([_dev(sc.x) for sc in scs], [_dev(sc.nx) for sc in scs], [_dev(sc.y) for sc in scs])
+    assert e.value.code == _lib.MFS_ERR_STATE
