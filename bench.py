@@ -49,9 +49,13 @@ def _parse():
     ap.add_argument("--inner-tol", type=float, default=0.5)
     ap.add_argument("--points-steps", type=int, default=1,
                     help="timed solves of the matrix-free points form (0 skips it)")
-    ap.add_argument("--workload", choices=["ellipsoids64", "timestep"], default="ellipsoids64",
+    ap.add_argument("--workload", choices=["ellipsoids64", "timestep", "batch"], default="ellipsoids64",
                     help="timestep: 128 mixed bodies x 2048 points (N=262,144) in rigid motion, "
-                         "rho 4, tol 1e-8, refactor every step, warm start (points form)")
+                         "rho 4, tol 1e-8, refactor every step, warm start (points form); "
+                         "batch: 256 uneven scenes (2-32 bodies x 256-2048 points), one batch")
+    ap.add_argument("--batch-scenes", type=int, default=256)
+    ap.add_argument("--batch-sequential", type=int, default=1,
+                    help="batch: also solve the scenes one after another (0 skips)")
     ap.add_argument("--ts-steps", type=int, default=4, help="timestep: steps after the initial solve")
     ap.add_argument("--ts-bodies", type=int, default=128)
     ap.add_argument("--ts-points", type=int, default=2048)
@@ -133,6 +137,80 @@ def timestep_main(args):
     return 0
 
 
+def batch_main(args):
+    """256 independent uneven scenes as one batch (points form, mixed precision)."""
+    import numpy as np
+    import torch
+
+    from fastmbcui_amd import _lib, autotune, scenes
+
+    torch.cuda.set_device(0)
+    dev = torch.device("cuda", 0)
+    scs = [scenes.batch_scene(s) for s in range(args.batch_scenes)]
+    ns = [sc.n for sc in scs]
+    _log(f"bench: batch of {len(scs)} scenes, N from {min(ns)} to {max(ns)}, total {sum(ns)}")
+    xs = [torch.from_numpy(sc.x).to(dev) for sc in scs]
+    nxs = [torch.from_numpy(sc.nx).to(dev) for sc in scs]
+    ys = [torch.from_numpy(sc.y).to(dev) for sc in scs]
+    Bs = [torch.from_numpy(sc.b.astype(np.float32)).to(dev) for sc in scs]
+    threads = min(16, os.cpu_count() or 1)
+    b = _lib.Batch(len(scs))
+    b.plan([sc.y for sc in scs], 3.0, threads=threads)
+    st = b.stats()
+    _log(f"bench: batch plan {st.plan_ms:.0f} ms on {threads} threads")
+    b.factor_points(xs, nxs, ys)            # warm-up (buffers)
+    b.solve_points(Bs, tol=args.tol, chunk=4)
+    tuned = autotune.tune_batch_chunk(b, Bs, candidates=(4, 8), tol=args.tol)
+    chunk, use_graph = tuned["best"]
+    _log(f"bench: launch tuning {tuned}")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    b.factor_points(xs, nxs, ys)
+    b.solve_points(Bs, tol=args.tol, chunk=chunk, use_graph=use_graph)
+    torch.cuda.synchronize()
+    batch_ms = (time.perf_counter() - t0) * 1e3
+    st = b.stats()
+    iters = [b.scene_stats(i).iters[0] for i in range(len(scs))]
+    relres = [b.scene_stats(i).relres[0] for i in range(len(scs))]
+    _log(f"bench: batch factor+solve {batch_ms:.0f} ms (factor {st.factor_ms:.0f}, solve {st.solve_ms:.0f}; "
+         f"{st.factor_launches} factor launches over {st.factor_items} supernodes; chunk {chunk}, "
+         f"{'graph' if use_graph else 'direct launches'}, {st.graph_replays} replays, {st.sweeps} sweeps)")
+    seq_ms = None
+    if args.batch_sequential:
+        ctx = _lib.Context(0)
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        for i, sc in enumerate(scs):
+            ctx.plan(sc.y, 3.0)
+            tp = ctx.stats().plan_ms
+            ctx.factor_points(xs[i], nxs[i], ys[i], _lib.MFS_FP32)
+            ctx.solve_points(Bs[i], tol=args.tol)
+            t2 += tp / 1e3      # exclude the plan from the sequential factor+solve time
+        torch.cuda.synchronize()
+        seq_ms = (time.perf_counter() - t2) * 1e3
+        _log(f"bench: sequential factor+solve of the same scenes {seq_ms:.0f} ms")
+    line = {
+        "metric": "ms per batch (factor+PCG) of 256 uneven scenes, res<=1e-6, points form",
+        "value": round(batch_ms, 1), "unit": "ms/batch", "n_gpus": 1, "steps": 1, "warmup": 1,
+        "ms_per_step": round(batch_ms, 1), "higher_is_better": False, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic",
+        "config": {"workload": f"{len(scs)} scenes (2-32 coins/ellipsoids x 256-2048 points, log-uniform, "
+                               f"seeds 0-{len(scs) - 1}), rho 3, tol {args.tol:g}, 1 RHS each",
+                   "n_total": int(sum(ns)), "n_min": int(min(ns)), "n_max": int(max(ns))},
+        "scenes_per_s": round(len(scs) / (batch_ms / 1e3), 1),
+        "plan_ms": round(b.stats().plan_ms, 1), "plan_threads": threads,
+        "factor_ms": round(st.factor_ms, 1), "solve_ms": round(st.solve_ms, 1),
+        "sequential_factor_solve_ms": round(seq_ms, 1) if seq_ms else None,
+        "launch": {"chunk": chunk, "graph": bool(use_graph), "replays": st.graph_replays,
+                   "tuning_ms": tuned["ms"]},
+        "factor_launches": st.factor_launches,
+        "iters_max": int(max(iters)), "iters_mean": float(np.mean(iters)),
+        "residual_max": float(max(relres)),
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def main():
     args = _parse()
     if args.gpus != 1:
@@ -141,6 +219,8 @@ def main():
         return 2
     if args.workload == "timestep":
         return timestep_main(args)
+    if args.workload == "batch":
+        return batch_main(args)
 
     import numpy as np
     import torch

@@ -19,6 +19,7 @@ MFS_OK, MFS_ERR_ARG, MFS_ERR_HIP, MFS_ERR_DUPLICATE, MFS_ERR_NOT_SPD, MFS_ERR_ST
 MFS_FP64, MFS_FP32 = 0, 1
 MFS_RHS_BOUNDARY, MFS_RHS_SOURCE = 0, 1
 MFS_K_NONE, MFS_K_MATVEC_ROWS, MFS_K_MATVEC_COLS, MFS_K_GRAM, MFS_K_RESID_ROWS, MFS_K_RESID_COLS = range(6)
+MFS_PARAM_POINTS_BLOCKS, MFS_PARAM_COLS_BLOCKS = 1, 2
 
 _STATUS_NAMES = {1: "MFS_ERR_ARG", 2: "MFS_ERR_HIP", 3: "MFS_ERR_DUPLICATE", 4: "MFS_ERR_NOT_SPD",
                  5: "MFS_ERR_STATE", 6: "MFS_ERR_ALLOC"}
@@ -50,7 +51,7 @@ class BatchStats(ctypes.Structure):
         ("plan_ms", ctypes.c_double), ("factor_ms", ctypes.c_double), ("solve_ms", ctypes.c_double),
         ("factor_launches", ctypes.c_int32), ("factor_items", ctypes.c_int32),
         ("graph_nodes", ctypes.c_int32), ("graph_replays", ctypes.c_int32),
-        ("sweeps", ctypes.c_int32), ("pad", ctypes.c_int32),
+        ("sweeps", ctypes.c_int32), ("graph_captures", ctypes.c_int32),
     ]
 
 
@@ -101,6 +102,7 @@ def lib():
             "mfs_batch_plan": (ci, [vp, vp, vp, f64, f64, i32, i32]),
             "mfs_batch_factor_points": (ci, [vp, vp, vp, vp, vp, ci]),
             "mfs_batch_solve_points": (ci, [vp, vp, vp, vp, i32, f64, f64, i32, ci]),
+            "mfs_ctx_set_param": (ci, [vp, ci, i64]),
             "mfs_ctx_track_kernel": (ci, [vp, ci]),
             "mfs_ctx_tracked": (ci, [vp, ci, ctypes.POINTER(i64), ctypes.POINTER(f64), ctypes.POINTER(f64)]),
         }
@@ -334,6 +336,10 @@ class Context:
         self._check(lib().mfs_points_apply(self._h, int(bool(trans)), int(precision), _ptr(v), v.shape[1],
                                            _ptr(out)), "mfs_points_apply")
         return out
+
+    def set_param(self, param, value):
+        """Launch-shape parameter (MFS_PARAM_*; 0 = default)."""
+        self._check(lib().mfs_ctx_set_param(self._h, int(param), int(value)), "mfs_ctx_set_param")
 
     # ---- statistics -----------------------------------------------------------
     def stats(self):

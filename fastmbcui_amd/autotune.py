@@ -1,0 +1,86 @@
+"""Autotuner for the launch shapes of the solve.
+
+The shapes that decide how work is spread over the chip depend on the problem
+(m, n, right-hand sides, scene mix), so they are measured on the problem at
+hand rather than fixed:
+
+* MFS_PARAM_POINTS_BLOCKS -- blocks per matrix-free pass (owners x splits of
+  the other side): more splits fill the CUs and balance XCDs, fewer keep the
+  partial sums small;
+* the batch's PCG chunk -- iterations per replay: longer chunks cost fewer
+  polls, shorter ones launch fewer no-op kernels after a scene converged --
+  and whether the chunk is replayed from a captured graph or launched
+  directly on the scenes' streams.
+
+Each candidate is timed with HIP events on the work it controls; the fastest
+is set and returned.  Results are kept per shape key for the process.
+"""
+from __future__ import annotations
+
+import time
+
+from . import _lib
+
+_cache: dict = {}
+
+
+def _time_ms(fn, reps):
+    import torch
+    fn()  # warm-up (buffers, code objects)
+    torch.cuda.synchronize()
+    best = float("inf")
+    for _ in range(reps):
+        start = torch.cuda.Event(enable_timing=True)
+        end = torch.cuda.Event(enable_timing=True)
+        start.record()
+        fn()
+        end.record()
+        end.synchronize()
+        best = min(best, start.elapsed_time(end))
+    return best
+
+
+def tune_points_blocks(ctx: _lib.Context, k: int = 8, candidates=(1024, 2048, 4096, 8192, 16384),
+                       reps: int = 3) -> dict:
+    """Time M p and M^T y (FP32, k columns) from the context's packed points for
+    each candidate block count; keep the fastest.  Needs mfs_factor_points."""
+    import torch
+    key = ("points_blocks", ctx.m_points, ctx.n, k)
+    if key in _cache:
+        ctx.set_param(_lib.MFS_PARAM_POINTS_BLOCKS, _cache[key]["best"])
+        return _cache[key]
+    dev = ctx.device
+    p = torch.randn(ctx.n, k, device=dev)
+    y = torch.randn(ctx.m_points, k, device=dev)
+    times = {}
+    for c in candidates:
+        ctx.set_param(_lib.MFS_PARAM_POINTS_BLOCKS, c)
+        times[c] = _time_ms(lambda: (ctx.points_apply(p), ctx.points_apply(y, trans=True)), reps)
+    best = min(times, key=times.get)
+    ctx.set_param(_lib.MFS_PARAM_POINTS_BLOCKS, best)
+    _cache[key] = {"best": best, "ms": times}
+    return _cache[key]
+
+
+def tune_batch_chunk(batch: _lib.Batch, Bs, candidates=(4, 8), graphs=(True, False), maxit=1000,
+                     tol=1e-6) -> dict:
+    """Solve the batch once per (chunk length, graph or direct launches) --
+    whole solves, since the chunk trades polls against launches over the whole
+    convergence history, and whether a captured graph beats direct launches on
+    many streams depends on how the runtime schedules the graph's branches;
+    keep the fastest."""
+    import torch
+    key = ("batch_chunk", tuple(batch.n), tuple(int(b.shape[1]) for b in Bs))
+    if key in _cache:
+        return _cache[key]
+    times = {}
+    for g in graphs:
+        for c in candidates:
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            batch.solve_points(Bs, maxit=maxit, tol=tol, chunk=c, use_graph=g)
+            times[(c, g)] = (time.perf_counter() - t0) * 1e3
+    best = min(times, key=times.get)
+    _cache[key] = {"best": best, "ms": {f"chunk {c}, {'graph' if g else 'direct'}": t
+                                        for (c, g), t in times.items()}}
+    return _cache[key]

@@ -267,9 +267,10 @@ inline unsigned blocks_for(int64_t count) { return unsigned((count + kThreads - 
 
 }  // namespace
 
-int32_t points_splits(int64_t owners, int64_t others) {
+int32_t points_splits(int64_t owners, int64_t others, int32_t target_blocks) {
   const int64_t blocks = (owners + kThreads - 1) / kThreads;
-  int64_t s = (4096 + blocks - 1) / blocks;
+  const int64_t target = target_blocks > 0 ? target_blocks : kPointsBlocks;
+  int64_t s = (target + blocks - 1) / blocks;
   const int64_t cap = (others + 255) / 256;  // at least 256 terms per split
   if (s > cap) s = cap;
   if (s > 256) s = 256;
@@ -287,7 +288,7 @@ template <typename TA>
 hipError_t points_rows(hipStream_t st, const PointsOp &op, int K, const TA *X, const float *Bsub,
                        int64_t ldb, int64_t col0, int kval, TA *Y, double *part,
                        const PcgState *state) {
-  const int32_t S = points_splits(op.m, op.n);
+  const int32_t S = points_splits(op.m, op.n, op.target_blocks);
   const int64_t per = (op.n + S - 1) / S;
   const dim3 grid(blocks_for(op.m), unsigned(S));
   if constexpr (sizeof(TA) == 4) {
@@ -305,7 +306,7 @@ hipError_t points_rows(hipStream_t st, const PointsOp &op, int K, const TA *X, c
 template <typename TA>
 hipError_t points_cols(hipStream_t st, const PointsOp &op, int K, const TA *Y, double *part,
                        const PcgState *state) {
-  const int32_t S = points_splits(op.n, op.m);
+  const int32_t S = points_splits(op.n, op.m, op.target_blocks);
   const int64_t per = (op.m + S - 1) / S;
   const dim3 grid(blocks_for(op.n), unsigned(S));
   if constexpr (sizeof(TA) == 4) {
@@ -321,7 +322,7 @@ hipError_t points_cols(hipStream_t st, const PointsOp &op, int K, const TA *Y, d
 template <typename TA>
 hipError_t points_cols_reduce(hipStream_t st, const PointsOp &op, int K, const double *part,
                               TA *out) {
-  const int32_t S = points_splits(op.n, op.m);
+  const int32_t S = points_splits(op.n, op.m, op.target_blocks);
   PTS_DISPATCH(K, (k_pts_reduce<TA, KK><<<blocks_for(op.n), kThreads, 0, st>>>(
                       op.n, S, part, nullptr, 0, 0, K, out, nullptr)));
   return hipGetLastError();

@@ -91,6 +91,8 @@ struct mfs_ctx {
   // points form: packed target/source records (kernel_eval.h layouts)
   bool has_points = false;
   PointsOp pts{};
+  int32_t points_blocks = 0;  // tuning: blocks per points pass (0 = default)
+  int32_t cols_blocks = 0;    // tuning: blocks of the dense M^T pass (0 = default)
   DevBuf dt32, dt64, ds32, ds64, dapin, dapout;
 
   // solve workspaces
@@ -464,10 +466,12 @@ int values_for(mfs_ctx *ctx, int want, const void **lval, const void **ltval) {
   return MFS_OK;
 }
 
-// Row blocks of the M^T pass: about 2048 blocks in all, at least 64 rows each.
-int32_t row_blocks_for(int64_t m, int64_t n) {
+// Row blocks of the M^T pass: about `target` (default 2048) blocks in all, at
+// least 64 rows each.
+int32_t row_blocks_for(int64_t m, int64_t n, int32_t target) {
   const int64_t cols = (n + 1023) / 1024;
-  int64_t rb = (2048 + cols - 1) / cols;
+  const int64_t want = target > 0 ? target : 2048;
+  int64_t rb = (want + cols - 1) / cols;
   rb = std::min<int64_t>(rb, std::max<int64_t>(m / 64, 1));
   return int32_t(std::max<int64_t>(rb, 1));
 }
@@ -500,7 +504,7 @@ int workspace(mfs_ctx *ctx, const float *M, const PointsOp *pts, int64_t m, int6
   a.tcol = ctx->dtcol.as<int32_t>();
   a.lval = static_cast<const TA *>(lv);
   a.ltval = static_cast<const TA *>(ltv);
-  a.row_blocks = pts ? points_splits(n, m) : row_blocks_for(m, n);
+  a.row_blocks = pts ? points_splits(n, m, pts->target_blocks) : row_blocks_for(m, n, ctx->cols_blocks);
   a.pts = pts ? *pts : PointsOp{};
   const size_t nv = size_t(n) * K, mv = size_t(m) * K;
   HIPCHK(ctx->dvec.ensure((7 * nv + mv) * sizeof(TA) + 64), "alloc vectors");
@@ -514,7 +518,7 @@ int workspace(mfs_ctx *ctx, const float *M, const PointsOp *pts, int64_t m, int6
   a.pv = base + 6 * nv;
   a.y = base + 7 * nv;
   size_t part_elems = size_t(a.row_blocks) * nv;
-  if (pts) part_elems = std::max(part_elems, size_t(points_splits(m, n)) * mv);
+  if (pts) part_elems = std::max(part_elems, size_t(points_splits(m, n, pts->target_blocks)) * mv);
   HIPCHK(ctx->dpart.ensure(part_elems * sizeof(double)), "alloc partials");
   a.part = ctx->dpart.as<double>();
   const size_t maxblocks = size_t((n * 8 + kThreads - 1) / kThreads) + size_t((m + kThreads - 1) / kThreads) + 16;
@@ -1159,6 +1163,7 @@ int mfs_factor_points(mfs_ctx *ctx, const double *x, const double *nx, int64_t m
   ctx->pts.src64 = ctx->ds64.as<double>();
   ctx->pts.m = m;
   ctx->pts.n = n;
+  ctx->pts.target_blocks = ctx->points_blocks;
   ctx->has_points = true;
   return factor_impl(ctx, nullptr, &ctx->pts, m, n, precision);
 }
@@ -1207,8 +1212,8 @@ int mfs_points_apply(mfs_ctx *ctx, int trans, int precision, const void *in, int
   const PointsOp &op = ctx->pts;
   const size_t w = precision == MFS_FP64 ? 8 : 4;
   const int64_t rows_in = trans ? op.m : op.n, rows_out = trans ? op.n : op.m;
-  const size_t part_rows = std::max(size_t(points_splits(op.m, op.n)) * size_t(op.m),
-                                    size_t(points_splits(op.n, op.m)) * size_t(op.n));
+  const size_t part_rows = std::max(size_t(points_splits(op.m, op.n, op.target_blocks)) * size_t(op.m),
+                                    size_t(points_splits(op.n, op.m, op.target_blocks)) * size_t(op.n));
   for (int64_t col0 = 0; col0 < k; col0 += kMaxRhs) {
     const int kval = int(std::min<int64_t>(kMaxRhs, k - col0));
     const int K = batch_width(kval);
@@ -1254,6 +1259,21 @@ int mfs_assemble_dense(mfs_ctx *ctx, const double *x, const double *nx, const do
   HIPCHK(hipSetDevice(ctx->device), "hipSetDevice");
   HIPCHK(launch_assemble(ctx->stream, x, nx, y, m, n, M), "assemble");
   return MFS_OK;
+}
+
+int mfs_ctx_set_param(mfs_ctx *ctx, int param, int64_t value) {
+  if (ctx == nullptr || value < 0 || value > (int64_t(1) << 24)) return MFS_ERR_ARG;
+  switch (param) {
+    case MFS_PARAM_POINTS_BLOCKS:
+      ctx->points_blocks = int32_t(value);
+      ctx->pts.target_blocks = int32_t(value);
+      return MFS_OK;
+    case MFS_PARAM_COLS_BLOCKS:
+      ctx->cols_blocks = int32_t(value);
+      return MFS_OK;
+    default:
+      return ctx->fail(MFS_ERR_ARG, "set_param: unknown parameter");
+  }
 }
 
 int mfs_ctx_track_kernel(mfs_ctx *ctx, int mask) {
@@ -1486,7 +1506,7 @@ int mfs_batch_factor_points(mfs_batch *b, const double *const *x, const double *
                         c->dt64.as<double>(), c->ds32.as<float>(), c->ds64.as<double>()),
             "pack points");
     c->pts = PointsOp{c->dt32.as<float>(), c->ds32.as<float>(), c->dt64.as<double>(),
-                      c->ds64.as<double>(), m[i], ni};
+                      c->ds64.as<double>(), m[i], ni, c->points_blocks};
     c->has_points = true;
     const int rc = factor_prepare(c, nullptr, &c->pts, m[i], ni, precision);
     if (rc != MFS_OK) return scene_error(b, i, rc);
@@ -1575,12 +1595,16 @@ int mfs_batch_solve_points(mfs_batch *b, const float *const *B, const int64_t *k
   BHIPCHK(hipMemcpyAsync(b->dstates.p, b->states.data(), count * sizeof(PcgState *),
                          hipMemcpyHostToDevice, b->root),
           "upload");
-  // The iteration graph: a branch per scene on the scene's stream.
+  // Iterations: `chunk` per scene on the scene's stream, for the scenes in
+  // `set`, then a gather of every scene's active count on root.  Either
+  // captured once into a graph (re-captured when the set has shrunk by a
+  // quarter, so converged scenes stop costing launches) or issued directly.
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
-  auto launch_chunk = [&]() -> int {
+  std::vector<size_t> graph_set;
+  auto launch_chunk = [&](const std::vector<size_t> &set) -> int {
     if (fork_scenes(b) != MFS_OK) return MFS_ERR_HIP;
-    for (size_t i = 0; i < count; ++i)
+    for (size_t i : set)
       for (int32_t c = 0; c < chunk; ++c)
         BHIPCHK(pcg_iteration<float>(b->streams[i], runs[i].a32, nullptr, nullptr), "pcg iteration");
     if (join_scenes(b) != MFS_OK) return MFS_ERR_HIP;
@@ -1589,19 +1613,28 @@ int mfs_batch_solve_points(mfs_batch *b, const float *const *B, const int64_t *k
     BHIPCHK(hipGetLastError(), "gather");
     return MFS_OK;
   };
-  if (use_graph) {
-    if (join_scenes(b) != MFS_OK) return MFS_ERR_HIP;  // capture starts after all begin() work
+  auto capture = [&](const std::vector<size_t> &set) -> int {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    exec = nullptr;
+    graph = nullptr;
+    if (join_scenes(b) != MFS_OK) return MFS_ERR_HIP;
     BHIPCHK(hipStreamSynchronize(b->root), "sync");
     BHIPCHK(hipStreamBeginCapture(b->root, hipStreamCaptureModeThreadLocal), "begin capture");
-    const int rc = launch_chunk();
+    const int rc = launch_chunk(set);
     hipError_t e = hipStreamEndCapture(b->root, &graph);
     if (rc != MFS_OK) return rc;
     BHIPCHK(e, "end capture");
     size_t nodes = 0;
     BHIPCHK(hipGraphGetNodes(graph, nullptr, &nodes), "graph nodes");
-    b->stats.graph_nodes = int32_t(nodes);
+    b->stats.graph_nodes = std::max<int32_t>(b->stats.graph_nodes, int32_t(nodes));
     BHIPCHK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0), "instantiate");
-  }
+    graph_set = set;
+    ++b->stats.graph_captures;
+    return MFS_OK;
+  };
+  b->stats.graph_nodes = 0;
+  b->stats.graph_captures = 0;
   int32_t replays = 0, sweeps = 0;
   int rc = MFS_OK;
   for (;;) {
@@ -1616,25 +1649,35 @@ int mfs_batch_solve_points(mfs_batch *b, const float *const *B, const int64_t *k
     ++sweeps;
     int32_t budget = 0;
     for (size_t i : active) budget = std::max(budget, maxit - runs[i].inner_used);
+    std::vector<size_t> running = active;
     for (int32_t it = 0;; it += chunk) {
       if (it > budget + 64) {
         rc = b->fail(MFS_ERR_STATE, "batch pcg: iteration bookkeeping diverged");
         break;
       }
       if (use_graph) {
+        // the graph must cover every running scene; re-capture when a quarter
+        // of its branches only launch no-op kernels
+        bool covered = true;
+        for (size_t i : running)
+          covered = covered && std::binary_search(graph_set.begin(), graph_set.end(), i);
+        if (exec == nullptr || !covered || 4 * running.size() <= 3 * graph_set.size())
+          if ((rc = capture(running)) != MFS_OK) break;
         if (join_scenes(b) != MFS_OK) { rc = MFS_ERR_HIP; break; }
         hipError_t e = hipGraphLaunch(exec, b->root);
         if (e != hipSuccess) { rc = b->fail(MFS_ERR_HIP, std::string("graph launch: ") + hipGetErrorString(e)); break; }
-      } else if ((rc = launch_chunk()) != MFS_OK) {
+      } else if ((rc = launch_chunk(running)) != MFS_OK) {
         break;
       }
       ++replays;
       hipError_t e = hipMemcpyAsync(b->hactive, b->dactive.p, count * 4, hipMemcpyDeviceToHost, b->root);
       if (e == hipSuccess) e = hipStreamSynchronize(b->root);
       if (e != hipSuccess) { rc = b->fail(MFS_ERR_HIP, std::string("poll: ") + hipGetErrorString(e)); break; }
-      bool any = false;
-      for (size_t i : active) any = any || b->hactive[i] != 0;
-      if (!any) break;
+      std::vector<size_t> still;
+      for (size_t i : running)
+        if (b->hactive[i] != 0) still.push_back(i);
+      running.swap(still);
+      if (running.empty()) break;
     }
     if (rc != MFS_OK) break;
     if (fork_scenes(b) != MFS_OK) { rc = MFS_ERR_HIP; break; }

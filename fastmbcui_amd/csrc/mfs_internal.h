@@ -49,6 +49,7 @@ struct PointsOp {
   const double *tgt64 = nullptr;  // m x 8
   const double *src64 = nullptr;  // n x 4
   int64_t m = 0, n = 0;
+  int32_t target_blocks = 0;      // blocks per pass (owners x splits); 0: kPointsBlocks
 };
 
 // ----------------------------------------------------------------- plan kernels
@@ -149,16 +150,17 @@ hipError_t residual64(hipStream_t st, const SolveArgs<double> &a, const float *B
 
 // ---------------------------------------------------------------- points form
 // Splits of the other side so that owners x splits fills the chip.
-int32_t points_splits(int64_t owners, int64_t others);
+constexpr int32_t kPointsBlocks = 4096;
+int32_t points_splits(int64_t owners, int64_t others, int32_t target_blocks);
 hipError_t pack_points(hipStream_t st, const double *x, const double *nx, const double *y,
                        int64_t m, int64_t n, float *t32, double *t64, float *s32, double *s64);
 // Y (m x K) = M X, or B - M X when Bsub is given; part holds
-// points_splits(m, n) * m * K doubles.
+// points_splits(m, n, op.target_blocks) * m * K doubles.
 template <typename TA>
 hipError_t points_rows(hipStream_t st, const PointsOp &op, int K, const TA *X, const float *Bsub,
                        int64_t ldb, int64_t col0, int kval, TA *Y, double *part,
                        const PcgState *state);
-// part[points_splits(n, m)][n][K] = split sums of M^T Y.
+// part[points_splits(n, m, op.target_blocks)][n][K] = split sums of M^T Y.
 template <typename TA>
 hipError_t points_cols(hipStream_t st, const PointsOp &op, int K, const TA *Y, double *part,
                        const PcgState *state);

@@ -160,6 +160,15 @@ int mfs_points_apply(mfs_ctx *ctx, int trans, int precision, const void *in, int
 int mfs_assemble_dense(mfs_ctx *ctx, const double *x, const double *nx, const double *y,
                        int64_t m, int64_t n, float *M);
 
+/* Launch-shape parameters (autotuned by fastmbcui_amd/autotune.py; 0 restores
+ * the default).  They change how work is split over blocks, not the results
+ * beyond summation order. */
+enum mfs_param {
+    MFS_PARAM_POINTS_BLOCKS = 1,  /* blocks per points pass, owners x splits (4096) */
+    MFS_PARAM_COLS_BLOCKS = 2     /* blocks of the dense M^T pass (2048) */
+};
+int mfs_ctx_set_param(mfs_ctx *ctx, int param, int64_t value);
+
 /* Kernel timing for the bench: while a class is enabled (bit (1 << class) of
  * `mask`), every launch of that class is bracketed by HIP events on the context
  * stream; mfs_ctx_tracked returns the launch count, the summed event time and
@@ -193,7 +202,7 @@ typedef struct mfs_batch_stats {
     int32_t graph_nodes;        /* kernels + dependencies captured per replay */
     int32_t graph_replays;
     int32_t sweeps;             /* refinement sweeps (max over scenes) */
-    int32_t pad;
+    int32_t graph_captures;     /* graphs captured (the set of running scenes shrinks) */
 } mfs_batch_stats;
 int mfs_batch_create(mfs_batch **out, int device, void *stream, int32_t count);
 void mfs_batch_destroy(mfs_batch *b);
@@ -209,9 +218,9 @@ int mfs_batch_plan(mfs_batch *b, const double *const *xyz, const int64_t *n, dou
 int mfs_batch_factor_points(mfs_batch *b, const double *const *x, const double *const *nx,
                             const int64_t *m, const double *const *y, int precision);
 /* mfs_solve_points for every scene (B[i]: m[i] x k[i], X[i]: n[i] x k[i], FP32
- * device, k[i] <= 16), the PCG iterations of all scenes replayed from one
- * captured hipGraph of `chunk` iterations per scene (use_graph 0: the same
- * launches issued directly, for comparison). */
+ * device, k[i] <= 16), the PCG iterations of the running scenes replayed from
+ * a captured hipGraph of `chunk` iterations per scene, re-captured as scenes
+ * converge (use_graph 0: the same launches issued directly). */
 int mfs_batch_solve_points(mfs_batch *b, const float *const *B, const int64_t *k, float *const *X,
                            int32_t maxit, double tol, double inner_tol, int32_t chunk,
                            int use_graph);

@@ -227,3 +227,17 @@ def test_time_stepper_warm_start_meets_tolerance_each_step():
         assert s.residual <= 1e-8
     assert all(s.warm for s in infos[1:])
     assert max(s.inner_iters for s in infos[1:]) < infos[0].inner_iters
+
+
+def test_autotune_points_blocks_keeps_results():
+    from fastmbcui_amd import autotune
+    sc = _ellips(k=2)
+    ctx = _points_ctx(sc)
+    B = _dev(sc.b.astype(np.float32))
+    X0 = ctx.solve_points(B, tol=1e-6)
+    r = autotune.tune_points_blocks(ctx, k=2, candidates=(512, 2048, 8192), reps=2)
+    print(f"points blocks tuning: {r}", flush=True)
+    assert r["best"] in (512, 2048, 8192)
+    X1 = ctx.solve_points(B, tol=1e-6)
+    assert ctx.points_residual(B, X1).max() <= 1e-6
+    assert torch.allclose(X0, X1, rtol=1e-3, atol=1e-3 * X0.abs().max().item())
