@@ -103,6 +103,22 @@ def lib():
             "mfs_batch_factor_points": (ci, [vp, vp, vp, vp, vp, ci]),
             "mfs_batch_solve_points": (ci, [vp, vp, vp, vp, i32, f64, f64, i32, ci]),
             "mfs_ctx_set_param": (ci, [vp, ci, i64]),
+            "mfs_mlp_create": (ci, [ctypes.POINTER(vp), ci, vp, ctypes.c_uint64]),
+            "mfs_mlp_destroy": (None, [vp]),
+            "mfs_mlp_last_error": (ctypes.c_char_p, [vp]),
+            "mfs_mlp_set_stream": (ci, [vp, vp]),
+            "mfs_mlp_param_count": (i64, [vp]),
+            "mfs_mlp_feature_dim": (i32, []),
+            "mfs_mlp_get_params": (ci, [vp, vp]),
+            "mfs_mlp_set_params": (ci, [vp, vp]),
+            "mfs_mlp_get_grads": (ci, [vp, vp]),
+            "mfs_mlp_features": (ci, [vp, vp, vp, vp, vp, vp, vp, vp, vp, i64, vp, vp]),
+            "mfs_mlp_loss_grad": (ci, [vp, vp, vp, vp, i64, ctypes.c_float, ctypes.POINTER(f64)]),
+            "mfs_mlp_adam_step": (ci, [vp, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float]),
+            "mfs_mlp_train_step": (ci, [vp, vp, vp, vp, i64, ctypes.c_float, ctypes.c_float,
+                                        ctypes.POINTER(f64)]),
+            "mfs_mlp_predict": (ci, [vp, vp, vp, i64, vp]),
+            "mfs_mlp_infer": (ci, [vp, vp, vp, i64, vp]),
             "mfs_ctx_track_kernel": (ci, [vp, ci]),
             "mfs_ctx_tracked": (ci, [vp, ci, ctypes.POINTER(i64), ctypes.POINTER(f64), ctypes.POINTER(f64)]),
         }

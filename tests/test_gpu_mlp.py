@@ -1,0 +1,128 @@
+"""GPU: the learned warm start's HIP kernels against the CPU oracle, and what
+the trained network gains over a zero start."""
+import numpy as np
+import pytest
+import torch
+
+from fastmbcui_amd import _lib, warmstart
+from oracle import mlp_oracle as mo
+
+pytestmark = pytest.mark.gpu
+
+DEV = torch.device("cuda", 0)
+
+
+def _host_inputs(rs):
+    sc = rs.scene()
+    body = rs.body_of_point.astype(np.int64)
+    starts = np.searchsorted(body, np.arange(rs.centres.shape[0] + 1))
+    vw = np.concatenate([rs.v, rs.w], axis=1)
+    return sc, body, starts, vw
+
+
+def test_features_match_oracle():
+    rs = warmstart.coin_trajectory(7, n_bodies=4, pts_per_body=64, box=8.0)
+    rs.advance(0.1)
+    net = warmstart.Network(seed=1)
+    feat, q = net.features(rs)
+    sc, body, starts, vw = _host_inputs(rs)
+    f_ref, q_ref = mo.features(sc.y, sc.nx, sc.b[:, 0], body, starts, rs.rot.reshape(-1, 9), rs.centres, vw)
+    err_f = np.abs(feat.cpu().numpy() - f_ref).max() / np.abs(f_ref).max()
+    err_q = np.abs(q.cpu().numpy() - q_ref).max() / np.abs(q_ref).max()
+    print(f"features: max rel err {err_f:.2e}, q {err_q:.2e}", flush=True)
+    assert err_f < 1e-5 and err_q < 1e-6
+
+
+@pytest.mark.parametrize("fused", [True, False])
+def test_forward_matches_oracle(fused):
+    rng = np.random.default_rng(2)
+    net = warmstart.Network(seed=3)
+    p = net.params()
+    feat = rng.standard_normal((300, 20)).astype(np.float32)
+    q = rng.standard_normal((300, 6)).astype(np.float32)
+    x = net.predict(torch.from_numpy(feat).to(DEV), torch.from_numpy(q).to(DEV), fused=fused).cpu().numpy()
+    out, _ = mo.forward(p, feat)
+    ref = np.sum(out * q, axis=1)
+    err = np.abs(x - ref).max() / np.abs(ref).max()
+    print(f"forward fused={fused}: max rel err {err:.2e}", flush=True)
+    assert err < 1e-5
+
+
+def test_gradients_match_oracle():
+    rng = np.random.default_rng(4)
+    net = warmstart.Network(seed=5)
+    p = net.params()
+    P = 2000   # several split-K chunks in the weight gradients
+    feat = rng.standard_normal((P, 20)).astype(np.float32)
+    q = rng.standard_normal((P, 6)).astype(np.float32)
+    t = rng.standard_normal(P).astype(np.float32)
+    loss = net.loss_grad(torch.from_numpy(feat).to(DEV), torch.from_numpy(q).to(DEV),
+                         torch.from_numpy(t).to(DEV), 1.0 / P)
+    g = net.grads()
+    loss_ref, g_ref = mo.loss_grad(p, feat, q, t, 1.0 / P)
+    assert abs(loss - loss_ref) <= 1e-5 * loss_ref
+    w_off, b_off, _, n_all = mo.layout()
+    for l in range(4):
+        for off, size in ((w_off[l], mo.DIMS[l] * mo.DIMS[l + 1]), (b_off[l], mo.DIMS[l + 1])):
+            a, b = g[off:off + size], g_ref[off:off + size]
+            err = np.linalg.norm(a - b) / np.linalg.norm(b)
+            print(f"layer {l} block at {off}: grad rel err {err:.2e}", flush=True)
+            assert err < 1e-4
+
+
+def test_adam_step_matches_oracle():
+    rng = np.random.default_rng(6)
+    net = warmstart.Network(seed=7)
+    p = net.params().astype(np.float64)
+    P = 500
+    args = [torch.from_numpy(rng.standard_normal(s).astype(np.float32)).to(DEV)
+            for s in ((P, 20), (P, 6), (P,))]
+    m = np.zeros_like(p)
+    v = np.zeros_like(p)
+    for step in (1, 2, 3):
+        net.train_step(*args, 1.0 / P, 1e-3)
+        g = net.grads().astype(np.float64)
+        p, m, v = mo.adam(p, g, m, v, step, 1e-3)
+    got = net.params()
+    assert np.abs(got - p).max() <= 1e-6 * max(1.0, np.abs(p).max())
+
+
+def test_trained_network_guess_is_judged_by_the_residual():
+    # Trains on the method's own solves along coin trajectories, then starts
+    # held-out solves from the network's guess.  The guess only changes the
+    # iteration count: every answer meets the same FP64 residual tolerance.
+    # What it gains is reported beside the zero and previous-step starts (on
+    # these ill-conditioned MFS systems even the previous pose's exact
+    # solution saves only a few iterations, and a per-point guess with tens of
+    # percent error saves none).
+    net = warmstart.Network(seed=11)
+    data = warmstart.generate(net, seeds=[1000, 1001, 1002], steps=60, every=3)
+    losses = warmstart.train(net, data, epochs=30, batch_samples=6, lr=1e-3)
+    print(f"samples {data.count}, relative loss {losses[0]:.3e} -> {losses[-1]:.3e}", flush=True)
+    assert losses[-1] < 0.5 * losses[0]
+    rs = warmstart.coin_trajectory(1009)
+    ctx = _lib.Context(0)
+    prev = None
+    for step in range(3):
+        if step:
+            rs.advance(1e-2)
+        sc = rs.scene()
+        ctx.plan(sc.y, 3.0)
+        ctx.factor_points(torch.from_numpy(sc.x).to(DEV), torch.from_numpy(sc.nx).to(DEV),
+                          torch.from_numpy(sc.y).to(DEV), _lib.MFS_FP32)
+        B = torch.from_numpy(sc.b.astype(np.float32)).to(DEV)
+        X = ctx.solve_points(B, tol=1e-6)
+        zero = ctx.stats().iters[0]
+        g = warmstart.guess(net, rs)
+        Xl = ctx.solve_points(B, tol=1e-6, x0=g)
+        learned = ctx.stats().iters[0]
+        res = ctx.points_residual(B, Xl).max()
+        pv = None
+        if prev is not None:
+            ctx.solve_points(B, tol=1e-6, x0=prev)
+            pv = ctx.stats().iters[0]
+        prev = X
+        print(f"held-out step {step}: zero start {zero} iterations, learned {learned} (guess residual "
+              f"{ctx.points_residual(B, g).max():.2e}), previous step {pv}; residual {res:.2e}", flush=True)
+        assert res <= 1e-6
+        assert learned <= zero + 3
