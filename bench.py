@@ -49,7 +49,11 @@ def _parse():
     ap.add_argument("--inner-tol", type=float, default=0.5)
     ap.add_argument("--points-steps", type=int, default=1,
                     help="timed solves of the matrix-free points form (0 skips it)")
-    ap.add_argument("--workload", choices=["ellipsoids64", "timestep", "batch"], default="ellipsoids64",
+    ap.add_argument("--learned-samples-every", type=int, default=10,
+                    help="learned: solve every k-th pose of the 2000-step training trajectories")
+    ap.add_argument("--learned-epochs", type=int, default=50)
+    ap.add_argument("--workload", choices=["ellipsoids64", "timestep", "batch", "learned"],
+                    default="ellipsoids64",
                     help="timestep: 128 mixed bodies x 2048 points (N=262,144) in rigid motion, "
                          "rho 4, tol 1e-8, refactor every step, warm start (points form); "
                          "batch: 256 uneven scenes (2-32 bodies x 256-2048 points), one batch")
@@ -211,6 +215,80 @@ def batch_main(args):
     return 0
 
 
+def learned_main(args):
+    """Learned warm start: train on coin trajectories (seeds 1000-1009), then
+    compare zero, previous-step and learned starts on held-out trajectories."""
+    import numpy as np
+    import torch
+
+    from fastmbcui_amd import _lib, warmstart
+
+    torch.cuda.set_device(0)
+    dev = torch.device("cuda", 0)
+    t0 = time.perf_counter()
+    net = warmstart.Network(seed=0)
+    data = warmstart.generate(net, seeds=list(range(1000, 1010)), steps=2000, every=args.learned_samples_every)
+    gen_s = time.perf_counter() - t0
+    _log(f"bench: {data.count} training samples (N={data.n}) in {gen_s:.1f} s")
+    t1 = time.perf_counter()
+    mse = warmstart.train(net, data, epochs=10, batch_samples=8, lr=1e-3)
+    fit = warmstart.train_fit(net, data, epochs=args.learned_epochs, batch_samples=8, lr=3e-4, normal=True)
+    train_s = time.perf_counter() - t1
+    _log(f"bench: trained in {train_s:.1f} s: strength loss {mse[0]:.3f} -> {mse[-1]:.3f}, "
+         f"normal-residual loss {fit[0]:.4f} -> {fit[-1]:.4f}")
+
+    def evaluate(n_bodies, box, seeds, steps=5):
+        ctx = _lib.Context(0)
+        rows = {"zero": [], "previous": [], "learned": []}
+        for seed in seeds:
+            rs = warmstart.coin_trajectory(seed, n_bodies=n_bodies, box=box)
+            prev = None
+            for step in range(steps):
+                if step:
+                    rs.advance(1e-2)
+                sc = rs.scene()
+                ctx.plan(sc.y, 3.0)
+                ctx.factor_points(*(torch.from_numpy(a).to(dev) for a in (sc.x, sc.nx, sc.y)), _lib.MFS_FP32)
+                B = torch.from_numpy(sc.b.astype(np.float32)).to(dev)
+                starts = {"zero": None, "learned": None}
+                if prev is not None:
+                    starts["previous"] = prev
+                for name in list(starts):
+                    torch.cuda.synchronize()
+                    ts = time.perf_counter()
+                    x0 = warmstart.guess(net, rs) if name == "learned" else starts[name]
+                    X = ctx.solve_points(B, tol=1e-6, x0=x0)
+                    torch.cuda.synchronize()
+                    ms = (time.perf_counter() - ts) * 1e3
+                    rows[name].append((ctx.stats().iters[0], ms, float(ctx.stats().relres[0])))
+                    if name == "zero":
+                        zero_X = X
+                prev = zero_X
+        return {k: {"iters_mean": float(np.mean([r[0] for r in v])), "ms_mean": float(np.mean([r[1] for r in v])),
+                    "residual_max": float(max(r[2] for r in v))} for k, v in rows.items() if v}
+
+    small = evaluate(8, 10.0, [1010, 1011])
+    large = evaluate(128, 25.0, [1010])
+    _log(f"bench: held-out N=4096: {small}")
+    _log(f"bench: held-out N=65536: {large}")
+    value = large["learned"]["ms_mean"]
+    line = {
+        "metric": "ms/solve to res<=1e-6 from the learned start, held-out coin scenes at N=65,536",
+        "value": round(value, 2), "unit": "ms/solve", "n_gpus": 1, "steps": 5, "warmup": 0,
+        "ms_per_step": round(value, 2), "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": "bf16 weights, f32+f64 solve", "data": "synthetic",
+        "config": {"workload": "train: 8 coins x 512 points, trajectories of 2000 steps dt 1e-2, seeds "
+                               f"1000-1009, every {args.learned_samples_every}th pose solved ({data.count} "
+                               f"samples); eval: held-out seeds, 128 coins x 512 points (N=65,536), rho 3"},
+        "train_samples": data.count, "generate_s": round(gen_s, 1), "train_s": round(train_s, 1),
+        "loss_strength": [round(mse[0], 4), round(mse[-1], 4)],
+        "loss_normal_residual": [round(fit[0], 5), round(fit[-1], 5)],
+        "heldout_N4096": small, "heldout_N65536": large,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def main():
     args = _parse()
     if args.gpus != 1:
@@ -221,6 +299,8 @@ def main():
         return timestep_main(args)
     if args.workload == "batch":
         return batch_main(args)
+    if args.workload == "learned":
+        return learned_main(args)
 
     import numpy as np
     import torch

@@ -118,6 +118,8 @@ def lib():
             "mfs_mlp_train_step": (ci, [vp, vp, vp, vp, i64, ctypes.c_float, ctypes.c_float,
                                         ctypes.POINTER(f64)]),
             "mfs_mlp_predict": (ci, [vp, vp, vp, i64, vp]),
+            "mfs_mlp_fit_step": (ci, [vp, vp, vp, i32, vp, vp, vp, vp, vp, ctypes.c_float, ctypes.c_float,
+                                      ctypes.c_float, ci, ctypes.POINTER(f64)]),
             "mfs_mlp_infer": (ci, [vp, vp, vp, i64, vp]),
             "mfs_ctx_track_kernel": (ci, [vp, ci]),
             "mfs_ctx_tracked": (ci, [vp, ci, ctypes.POINTER(i64), ctypes.POINTER(f64), ctypes.POINTER(f64)]),

@@ -303,9 +303,51 @@ __global__ __launch_bounds__(kThreads) void k_infer(int64_t P, const float *__re
   }
 }
 
+__global__ void k_scale(int64_t P, float s, float *__restrict__ x) {
+  const int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x;
+  if (i < P) x[i] *= s;
+}
+
+__global__ void k_sqnorm(int64_t P, const float *__restrict__ r, float w, double *__restrict__ acc) {
+  __shared__ double red[kThreads];
+  const int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x;
+  const double v = (i < P) ? double(r[i]) : 0.0;
+  red[threadIdx.x] = v * v * double(w);
+  __syncthreads();
+  for (int o = kThreads / 2; o > 0; o >>= 1) {
+    if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) atomicAdd(acc, red[0]);
+}
+
+__global__ void k_fit_dout(int64_t P, const float *__restrict__ g, const float *__restrict__ q,
+                           float c, float *__restrict__ dout) {
+  const int64_t j = int64_t(blockIdx.x) * kThreads + threadIdx.x;
+  if (j >= P) return;
+  const float d = c * g[j];
+  for (int e = 0; e < 6; ++e) dout[j * 6 + e] = d * q[j * 6 + e];
+}
+
 inline unsigned blocks_for(int64_t count) { return unsigned((count + kThreads - 1) / kThreads); }
 
 }  // namespace
+
+hipError_t mlp_scale(hipStream_t st, int64_t P, float s, float *x) {
+  k_scale<<<blocks_for(P), kThreads, 0, st>>>(P, s, x);
+  return hipGetLastError();
+}
+
+hipError_t mlp_sqnorm(hipStream_t st, int64_t P, const float *r, float w, double *acc) {
+  k_sqnorm<<<blocks_for(P), kThreads, 0, st>>>(P, r, w, acc);
+  return hipGetLastError();
+}
+
+hipError_t mlp_fit_dout(hipStream_t st, int64_t P, const float *g, const float *q, float c,
+                        float *dout) {
+  k_fit_dout<<<blocks_for(P), kThreads, 0, st>>>(P, g, q, c, dout);
+  return hipGetLastError();
+}
 
 int32_t mlp_feature_dim() { return kFeat; }
 

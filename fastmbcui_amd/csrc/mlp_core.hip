@@ -8,6 +8,7 @@
 // the forward and backward passes read.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <random>
@@ -37,6 +38,9 @@ hipError_t mlp_adam(hipStream_t st, int64_t count, int64_t count_w, float *w, ui
 hipError_t mlp_to_bf16(hipStream_t st, int64_t count, const float *w, uint16_t *wb);
 hipError_t mlp_infer(hipStream_t st, int64_t P, const float *feat, const float *q,
                      const uint16_t *wb, const float *bias_all, float *x);
+hipError_t mlp_scale(hipStream_t st, int64_t P, float s, float *x);
+hipError_t mlp_sqnorm(hipStream_t st, int64_t P, const float *r, float w, double *acc);
+hipError_t mlp_fit_dout(hipStream_t st, int64_t P, const float *g, const float *q, float c, float *dout);
 }  // namespace mfs
 
 using namespace mfs;
@@ -77,7 +81,7 @@ struct mfs_mlp {
   std::string err;
   int64_t w_off[kLayers] = {}, b_off[kLayers] = {};
   int64_t n_w = 0, n_all = 0;
-  Buf w, wb, g, m, v, act, gbuf0, gbuf1, out, dout, loss;
+  Buf w, wb, g, m, v, act, gbuf0, gbuf1, out, dout, loss, fit;
   int64_t cap_points = 0;
   int32_t step = 0;
   double *hloss = nullptr;  // pinned
@@ -200,7 +204,7 @@ void mfs_mlp_destroy(mfs_mlp *net) {
   (void)hipSetDevice(net->device);
   (void)hipStreamSynchronize(net->stream);
   for (Buf *b : {&net->w, &net->wb, &net->g, &net->m, &net->v, &net->act, &net->gbuf0, &net->gbuf1,
-                 &net->out, &net->dout, &net->loss})
+                 &net->out, &net->dout, &net->loss, &net->fit})
     b->release();
   if (net->hloss) (void)hipHostFree(net->hloss);
   delete net;
@@ -287,6 +291,73 @@ int mfs_mlp_train_step(mfs_mlp *net, const float *feat, const float *q, const fl
                        int64_t P, float weight, float lr, double *loss) {
   int rc = mfs_mlp_loss_grad(net, feat, q, target, P, weight, loss);
   if (rc != MFS_OK) return rc;
+  return mfs_mlp_adam_step(net, lr, 0.9f, 0.999f, 1e-8f);
+}
+
+// Fit loss: sum over samples s of w ||M_s (scale x_s) - b_s||^2, x_s the head's
+// output on sample s's n_s points (samples back to back in feat / q), M_s
+// applied matrix-free from the packed records (k_points.hip).  Gradient:
+// dL/dx_s = 2 w scale M_s^T (M_s scale x_s - b_s).
+int mfs_mlp_fit_step(mfs_mlp *net, const float *feat, const float *q, int32_t samples,
+                     const int64_t *offsets, const float *const *tgt32, const float *const *src32,
+                     const int64_t *m, const float *const *b, float scale, float weight, float lr,
+                     int normal, double *loss) {
+  if (net == nullptr || feat == nullptr || q == nullptr || samples <= 0 || offsets == nullptr ||
+      tgt32 == nullptr || src32 == nullptr || m == nullptr || b == nullptr)
+    return MFS_ERR_ARG;
+  MHIPCHK(hipSetDevice(net->device), "hipSetDevice");
+  const int64_t P = offsets[samples];
+  int rc = ensure_points(net, P);
+  if (rc != MFS_OK) return rc;
+  if ((rc = forward(net, feat, P)) != MFS_OK) return rc;
+  int64_t max_m = 0, max_n = 0;
+  for (int32_t s = 0; s < samples; ++s) {
+    max_m = std::max<int64_t>(max_m, m[s]);
+    max_n = std::max<int64_t>(max_n, offsets[s + 1] - offsets[s]);
+  }
+  // scratch: x (P), r (max_m), g (P), split partials
+  const size_t part = size_t(std::max(points_splits(max_m, max_n, 0) * max_m,
+                                      points_splits(max_n, max_m, 0) * max_n));
+  MHIPCHK(net->fit.ensure((size_t(P) * 3 + size_t(max_m)) * sizeof(float) + part * sizeof(double)),
+          "alloc fit scratch");
+  float *x = net->fit.as<float>();
+  float *g = x + P;
+  float *h = g + P;
+  float *r = h + P;
+  double *partial = reinterpret_cast<double *>(r + ((max_m + 1) / 2) * 2);
+  MHIPCHK(mlp_head(net->stream, P, net->out.as<float>(), q, nullptr, 0.f, x, nullptr, nullptr), "head");
+  MHIPCHK(mlp_scale(net->stream, P, scale, x), "scale");
+  MHIPCHK(hipMemsetAsync(net->loss.p, 0, 8, net->stream), "clear loss");
+  for (int32_t s = 0; s < samples; ++s) {
+    PointsOp op;
+    op.tgt32 = tgt32[s];
+    op.src32 = src32[s];
+    op.m = m[s];
+    op.n = offsets[s + 1] - offsets[s];
+    const float *xs = x + offsets[s];
+    // r = b - M x (rows pass with the boundary data subtracted from)
+    MHIPCHK(points_rows<float>(net->stream, op, 1, xs, b[s], 1, 0, 1, r, partial, nullptr), "M x");
+    if (!normal) MHIPCHK(mlp_sqnorm(net->stream, op.m, r, weight, net->loss.as<double>()), "loss");
+    // g = M^T r
+    MHIPCHK(points_cols<float>(net->stream, op, 1, r, partial, nullptr), "M^T r");
+    MHIPCHK(points_cols_reduce<float>(net->stream, op, 1, partial, g + offsets[s]), "M^T r");
+    if (normal) {
+      // loss w |M^T r|^2, dL/dx = -2 w M^T M (M^T r): h = M g, g <- M^T h
+      MHIPCHK(mlp_sqnorm(net->stream, op.n, g + offsets[s], weight, net->loss.as<double>()), "loss");
+      MHIPCHK(points_rows<float>(net->stream, op, 1, g + offsets[s], nullptr, 1, 0, 1, h, partial, nullptr),
+              "M g");
+      MHIPCHK(points_cols<float>(net->stream, op, 1, h, partial, nullptr), "M^T M g");
+      MHIPCHK(points_cols_reduce<float>(net->stream, op, 1, partial, g + offsets[s]), "M^T M g");
+    }
+  }
+  // dL/dx = -2 w scale g ; dout = dL/dx q
+  MHIPCHK(mlp_fit_dout(net->stream, P, g, q, -2.f * weight * scale, net->dout.as<float>()), "dout");
+  if ((rc = backward(net, feat, P)) != MFS_OK) return rc;
+  if (loss != nullptr) {
+    MHIPCHK(hipMemcpyAsync(net->hloss, net->loss.p, 8, hipMemcpyDeviceToHost, net->stream), "copy");
+    MHIPCHK(hipStreamSynchronize(net->stream), "sync");
+    *loss = *net->hloss;
+  }
   return mfs_mlp_adam_step(net, lr, 0.9f, 0.999f, 1e-8f);
 }
 

@@ -127,6 +127,21 @@ class Samples:
     n: int            # points per sample
     count: int
     solve_ms: float
+    t32: object = None  # S x n x 12 packed targets (boundary-fit loss)
+    s32: object = None  # S x n x 8 packed sources
+    b: object = None    # S x n boundary data (FP32)
+    mtb: object = None  # S x n  M^T b (FP32)
+
+
+def _pack(x, nx, y):
+    """FP32 point records of the matrix-free kernels (hi/lo coordinates)."""
+    import torch
+    xh = x.float()
+    yh = y.float()
+    z = torch.zeros_like(xh)
+    t32 = torch.cat([xh, (x - xh.double()).float(), nx.float(), z], dim=1).contiguous()
+    s32 = torch.cat([yh, (y - yh.double()).float(), z[:, :2]], dim=1).contiguous()
+    return t32, s32
 
 
 def coin_trajectory(seed, n_bodies=8, pts_per_body=512, box=10.0):
@@ -140,7 +155,7 @@ def generate(net: Network, seeds, steps, every=1, dt=1e-2, rho=3.0, tol=1e-6, n_
     """Solve every `every`-th pose of `steps`-step trajectories (points form,
     warm-started from the previous solve) and keep (features, q, strengths)."""
     import torch
-    feats, qs, xs = [], [], []
+    feats, qs, xs, t32s, s32s, bs, mtbs = [], [], [], [], [], [], []
     t_solve = 0.0
     ctx = _lib.Context(net.device.index)
     dev = net.device
@@ -155,16 +170,23 @@ def generate(net: Network, seeds, steps, every=1, dt=1e-2, rho=3.0, tol=1e-6, n_
                 continue
             sc = rs.scene()
             t0 = time.perf_counter()
+            x, nx, y = (torch.from_numpy(a).to(dev) for a in (sc.x, sc.nx, sc.y))
+            B = torch.from_numpy(sc.b.astype(np.float32)).to(dev)
             ctx.plan(sc.y, rho)
-            ctx.factor_points(torch.from_numpy(sc.x).to(dev), torch.from_numpy(sc.nx).to(dev),
-                              torch.from_numpy(sc.y).to(dev), _lib.MFS_FP32)
-            X = ctx.solve_points(torch.from_numpy(sc.b.astype(np.float32)).to(dev), tol=tol, x0=X)
+            ctx.factor_points(x, nx, y, _lib.MFS_FP32)
+            X = ctx.solve_points(B, tol=tol, x0=X)
             t_solve += time.perf_counter() - t0
             f, q = net.features(rs)
             feats.append(f)
             qs.append(q)
             xs.append(X[:, 0].clone())
-    return Samples(torch.cat(feats), torch.cat(qs), torch.cat(xs), n, len(xs), t_solve * 1e3)
+            t32, s32 = _pack(x, nx, y)
+            t32s.append(t32)
+            s32s.append(s32)
+            bs.append(B[:, 0].clone())
+            mtbs.append(ctx.points_apply(B, trans=True)[:, 0].clone())
+    return Samples(torch.cat(feats), torch.cat(qs), torch.cat(xs), n, len(xs), t_solve * 1e3,
+                   torch.stack(t32s), torch.stack(s32s), torch.stack(bs), torch.stack(mtbs))
 
 
 def train(net: Network, data: Samples, epochs=50, batch_samples=8, lr=1e-3, seed=0, log=None):
@@ -191,6 +213,54 @@ def train(net: Network, data: Samples, epochs=50, batch_samples=8, lr=1e-3, seed
         losses.append(tot / data.count)
         if log:
             log(f"epoch {ep + 1}/{epochs}: relative loss {losses[-1]:.4e}")
+    return losses
+
+
+def train_fit(net: Network, data: Samples, epochs=50, batch_samples=8, lr=3e-4, seed=0, log=None,
+              normal=True, lr_final_frac=0.1):
+    """Adam on the data-space loss of the guesses over minibatches of whole
+    samples, M applied matrix-free on the GPU: the normal-equation residual
+    sum ||M^T (M x_pred - b)||^2 / sum ||M^T b||^2 (normal=True), which is
+    where PCG starts from, or the boundary misfit sum ||M x_pred - b||^2 /
+    sum ||b||^2."""
+    import torch
+    rng = np.random.default_rng(seed)
+    n = data.n
+    if net.scale == 1.0:
+        net.scale = float(torch.sqrt((data.x.double() ** 2).mean()).item()) or 1.0
+    losses = []
+    elt32 = data.t32.element_size() * n * data.t32.shape[2]
+    els32 = data.s32.element_size() * n * data.s32.shape[2]
+    elb = data.b.element_size() * n
+    for ep in range(epochs):
+        order = rng.permutation(data.count)
+        tot = 0.0
+        lr_ep = lr * lr_final_frac ** (ep / max(epochs - 1, 1))   # exponential decay
+        for b0 in range(0, data.count, batch_samples):
+            idx = order[b0:b0 + batch_samples]
+            S = len(idx)
+            rows = (torch.from_numpy(idx).to(data.feat.device)[:, None] * n +
+                    torch.arange(n, device=data.feat.device)[None, :]).reshape(-1)
+            f = data.feat.index_select(0, rows).contiguous()
+            q = data.q.index_select(0, rows).contiguous()
+            offs = np.arange(S + 1, dtype=np.int64) * n
+            ms = np.full(S, n, dtype=np.int64)
+            T = (ctypes.c_void_p * S)(*[data.t32.data_ptr() + int(i) * elt32 for i in idx])
+            Sr = (ctypes.c_void_p * S)(*[data.s32.data_ptr() + int(i) * els32 for i in idx])
+            Bp = (ctypes.c_void_p * S)(*[data.b.data_ptr() + int(i) * elb for i in idx])
+            sel = torch.from_numpy(idx).to(data.b.device)
+            ref = data.mtb if normal else data.b
+            w = 1.0 / max(float((ref[sel].double() ** 2).sum()), 1e-30)
+            loss = ctypes.c_double(0)
+            net._stream()
+            net._check(_lib.lib().mfs_mlp_fit_step(net._h, _lib._ptr(f), _lib._ptr(q), S, offs.ctypes.data, T,
+                                                   Sr, ms.ctypes.data, Bp, float(net.scale), float(w),
+                                                   float(lr_ep), int(bool(normal)), ctypes.byref(loss)),
+                       "fit_step")
+            tot += loss.value * S
+        losses.append(tot / data.count)
+        if log:
+            log(f"epoch {ep + 1}/{epochs}: relative boundary misfit {losses[-1]:.4e}")
     return losses
 
 

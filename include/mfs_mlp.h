@@ -54,6 +54,19 @@ int mfs_mlp_adam_step(mfs_mlp *net, float lr, float beta1, float beta2, float ep
 /* loss_grad + Adam (beta 0.9, 0.999, eps 1e-8). */
 int mfs_mlp_train_step(mfs_mlp *net, const float *feat, const float *q, const float *target,
                        int64_t P, float weight, float lr, double *loss);
+/* One Adam step on the boundary-fit loss sum_s weight ||M_s x_s - b_s||^2 of
+ * `samples` scenes whose points lie back to back in feat / q (sample s:
+ * rows offsets[s] .. offsets[s+1]), x_s = scale * (head output), M_s applied
+ * matrix-free from its packed FP32 records tgt32[s] (m[s] x 12) and src32[s]
+ * (n_s x 8) as in mfs_factor_points, b_s (m[s]) FP32 boundary data.  With
+ * normal != 0 the loss is sum_s weight ||M_s^T (M_s x_s - b_s)||^2 instead:
+ * the normal-equation residual a PCG started from x_s begins with.  Either
+ * loss weighs the strengths by what they do to the data, which is what a
+ * PCG start needs, instead of by their size. */
+int mfs_mlp_fit_step(mfs_mlp *net, const float *feat, const float *q, int32_t samples,
+                     const int64_t *offsets, const float *const *tgt32, const float *const *src32,
+                     const int64_t *m, const float *const *b, float scale, float weight, float lr,
+                     int normal, double *loss);
 /* Strengths x (P, FP32): layer-by-layer GEMMs / the fused inference kernel. */
 int mfs_mlp_predict(mfs_mlp *net, const float *feat, const float *q, int64_t P, float *x);
 int mfs_mlp_infer(mfs_mlp *net, const float *feat, const float *q, int64_t P, float *x);

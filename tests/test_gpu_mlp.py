@@ -97,9 +97,12 @@ def test_trained_network_guess_is_judged_by_the_residual():
     # percent error saves none).
     net = warmstart.Network(seed=11)
     data = warmstart.generate(net, seeds=[1000, 1001, 1002], steps=60, every=3)
-    losses = warmstart.train(net, data, epochs=30, batch_samples=6, lr=1e-3)
-    print(f"samples {data.count}, relative loss {losses[0]:.3e} -> {losses[-1]:.3e}", flush=True)
+    losses = warmstart.train(net, data, epochs=10, batch_samples=6, lr=1e-3)
+    fit = warmstart.train_fit(net, data, epochs=20, batch_samples=6)
+    print(f"samples {data.count}, strength loss {losses[0]:.3e} -> {losses[-1]:.3e}, normal-residual "
+          f"loss {fit[0]:.3e} -> {fit[-1]:.3e}", flush=True)
     assert losses[-1] < 0.5 * losses[0]
+    assert fit[-1] < fit[0]
     rs = warmstart.coin_trajectory(1009)
     ctx = _lib.Context(0)
     prev = None
