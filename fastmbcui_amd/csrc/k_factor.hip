@@ -287,37 +287,49 @@ __global__ __launch_bounds__(kThreads) void k_super_factor(const FactorScene *__
       ok[a] = p < ns;
       col[a] = Mt + int64_t(ok[a] ? src[p] : src[0]) * ldmt + 4 * (lane >> 4);
     }
-    constexpr int FSTEP = 16 * KS;  // ldmt is a multiple of 128, hence of FSTEP
-    auto load = [&](int64_t f0, float4 (&v)[T]) {
+    // W float4 per tile and step: each wave reads 16 W consecutive rows (64 W
+    // bytes) of every column -- whole 128-byte lines from W = 2 on; the small
+    // tiles afford W = 4 registers.
+    constexpr int W = LARGE ? 1 : (T <= 4 ? 4 : 2);
+    constexpr int FSTEP = 16 * W * KS;  // ldmt is a multiple of 128, hence of FSTEP
+    auto load = [&](int64_t f0, float4 (&v)[T][W]) {
 #pragma unroll
       for (int a = 0; a < T; ++a)
-        v[a] = ok[a] ? *reinterpret_cast<const float4 *>(col[a] + f0) : make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+        for (int u = 0; u < W; ++u)
+          v[a][u] = ok[a] ? *reinterpret_cast<const float4 *>(col[a] + f0 + 16 * u)
+                          : make_float4(0.f, 0.f, 0.f, 0.f);
     };
     constexpr bool PREFETCH = !LARGE;  // the large tiles need the registers
-    float4 cur[T], nxt[PREFETCH ? T : 1];
-    if (PREFETCH) load(kph * 16, cur);
-    for (int64_t f0 = kph * 16; f0 < ldmt; f0 += FSTEP) {
+    float4 cur[T][W], nxt[PREFETCH ? T : 1][W];
+    if (PREFETCH) load(kph * 16 * W, cur);
+    for (int64_t f0 = kph * 16 * W; f0 < ldmt; f0 += FSTEP) {
       if constexpr (PREFETCH) {
         if (f0 + FSTEP < ldmt) load(f0 + FSTEP, nxt);
       } else {
         load(f0, cur);
       }
-      TC fr[T];
 #pragma unroll
-      for (int a = 0; a < T; ++a) fr[a] = TC(cur[a].x);
-      step(fr);
+      for (int u = 0; u < W; ++u) {
+        TC fr[T];
 #pragma unroll
-      for (int a = 0; a < T; ++a) fr[a] = TC(cur[a].y);
-      step(fr);
+        for (int a = 0; a < T; ++a) fr[a] = TC(cur[a][u].x);
+        step(fr);
 #pragma unroll
-      for (int a = 0; a < T; ++a) fr[a] = TC(cur[a].z);
-      step(fr);
+        for (int a = 0; a < T; ++a) fr[a] = TC(cur[a][u].y);
+        step(fr);
 #pragma unroll
-      for (int a = 0; a < T; ++a) fr[a] = TC(cur[a].w);
-      step(fr);
+        for (int a = 0; a < T; ++a) fr[a] = TC(cur[a][u].z);
+        step(fr);
+#pragma unroll
+        for (int a = 0; a < T; ++a) fr[a] = TC(cur[a][u].w);
+        step(fr);
+      }
       if constexpr (PREFETCH) {
 #pragma unroll
-        for (int a = 0; a < T; ++a) cur[a] = nxt[a];
+        for (int a = 0; a < T; ++a)
+#pragma unroll
+          for (int u = 0; u < W; ++u) cur[a][u] = nxt[a][u];
       }
     }
   }
