@@ -65,6 +65,7 @@ def _parse():
     ap.add_argument("--ts-points", type=int, default=2048)
     ap.add_argument("--ts-dt", type=float, default=1e-2)
     ap.add_argument("--replan-every", type=int, default=1)
+    ap.add_argument("--ts-maxit", type=int, default=4000, help="timestep: inner PCG iteration cap per step")
     ap.add_argument("--copy-gb", type=float, default=0.0,
                     help="run one plain device copy of this many GB first (counter calibration)")
     return ap.parse_args()
@@ -115,7 +116,7 @@ def timestep_main(args):
     rs = scenes.mixed_bodies(seed=0, n_bodies=args.ts_bodies, pts_per_body=args.ts_points)
     n = rs.local_x.shape[0]
     _log(f"bench: timestep scene {args.ts_bodies} bodies x {args.ts_points} points (N={n})")
-    ts = TimeStepper(rs, rho=4.0, tol=1e-8, replan_every=args.replan_every)
+    ts = TimeStepper(rs, rho=4.0, tol=1e-8, replan_every=args.replan_every, maxit=args.ts_maxit)
     first = ts.step()
     _log(f"bench: step 0 (cold) {first}")
     infos = []
