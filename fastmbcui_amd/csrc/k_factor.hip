@@ -104,7 +104,10 @@ __global__ __launch_bounds__(kThreads) void k_transpose(const float *__restrict_
     float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
     if (f < m) {
       if (VEC && c + 3 < n) {
-        v = *reinterpret_cast<const float4 *>(M + f * n + c);
+        // M is read once: non-temporal, so it does not evict what L2 reuses
+        typedef float f4v __attribute__((ext_vector_type(4)));
+        const f4v t = __builtin_nontemporal_load(reinterpret_cast<const f4v *>(M + f * n + c));
+        v = make_float4(t.x, t.y, t.z, t.w);
       } else {
         if (c + 0 < n) v.x = M[f * n + c + 0];
         if (c + 1 < n) v.y = M[f * n + c + 1];

@@ -96,6 +96,14 @@ __device__ inline bool batch_done(const PcgState *st) { return st != nullptr && 
 // Optional Bsub gives y = b - M x.
 constexpr int kRowsThreads = 512;
 
+// M is streamed once per pass: non-temporal loads keep it from displacing the
+// vectors (x, y, partials) that are reused from L2.
+typedef float f4v __attribute__((ext_vector_type(4)));
+__device__ inline float4 load_stream4(const float *p) {
+  const f4v v = __builtin_nontemporal_load(reinterpret_cast<const f4v *>(p));
+  return make_float4(v.x, v.y, v.z, v.w);
+}
+
 template <typename TA, int K>
 struct RowsCfg {
   static constexpr int RW = (K == 1) ? 8 : (K * int(sizeof(TA)) >= 64 ? 2 : 4);
@@ -162,7 +170,7 @@ __global__ __launch_bounds__(kRowsThreads) void k_mv_rows(const float *__restric
         const int64_t f = row0 + i;
         const float *rowp = M + f * n;
         if (f < m && VEC && cc + 3 < n) {
-          mv[i] = *reinterpret_cast<const float4 *>(rowp + cc);
+          mv[i] = load_stream4(rowp + cc);
         } else if (f < m) {
           mv[i].x = (cc + 0 < n) ? rowp[cc + 0] : 0.f;
           mv[i].y = (cc + 1 < n) ? rowp[cc + 1] : 0.f;
@@ -262,7 +270,7 @@ __global__ __launch_bounds__(kThreads) void k_mv_cols(const float *__restrict__ 
         for (int u = 0; u < RU; ++u) {
           const float *rowp = M + (fc + f + u) * n;
           if (VEC && i0 + 3 < n) {
-            v[u] = *reinterpret_cast<const float4 *>(rowp + i0);
+            v[u] = load_stream4(rowp + i0);
           } else {
             v[u].x = rowp[i0];
             v[u].y = (i0 + 1 < n) ? rowp[i0 + 1] : 0.f;
