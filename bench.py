@@ -52,11 +52,13 @@ def _parse():
     ap.add_argument("--learned-samples-every", type=int, default=10,
                     help="learned: solve every k-th pose of the 2000-step training trajectories")
     ap.add_argument("--learned-epochs", type=int, default=50)
-    ap.add_argument("--workload", choices=["ellipsoids64", "timestep", "batch", "learned"],
+    ap.add_argument("--workload", choices=["ellipsoids64", "timestep", "batch", "learned", "caller"],
                     default="ellipsoids64",
                     help="timestep: 128 mixed bodies x 2048 points (N=262,144) in rigid motion, "
                          "rho 4, tol 1e-8, refactor every step, warm start (points form); "
                          "batch: 256 uneven scenes (2-32 bodies x 256-2048 points), one batch")
+    ap.add_argument("--caller-bodies", type=int, default=6,
+                    help="caller: bodies of 10,242 vertices in the large case")
     ap.add_argument("--batch-scenes", type=int, default=256)
     ap.add_argument("--batch-sequential", type=int, default=1,
                     help="batch: also solve the scenes one after another (0 skips)")
@@ -290,6 +292,208 @@ def learned_main(args):
     return 0
 
 
+def _torch_solid_angle(V, F, S):
+    """The reference's torch expression of compute_solid_angle
+    (utils/torch_utils.py:306-334), run by PyTorch on the same GPU: the
+    comparison point for the HIP kernel."""
+    import torch
+    F = F.long()
+    s = S.unsqueeze(1)
+    a, b, c = V[F[:, 0]] - s, V[F[:, 1]] - s, V[F[:, 2]] - s
+    num = torch.einsum("ijk,ijk->ij", torch.cross(b, c, dim=2), a)
+    l1, l2, l3 = a.norm(dim=2), b.norm(dim=2), c.norm(dim=2)
+    den = (l1 * l2 * l3 + torch.einsum("ijk,ijk->ij", a, b) * l3
+           + torch.einsum("ijk,ijk->ij", a, c) * l2 + torch.einsum("ijk,ijk->ij", b, c) * l1)
+    return 2 * torch.atan2(num, den).transpose(0, 1)
+
+
+def _torch_single_layer(S, C):
+    """The reference's compute_single_layer (utils/torch_utils.py:254-276): the dense matrix."""
+    return 1.0 / (C.unsqueeze(1) - S.unsqueeze(0)).norm(dim=2)
+
+
+def _gpu_ms(fn, reps, warmup=1):
+    import torch
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        fn()
+    e1.record()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / reps
+
+
+# FP32 operations per solid-angle entry in k_solid_angle, counted from its
+# expression (9 differences, 3 squared norms at 5, 3 square roots, N at 5,
+# 3 dot products at 5, D at 8, atan2 and the doubling at 1 each).
+SA_FLOPS_PER_ENTRY = 57
+VALU_FP32_PEAK_TFLOPS = 157.3
+
+
+def caller_main(args):
+    """Caller steps of the Kirchhoff-tensor step (KTAB:97-128) on one MI355X:
+    face solid angles M (forward, backward) and the single-layer potential
+    SL @ X (forward, backward) at the shipped scene size (3 bodies of 642
+    vertices / 1280 faces, nv = 18) and at a large size (`--caller-bodies`
+    bodies of 10,242 vertices / 20,480 faces), the reference's torch
+    expressions on the same GPU beside them, and the whole step with
+    L.backward() at the shipped size."""
+    import numpy as np
+    import torch
+
+    from fastmbcui_amd import _lib, caller, meshes, runtime
+
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    ctx = _lib.Context(0)
+    out = {}
+
+    def scene(nb, subdiv):
+        Vs, Fs, Ss, ys, off = [], [], [], [], 0
+        for b in meshes.bodies(nb, subdiv, seed=0, spacing=3.0):
+            Vs.append(b["V0"] @ b["R"].T + b["y"])
+            Ss.append(b["S0"] @ b["R"].T + b["y"])
+            Fs.append(b["F"])
+            ys.append(b["y"])
+        return Vs, Fs, Ss, ys
+
+    def flat(Vs, Fs, Ss):
+        off, F = 0, []
+        for V, Fi in zip(Vs, Fs):
+            F.append(Fi + off)
+            off += V.shape[0]
+        g = lambda a, dt: torch.as_tensor(np.ascontiguousarray(a)).to(dev, dt)
+        return (g(np.concatenate(Vs), torch.float32), g(np.concatenate(F).astype(np.int32), torch.int32),
+                g(np.concatenate(Ss), torch.float32))
+
+    for label, nb, subdiv, reps in (("shipped", 3, 3, 20), ("large", args.caller_bodies, 5, 3)):
+        Vs, Fs, Ss, ys = scene(nb, subdiv)
+        V, F, S = flat(Vs, Fs, Ss)
+        nf, n = F.shape[0], S.shape[0]
+        k = 6 * nb
+        entries = nf * n
+        C = caller.compute_face_centers(V, F)
+        X = torch.randn((n, k), device=dev)
+        G = torch.randn((nf, k), device=dev)
+        _log(f"bench caller {label}: {nb} bodies, faces {nf}, sources {n}, k {k}")
+        M = ctx.solid_angle(V, F, S)
+        sa_ms = _gpu_ms(lambda: ctx.solid_angle(V, F, S), reps)
+        gM = torch.randn_like(M)
+        sab_ms = _gpu_ms(lambda: ctx.solid_angle_backward(V, F, S, gM), max(1, reps // 2))
+        del gM, M
+        torch.cuda.empty_cache()
+        sl_ms = _gpu_ms(lambda: ctx.single_layer_apply(S, C, X), reps)
+        slb_ms = _gpu_ms(lambda: ctx.single_layer_backward(S, C, X, G), reps)
+        r = dict(faces=nf, sources=n, k=k, solid_angle_ms=round(sa_ms, 4),
+                 solid_angle_backward_ms=round(sab_ms, 4), single_layer_ms=round(sl_ms, 4),
+                 single_layer_backward_ms=round(slb_ms, 4),
+                 solid_angle_entries_per_s=entries / (sa_ms * 1e-3),
+                 solid_angle_tflops=SA_FLOPS_PER_ENTRY * entries / (sa_ms * 1e-3) / 1e12,
+                 solid_angle_write_gbs=entries * 4 / (sa_ms * 1e-3) / 1e9)
+        _log(f"bench caller {label}: solid angle {sa_ms:.3f} ms "
+             f"({r['solid_angle_tflops']:.1f} TFLOP/s, {r['solid_angle_write_gbs']:.0f} GB/s written), "
+             f"backward {sab_ms:.3f} ms; single layer {sl_ms:.3f} ms, backward {slb_ms:.3f} ms")
+        if label == "shipped" or nf * n * 3 * 4 * 12 < 200e9:
+            Fl = F.long()
+            tsa = _gpu_ms(lambda: _torch_solid_angle(V, Fl, S), max(1, reps // 2))
+            Vg, Sg = V.clone().requires_grad_(), S.clone().requires_grad_()
+            gM = torch.randn((nf, n), device=dev)
+
+            def torch_bwd():
+                Vg.grad = Sg.grad = None
+                (_torch_solid_angle(Vg, Fl, Sg) * gM).sum().backward()
+
+            tsab = _gpu_ms(torch_bwd, max(1, reps // 2))
+            tsl = _gpu_ms(lambda: _torch_single_layer(S, C) @ X, reps)
+            r.update(torch_solid_angle_ms=round(tsa, 4), torch_solid_angle_fwd_bwd_ms=round(tsab, 4),
+                     torch_single_layer_ms=round(tsl, 4))
+            _log(f"bench caller {label}: reference torch expressions on this GPU: solid angle "
+                 f"{tsa:.3f} ms, forward+backward {tsab:.3f} ms, single layer + matmul {tsl:.3f} ms")
+            del gM, Vg, Sg
+            torch.cuda.empty_cache()
+        out[label] = r
+
+    # the whole Kirchhoff step (forward + L.backward()) at the shipped size
+    runtime.reset()
+    Vs, Fs, Ss, ys = scene(3, 3)
+    g = lambda a: torch.as_tensor(np.ascontiguousarray(a)).to(dev, torch.float32)
+    V_s, S_s, y_s = [g(v) for v in Vs], [g(s) for s in Ss], [g(y) for y in ys]
+    F_s = [torch.as_tensor(f).to(dev) for f in Fs]
+    Qp = torch.eye(18, device=dev)
+    v = torch.linspace(-1, 1, 18, device=dev)
+
+    def step(hip):
+        shift = torch.zeros((3, 3), device=dev, requires_grad=True)
+        Vm = [V + shift[i] for i, V in enumerate(V_s)]
+        Sm = [S + shift[i] for i, S in enumerate(S_s)]
+        ym = [y + shift[i] for i, y in enumerate(y_s)]
+        kw = {} if hip else dict(solid_angle=_torch_solid_angle, single_layer=_torch_single_layer)
+        # the coin driver's solver settings (simuspclf1d_mfs_coins.py:21-23)
+        Mf = caller.kirchhoff_tensor(Vm, F_s, Sm, ym, Qp, 1.0, 1.2, 75, 2.7e-3, **kw)[0]
+        L = 0.5 * v @ Mf @ v
+        L.backward()
+        return shift.grad
+
+    def wall(fn, reps):
+        fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / reps * 1e3
+
+    ms_hip = wall(lambda: step(True), 5)
+    solve_ms = runtime.last_info.solve_ms
+    ms_torch = wall(lambda: step(False), 5)
+    gh, gt = step(True), step(False)
+    agree = float((gh - gt).norm() / gt.norm())
+    out["step"] = dict(ms_hip=round(ms_hip, 2), ms_torch_expressions=round(ms_torch, 2),
+                       grad_rel_diff=agree, last_backward_solve_ms=round(solve_ms, 2))
+    _log(f"bench caller step (3 bodies, forward + L.backward()): {ms_hip:.1f} ms with the HIP "
+         f"kernels, {ms_torch:.1f} ms with the reference's torch expressions (gradients agree to {agree:.1e})")
+
+    cpu = None
+    if not args.skip_cpu:
+        from oracle import caller_oracle as co
+        Vs, Fs, Ss, ys = scene(3, 3)
+        Vc = torch.as_tensor(np.concatenate(Vs))
+        off = np.cumsum([0] + [v.shape[0] for v in Vs[:-1]])
+        Fc = torch.as_tensor(np.concatenate([f + o for f, o in zip(Fs, off)]))
+        Sc = torch.as_tensor(np.concatenate(Ss))
+        torch.set_num_threads(16)
+        t0 = time.perf_counter()
+        reps = 0
+        while time.perf_counter() - t0 < 10.0:
+            co.solid_angle(Vc, Fc, Sc)
+            reps += 1
+        per = (time.perf_counter() - t0) / reps
+        cpu = dict(value=round(per * 1e3, 2), unit="ms/solid-angle matrix (3 bodies, shipped size)",
+                   cores=16, kind="port",
+                   sample=f"FP64 torch-on-CPU oracle of compute_solid_angle, {reps} calls in ~10 s")
+    big = out["large"]
+    line = {
+        "metric": "caller steps around the solve: face solid angles M and single layer SL@X on MI355X",
+        "value": round(big["solid_angle_ms"], 4), "unit": "ms per solid-angle matrix (large)",
+        "n_gpus": 1, "steps": 3, "warmup": 1, "ms_per_step": round(big["solid_angle_ms"], 4),
+        "higher_is_better": False, "scaling": "strong", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic icosphere/ellipsoid bodies",
+        "config": {"workload": f"{args.caller_bodies} bodies x 10,242 vertices / 20,480 faces; "
+                               "3 bodies x 642 / 1280 (shipped size)"},
+        "roofline": {"bound": "valu", "achieved": round(big["solid_angle_tflops"], 2),
+                     "peak": VALU_FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(big["solid_angle_tflops"] / VALU_FP32_PEAK_TFLOPS, 4),
+                     "traffic": None},
+        "cpu_baseline": cpu,
+        "detail": out,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def main():
     args = _parse()
     if args.gpus != 1:
@@ -302,6 +506,8 @@ def main():
         return batch_main(args)
     if args.workload == "learned":
         return learned_main(args)
+    if args.workload == "caller":
+        return caller_main(args)
 
     import numpy as np
     import torch

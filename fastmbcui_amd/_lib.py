@@ -94,6 +94,10 @@ def lib():
             "mfs_solve_points64": (ci, [vp, vp, i64, vp, vp, i32, f64, f64]),
             "mfs_points_residual64": (ci, [vp, vp, i64, vp, vp]),
             "mfs_points_apply": (ci, [vp, ci, ci, vp, i64, vp]),
+            "mfs_solid_angle": (ci, [vp, ci, vp, i64, vp, i64, vp, i64, vp]),
+            "mfs_solid_angle_backward": (ci, [vp, ci, vp, i64, vp, i64, vp, i64, vp, vp, vp]),
+            "mfs_single_layer_apply": (ci, [vp, ci, vp, i64, vp, i64, vp, i64, vp]),
+            "mfs_single_layer_backward": (ci, [vp, ci, vp, i64, vp, i64, vp, i64, vp, vp, vp, vp]),
             "mfs_batch_create": (ci, [ctypes.POINTER(vp), ci, vp, i32]),
             "mfs_batch_destroy": (None, [vp]),
             "mfs_batch_scene": (vp, [vp, i32]),
@@ -355,6 +359,80 @@ class Context:
                                            _ptr(out)), "mfs_points_apply")
         return out
 
+    # ---- caller steps: face solid angles and the single-layer potential ------
+    @staticmethod
+    def _caller_precision(t):
+        import torch
+        if t.dtype == torch.float32:
+            return MFS_FP32
+        if t.dtype == torch.float64:
+            return MFS_FP64
+        raise TypeError("expected a float32 or float64 tensor")
+
+    def solid_angle(self, V, F, S):
+        """M (nf x n): solid angle of each face of (V, F) seen from each source S."""
+        import torch
+        self._sync_stream()
+        prec = self._caller_precision(V)
+        V, S = V.contiguous(), S.to(V.dtype).contiguous()
+        F = F.to(torch.int32).contiguous()
+        _check_xyz(V, "V"), _check_xyz(S, "S"), _check_faces(F)
+        M = torch.empty((F.shape[0], S.shape[0]), dtype=V.dtype, device=V.device)
+        self._check(lib().mfs_solid_angle(self._h, prec, _ptr(V), V.shape[0], _ptr(F), F.shape[0],
+                                          _ptr(S), S.shape[0], _ptr(M)), "mfs_solid_angle")
+        return M
+
+    def solid_angle_backward(self, V, F, S, gM, need_v=True, need_s=True):
+        """(dL/dV, dL/dS) from dL/dM; either is None when not asked for."""
+        import torch
+        self._sync_stream()
+        prec = self._caller_precision(V)
+        V, S = V.contiguous(), S.to(V.dtype).contiguous()
+        F = F.to(torch.int32).contiguous()
+        gM = gM.to(V.dtype).contiguous()
+        _check_xyz(V, "V"), _check_xyz(S, "S"), _check_faces(F)
+        if tuple(gM.shape) != (F.shape[0], S.shape[0]):
+            raise ValueError("gM must be nf x n")
+        gV = torch.empty_like(V) if need_v else None
+        gS = torch.empty_like(S) if need_s else None
+        self._check(lib().mfs_solid_angle_backward(self._h, prec, _ptr(V), V.shape[0], _ptr(F),
+                                                   F.shape[0], _ptr(S), S.shape[0], _ptr(gM),
+                                                   _ptr(gV), _ptr(gS)), "mfs_solid_angle_backward")
+        return gV, gS
+
+    def single_layer_apply(self, S, C, X):
+        """phi (nf x k) = SL X, SL[f][s] = 1 / |C[f] - S[s]|, without storing SL."""
+        import torch
+        self._sync_stream()
+        prec = self._caller_precision(X)
+        S, C, X = S.to(X.dtype).contiguous(), C.to(X.dtype).contiguous(), X.contiguous()
+        _check_xyz(S, "S"), _check_xyz(C, "C")
+        if X.dim() != 2 or X.shape[0] != S.shape[0]:
+            raise ValueError("X must be n x k")
+        phi = torch.empty((C.shape[0], X.shape[1]), dtype=X.dtype, device=X.device)
+        self._check(lib().mfs_single_layer_apply(self._h, prec, _ptr(S), S.shape[0], _ptr(C),
+                                                 C.shape[0], _ptr(X), X.shape[1], _ptr(phi)),
+                    "mfs_single_layer_apply")
+        return phi
+
+    def single_layer_backward(self, S, C, X, gphi, need_s=True, need_c=True, need_x=True):
+        """(dL/dS, dL/dC, dL/dX) from dL/dphi; entries not asked for are None."""
+        import torch
+        self._sync_stream()
+        prec = self._caller_precision(X)
+        S, C, X = S.to(X.dtype).contiguous(), C.to(X.dtype).contiguous(), X.contiguous()
+        gphi = gphi.to(X.dtype).contiguous()
+        if tuple(gphi.shape) != (C.shape[0], X.shape[1]):
+            raise ValueError("gphi must be nf x k")
+        gS = torch.empty_like(S) if need_s else None
+        gC = torch.empty_like(C) if need_c else None
+        gX = torch.empty_like(X) if need_x else None
+        self._check(lib().mfs_single_layer_backward(self._h, prec, _ptr(S), S.shape[0], _ptr(C),
+                                                    C.shape[0], _ptr(X), X.shape[1], _ptr(gphi),
+                                                    _ptr(gS), _ptr(gC), _ptr(gX)),
+                    "mfs_single_layer_backward")
+        return gS, gC, gX
+
     def set_param(self, param, value):
         """Launch-shape parameter (MFS_PARAM_*; 0 = default)."""
         self._check(lib().mfs_ctx_set_param(self._h, int(param), int(value)), "mfs_ctx_set_param")
@@ -380,6 +458,16 @@ class Context:
         self._check(lib().mfs_ctx_tracked(self._h, int(kernel_class), ctypes.byref(n), ctypes.byref(ms),
                                           ctypes.byref(by)), "mfs_ctx_tracked")
         return n.value, ms.value, by.value
+
+
+def _check_xyz(t, name):
+    if t.dim() != 2 or t.shape[1] != 3:
+        raise ValueError(f"{name} must be (count, 3)")
+
+
+def _check_faces(F):
+    if F.dim() != 2 or F.shape[1] != 3:
+        raise ValueError("F must be (num_faces, 3)")
 
 
 def _check_dense(M, n):

@@ -95,6 +95,9 @@ struct mfs_ctx {
   int32_t cols_blocks = 0;    // tuning: blocks of the dense M^T pass (0 = default)
   DevBuf dt32, dt64, ds32, ds64, dapin, dapout;
 
+  // caller steps (solid angles, single layer): scratch and the index flag
+  DevBuf dcaller, dflag;
+
   // solve workspaces
   DevBuf dvec, dpart, ddpart, dstate, dx64, dr64, dnorm;
   PcgState *hstate = nullptr;  // pinned
@@ -1006,7 +1009,7 @@ void mfs_ctx_destroy(mfs_ctx *ctx) {
                     &ctx->dtotal, &ctx->dbands, &ctx->dcells, &ctx->dcpts, &ctx->dlong, &ctx->dval, &ctx->dtval, &ctx->dMt, &ctx->dval_alt,
                     &ctx->dtval_alt, &ctx->dthg, &ctx->dfscene, &ctx->dvec, &ctx->dpart, &ctx->ddpart, &ctx->dstate,
                     &ctx->dx64, &ctx->dr64, &ctx->dnorm, &ctx->dt32, &ctx->dt64, &ctx->ds32,
-                    &ctx->ds64, &ctx->dapin, &ctx->dapout})
+                    &ctx->ds64, &ctx->dapin, &ctx->dapout, &ctx->dcaller, &ctx->dflag})
     b->release();
   if (ctx->hstate) (void)hipHostFree(ctx->hstate);
   if (ctx->hnorm) (void)hipHostFree(ctx->hnorm);
@@ -1258,6 +1261,131 @@ int mfs_assemble_dense(mfs_ctx *ctx, const double *x, const double *nx, const do
     return MFS_ERR_ARG;
   HIPCHK(hipSetDevice(ctx->device), "hipSetDevice");
   HIPCHK(launch_assemble(ctx->stream, x, nx, y, m, n, M), "assemble");
+  return MFS_OK;
+}
+
+// ------------------------------------------------------------ caller steps
+
+namespace {
+
+bool caller_dims_ok(int64_t a, int64_t b) {
+  return a >= 0 && b >= 0 && a <= INT32_MAX && b <= INT32_MAX;
+}
+
+// Reads the device index flag back (synchronises the context stream).
+int caller_check_flag(mfs_ctx *ctx) {
+  int32_t bad = 0;
+  HIPCHK(hipMemcpyAsync(&bad, ctx->dflag.p, sizeof(bad), hipMemcpyDeviceToHost, ctx->stream),
+         "read index flag");
+  HIPCHK(hipStreamSynchronize(ctx->stream), "synchronize");
+  if (bad) return ctx->fail(MFS_ERR_ARG, "solid_angle: face index outside [0, nv)");
+  return MFS_OK;
+}
+
+}  // namespace
+
+int mfs_solid_angle(mfs_ctx *ctx, int precision, const void *V, int64_t nv, const int32_t *F,
+                    int64_t nf, const void *S, int64_t n, void *M) {
+  if (ctx == nullptr) return MFS_ERR_ARG;
+  if ((precision != MFS_FP32 && precision != MFS_FP64) || !caller_dims_ok(nv, nf) ||
+      !caller_dims_ok(n, 0) || (nf > 0 && (F == nullptr || V == nullptr)) ||
+      (n > 0 && S == nullptr) || (nf > 0 && n > 0 && M == nullptr))
+    return ctx->fail(MFS_ERR_ARG, "solid_angle: bad arguments");
+  if (nf > 0 && nv == 0) return ctx->fail(MFS_ERR_ARG, "solid_angle: faces without vertices");
+  const int64_t blocks = (n + kThreads - 1) / kThreads * ((nf + 63) / 64);
+  if (blocks > INT32_MAX) return ctx->fail(MFS_ERR_ARG, "solid_angle: too large");
+  HIPCHK(hipSetDevice(ctx->device), "hipSetDevice");
+  HIPCHK(ctx->dcaller.ensure(solid_angle_scratch_bytes(nv, nf, n)), "alloc caller scratch");
+  HIPCHK(ctx->dflag.ensure(256), "alloc");
+  HIPCHK(hipMemsetAsync(ctx->dflag.p, 0, sizeof(int32_t), ctx->stream), "clear flag");
+  int32_t *bad = ctx->dflag.as<int32_t>();
+  if (precision == MFS_FP32)
+    HIPCHK(solid_angle_fwd<float>(ctx->stream, static_cast<const float *>(V), nv, F, nf,
+                                  static_cast<const float *>(S), n, static_cast<float *>(M),
+                                  ctx->dcaller.p, bad), "solid_angle");
+  else
+    HIPCHK(solid_angle_fwd<double>(ctx->stream, static_cast<const double *>(V), nv, F, nf,
+                                   static_cast<const double *>(S), n, static_cast<double *>(M),
+                                   ctx->dcaller.p, bad), "solid_angle");
+  return caller_check_flag(ctx);
+}
+
+int mfs_solid_angle_backward(mfs_ctx *ctx, int precision, const void *V, int64_t nv,
+                             const int32_t *F, int64_t nf, const void *S, int64_t n,
+                             const void *gM, void *gV, void *gS) {
+  if (ctx == nullptr) return MFS_ERR_ARG;
+  if ((precision != MFS_FP32 && precision != MFS_FP64) || !caller_dims_ok(nv, nf) ||
+      !caller_dims_ok(n, 0) || (nf > 0 && (F == nullptr || V == nullptr)) ||
+      (n > 0 && S == nullptr) || (nf > 0 && n > 0 && gM == nullptr))
+    return ctx->fail(MFS_ERR_ARG, "solid_angle_backward: bad arguments");
+  if (nf > 0 && nv == 0)
+    return ctx->fail(MFS_ERR_ARG, "solid_angle_backward: faces without vertices");
+  HIPCHK(hipSetDevice(ctx->device), "hipSetDevice");
+  HIPCHK(ctx->dcaller.ensure(solid_angle_scratch_bytes(nv, nf, n)), "alloc caller scratch");
+  HIPCHK(ctx->dflag.ensure(256), "alloc");
+  HIPCHK(hipMemsetAsync(ctx->dflag.p, 0, sizeof(int32_t), ctx->stream), "clear flag");
+  int32_t *bad = ctx->dflag.as<int32_t>();
+  if (precision == MFS_FP32)
+    HIPCHK(solid_angle_bwd<float>(ctx->stream, static_cast<const float *>(V), nv, F, nf,
+                                  static_cast<const float *>(S), n,
+                                  static_cast<const float *>(gM), static_cast<float *>(gV),
+                                  static_cast<float *>(gS), ctx->dcaller.p, bad),
+           "solid_angle_backward");
+  else
+    HIPCHK(solid_angle_bwd<double>(ctx->stream, static_cast<const double *>(V), nv, F, nf,
+                                   static_cast<const double *>(S), n,
+                                   static_cast<const double *>(gM), static_cast<double *>(gV),
+                                   static_cast<double *>(gS), ctx->dcaller.p, bad),
+           "solid_angle_backward");
+  return caller_check_flag(ctx);
+}
+
+int mfs_single_layer_apply(mfs_ctx *ctx, int precision, const void *S, int64_t n, const void *C,
+                           int64_t nf, const void *X, int64_t k, void *phi) {
+  if (ctx == nullptr) return MFS_ERR_ARG;
+  if ((precision != MFS_FP32 && precision != MFS_FP64) || !caller_dims_ok(n, nf) ||
+      !caller_dims_ok(k, 0) || (n > 0 && S == nullptr) || (nf > 0 && C == nullptr) ||
+      (n > 0 && k > 0 && X == nullptr) || (nf > 0 && k > 0 && phi == nullptr))
+    return ctx->fail(MFS_ERR_ARG, "single_layer_apply: bad arguments");
+  HIPCHK(hipSetDevice(ctx->device), "hipSetDevice");
+  HIPCHK(ctx->dcaller.ensure(single_layer_scratch_bytes(nf, n, k)), "alloc caller scratch");
+  if (precision == MFS_FP32)
+    HIPCHK(single_layer_fwd<float>(ctx->stream, static_cast<const float *>(S), n,
+                                   static_cast<const float *>(C), nf,
+                                   static_cast<const float *>(X), k, static_cast<float *>(phi),
+                                   ctx->dcaller.p), "single_layer_apply");
+  else
+    HIPCHK(single_layer_fwd<double>(ctx->stream, static_cast<const double *>(S), n,
+                                    static_cast<const double *>(C), nf,
+                                    static_cast<const double *>(X), k, static_cast<double *>(phi),
+                                    ctx->dcaller.p), "single_layer_apply");
+  return MFS_OK;
+}
+
+int mfs_single_layer_backward(mfs_ctx *ctx, int precision, const void *S, int64_t n,
+                              const void *C, int64_t nf, const void *X, int64_t k,
+                              const void *gphi, void *gS, void *gC, void *gX) {
+  if (ctx == nullptr) return MFS_ERR_ARG;
+  if ((precision != MFS_FP32 && precision != MFS_FP64) || !caller_dims_ok(n, nf) ||
+      !caller_dims_ok(k, 0) || (n > 0 && S == nullptr) || (nf > 0 && C == nullptr) ||
+      (n > 0 && k > 0 && X == nullptr) || (nf > 0 && k > 0 && gphi == nullptr))
+    return ctx->fail(MFS_ERR_ARG, "single_layer_backward: bad arguments");
+  HIPCHK(hipSetDevice(ctx->device), "hipSetDevice");
+  HIPCHK(ctx->dcaller.ensure(single_layer_scratch_bytes(nf, n, k)), "alloc caller scratch");
+  if (precision == MFS_FP32)
+    HIPCHK(single_layer_bwd<float>(ctx->stream, static_cast<const float *>(S), n,
+                                   static_cast<const float *>(C), nf,
+                                   static_cast<const float *>(X), k,
+                                   static_cast<const float *>(gphi), static_cast<float *>(gS),
+                                   static_cast<float *>(gC), static_cast<float *>(gX),
+                                   ctx->dcaller.p), "single_layer_backward");
+  else
+    HIPCHK(single_layer_bwd<double>(ctx->stream, static_cast<const double *>(S), n,
+                                    static_cast<const double *>(C), nf,
+                                    static_cast<const double *>(X), k,
+                                    static_cast<const double *>(gphi), static_cast<double *>(gS),
+                                    static_cast<double *>(gC), static_cast<double *>(gX),
+                                    ctx->dcaller.p), "single_layer_backward");
   return MFS_OK;
 }
 

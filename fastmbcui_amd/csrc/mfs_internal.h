@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cstddef>
 #include <cstdint>
 
 namespace mfs {
@@ -171,5 +172,27 @@ hipError_t points_cols_reduce(hipStream_t st, const PointsOp &op, int K, const d
 // Kernel assembly of M from FP64 points.
 hipError_t launch_assemble(hipStream_t st, const double *x, const double *nx, const double *y,
                            int64_t m, int64_t n, float *M);
+
+// ------------------------------------------------- caller steps (k_caller.hip)
+// Face solid angles M (nf x n) of the mesh (V, F) seen from the sources S and
+// their backward; scratch holds solid_angle_scratch_bytes(nv, nf, n) bytes and
+// *bad (device) is set to 1 when F holds an index outside [0, nv).
+size_t solid_angle_scratch_bytes(int64_t nv, int64_t nf, int64_t n);
+template <typename T>
+hipError_t solid_angle_fwd(hipStream_t st, const T *V, int64_t nv, const int32_t *F, int64_t nf,
+                           const T *S, int64_t n, T *M, void *scratch, int32_t *bad);
+template <typename T>
+hipError_t solid_angle_bwd(hipStream_t st, const T *V, int64_t nv, const int32_t *F, int64_t nf,
+                           const T *S, int64_t n, const T *G, T *gV, T *gS, void *scratch,
+                           int32_t *bad);
+// phi (nf x k) = SL X with SL[f][s] = 1 / |C_f - S_s|, and its backward.
+size_t single_layer_scratch_bytes(int64_t nf, int64_t n, int64_t k);
+template <typename T>
+hipError_t single_layer_fwd(hipStream_t st, const T *S, int64_t n, const T *C, int64_t nf,
+                            const T *X, int64_t k, T *phi, void *scratch);
+template <typename T>
+hipError_t single_layer_bwd(hipStream_t st, const T *S, int64_t n, const T *C, int64_t nf,
+                            const T *X, int64_t k, const T *Gp, T *gS, T *gC, T *gX,
+                            void *scratch);
 
 }  // namespace mfs

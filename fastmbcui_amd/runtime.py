@@ -99,6 +99,12 @@ def _slot(device):
         return s
 
 
+def caller_context(device):
+    """The context of (device, current stream), for the caller-step kernels
+    (solid angles, single layer); they leave its plan and factor alone."""
+    return _slot(device).ctx
+
+
 def prepared(M, Vs_cpu, rho, precision):
     """Context with a plan for Vs_cpu and a factor of M, reusing what matches."""
     global last_info

@@ -160,6 +160,41 @@ int mfs_points_apply(mfs_ctx *ctx, int trans, int precision, const void *in, int
 int mfs_assemble_dense(mfs_ctx *ctx, const double *x, const double *nx, const double *y,
                        int64_t m, int64_t n, float *M);
 
+/* ---- Steps either side of the solve in the caller (device arrays of float
+ * for MFS_FP32 or double for MFS_FP64; F is int32, row-major throughout).
+ * The two solid-angle calls return after their work is done (they check F's
+ * indices on the device and read the verdict back; an index outside [0, nv)
+ * gives MFS_ERR_ARG); the single-layer calls are asynchronous. */
+
+/* M (nf x n): M[f][s] = 2 atan2(N, D), the solid angle of face f of the mesh
+ * (V: nv x 3, F: nf x 3) seen from source s (S: n x 3), with R_i = V[F[f][i]]
+ * - S[s], N = R1.(R2 x R3), D = l1 l2 l3 + (R1.R2) l3 + (R1.R3) l2 +
+ * (R2.R3) l1.  Replaces compute_solid_angle (utils/torch_utils.py:306-334) as
+ * called by core/KTAB_Mfs_spclf1d.py:114 (one call per body, rows stacked). */
+int mfs_solid_angle(mfs_ctx *ctx, int precision, const void *V, int64_t nv, const int32_t *F,
+                    int64_t nf, const void *S, int64_t n, void *M);
+
+/* Backward of mfs_solid_angle: given gM = dL/dM (nf x n), writes gV = dL/dV
+ * (nv x 3) and gS = dL/dS (n x 3); either may be NULL.  Gradients are formed
+ * in FP64.  The autograd of compute_solid_angle (utils/torch_utils.py:306-334)
+ * through L.backward() in core/KTAB_Mfs_spclf1d.py:136-139. */
+int mfs_solid_angle_backward(mfs_ctx *ctx, int precision, const void *V, int64_t nv,
+                             const int32_t *F, int64_t nf, const void *S, int64_t n,
+                             const void *gM, void *gV, void *gS);
+
+/* phi (nf x k) = SL X with SL[f][s] = 1 / |C[f] - S[s]| (S: n x 3 sources,
+ * C: nf x 3 face centres, X: n x k strengths), without storing SL.  Replaces
+ * compute_single_layer (utils/torch_utils.py:254-276) followed by SL @
+ * strengths (core/KTAB_Mfs_spclf1d.py:123-124). */
+int mfs_single_layer_apply(mfs_ctx *ctx, int precision, const void *S, int64_t n, const void *C,
+                           int64_t nf, const void *X, int64_t k, void *phi);
+
+/* Backward of mfs_single_layer_apply: given gphi = dL/dphi (nf x k), writes
+ * gS (n x 3), gC (nf x 3) and gX (n x k); any may be NULL. */
+int mfs_single_layer_backward(mfs_ctx *ctx, int precision, const void *S, int64_t n,
+                              const void *C, int64_t nf, const void *X, int64_t k,
+                              const void *gphi, void *gS, void *gC, void *gX);
+
 /* Launch-shape parameters (autotuned by fastmbcui_amd/autotune.py; 0 restores
  * the default).  They change how work is split over blocks, not the results
  * beyond summation order. */
