@@ -162,7 +162,12 @@ __global__ __launch_bounds__(kRowsThreads) void k_mv_rows(const float *__restric
     const TA *xb = xs[buf];
 #pragma unroll 2
     for (int u = 0; u < CH / 256; ++u) {
-      const int cl = u * 256 + lane * 4;
+      // VEC (n % 4 == 0): a lane owns 4 consecutive columns (one float4).
+      // Otherwise rows are not 16-byte aligned and a lane owns columns
+      // lane, lane + 64, lane + 128, lane + 192 of the 256-column step, so
+      // each scalar load instruction still reads 256 contiguous bytes.
+      const int cl = VEC ? u * 256 + lane * 4 : u * 256 + lane;
+      constexpr int cs = VEC ? 1 : 64;  // column stride between a lane's 4 values
       const int64_t cc = c0 + cl;
       float4 mv[RW];
 #pragma unroll
@@ -172,10 +177,10 @@ __global__ __launch_bounds__(kRowsThreads) void k_mv_rows(const float *__restric
         if (f < m && VEC && cc + 3 < n) {
           mv[i] = load_stream4(rowp + cc);
         } else if (f < m) {
-          mv[i].x = (cc + 0 < n) ? rowp[cc + 0] : 0.f;
-          mv[i].y = (cc + 1 < n) ? rowp[cc + 1] : 0.f;
-          mv[i].z = (cc + 2 < n) ? rowp[cc + 2] : 0.f;
-          mv[i].w = (cc + 3 < n) ? rowp[cc + 3] : 0.f;
+          mv[i].x = (cc + 0 * cs < n) ? rowp[cc + 0 * cs] : 0.f;
+          mv[i].y = (cc + 1 * cs < n) ? rowp[cc + 1 * cs] : 0.f;
+          mv[i].z = (cc + 2 * cs < n) ? rowp[cc + 2 * cs] : 0.f;
+          mv[i].w = (cc + 3 * cs < n) ? rowp[cc + 3 * cs] : 0.f;
         } else {
           mv[i] = make_float4(0.f, 0.f, 0.f, 0.f);
         }
@@ -183,7 +188,7 @@ __global__ __launch_bounds__(kRowsThreads) void k_mv_rows(const float *__restric
 #pragma unroll
       for (int r = 0; r < K; ++r) {
         const TA *xr = xb + r * CH + cl;
-        const TA x0 = xr[0], x1 = xr[1], x2 = xr[2], x3 = xr[3];
+        const TA x0 = xr[0], x1 = xr[cs], x2 = xr[2 * cs], x3 = xr[3 * cs];
 #pragma unroll
         for (int i = 0; i < RW; ++i) {
           TA s = acc[i][r];
@@ -223,7 +228,11 @@ __global__ __launch_bounds__(kRowsThreads) void k_mv_rows(const float *__restric
 // sums are carried in FP64, so the FP32 path keeps a compensated running sum.
 constexpr int kRowChunk = 32;
 
-template <typename TA, int K, bool VEC>
+// VEC (n % 4 == 0): a thread owns 4 consecutive columns (one float4 per row);
+// otherwise it owns columns c, c + 256, c + 512, c + 768 of its block's 1024,
+// so each scalar load instruction reads 256 contiguous bytes.  RU rows are in
+// flight per thread (16 for small grids, where each thread walks many rows).
+template <typename TA, int K, bool VEC, int RU>
 __global__ __launch_bounds__(kThreads) void k_mv_cols(const float *__restrict__ M, int64_t m,
                                                       int64_t n, const TA *__restrict__ Y,
                                                       int64_t rows_per_block,
@@ -234,7 +243,9 @@ __global__ __launch_bounds__(kThreads) void k_mv_cols(const float *__restrict__ 
   // carried in FP64; FP64 products go straight into the FP64 sums.
   constexpr bool kTwoLevel = sizeof(TA) == 4;
   __shared__ TA ys[kRowChunk * K];
-  const int64_t i0 = (int64_t(blockIdx.x) * kThreads + threadIdx.x) * 4;
+  const int64_t i0 = VEC ? (int64_t(blockIdx.x) * kThreads + threadIdx.x) * 4
+                         : int64_t(blockIdx.x) * 4 * kThreads + threadIdx.x;
+  constexpr int64_t cs = VEC ? 1 : kThreads;  // column stride between a thread's 4 columns
   const int64_t fb = int64_t(blockIdx.y) * rows_per_block;
   const int64_t fe = (fb + rows_per_block < m) ? fb + rows_per_block : m;
   double acc[4][K];
@@ -262,7 +273,6 @@ __global__ __launch_bounds__(kThreads) void k_mv_cols(const float *__restrict__ 
         acc[j][r] += double(v);
     };
     if (i0 < n) {
-      constexpr int RU = 4;  // rows in flight per thread
       int f = 0;
       for (; f + RU <= rows; f += RU) {
         float4 v[RU];
@@ -273,9 +283,9 @@ __global__ __launch_bounds__(kThreads) void k_mv_cols(const float *__restrict__ 
             v[u] = load_stream4(rowp + i0);
           } else {
             v[u].x = rowp[i0];
-            v[u].y = (i0 + 1 < n) ? rowp[i0 + 1] : 0.f;
-            v[u].z = (i0 + 2 < n) ? rowp[i0 + 2] : 0.f;
-            v[u].w = (i0 + 3 < n) ? rowp[i0 + 3] : 0.f;
+            v[u].y = (i0 + cs < n) ? rowp[i0 + cs] : 0.f;
+            v[u].z = (i0 + 2 * cs < n) ? rowp[i0 + 2 * cs] : 0.f;
+            v[u].w = (i0 + 3 * cs < n) ? rowp[i0 + 3 * cs] : 0.f;
           }
         }
 #pragma unroll
@@ -293,9 +303,9 @@ __global__ __launch_bounds__(kThreads) void k_mv_cols(const float *__restrict__ 
         const float *rowp = M + (fc + f) * n;
         float4 v;
         v.x = rowp[i0];
-        v.y = (i0 + 1 < n) ? rowp[i0 + 1] : 0.f;
-        v.z = (i0 + 2 < n) ? rowp[i0 + 2] : 0.f;
-        v.w = (i0 + 3 < n) ? rowp[i0 + 3] : 0.f;
+        v.y = (i0 + cs < n) ? rowp[i0 + cs] : 0.f;
+        v.z = (i0 + 2 * cs < n) ? rowp[i0 + 2 * cs] : 0.f;
+        v.w = (i0 + 3 * cs < n) ? rowp[i0 + 3 * cs] : 0.f;
 #pragma unroll
         for (int r = 0; r < K; ++r) {
           const TA yv = ys[f * K + r];
@@ -316,8 +326,8 @@ __global__ __launch_bounds__(kThreads) void k_mv_cols(const float *__restrict__ 
   if (i0 < n) {
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
-      if (i0 + j < n) {
-        double *dst = part + (int64_t(blockIdx.y) * n + i0 + j) * K;
+      if (i0 + j * cs < n) {
+        double *dst = part + (int64_t(blockIdx.y) * n + i0 + j * cs) * K;
 #pragma unroll
         for (int r = 0; r < K; ++r) dst[r] = acc[j][r];
       }
@@ -347,6 +357,7 @@ __global__ __launch_bounds__(kThreads) void k_mv_reduce(
     double s[K];
 #pragma unroll
     for (int r = 0; r < K; ++r) s[r] = 0.0;
+#pragma unroll 8
     for (int32_t b = 0; b < row_blocks; ++b) {
       const double *src = part + (int64_t(b) * n + i) * K;
 #pragma unroll
@@ -648,10 +659,19 @@ void mv_rows(hipStream_t s, const SolveArgs<TA> &a, const TA *X, const float *Bs
     (void)points_rows<TA>(s, a.pts, K, X, Bsub, ldb, col0, kval, Y, a.part, st);
     return;
   }
+  // Fewer rows per wave when the grid would not give every CU a few blocks.
   constexpr int RW = RowsCfg<TA, K>::RW;
-  constexpr int rows_per_block = (kRowsThreads / 64) * RW;
-  const unsigned grid = unsigned((a.m + rows_per_block - 1) / rows_per_block);
-  if (a.n % 4 == 0)
+  constexpr int RWS = RW > 2 ? 2 : RW;
+  constexpr int waves = kRowsThreads / 64;
+  const bool small = (a.m + waves * RW - 1) / (waves * RW) < 1024;
+  const int64_t rpb = int64_t(waves) * (small ? RWS : RW);
+  const unsigned grid = unsigned((a.m + rpb - 1) / rpb);
+  const bool vec = a.n % 4 == 0;
+  if (small && vec)
+    k_mv_rows<TA, K, RWS, true><<<grid, kRowsThreads, 0, s>>>(a.M, a.m, a.n, X, Bsub, ldb, col0, kval, Y, st);
+  else if (small)
+    k_mv_rows<TA, K, RWS, false><<<grid, kRowsThreads, 0, s>>>(a.M, a.m, a.n, X, Bsub, ldb, col0, kval, Y, st);
+  else if (vec)
     k_mv_rows<TA, K, RW, true><<<grid, kRowsThreads, 0, s>>>(a.M, a.m, a.n, X, Bsub, ldb, col0, kval, Y, st);
   else
     k_mv_rows<TA, K, RW, false><<<grid, kRowsThreads, 0, s>>>(a.M, a.m, a.n, X, Bsub, ldb, col0, kval, Y, st);
@@ -662,10 +682,20 @@ void mv_cols_raw(hipStream_t s, const float *M, int64_t m, int64_t n, int32_t ro
                  const TY *Y, double *part, const PcgState *st) {
   const int64_t rpb = (m + row_blocks - 1) / row_blocks;
   dim3 grid(unsigned((n + 4 * kThreads - 1) / (4 * kThreads)), unsigned(row_blocks));
-  if (n % 4 == 0)
-    k_mv_cols<TY, K, true><<<grid, kThreads, 0, s>>>(M, m, n, Y, rpb, part, st);
-  else
-    k_mv_cols<TY, K, false><<<grid, kThreads, 0, s>>>(M, m, n, Y, rpb, part, st);
+  // Small grids: each thread walks many rows, so keep more of them in flight.
+  constexpr int RUS = K <= 2 ? 16 : 4;
+  const bool small = int64_t(grid.x) * grid.y < 1024;
+  if (n % 4 == 0) {
+    if (small)
+      k_mv_cols<TY, K, true, RUS><<<grid, kThreads, 0, s>>>(M, m, n, Y, rpb, part, st);
+    else
+      k_mv_cols<TY, K, true, 4><<<grid, kThreads, 0, s>>>(M, m, n, Y, rpb, part, st);
+  } else {
+    if (small)
+      k_mv_cols<TY, K, false, RUS><<<grid, kThreads, 0, s>>>(M, m, n, Y, rpb, part, st);
+    else
+      k_mv_cols<TY, K, false, 4><<<grid, kThreads, 0, s>>>(M, m, n, Y, rpb, part, st);
+  }
 }
 
 // part = split sums of M^T Y (a.row_blocks splits in both forms).

@@ -163,6 +163,29 @@ def test_pcg_fixed_iterations_match_oracle_trajectory():
             assert rel < 1e-6, (iters, c, rel)   # FP32 output rounding dominates
 
 
+@pytest.mark.parametrize("ppb", [161, 162, 163])
+def test_pcg_ragged_sizes_match_oracle_trajectory(ppb):
+    """n % 4 = 3, 2, 1: the matvec passes' scalar paths (rows not 16-byte
+    aligned) and their small-grid launch shapes, against the oracle."""
+    sc = scenes.coins(seed=2, n_bodies=3, pts_per_body=ppb, k=2, box=6.0)
+    assert sc.n % 4 != 0
+    ctx, Md, M, pl, val, ref = _factor_pair(sc)
+    Mp = oracle.permuted_colmajor(M.astype(np.float64), pl.Pinv)
+    B = torch.from_numpy(sc.b.astype(np.float32)).to(DEV)
+    for iters in (1, 4):
+        X = ctx.solve_dense(Md, B, maxit=iters, tol=0.0).cpu().numpy()
+        for c in range(sc.k):
+            b = sc.b[:, c].astype(np.float32).astype(np.float64)
+            xo, it, _ = oracle.pcg(Mp, M.shape[0], pl, ref, b, 0, iters, 0.0)
+            xo = xo[pl.P]
+            rel = np.linalg.norm(X[:, c] - xo) / np.linalg.norm(xo)
+            assert rel < 1e-6, (ppb, iters, c, rel)
+    ctx.factor_dense(Md, _lib.MFS_FP32)
+    Xm = ctx.solve_mixed(Md, B, maxit=500, tol=1e-6).cpu().numpy().astype(np.float64)
+    res = oracle.normal_residual(M.astype(np.float64), sc.b.astype(np.float32), Xm)
+    assert res < 1e-6, res
+
+
 def test_pcg_converged_matches_direct_solve():
     sc = _small(k=2)
     ctx, Md, M, pl, val, ref = _factor_pair(sc)
