@@ -456,6 +456,59 @@ def caller_main(args):
     _log(f"bench caller step (3 bodies, forward + L.backward()): {ms_hip:.1f} ms with the HIP "
          f"kernels, {ms_torch:.1f} ms with the reference's torch expressions (gradients agree to {agree:.1e})")
 
+    # the whole caller step with kinematics, L.backward() and the Jacobian of
+    # Mf_KDetached (KTAB:88-163): HIP kernels + one batched backward, against the
+    # reference's composition (torch expressions, nv^2 backward() calls)
+    from fastmbcui_amd import kinematics as kin
+
+    def ref_step(table, bodies, q, v):
+        qt = torch.as_tensor(q, dtype=torch.float32, device=dev).clone().requires_grad_(True)
+        vt = torch.as_tensor(v, dtype=torch.float32, device=dev)
+        R_s, y_s, Qp_s = kin.poses(table, qt)
+        Qp = torch.cat(Qp_s, 0)
+        V_s = [V0 @ R.T + y for (V0, _, _), R, y in zip(bodies, R_s, y_s)]
+        S_s = [S0 @ R.T + y for (_, _, S0), R, y in zip(bodies, R_s, y_s)]
+        Mf, Ks, _ = caller.kirchhoff_tensor(V_s, [f for _, f, _ in bodies], S_s, y_s, Qp, 1.0, 1.2, 75,
+                                            2.7e-3, solid_angle=_torch_solid_angle,
+                                            single_layer=_torch_single_layer)
+        (0.5 * vt @ Mf @ vt).backward(retain_graph=True)
+        R6n = kin.r6n(R_s)
+        K = (R6n.T @ caller.iq_matrix(len(bodies), torch.float32, dev) @ Ks).detach()
+        MfK = Qp.T @ R6n @ K
+        J = torch.zeros((table.nv, table.nv, table.nq), device=dev)
+        for i in range(table.nv):
+            for j in range(table.nv):
+                qt.grad = None
+                MfK[i, j].backward(retain_graph=True)
+                J[i, j] = qt.grad
+        return J
+
+    for nb in (2, 6):
+        table = kin.JointTable([kin.Joint(kind=kin.KINDS["PlanarYZV2"]) for _ in range(nb)])
+        bodies = []
+        for b in range(nb):
+            V0, F0 = meshes.ellipsoid((1.0, 1.0, 0.3) if b % 2 == 0 else (1.0, 0.6, 0.6), 3)
+            bodies.append((torch.as_tensor(V0, dtype=torch.float32, device=dev),
+                           torch.as_tensor(F0, device=dev),
+                           torch.as_tensor(meshes.sources(V0, F0, 0.1), dtype=torch.float32, device=dev)))
+        q = np.zeros(table.nq)
+        q[0::3] = 2.6 * np.arange(nb)
+        q[2::3] = 0.3
+        v = np.linspace(-1, 1, table.nv)
+        runtime.reset()
+        ms_step = wall(lambda: caller.kirchhoff_step(table, bodies, q, v, 1.0, 1.2, 75, 2.7e-3), 3)
+        ms_ref = wall(lambda: ref_step(table, bodies, q, v), 2)
+        Jh = caller.kirchhoff_step(table, bodies, q, v, 1.0, 1.2, 75, 2.7e-3)["grad_Mf_q_KDetached"]
+        Jr = ref_step(table, bodies, q, v)
+        dj = float((Jh - Jr).norm() / Jr.norm())
+        out[f"ktab_step_{nb}_links"] = dict(nv=table.nv, ms_hip_batched=round(ms_step, 2),
+                                            ms_reference_composition=round(ms_ref, 2),
+                                            jacobian_rel_diff=dj)
+        _log(f"bench caller KTAB step ({nb} planar links, nv {table.nv}, kinematics + Mf + L.backward() "
+             f"+ d(Mf_KDetached)/dq): {ms_step:.1f} ms (HIP kernels, one batched backward) vs "
+             f"{ms_ref:.1f} ms (torch expressions, {table.nv ** 2} backward() calls); Jacobians agree "
+             f"to {dj:.1e}")
+
     cpu = None
     if not args.skip_cpu:
         from oracle import caller_oracle as co

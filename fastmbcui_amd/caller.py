@@ -27,7 +27,7 @@ from . import runtime
 
 __all__ = ["compute_solid_angle", "compute_single_layer", "single_layer_apply",
            "compute_face_centers", "compute_FL_matrix", "compute_quadrature", "iq_matrix",
-           "kirchhoff_tensor"]
+           "kirchhoff_tensor", "kirchhoff_step"]
 
 
 def _require_device(*ts):
@@ -171,3 +171,37 @@ def kirchhoff_tensor(V_s, F_s, S_s, y_s, Qp, rho_fluid, rho_mfs, pcg_max_iter, t
     Ks = torch.cat([Q @ phi * rho_fluid for Q, phi in zip(Q_s, phi_s)], dim=0)
     Mf = Qp.T @ IQ @ Ks
     return Mf, Ks, strengths
+
+
+def kirchhoff_step(table, bodies, q, v, rho_fluid, rho_mfs, pcg_max_iter, tol_r, grad=True):
+    """One call of the reference's compute_kirchhoff_tensor (KTAB:88-169,
+    without drag) on the GPU.  `table` is a kinematics.JointTable with one
+    link per body; `bodies` a list of (V0, F, S0) device tensors in the link
+    frames (vertices, faces, sources); q (nq) and v (nv) numpy or tensors.
+
+    Returns a dict with Mf (nv x nv), K (6 num_links x nv, the detached K_ of
+    :146-147) and, when `grad`, grad_L_q = dL/dq for L = 0.5 v^T Mf v
+    (:136-139) and grad_Mf_q_KDetached = d(Qp^T R6n K_)/dq (nv x nv x nq,
+    :152-163) from one batched backward pass instead of nv^2 backward() calls."""
+    from . import kinematics as kin
+
+    V0 = bodies[0][0]
+    dt, dev = V0.dtype, V0.device
+    q_t = torch.as_tensor(q, dtype=dt, device=dev).detach().clone().requires_grad_(grad)
+    v_t = torch.as_tensor(v, dtype=dt, device=dev)
+    R_s, y_s, Qp_s = kin.poses(table, q_t)
+    Qp = torch.cat(tuple(Qp_s), dim=0)
+    V_s = [V0_ @ R.T + y for (V0_, _, _), R, y in zip(bodies, R_s, y_s)]
+    S_s = [S0 @ R.T + y for (_, _, S0), R, y in zip(bodies, R_s, y_s)]
+    F_s = [F for (_, F, _) in bodies]
+    Mf, Ks, _ = kirchhoff_tensor(V_s, F_s, S_s, y_s, Qp, rho_fluid, rho_mfs, pcg_max_iter, tol_r)
+    IQ = iq_matrix(len(bodies), dt, dev)
+    R6n = kin.r6n(R_s)
+    K = (R6n.T @ IQ @ Ks).detach()
+    out = dict(Mf=Mf.detach(), K=K)
+    if grad:
+        L = 0.5 * v_t @ Mf @ v_t
+        (gq,) = torch.autograd.grad(L, q_t, retain_graph=True)
+        out["grad_L_q"] = gq
+        out["grad_Mf_q_KDetached"] = kin.mf_kdetached_jacobian(Qp, R6n, K, q_t)[1]
+    return out

@@ -13,6 +13,7 @@ reference obtains them (L.backward(), core/KTAB_Mfs_spclf1d.py:136-139):
   fl_matrix          utils/torch_utils.py:62-75    (compute_FL_matrix)
   quadrature         utils/torch_utils.py:336-347  (compute_quadrature)
   vertex_normals     utils/torch_utils.py:4-47     (compute_normal_vertex)
+  jacobian_loop      core/KTAB_Mfs_spclf1d.py:155-163 (one backward() per entry)
   kirchhoff          core/KTAB_Mfs_spclf1d.py:97-128, the strengths from a dense
                      least-squares solve of M X = F (its renew_grad=False branch,
                      :119-121, solves with lstsq as well)
@@ -119,3 +120,18 @@ def kirchhoff(V_s, F_s, S_s, y_s, Qp, rho_fluid):
     Ks = torch.cat([Q @ phi * rho_fluid for Q, phi in zip(Q_s, phi_s)], 0)
     Mf = Qp.T @ IQ @ Ks
     return Mf, Ks, strengths
+
+
+def jacobian_loop(out, q):
+    """d out / d q entry by entry, as the caller does: out[i, j].backward()
+    then q.grad (KTAB:155-163).  Returns out.shape + q.shape."""
+    J = torch.zeros(tuple(out.shape) + tuple(q.shape), dtype=q.dtype, device=q.device)
+    for idx in np.ndindex(*out.shape):
+        if q.grad is not None:
+            q.grad.zero_()
+        out[idx].backward(retain_graph=True)
+        if q.grad is not None:
+            J[idx] = q.grad.clone()
+    if q.grad is not None:
+        q.grad.zero_()
+    return J

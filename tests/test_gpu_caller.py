@@ -242,3 +242,40 @@ def test_kirchhoff_gradient_vs_oracle():
     want = run(F64, "cpu", True)
     rel = ((got - want).norm() / want.norm()).item()
     assert rel < 5e-3, (rel, got, want)
+
+
+def test_kirchhoff_step_vs_oracle():
+    """The whole caller step (KTAB:88-163, no drag) for the coin driver's model
+    (two PlanarYZV2 links): Mf, dL/dq and d(Mf_KDetached)/dq against the FP64
+    oracle pipeline with an exact least-squares solve and the entry-by-entry
+    backward loop."""
+    from fastmbcui_amd import kinematics as kin
+    runtime.reset()
+    table = kin.JointTable([kin.Joint(kind=kin.KINDS["PlanarYZV2"]),
+                            kin.Joint(kind=kin.KINDS["PlanarYZV2"])])
+    shapes = [meshes.ellipsoid((1.0, 1.0, 0.3), 2), meshes.ellipsoid((1.0, 0.6, 0.6), 2)]
+    bodies64 = [(torch.tensor(V), torch.tensor(F), torch.tensor(meshes.sources(V, F, 0.1)))
+                for V, F in shapes]
+    bodies = [(V.to(DEV, torch.float32), F.to(DEV), S.to(DEV, torch.float32)) for V, F, S in bodies64]
+    q = np.array([0.1, -0.2, 0.3, 2.4, 0.4, -0.5])
+    v = np.array([0.3, -0.2, 0.5, 0.1, 0.2, -0.4])
+    out = caller.kirchhoff_step(table, bodies, q, v, 1.0, 1.2, 4000, 1e-8)
+
+    qt = torch.tensor(q, requires_grad=True)
+    R_s, y_s, Qp_s = kin.poses(table, qt)
+    Qp = torch.cat(Qp_s, 0)
+    r32 = lambda t: t.float().double()     # the inputs as the GPU saw them
+    V_s = [r32(V) @ R.T + y for (V, _, _), R, y in zip(bodies64, R_s, y_s)]
+    S_s = [r32(S) @ R.T + y for (_, _, S), R, y in zip(bodies64, R_s, y_s)]
+    Mf, Ks, _ = co.kirchhoff(V_s, [F for _, F, _ in bodies64], S_s, y_s, Qp, 1.0)
+    vt = torch.tensor(v)
+    (gq,) = torch.autograd.grad(0.5 * vt @ Mf @ vt, qt, retain_graph=True)
+    R6n = kin.r6n(R_s)
+    K = (R6n.T @ co.iq_matrix(2) @ Ks).detach()
+    J = co.jacobian_loop(Qp.T @ R6n @ K, qt)
+
+    rel = lambda a, b: ((a.cpu().double() - b).norm() / b.norm()).item()
+    assert rel(out["Mf"], Mf.detach()) < 2e-4
+    assert rel(out["K"], K) < 2e-4
+    assert rel(out["grad_L_q"], gq) < 5e-3, (out["grad_L_q"], gq)
+    assert rel(out["grad_Mf_q_KDetached"], J) < 2e-4
