@@ -509,6 +509,24 @@ def caller_main(args):
              f"{ms_ref:.1f} ms (torch expressions, {table.nv ** 2} backward() calls); Jacobians agree "
              f"to {dj:.1e}")
 
+    # the shipped problems' models (stand-in bodies of the meshes' sizes and
+    # extents) at the drivers' q_init and solver settings
+    from fastmbcui_amd import problems
+    for name in sorted(problems.PROBLEMS):
+        P = problems.load_problem(name)
+        bodies = [(torch.as_tensor(V, dtype=torch.float32, device=dev), torch.as_tensor(F, device=dev),
+                   torch.as_tensor(S, dtype=torch.float32, device=dev)) for V, F, S in P["bodies"]]
+        v = np.linspace(-1, 1, P["table"].nv)
+        runtime.reset()
+        run = lambda: caller.kirchhoff_step(P["table"], bodies, P["q_init"], v, P["rho_fluid"],
+                                            P["rho_mfs"], P["pcg_max_iter"], P["tol_r"])
+        ms = wall(run, 3)
+        n = sum(b[0].shape[0] for b in P["bodies"])
+        nf = sum(b[1].shape[0] for b in P["bodies"])
+        out[f"problem_{name}"] = dict(ms_step=round(ms, 2), sources=n, faces=nf, nv=P["table"].nv)
+        _log(f"bench caller problem {name} (stand-in bodies, {nf} faces, {n} sources, nv "
+             f"{P['table'].nv}): Kirchhoff step with L.backward() and d(Mf_KDetached)/dq {ms:.1f} ms")
+
     cpu = None
     if not args.skip_cpu:
         from oracle import caller_oracle as co
