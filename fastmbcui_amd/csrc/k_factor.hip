@@ -294,7 +294,11 @@ __global__ __launch_bounds__(kThreads) void k_super_factor(const FactorScene *__
     // bytes) of every column -- whole 128-byte lines from W = 2 on; the small
     // tiles afford W = 4 registers.
     constexpr int W = LARGE ? 1 : (T <= 4 ? 4 : 2);
-    constexpr int FSTEP = 16 * W * KS;  // ldmt is a multiple of 128, hence of FSTEP
+    // A wave reads slabs [f0, f0 + 16 W) with f0 = kph 16 W + j FSTEP.  16 W
+    // divides 128 and ldmt is a multiple of 128, so a slab that starts below
+    // ldmt ends within the row; FSTEP (up to 256) need not divide ldmt, so
+    // every load, the first prefetch included, is issued only for f0 < ldmt.
+    constexpr int FSTEP = 16 * W * KS;
     auto load = [&](int64_t f0, float4 (&v)[T][W]) {
 #pragma unroll
       for (int a = 0; a < T; ++a)
@@ -305,7 +309,7 @@ __global__ __launch_bounds__(kThreads) void k_super_factor(const FactorScene *__
     };
     constexpr bool PREFETCH = !LARGE;  // the large tiles need the registers
     float4 cur[T][W], nxt[PREFETCH ? T : 1][W];
-    if (PREFETCH) load(kph * 16 * W, cur);
+    if (PREFETCH && kph * 16 * W < ldmt) load(kph * 16 * W, cur);
     for (int64_t f0 = kph * 16 * W; f0 < ldmt; f0 += FSTEP) {
       if constexpr (PREFETCH) {
         if (f0 + FSTEP < ldmt) load(f0 + FSTEP, nxt);
