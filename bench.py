@@ -497,17 +497,23 @@ def caller_main(args):
         v = np.linspace(-1, 1, table.nv)
         runtime.reset()
         ms_step = wall(lambda: caller.kirchhoff_step(table, bodies, q, v, 1.0, 1.2, 75, 2.7e-3), 3)
+        runtime.set_solver_mode("mixed")
+        runtime.reset()
+        ms_mixed = wall(lambda: caller.kirchhoff_step(table, bodies, q, v, 1.0, 1.2, 75, 2.7e-3), 3)
+        runtime.set_solver_mode("plain")
+        runtime.reset()
         ms_ref = wall(lambda: ref_step(table, bodies, q, v), 2)
         Jh = caller.kirchhoff_step(table, bodies, q, v, 1.0, 1.2, 75, 2.7e-3)["grad_Mf_q_KDetached"]
         Jr = ref_step(table, bodies, q, v)
         dj = float((Jh - Jr).norm() / Jr.norm())
         out[f"ktab_step_{nb}_links"] = dict(nv=table.nv, ms_hip_batched=round(ms_step, 2),
+                                            ms_hip_batched_mixed_solve=round(ms_mixed, 2),
                                             ms_reference_composition=round(ms_ref, 2),
                                             jacobian_rel_diff=dj)
         _log(f"bench caller KTAB step ({nb} planar links, nv {table.nv}, kinematics + Mf + L.backward() "
-             f"+ d(Mf_KDetached)/dq): {ms_step:.1f} ms (HIP kernels, one batched backward) vs "
-             f"{ms_ref:.1f} ms (torch expressions, {table.nv ** 2} backward() calls); Jacobians agree "
-             f"to {dj:.1e}")
+             f"+ d(Mf_KDetached)/dq): {ms_step:.1f} ms (HIP kernels, one batched backward; "
+             f"{ms_mixed:.1f} ms with the mixed solve) vs {ms_ref:.1f} ms (torch expressions, "
+             f"{table.nv ** 2} backward() calls); Jacobians agree to {dj:.1e}")
 
     # the shipped problems' models (stand-in bodies of the meshes' sizes and
     # extents) at the drivers' q_init and solver settings

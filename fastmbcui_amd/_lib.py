@@ -244,8 +244,12 @@ class Context:
                     "mfs_solve_dense")
         return out
 
-    def solve_mixed(self, M, B, maxit=1000, tol=1e-6, inner_tol=0.5, x0=None, out=None):
-        """Batched FP32 PCG with FP64 refinement.  B: m x k float32 cuda."""
+    def solve_mixed(self, M, B, maxit=1000, tol=1e-6, inner_tol=0.5, x0=None, out=None,
+                    rhs_space=None):
+        """Batched FP32 PCG with FP64 refinement.  B: m x k float32 cuda
+        (MFS_RHS_BOUNDARY, the default) or n x k (MFS_RHS_SOURCE, VpL)."""
+        if rhs_space is None:
+            rhs_space = MFS_RHS_BOUNDARY
         import torch
         self._sync_stream()
         _check_dense(M, self.n)
@@ -254,8 +258,11 @@ class Context:
             out = torch.empty((M.shape[1], B.shape[1]), dtype=torch.float32, device=M.device)
         if x0 is not None:
             x0 = _as_2d_f32(x0)
+        rows = M.shape[0] if rhs_space == MFS_RHS_BOUNDARY else M.shape[1]
+        if B.shape[0] != rows:
+            raise ValueError("right-hand sides have the wrong number of rows")
         self._check(lib().mfs_solve_mixed(self._h, _ptr(M), M.shape[0], M.shape[1], _ptr(B), B.shape[1],
-                                          MFS_RHS_BOUNDARY, _ptr(x0), _ptr(out), int(maxit), float(tol),
+                                          int(rhs_space), _ptr(x0), _ptr(out), int(maxit), float(tol),
                                           float(inner_tol)), "mfs_solve_mixed")
         return out
 

@@ -469,6 +469,34 @@ __global__ __launch_bounds__(kThreads) void k_load_boundary(int64_t m, const flo
   for (int r = 0; r < K; ++r) y[f * K + r] = (r < kval) ? TA(B[f * ldb + col0 + r]) : TA(0);
 }
 
+// R = b - R (sub) or R = b, b the source-space FP32 data in original order;
+// per-column ||R||^2 into out2 (last-block reduction on counter C_NORM).
+template <int K>
+__global__ __launch_bounds__(kThreads) void k_source_residual(
+    int64_t n, const float *__restrict__ B, int64_t ldb, int64_t col0, int kval, int sub,
+    double *__restrict__ R, double *__restrict__ dpart, PcgState *st, double *out2) {
+  __shared__ double red[5 * K];
+  __shared__ int last;
+  const int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x;
+  double d[K];
+#pragma unroll
+  for (int r = 0; r < K; ++r) d[r] = 0.0;
+  if (i < n) {
+#pragma unroll
+    for (int r = 0; r < K; ++r) {
+      const double b = (r < kval) ? double(B[i * ldb + col0 + r]) : 0.0;
+      const double v = sub ? b - R[i * K + r] : b;
+      R[i * K + r] = v;
+      d[r] = v * v;
+    }
+  }
+  block_sum<K>(d, red);
+  if (publish_and_arrive<K>(red + 4 * K, dpart, &st->counter[C_NORM], gridDim.x, &last)) {
+    gather_partials<K>(dpart, gridDim.x, red);
+    if (threadIdx.x < K) out2[threadIdx.x] = red[4 * K + threadIdx.x];
+  }
+}
+
 template <typename TA>
 __global__ __launch_bounds__(kThreads) void k_zero(int64_t count, TA *__restrict__ a) {
   const int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x;
@@ -793,6 +821,28 @@ hipError_t residual_k(hipStream_t s, const SolveArgs<double> &a, const float *B,
   return hipGetLastError();
 }
 
+// Source-space FP64 residual R = B - M^T M X in original order (R = B when
+// X64 is null), per-column ||R||^2 into out2.
+template <int K>
+hipError_t residual_src_k(hipStream_t s, const SolveArgs<double> &a, const float *B, int64_t ldb,
+                          int64_t col0, int kval, const double *X64, double *R, double *out2,
+                          TrackFn fn, void *tctx) {
+  if (X64 != nullptr) {
+    track(fn, tctx, 4, 0);
+    mv_rows<double, K>(s, a, X64, nullptr, 0, 0, kval, a.y, nullptr);
+    track(fn, tctx, 4, 1);
+    track(fn, tctx, 5, 0);
+    mv_cols<double, double, K>(s, a, a.y, nullptr);
+    track(fn, tctx, 5, 1);
+    k_mv_reduce<double, K, RED_ORIG><<<blocks_for(a.n), kThreads, 0, s>>>(
+        a.part, a.row_blocks, a.n, a.P, nullptr, nullptr, R, a.dpart, a.st, out2, 0.0, kval, 0);
+  }
+  k_source_residual<K><<<blocks_for(a.n), kThreads, 0, s>>>(a.n, B, ldb, col0, kval,
+                                                           X64 != nullptr ? 1 : 0, R, a.dpart,
+                                                           a.st, out2);
+  return hipGetLastError();
+}
+
 }  // namespace
 
 #define MFS_DISPATCH_K(K, CALL)                  \
@@ -827,6 +877,12 @@ hipError_t residual64(hipStream_t s, const SolveArgs<double> &a, const float *B,
                       int64_t col0, int kval, const double *X64, double *R, double *out2,
                       TrackFn fn, void *tctx) {
   MFS_DISPATCH_K(a.K, (residual_k<KK>(s, a, B, ldb, col0, kval, X64, R, out2, fn, tctx)));
+}
+
+hipError_t residual64_source(hipStream_t s, const SolveArgs<double> &a, const float *B, int64_t ldb,
+                             int64_t col0, int kval, const double *X64, double *R, double *out2,
+                             TrackFn fn, void *tctx) {
+  MFS_DISPATCH_K(a.K, (residual_src_k<KK>(s, a, B, ldb, col0, kval, X64, R, out2, fn, tctx)));
 }
 
 template hipError_t pcg_init<double>(hipStream_t, const SolveArgs<double> &, const float *, int64_t,

@@ -793,6 +793,7 @@ int factor_impl(mfs_ctx *ctx, const float *M, const PointsOp *pts, int64_t m, in
 struct MixedRun {
   mfs_ctx *ctx = nullptr;
   const float *B = nullptr;
+  int rhs_space = 0;  // MFS_RHS_BOUNDARY: B m x k; MFS_RHS_SOURCE: B n x k
   int64_t ldb = 0, col0 = 0, n = 0;
   int kval = 0, K = 1;
   int32_t maxit = 0;
@@ -804,10 +805,18 @@ struct MixedRun {
   int outer = 0;
   std::vector<int32_t> iters;
 
+  // FP64 residual of the right-hand-side space: M^T (b - M x) or b - M^T M x.
+  hipError_t residual(const double *X, double *out2) {
+    return rhs_space == MFS_RHS_SOURCE
+               ? residual64_source(ctx->stream, a64, B, ldb, col0, kval, X, R64, out2, track_cb, ctx)
+               : residual64(ctx->stream, a64, B, ldb, col0, kval, X, R64, out2, track_cb, ctx);
+  }
+
   int begin(mfs_ctx *c, const float *M, const PointsOp *pts, int64_t m, int64_t n_, const float *B_,
             int64_t ldb_, int64_t col0_, int kval_, int32_t maxit_, double tol_, double inner_tol_,
-            const float *X0, const double *X0d) {
+            const float *X0, const double *X0d, int rhs_space_ = MFS_RHS_BOUNDARY) {
     ctx = c; B = B_; ldb = ldb_; col0 = col0_; n = n_; kval = kval_; K = batch_width(kval);
+    rhs_space = rhs_space_;
     maxit = maxit_; tol = tol_; inner_tol = inner_tol_;
     iters.assign(kval, 0);
     int rc = workspace<double>(ctx, M, pts, m, n, K, a64);
@@ -844,10 +853,11 @@ struct MixedRun {
       k_x64_from_f64<<<gn, kThreads, 0, ctx->stream>>>(n, X0d, ldb, col0, kval, K, X64);
     else
       k_x64_from_f32<<<gn, kThreads, 0, ctx->stream>>>(n, X0, ldb, col0, kval, K, X64);
-    // ||M^T b||^2 and, for a zero start, the first residual.
-    HIPCHK(residual64(ctx->stream, a64, B, ldb, col0, kval, nullptr, R64, nb2, track_cb, ctx), "residual");
+    // ||M^T b||^2 (||b||^2 for source-space data) and, for a zero start, the
+    // first residual.
+    HIPCHK(residual(nullptr, nb2), "residual");
     if (X0 != nullptr || X0d != nullptr)
-      HIPCHK(residual64(ctx->stream, a64, B, ldb, col0, kval, X64, R64, res2, track_cb, ctx), "residual");
+      HIPCHK(residual(X64, res2), "residual");
     else
       HIPCHK(hipMemcpyAsync(res2, nb2, kMaxRhs * 8, hipMemcpyDeviceToDevice, ctx->stream), "copy");
     HIPCHK(hipMemcpyAsync(ctx->hnorm, nb2, 2 * kMaxRhs * 8, hipMemcpyDeviceToHost, ctx->stream), "copy");
@@ -886,7 +896,7 @@ struct MixedRun {
     }
     inner_used += std::max(used, 1);
     ++outer;
-    HIPCHK(residual64(ctx->stream, a64, B, ldb, col0, kval, X64, R64, res2, track_cb, ctx), "residual");
+    HIPCHK(residual(X64, res2), "residual");
     HIPCHK(hipMemcpyAsync(ctx->hnorm, nb2, 2 * kMaxRhs * 8, hipMemcpyDeviceToHost, ctx->stream), "copy");
     return MFS_OK;
   }
@@ -911,8 +921,9 @@ int solve_mixed_impl(mfs_ctx *ctx, const float *M, const PointsOp *pts, int64_t 
                      int32_t maxit, double tol, double inner_tol, const double *X0d = nullptr,
                      double *Xd = nullptr) {
   if (!ctx->has_factor) return ctx->fail(MFS_ERR_STATE, "no factor: factor before solving");
-  if (B == nullptr || (X == nullptr && Xd == nullptr) || k <= 0 || rhs_space != 0)
-    return ctx->fail(MFS_ERR_ARG, "solve_mixed: boundary right-hand sides only");
+  if (B == nullptr || (X == nullptr && Xd == nullptr) || k <= 0 ||
+      (rhs_space != MFS_RHS_BOUNDARY && rhs_space != MFS_RHS_SOURCE))
+    return ctx->fail(MFS_ERR_ARG, "solve_mixed: bad arguments");
   if (!(inner_tol > 0) || !(inner_tol <= 1)) inner_tol = 0.5;
   HIPCHK(hipSetDevice(ctx->device), "hipSetDevice");
   phase_begin(ctx);
@@ -921,7 +932,7 @@ int solve_mixed_impl(mfs_ctx *ctx, const float *M, const PointsOp *pts, int64_t 
   for (int64_t col0 = 0; col0 < k; col0 += kMaxRhs) {
     const int kval = int(std::min<int64_t>(kMaxRhs, k - col0));
     MixedRun run;
-    int rc = run.begin(ctx, M, pts, m, n, B, k, col0, kval, maxit, tol, inner_tol, X0, X0d);
+    int rc = run.begin(ctx, M, pts, m, n, B, k, col0, kval, maxit, tol, inner_tol, X0, X0d, rhs_space);
     if (rc != MFS_OK) return rc;
     for (;;) {
       HIPCHK(hipStreamSynchronize(ctx->stream), "sync");

@@ -148,6 +148,11 @@ hipError_t pcg_finish(hipStream_t st, const SolveArgs<TA> &a, float *X, int64_t 
 hipError_t residual64(hipStream_t st, const SolveArgs<double> &a, const float *B, int64_t ldb,
                       int64_t col0, int k_valid, const double *X64, double *R, double *out2,
                       void (*track)(void *, int, int), void *track_ctx);
+// Source-space (VpL) residual R = B - M^T M X (B: n x k, original order), or
+// R = B when X64 is null, with per-column ||R||^2 into out2.
+hipError_t residual64_source(hipStream_t st, const SolveArgs<double> &a, const float *B,
+                             int64_t ldb, int64_t col0, int k_valid, const double *X64, double *R,
+                             double *out2, void (*track)(void *, int, int), void *track_ctx);
 
 // ---------------------------------------------------------------- points form
 // Splits of the other side so that owners x splits fills the chip.

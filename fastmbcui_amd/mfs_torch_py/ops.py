@@ -59,14 +59,15 @@ def _report(info, timing_verbose, solve_verbose, ctx_stats):
 def _solve(M, B, Vs_s, rho, pcg_max_iter, tol_r, timing_verbose, solve_verbose, forward):
     Mc = M.contiguous()
     Bc = B.contiguous()
-    mode = runtime.solver_mode() if forward else "plain"
+    mode = runtime.solver_mode()
     prec = _lib.MFS_FP32 if mode == "mixed" else _lib.MFS_FP64
     ctx, info = runtime.prepared(Mc, Vs_s, rho, prec)
     n, k = Mc.shape[1], Bc.shape[1]
     out = torch.empty((n, k), dtype=torch.float32, device=M.device)
     if k > 0:
         if mode == "mixed":
-            ctx.solve_mixed(Mc, Bc, maxit=max(int(pcg_max_iter), 1) * 4, tol=float(tol_r), out=out)
+            ctx.solve_mixed(Mc, Bc, maxit=max(int(pcg_max_iter), 1) * 4, tol=float(tol_r), out=out,
+                            rhs_space=_lib.MFS_RHS_BOUNDARY if forward else _lib.MFS_RHS_SOURCE)
         else:
             ctx.solve_dense(Mc, Bc, _lib.MFS_RHS_BOUNDARY if forward else _lib.MFS_RHS_SOURCE,
                             int(pcg_max_iter), float(tol_r), out=out)

@@ -58,7 +58,8 @@ last_info = SolveInfo()
 
 def set_solver_mode(mode: str) -> None:
     """'plain' (default): FP64 factor and one FP64 PCG per column, the drop-in.
-    'mixed': FP32 factor, batched FP32 PCG with FP64 refinement (forward only)."""
+    'mixed': FP32 factor, all columns through one batched FP32 PCG with FP64
+    refinement, for the forward (MinvF) and the backward's VpL alike."""
     global _mode
     if mode not in ("plain", "mixed"):
         raise ValueError("mode must be 'plain' or 'mixed'")

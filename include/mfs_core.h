@@ -105,8 +105,10 @@ int mfs_solve_dense(mfs_ctx *ctx, const float *M, int64_t m, int64_t n,
  * ||M^T(b - M x)|| <= tol ||M^T b|| for every column (or maxit total inner
  * iterations).  Each inner sweep stops at an absolute residual of
  * inner_tol * tol * ||M^T b|| (inner_tol in (0, 1], default 0.5 when out of
- * range).  X0 (n x k, may be NULL) is a warm start.  No counterpart in the
- * reference. */
+ * range).  X0 (n x k, may be NULL) is a warm start.  rhs_space
+ * MFS_RHS_SOURCE takes B as n x k source-space data (the VpL system
+ * M^T M x = b) and certifies ||b - M^T M x|| <= tol ||b|| instead.  No
+ * counterpart in the reference. */
 int mfs_solve_mixed(mfs_ctx *ctx, const float *M, int64_t m, int64_t n,
                     const float *B, int64_t k, int rhs_space, const float *X0,
                     float *X, int32_t maxit, double tol, double inner_tol);

@@ -134,6 +134,17 @@ def test_factor_fp32_close_to_oracle():
     assert err < 5e-2
 
 
+@pytest.mark.parametrize("ppb", [24, 50, 64])
+def test_factor_small_m_matches_oracle(ppb):
+    """m <= 128 (M^T rows padded to 128 only): the Gram's row slabs stay within
+    a row for every wave of the small-tile path; FP64 and FP32 factors."""
+    sc = scenes.coins(seed=4, n_bodies=2, pts_per_body=ppb, k=1, box=4.0)
+    assert sc.m <= 128
+    for precision, tol in ((_lib.MFS_FP64, 1e-9), (_lib.MFS_FP32, 5e-2)):
+        ctx, Md, M, pl, val, ref = _factor_pair(sc, precision)
+        assert _col_rel_err(pl, val, ref) < tol
+
+
 def test_factor_dense_pattern_one_iteration():
     sc = scenes.coins(seed=1, n_bodies=2, pts_per_body=24, k=1, box=4.0)
     M = _M32(sc)
@@ -234,7 +245,55 @@ def test_source_space_solve_matches_oracle_vpl():
     assert np.linalg.norm(X - xd) / np.linalg.norm(xd) < 1e-6
 
 
-# ----------------------------------------------------------------------------- mixed
+@pytest.mark.parametrize("k", [3, 18])
+def test_mixed_source_space_solve_reaches_tolerance(k):
+    """The batched mixed solve of the VpL system M^T M x = b: FP64 residual
+    ||b - M^T M x|| <= tol ||b|| per column (recomputed here), two batches for
+    k = 18."""
+    sc = _small(k=1)
+    ctx, Md, M, pl, val, ref = _factor_pair(sc, _lib.MFS_FP32)
+    rng = np.random.default_rng(k)
+    G = rng.standard_normal((sc.n, k)).astype(np.float32)
+    X = ctx.solve_mixed(Md, torch.from_numpy(G).to(DEV), maxit=2000, tol=1e-6,
+                        rhs_space=_lib.MFS_RHS_SOURCE).cpu().numpy().astype(np.float64)
+    M64 = M.astype(np.float64)
+    G64 = G.astype(np.float64)
+    A = M64.T @ M64
+    res = lambda Y: np.linalg.norm(G64 - A @ Y, axis=0) / np.linalg.norm(G64, axis=0)
+    # the solver certifies its FP64 iterate to 1e-6; the FP32 output adds the
+    # residual that rounding the exact solution to FP32 already has
+    floor = res(np.linalg.solve(A, G64).astype(np.float32).astype(np.float64))
+    rel = res(X)
+    assert np.all(rel <= 1e-6 + 2 * floor), (rel.max(), floor.max())
+    s = ctx.stats()
+    assert all(s.relres[c] <= 1e-6 for c in range(min(k, 64)))
+    with pytest.raises(ValueError):
+        ctx.solve_mixed(Md, torch.zeros((sc.m + 1, 2), device=DEV), rhs_space=_lib.MFS_RHS_SOURCE)
+
+
+def test_op_autograd_mixed_mode_matches_plain():
+    """MinvF_float_fastGrad's forward and backward (VpL) in the mixed mode agree
+    with the plain mode to the solver tolerance."""
+    from fastmbcui_amd import runtime
+    from fastmbcui_amd.mfs_torch_py import ops
+    sc = _small(k=3)
+    M = torch.from_numpy(_M32(sc)).to(DEV).requires_grad_()
+    F = torch.from_numpy(sc.b.astype(np.float32)).to(DEV).requires_grad_()
+    Vs = torch.from_numpy(sc.y.astype(np.float32))
+    W = torch.randn((sc.n, sc.k), device=DEV, generator=torch.Generator(DEV).manual_seed(3))
+    out = {}
+    try:
+        for mode in ("plain", "mixed"):
+            runtime.reset()
+            runtime.set_solver_mode(mode)
+            X = ops.MinvF_float_fastGrad(M, F, Vs, 3.0, 2000, 1e-8)
+            gM, gF = torch.autograd.grad((X * W).sum(), (M, F))
+            out[mode] = (X.detach(), gM, gF)
+    finally:
+        runtime.set_solver_mode("plain")
+        runtime.reset()
+    for a, b in zip(out["mixed"], out["plain"]):
+        assert ((a - b).norm() / b.norm()).item() < 1e-4
 @pytest.mark.parametrize("maker,k", [(_small, 2), (_ellips, 5)])
 def test_mixed_solve_reaches_tolerance(maker, k):
     sc = maker(k=k)
