@@ -43,6 +43,8 @@ def _parse():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--bodies", type=int, default=64, help="ellipsoids (1024 points each)")
     ap.add_argument("--plain-steps", type=int, default=1)
+    ap.add_argument("--one-pass-steps", type=int, default=1,
+                    help="steps of the chosen form with the opt-in one-pass matvec, for comparison")
     ap.add_argument("--skip-plain", action="store_true")
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--tol", type=float, default=1e-6)
@@ -631,7 +633,7 @@ def main():
         chosen()
         _log(f"bench: warmup {i + 1}/{args.warmup} done")
     classes = [_lib.MFS_K_MATVEC_ROWS, _lib.MFS_K_MATVEC_COLS, _lib.MFS_K_GRAM,
-               _lib.MFS_K_RESID_ROWS, _lib.MFS_K_RESID_COLS]
+               _lib.MFS_K_RESID_ROWS, _lib.MFS_K_RESID_COLS, _lib.MFS_K_MATVEC_FUSED]
     ctx.track(classes)
     torch.cuda.synchronize()
     factor_ms = []
@@ -656,12 +658,14 @@ def main():
     names = {_lib.MFS_K_MATVEC_ROWS: "k_mv_rows (y = M p, FP32, 8 RHS)",
              _lib.MFS_K_MATVEC_COLS: "k_mv_cols (partial M^T y, FP32, 8 RHS)",
              _lib.MFS_K_RESID_ROWS: "k_mv_rows (y = b - M x, FP64, 8 RHS)",
-             _lib.MFS_K_RESID_COLS: "k_mv_cols (partial M^T y, FP64, 8 RHS)"}
+             _lib.MFS_K_RESID_COLS: "k_mv_cols (partial M^T y, FP64, 8 RHS)",
+             _lib.MFS_K_MATVEC_FUSED: "k_mv_fused8 (q = M^T M p in one pass over M, FP32, 8 RHS)"}
     dom = max(names, key=lambda c: tracked[c][1])
     launches, tot_ms, tot_bytes = tracked[dom]
     avg_ms = tot_ms / max(launches, 1)
     achieved = (tot_bytes / max(launches, 1)) / (avg_ms * 1e-3) / 1e9 if launches else 0.0
     g_l, g_ms, g_bytes = tracked[_lib.MFS_K_GRAM]
+    fused_launches = tracked[_lib.MFS_K_MATVEC_FUSED][0]
     kernel_ms = {names.get(c, "supernode factor (Gram+Cholesky+L)"): round(tracked[c][1] / args.steps, 3)
                  for c in classes}
 
@@ -672,6 +676,30 @@ def main():
             traffic = json.load(open(tpath)).get(names[dom].split(" ")[0])
         except Exception:
             traffic = None
+
+    # The same step with the one-pass matvec (k_fused.hip, opt-in), for comparison.
+    one_pass = None
+    if fused_launches == 0 and args.one_pass_steps > 0:
+        ctx.set_param(_lib.MFS_PARAM_FUSED, 2)
+        chosen()
+        torch.cuda.synchronize()
+        ctx.track([_lib.MFS_K_MATVEC_FUSED])
+        t2 = time.perf_counter()
+        for i in range(args.one_pass_steps):
+            X2, _ = chosen()
+        torch.cuda.synchronize()
+        one_ms = (time.perf_counter() - t2) * 1e3 / args.one_pass_steps
+        fl, fms, _ = ctx.tracked(_lib.MFS_K_MATVEC_FUSED)
+        ctx.track([])
+        s2 = ctx.stats()
+        res2 = ctx.normal_residual(M, B, X2)
+        ctx.set_param(_lib.MFS_PARAM_FUSED, 0)
+        one_pass = dict(ms_per_solve=round(one_ms / k, 4), iters=[int(s2.iters[c]) for c in range(k)],
+                        residual_max=float(res2.max()),
+                        k_mv_fused8_avg_ms=round(fms / max(fl, 1), 4),
+                        max_rel_diff_x=float((X2.double() - X.double()).norm() / X.double().norm()))
+        _log(f"bench: one-pass matvec {one_ms:.2f} ms/step, k_mv_fused8 {fms / max(fl, 1):.3f} ms, "
+             f"iters {one_pass['iters']}")
 
     # ---------------------------------------------------------------- plain form
     plain = None
@@ -779,6 +807,7 @@ def main():
         "residual_max": float(res.max()),
         "kernel_ms_per_step": kernel_ms,
         "gram_gb_gathered_per_step": round(g_bytes / max(args.steps, 1) / 1e9, 2),
+        "one_pass": one_pass,
         "plain": plain,
         "speedup_vs_plain": (round(plain["ms_per_solve"] / value, 2) if plain else None),
         "points": points,

@@ -18,8 +18,9 @@ LIB_PATH = os.path.join(_HERE, "lib", "libmfs_core.so")
 MFS_OK, MFS_ERR_ARG, MFS_ERR_HIP, MFS_ERR_DUPLICATE, MFS_ERR_NOT_SPD, MFS_ERR_STATE, MFS_ERR_ALLOC = range(7)
 MFS_FP64, MFS_FP32 = 0, 1
 MFS_RHS_BOUNDARY, MFS_RHS_SOURCE = 0, 1
-MFS_K_NONE, MFS_K_MATVEC_ROWS, MFS_K_MATVEC_COLS, MFS_K_GRAM, MFS_K_RESID_ROWS, MFS_K_RESID_COLS = range(6)
-MFS_PARAM_POINTS_BLOCKS, MFS_PARAM_COLS_BLOCKS = 1, 2
+(MFS_K_NONE, MFS_K_MATVEC_ROWS, MFS_K_MATVEC_COLS, MFS_K_GRAM, MFS_K_RESID_ROWS, MFS_K_RESID_COLS,
+ MFS_K_MATVEC_FUSED) = range(7)
+MFS_PARAM_POINTS_BLOCKS, MFS_PARAM_COLS_BLOCKS, MFS_PARAM_FUSED = 1, 2, 3
 
 _STATUS_NAMES = {1: "MFS_ERR_ARG", 2: "MFS_ERR_HIP", 3: "MFS_ERR_DUPLICATE", 4: "MFS_ERR_NOT_SPD",
                  5: "MFS_ERR_STATE", 6: "MFS_ERR_ALLOC"}
@@ -94,6 +95,7 @@ def lib():
             "mfs_solve_points64": (ci, [vp, vp, i64, vp, vp, i32, f64, f64]),
             "mfs_points_residual64": (ci, [vp, vp, i64, vp, vp]),
             "mfs_points_apply": (ci, [vp, ci, ci, vp, i64, vp]),
+            "mfs_dense_normal_apply": (ci, [vp, vp, i64, i64, vp, i64, vp]),
             "mfs_solid_angle": (ci, [vp, ci, vp, i64, vp, i64, vp, i64, vp]),
             "mfs_solid_angle_backward": (ci, [vp, ci, vp, i64, vp, i64, vp, i64, vp, vp, vp]),
             "mfs_single_layer_apply": (ci, [vp, ci, vp, i64, vp, i64, vp, i64, vp]),
@@ -349,6 +351,21 @@ class Context:
             self._check(lib().mfs_points_residual(self._h, _ptr(B), B.shape[1], _ptr(X), _ptr(out)),
                         "mfs_points_residual")
         return out
+
+    def dense_normal_apply(self, M, X):
+        """Q = M^T (M X) in FP64 (X: n x k float32 cuda, k <= 16) through the
+        PCG's matvec (one pass over M when MFS_PARAM_FUSED allows it)."""
+        import torch
+        self._sync_stream()
+        if X.dim() == 1:
+            X = X.unsqueeze(1)
+        X = X.to(torch.float32).contiguous()
+        if M.dim() != 2 or X.shape[0] != M.shape[1]:
+            raise ValueError("X must have as many rows as M has columns")
+        Q = torch.empty((M.shape[1], X.shape[1]), dtype=torch.float64, device=X.device)
+        self._check(lib().mfs_dense_normal_apply(self._h, _ptr(M.contiguous()), M.shape[0], M.shape[1],
+                                                 _ptr(X), X.shape[1], _ptr(Q)), "mfs_dense_normal_apply")
+        return Q
 
     def points_apply(self, v, trans=False, precision=MFS_FP32):
         """M v (v: n x k) or M^T v (trans, v: m x k) from the packed points."""

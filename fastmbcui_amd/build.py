@@ -22,7 +22,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("MFS_OFFLOAD_ARCH", "gfx950")
 
 HIP_SOURCES = ["mfs_core.hip", "k_plan.hip", "k_factor.hip", "k_solve.hip", "k_points.hip",
-               "k_mlp.hip", "mlp_core.hip", "k_caller.hip"]
+               "k_mlp.hip", "mlp_core.hip", "k_caller.hip", "k_fused.hip"]
 HOST_SOURCES = ["plan_host.cpp"]
 HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

@@ -766,16 +766,37 @@ hipError_t init_k(hipStream_t s, const SolveArgs<TA> &a, const float *B, int64_t
   return hipGetLastError();
 }
 
+// q = M^T (M p): one pass over M (dense FP32 batch of 8 with the exchange
+// records set up) or two (y = M p, then split partials of M^T y).
 template <typename TA, int K>
-hipError_t iter_k(hipStream_t s, const SolveArgs<TA> &a, TrackFn fn, void *tctx) {
-  const unsigned gn = blocks_for(a.n);
+hipError_t normal_matvec(hipStream_t s, const SolveArgs<TA> &a, TrackFn fn, void *tctx,
+                         int32_t *splits) {
+  if constexpr (sizeof(TA) == 4 && K == 8) {
+    if (a.fused > 0 && a.pts.tgt32 == nullptr) {
+      track(fn, tctx, 6, 0);
+      const hipError_t e = mv_fused8(s, a.M, a.m, a.n, a.pv, a.part, a.frec, a.st, a.fused);
+      track(fn, tctx, 6, 1);
+      *splits = 1;
+      return e;
+    }
+  }
   track(fn, tctx, 1, 0);
   mv_rows<TA, K>(s, a, a.pv, nullptr, 0, 0, K, a.y, a.st);
   track(fn, tctx, 1, 1);
   track(fn, tctx, 2, 0);
   mv_cols<TA, TA, K>(s, a, a.y, a.st);
   track(fn, tctx, 2, 1);
-  k_mv_reduce<TA, K, RED_Q><<<gn, kThreads, 0, s>>>(a.part, a.row_blocks, a.n, a.P, a.pv, a.q,
+  *splits = a.row_blocks;
+  return hipSuccess;
+}
+
+template <typename TA, int K>
+hipError_t iter_k(hipStream_t s, const SolveArgs<TA> &a, TrackFn fn, void *tctx) {
+  const unsigned gn = blocks_for(a.n);
+  int32_t splits = 0;
+  const hipError_t e = normal_matvec<TA, K>(s, a, fn, tctx, &splits);
+  if (e != hipSuccess) return e;
+  k_mv_reduce<TA, K, RED_Q><<<gn, kThreads, 0, s>>>(a.part, splits, a.n, a.P, a.pv, a.q,
                                                      nullptr, a.dpart, a.st, nullptr, 0.0, K, 0);
   k_upd_xr<TA, K><<<gn, kThreads, 0, s>>>(a.n, a.x, a.r, a.p, a.q, a.dpart, a.st);
   const unsigned gs = unsigned((a.n * 8 + kThreads - 1) / kThreads);
@@ -871,6 +892,35 @@ template <typename TA>
 hipError_t pcg_finish(hipStream_t s, const SolveArgs<TA> &a, float *X, int64_t ldx, int64_t col0,
                       int kval, double *X64) {
   MFS_DISPATCH_K(a.K, (finish_k<TA, KK>(s, a, X, ldx, col0, kval, X64)));
+}
+
+// Q (n x kval, FP64, original order) = the sum of `splits` partials.
+__global__ void k_sum_splits(const double *__restrict__ part, int32_t splits, int64_t n, int K,
+                             int kval, double *__restrict__ Q) {
+  const int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x;
+  if (i >= n) return;
+  for (int r = 0; r < kval; ++r) {
+    double s = 0.0;
+    for (int32_t b = 0; b < splits; ++b) s += part[(int64_t(b) * n + i) * K + r];
+    Q[i * kval + r] = s;
+  }
+}
+
+hipError_t normal_apply32(hipStream_t s, const SolveArgs<float> &a, int kval, double *Q,
+                          TrackFn fn, void *tctx) {
+  int32_t splits = 0;
+  hipError_t e = hipErrorInvalidValue;
+  switch (a.K) {
+    case 1: e = normal_matvec<float, 1>(s, a, fn, tctx, &splits); break;
+    case 2: e = normal_matvec<float, 2>(s, a, fn, tctx, &splits); break;
+    case 4: e = normal_matvec<float, 4>(s, a, fn, tctx, &splits); break;
+    case 8: e = normal_matvec<float, 8>(s, a, fn, tctx, &splits); break;
+    case 16: e = normal_matvec<float, 16>(s, a, fn, tctx, &splits); break;
+    default: break;
+  }
+  if (e != hipSuccess) return e;
+  k_sum_splits<<<blocks_for(a.n), kThreads, 0, s>>>(a.part, splits, a.n, a.K, kval, Q);
+  return hipGetLastError();
 }
 
 hipError_t residual64(hipStream_t s, const SolveArgs<double> &a, const float *B, int64_t ldb,

@@ -93,6 +93,9 @@ struct mfs_ctx {
   PointsOp pts{};
   int32_t points_blocks = 0;  // tuning: blocks per points pass (0 = default)
   int32_t cols_blocks = 0;    // tuning: blocks of the dense M^T pass (0 = default)
+  int32_t fused_mode = 0;     // one-pass normal matvec: 0 default (off), 1 off, 2 whenever it applies
+  int cus = 0;                // compute units of the device
+  DevBuf dfrec;               // exchange records of the one-pass matvec
   DevBuf dt32, dt64, ds32, ds64, dapin, dapout;
 
   // caller steps (solid angles, single layer): scratch and the index flag
@@ -487,6 +490,28 @@ int batch_width(int64_t k) {
   return 16;
 }
 
+// One pass over M per FP32 iteration for dense batches of 8, when asked for
+// (MFS_PARAM_FUSED = 2) and the shape qualifies.  Not the default: on MI355X
+// at m = n = 65,536 the two passes take 5.6 ms and the one pass 8.4 ms, its
+// cross-block exchange of y being latency-bound.
+int setup_fused(mfs_ctx *ctx, const PointsOp *pts, int64_t m, int64_t n, int K, int32_t &blocks,
+                uint64_t *&records) {
+  blocks = 0;
+  records = nullptr;
+  if (pts != nullptr || ctx->fused_mode != 2) return MFS_OK;
+  const int32_t G = fused_blocks(m, n, K, ctx->cus);
+  if (G <= 0) return MFS_OK;
+  const size_t bytes = fused_records(G) * sizeof(uint64_t);
+  if (ctx->dfrec.cap < bytes || ctx->dfrec.p == nullptr) {
+    HIPCHK(ctx->dfrec.ensure(bytes), "alloc exchange records");
+    // records of an earlier owner of this memory must not carry live tags
+    HIPCHK(hipMemsetAsync(ctx->dfrec.p, 0, ctx->dfrec.cap, ctx->stream), "clear records");
+  }
+  blocks = G;
+  records = ctx->dfrec.as<uint64_t>();
+  return MFS_OK;
+}
+
 // Lay out a SolveArgs workspace for batch width K.
 template <typename TA>
 int workspace(mfs_ctx *ctx, const float *M, const PointsOp *pts, int64_t m, int64_t n, int K,
@@ -532,6 +557,9 @@ int workspace(mfs_ctx *ctx, const float *M, const PointsOp *pts, int64_t m, int6
     HIPCHK(hipMemsetAsync(ctx->dstate.p, 0, sizeof(PcgState), ctx->stream), "memset state");
   }
   a.st = ctx->dstate.as<PcgState>();
+  a.fused = 0;
+  a.frec = nullptr;
+  if (sizeof(TA) == 4) return setup_fused(ctx, pts, m, n, K, a.fused, a.frec);
   return MFS_OK;
 }
 
@@ -555,6 +583,13 @@ int iterate(mfs_ctx *ctx, const SolveArgs<TA> &a, int32_t maxit) {
   const int dead = launched - ctx->hstate->iter;
   track_discard(ctx, MFS_K_MATVEC_ROWS, dead);
   track_discard(ctx, MFS_K_MATVEC_COLS, dead);
+  track_discard(ctx, MFS_K_MATVEC_FUSED, dead);
+  if (ctx->hstate->fabort != 0u) {
+    const uint32_t zero = 0;
+    (void)hipMemcpyAsync(&a.st->fabort, &zero, sizeof(zero), hipMemcpyHostToDevice, ctx->stream);
+    (void)hipStreamSynchronize(ctx->stream);
+    return ctx->fail(MFS_ERR_HIP, "one-pass matvec: a wait for another block timed out");
+  }
   return MFS_OK;
 }
 
@@ -847,6 +882,8 @@ struct MixedRun {
     a32.x = base; a32.r = base + nv; a32.p = base + 2 * nv; a32.q = base + 3 * nv;
     a32.z = base + 4 * nv; a32.t = base + 5 * nv; a32.pv = base + 6 * nv; a32.y = base + 7 * nv;
     a32.part = a64.part; a32.dpart = a64.dpart; a32.st = a64.st;
+    rc = setup_fused(ctx, pts, m, n, K, a32.fused, a32.frec);
+    if (rc != MFS_OK) return rc;
 
     const unsigned gn = unsigned((n + kThreads - 1) / kThreads);
     if (X0d != nullptr)
@@ -1001,7 +1038,8 @@ int mfs_ctx_create(mfs_ctx **out, int device, void *stream) {
   if (hipSetDevice(device) != hipSuccess || hipEventCreate(&ctx->ev0) != hipSuccess ||
       hipEventCreate(&ctx->ev1) != hipSuccess ||
       hipHostMalloc(reinterpret_cast<void **>(&ctx->hstate), sizeof(PcgState)) != hipSuccess ||
-      hipHostMalloc(reinterpret_cast<void **>(&ctx->hnorm), 4 * kMaxRhs * sizeof(double)) != hipSuccess) {
+      hipHostMalloc(reinterpret_cast<void **>(&ctx->hnorm), 4 * kMaxRhs * sizeof(double)) != hipSuccess ||
+      hipDeviceGetAttribute(&ctx->cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) {
     mfs_ctx_destroy(ctx);
     return MFS_ERR_HIP;
   }
@@ -1020,7 +1058,7 @@ void mfs_ctx_destroy(mfs_ctx *ctx) {
                     &ctx->dtotal, &ctx->dbands, &ctx->dcells, &ctx->dcpts, &ctx->dlong, &ctx->dval, &ctx->dtval, &ctx->dMt, &ctx->dval_alt,
                     &ctx->dtval_alt, &ctx->dthg, &ctx->dfscene, &ctx->dvec, &ctx->dpart, &ctx->ddpart, &ctx->dstate,
                     &ctx->dx64, &ctx->dr64, &ctx->dnorm, &ctx->dt32, &ctx->dt64, &ctx->ds32,
-                    &ctx->ds64, &ctx->dapin, &ctx->dapout, &ctx->dcaller, &ctx->dflag})
+                    &ctx->ds64, &ctx->dapin, &ctx->dapout, &ctx->dcaller, &ctx->dflag, &ctx->dfrec})
     b->release();
   if (ctx->hstate) (void)hipHostFree(ctx->hstate);
   if (ctx->hnorm) (void)hipHostFree(ctx->hnorm);
@@ -1212,6 +1250,51 @@ int mfs_points_residual(mfs_ctx *ctx, const float *B, int64_t k, const float *X,
   int rc = check_points(ctx);
   if (rc != MFS_OK) return rc;
   return residual_impl(ctx, nullptr, &ctx->pts, ctx->pts.m, ctx->n, B, k, X, out);
+}
+
+int mfs_dense_normal_apply(mfs_ctx *ctx, const float *M, int64_t m, int64_t n, const float *X,
+                           int64_t k, double *Q) {
+  if (ctx == nullptr) return MFS_ERR_ARG;
+  if (M == nullptr || X == nullptr || Q == nullptr || m <= 0 || n <= 0 || k <= 0 || k > kMaxRhs)
+    return ctx->fail(MFS_ERR_ARG, "dense_normal_apply: bad arguments (1 <= k <= 16)");
+  HIPCHK(hipSetDevice(ctx->device), "hipSetDevice");
+  const int K = batch_width(k);
+  SolveArgs<float> a{};
+  a.M = M;
+  a.m = m;
+  a.n = n;
+  a.K = K;
+  a.row_blocks = row_blocks_for(m, n, ctx->cols_blocks);
+  a.pts = PointsOp{};
+  const size_t nv = size_t(n) * K, mv = size_t(m) * K;
+  HIPCHK(ctx->dvec.ensure((nv + mv) * sizeof(float) + 64), "alloc vectors");
+  a.pv = ctx->dvec.as<float>();
+  a.y = a.pv + nv;
+  HIPCHK(ctx->dpart.ensure(size_t(a.row_blocks) * nv * sizeof(double)), "alloc partials");
+  a.part = ctx->dpart.as<double>();
+  if (ctx->dstate.p == nullptr) {
+    HIPCHK(ctx->dstate.ensure(sizeof(PcgState)), "alloc state");
+    HIPCHK(hipMemsetAsync(ctx->dstate.p, 0, sizeof(PcgState), ctx->stream), "memset state");
+  }
+  a.st = ctx->dstate.as<PcgState>();
+  int rc = setup_fused(ctx, nullptr, m, n, K, a.fused, a.frec);
+  if (rc != MFS_OK) return rc;
+  const int32_t one = 1;
+  HIPCHK(hipMemcpyAsync(&a.st->n_active, &one, sizeof(one), hipMemcpyHostToDevice, ctx->stream),
+         "arm state");
+  if (k < K) HIPCHK(hipMemsetAsync(a.pv, 0, nv * sizeof(float), ctx->stream), "memset");
+  HIPCHK(hipMemcpy2DAsync(a.pv, K * sizeof(float), X, k * sizeof(float), k * sizeof(float), n,
+                          hipMemcpyDeviceToDevice, ctx->stream), "pack input");
+  for (int c = 1; c < 8; ++c) ctx->bytes_per_launch[c] = double(m) * double(n) * 4.0;
+  HIPCHK(normal_apply32(ctx->stream, a, int(k), Q, track_cb, ctx), "normal apply");
+  HIPCHK(hipStreamSynchronize(ctx->stream), "sync");
+  HIPCHK(hipMemcpy(ctx->hstate, a.st, sizeof(PcgState), hipMemcpyDeviceToHost), "read state");
+  if (ctx->hstate->fabort != 0u) {
+    const uint32_t zero = 0;
+    (void)hipMemcpy(&a.st->fabort, &zero, sizeof(zero), hipMemcpyHostToDevice);
+    return ctx->fail(MFS_ERR_HIP, "one-pass matvec: a wait for another block timed out");
+  }
+  return MFS_OK;
 }
 
 int mfs_points_apply(mfs_ctx *ctx, int trans, int precision, const void *in, int64_t k,
@@ -1409,6 +1492,10 @@ int mfs_ctx_set_param(mfs_ctx *ctx, int param, int64_t value) {
       return MFS_OK;
     case MFS_PARAM_COLS_BLOCKS:
       ctx->cols_blocks = int32_t(value);
+      return MFS_OK;
+    case MFS_PARAM_FUSED:
+      if (value > 2) return ctx->fail(MFS_ERR_ARG, "set_param: fused mode is 0, 1 or 2");
+      ctx->fused_mode = int32_t(value);
       return MFS_OK;
     default:
       return ctx->fail(MFS_ERR_ARG, "set_param: unknown parameter");

@@ -38,6 +38,12 @@ struct PcgState {
   int32_t maxit;
   int32_t pad;
   uint32_t counter[8];      // arrival counters of the last-block reductions
+  // one-pass normal matvec (k_fused.hip): launch epoch of its tagged records,
+  // waves finished in the current launch, set when a bounded wait gave up
+  uint32_t fepoch;
+  uint32_t fdone;
+  uint32_t fabort;
+  uint32_t fpad;
 };
 
 enum Counter { C_PQ = 0, C_RR = 1, C_RZ = 2, C_P = 3, C_R0 = 4, C_NORM = 5 };
@@ -127,7 +133,30 @@ struct SolveArgs {
   PcgState *st;
   int32_t row_blocks;          // number of row blocks (splits) of the M^T pass
   PointsOp pts;                // points form when pts.tgt32 != nullptr (M unused)
+  int32_t fused = 0;           // blocks of the one-pass normal matvec (0: two passes)
+  uint64_t *frec = nullptr;    // its exchange records (fused_records(fused) of them)
 };
+
+// One HBM pass over M for q = M^T (M p), dense FP32 batches of 8 (k_fused.hip).
+// Panels of 16 rows; y of a panel is used kFusedDefer panels later, and the
+// exchange records live in rings of kFusedSlots panels.
+constexpr int kFusedDefer = 8;
+constexpr int kFusedSlots = 16;
+// Blocks (one per 256-column slice, all resident) or 0 when the shape does not
+// qualify (K != 8, n % 4 != 0, fewer than 8 or more slices than CUs, 65,535
+// panels or more).
+int32_t fused_blocks(int64_t m, int64_t n, int K, int cus);
+size_t fused_records(int32_t G);
+// part[c][r] = (M^T M X)[c][r] in FP64, original column order (X: n x 8).
+hipError_t mv_fused8(hipStream_t st, const float *M, int64_t m, int64_t n, const float *X,
+                     double *part, uint64_t *records, PcgState *state, int32_t G);
+
+// Q (n x kval, FP64, original column order) = M^T (M X) for the FP32 batch in
+// a.pv (n x a.K, original order, zero beyond kval), through the iteration's
+// own matvec (one pass when a.fused > 0, else two); a.st->n_active must be
+// non-zero.
+hipError_t normal_apply32(hipStream_t st, const SolveArgs<float> &a, int kval, double *Q,
+                          void (*track)(void *, int, int), void *track_ctx);
 
 // thresh_abs (device, K values, may be null) replaces tol^2 ||r0||^2 as the
 // stopping threshold of each column (used by the refinement's inner solves).

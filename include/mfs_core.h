@@ -155,6 +155,16 @@ int mfs_points_residual64(mfs_ctx *ctx, const float *B, int64_t k, const double 
 int mfs_points_apply(mfs_ctx *ctx, int trans, int precision, const void *in, int64_t k,
                      void *out);
 
+/* Q (n x k, FP64, row-major) = M^T (M X) for the dense FP32 M (m x n,
+ * row-major, device) and X (n x k, FP32, device), 1 <= k <= 16, through the
+ * PCG iteration's own matvec: one HBM pass over M (k_fused.hip) when
+ * MFS_PARAM_FUSED allows it for this shape and k pads to 8, else y = M X then
+ * M^T y.  Returns after the work is done.  The matvec pair of
+ * src/kl_chol.cu:928-936 and :992-1002 (cublasSgemv with M then M^T) for a
+ * batch of right-hand sides. */
+int mfs_dense_normal_apply(mfs_ctx *ctx, const float *M, int64_t m, int64_t n, const float *X,
+                           int64_t k, double *Q);
+
 /* Dense point-kernel matrix M[i,j] = -(x_i - y_j).n_i / (4 pi |x_i - y_j|^3),
  * evaluated in FP64 and stored FP32.  x, nx: m x 3, y: n x 3, device FP64.
  * Point form of src/kernels.h:24-40 (laplace_nm_3d_sl, up to the factor
@@ -202,7 +212,10 @@ int mfs_single_layer_backward(mfs_ctx *ctx, int precision, const void *S, int64_
  * beyond summation order. */
 enum mfs_param {
     MFS_PARAM_POINTS_BLOCKS = 1,  /* blocks per points pass, owners x splits (4096) */
-    MFS_PARAM_COLS_BLOCKS = 2     /* blocks of the dense M^T pass (2048) */
+    MFS_PARAM_COLS_BLOCKS = 2,    /* blocks of the dense M^T pass (2048) */
+    MFS_PARAM_FUSED = 3           /* one-pass M^T M p for dense FP32 batches of 8:
+                                     0 default (two passes), 1 off, 2 whenever the shape
+                                     qualifies (n % 4 == 0, 8 <= n / 256 <= CUs) */
 };
 int mfs_ctx_set_param(mfs_ctx *ctx, int param, int64_t value);
 
@@ -218,7 +231,8 @@ enum mfs_kernel_class {
     MFS_K_MATVEC_COLS = 2,   /* PCG q = M^T y   (one HBM pass over M, whole batch) */
     MFS_K_GRAM = 3,          /* fused supernode Gram + Cholesky + columns of L */
     MFS_K_RESID_ROWS = 4,    /* FP64 residual y = b - M x */
-    MFS_K_RESID_COLS = 5     /* FP64 residual r = M^T y */
+    MFS_K_RESID_COLS = 5,    /* FP64 residual r = M^T y */
+    MFS_K_MATVEC_FUSED = 6   /* PCG q = M^T (M p) in one HBM pass over M (dense, FP32, 8 RHS) */
 };
 int mfs_ctx_track_kernel(mfs_ctx *ctx, int mask);
 int mfs_ctx_tracked(mfs_ctx *ctx, int kernel_class, int64_t *launches, double *total_ms,

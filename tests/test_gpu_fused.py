@@ -1,0 +1,187 @@
+"""The one-pass normal matvec (k_fused.hip) through the C-ABI.
+
+* Q = M^T (M X) from mfs_dense_normal_apply against the same product in FP64
+  (numpy / torch on FP64 copies of the FP32 inputs, whose products are
+  exact): the one-pass kernel and the two-pass matvec both to FP32-level
+  relative error, at ragged row panels (m not a multiple of 16), a partial
+  last column slice (n not a multiple of 256), a batch of 5 padded to 8 and
+  the full 65,536 x 65,536 workload (256 slices, one per CU);
+* the mixed solve, whose answer is certified by the FP64 two-pass residual
+  M^T (b - M x), reaches its tolerance with the one-pass matvec, boundary and
+  source-space right-hand sides, and the batch (which runs until its slowest
+  column converges) takes no more iterations than with two passes, up to one
+  (per column the counts scatter both ways with the FP32 rounding);
+* shapes the kernel does not take (n % 4 != 0, k = 4, fewer than 8 column
+  slices) use two passes.
+"""
+import numpy as np
+import pytest
+import torch
+
+from fastmbcui_amd import _lib, scenes
+
+pytestmark = pytest.mark.gpu
+
+DEV = torch.device("cuda", 0)
+DEFAULT, OFF, FORCE = 0, 1, 2
+
+
+def _solve(ctx, Md, B, mode, rhs_space=None, tol=1e-6):
+    ctx.set_param(_lib.MFS_PARAM_FUSED, mode)
+    ctx.track([_lib.MFS_K_MATVEC_FUSED])
+    X = ctx.solve_mixed(Md, B, maxit=2000, tol=tol, rhs_space=rhs_space)
+    torch.cuda.synchronize()
+    launches = ctx.tracked(_lib.MFS_K_MATVEC_FUSED)[0]
+    ctx.track([])
+    s = ctx.stats()
+    iters = [int(s.iters[c]) for c in range(B.shape[1])]
+    ctx.set_param(_lib.MFS_PARAM_FUSED, DEFAULT)
+    return X, launches, iters
+
+
+def _setup(sc, extra_rows=0, seed=0):
+    M = scenes.kernel_matrix(sc.x, sc.nx, sc.y).astype(np.float32)
+    B = sc.b.astype(np.float32)
+    if extra_rows:
+        rng = np.random.default_rng(seed)
+        idx = rng.choice(M.shape[0], extra_rows, replace=False)
+        M = np.vstack([M, 0.5 * M[idx]])
+        B = np.vstack([B, rng.standard_normal((extra_rows, B.shape[1])).astype(np.float32)])
+    ctx = _lib.Context(0)
+    ctx.plan(sc.y, 3.0)
+    Md = torch.from_numpy(M).to(DEV)
+    ctx.factor_dense(Md, _lib.MFS_FP32)
+    return ctx, Md, torch.from_numpy(B).to(DEV)
+
+
+def _compare(ctx, Md, B, want_fused=True, rhs_space=None):
+    Xf, lf, itf = _solve(ctx, Md, B, FORCE, rhs_space)
+    X2, l2, it2 = _solve(ctx, Md, B, OFF, rhs_space)
+    print(f"fused: m={Md.shape[0]} n={Md.shape[1]} k={B.shape[1]} launches {lf}, iters {itf} vs {it2}",
+          flush=True)
+    assert l2 == 0
+    assert (lf > 0) == want_fused
+    if rhs_space is None:
+        res = ctx.normal_residual(Md, B, Xf)
+        assert res.max() <= 1e-6, res
+    assert max(itf) <= max(it2) + 1, (itf, it2)
+    return Xf
+
+
+def _apply(ctx, Md, X, mode):
+    ctx.set_param(_lib.MFS_PARAM_FUSED, mode)
+    ctx.track([_lib.MFS_K_MATVEC_FUSED])
+    Q = ctx.dense_normal_apply(Md, X)
+    launches = ctx.tracked(_lib.MFS_K_MATVEC_FUSED)[0]
+    ctx.track([])
+    ctx.set_param(_lib.MFS_PARAM_FUSED, DEFAULT)
+    return Q, launches
+
+
+def _ref_normal(Md, X, chunk=8192):
+    """M^T (M X) in FP64 on the device, in row chunks of M."""
+    X64 = X.double()
+    Q = torch.zeros((Md.shape[1], X.shape[1]), dtype=torch.float64, device=Md.device)
+    for r0 in range(0, Md.shape[0], chunk):
+        Mc = Md[r0:r0 + chunk].double()
+        Q += Mc.t() @ (Mc @ X64)
+    return Q
+
+
+def _check_apply(Md, X, want_fused=True, tol=2e-6):
+    ctx = _lib.Context(0)
+    ref = _ref_normal(Md, X)
+    for mode in (FORCE, OFF):
+        Q, launches = _apply(ctx, Md, X, mode)
+        assert (launches > 0) == (want_fused and mode == FORCE), (mode, launches)
+        err = ((Q - ref).norm(dim=0) / ref.norm(dim=0)).max().item()
+        print(f"normal apply m={Md.shape[0]} n={Md.shape[1]} k={X.shape[1]} mode {mode}: "
+              f"one-pass launches {launches}, max rel. error {err:.2e}", flush=True)
+        assert err < tol, (mode, err)
+
+
+def test_normal_apply_matches_fp64_ragged_rows():
+    sc = scenes.coins(seed=0, n_bodies=8, pts_per_body=512, k=8)
+    _, Md, _ = _setup(sc, extra_rows=45)                              # m = 4141: last panel of 13 rows
+    X = torch.randn((sc.n, 8), device=DEV, generator=torch.Generator(DEV).manual_seed(0))
+    _check_apply(Md, X)
+
+
+def test_normal_apply_partial_slice_padded_batch():
+    sc = scenes.ellipsoids(seed=1, n_bodies=4, pts_per_body=500, k=5, box=8.0)   # n = 2000
+    _, Md, _ = _setup(sc)
+    X = torch.randn((sc.n, 5), device=DEV, generator=torch.Generator(DEV).manual_seed(1))
+    _check_apply(Md, X)
+
+
+def test_normal_apply_other_widths_use_two_passes():
+    sc = scenes.coins(seed=3, n_bodies=3, pts_per_body=161, k=8, box=6.0)    # n = 483
+    _, Md, _ = _setup(sc)
+    X = torch.randn((sc.n, 8), device=DEV, generator=torch.Generator(DEV).manual_seed(2))
+    _check_apply(Md, X, want_fused=False)
+    sc = scenes.coins(seed=3, n_bodies=4, pts_per_body=256, k=4)      # n = 1024: 4 slices
+    _, Md, _ = _setup(sc)
+    X = torch.randn((sc.n, 4), device=DEV, generator=torch.Generator(DEV).manual_seed(3))
+    _check_apply(Md, X, want_fused=False)
+    X = torch.randn((sc.n, 8), device=DEV, generator=torch.Generator(DEV).manual_seed(5))
+    _check_apply(Md, X, want_fused=False)
+
+
+def test_fused_matches_two_pass_coins_ragged_rows():
+    sc = scenes.coins(seed=0, n_bodies=8, pts_per_body=512, k=8)    # n = 4096: 16 slices
+    ctx, Md, B = _setup(sc, extra_rows=45)                           # m = 4141
+    _compare(ctx, Md, B)
+
+
+def test_fused_matches_two_pass_partial_slice_and_padded_batch():
+    sc = scenes.ellipsoids(seed=1, n_bodies=4, pts_per_body=500, k=5, box=8.0)  # n = 2000
+    ctx, Md, B = _setup(sc)
+    _compare(ctx, Md, B)
+
+
+def test_fused_source_space_right_hand_sides():
+    sc = scenes.coins(seed=2, n_bodies=8, pts_per_body=256, k=1)     # n = 2048: 8 slices
+    ctx, Md, _ = _setup(sc)
+    rng = np.random.default_rng(7)
+    G = torch.from_numpy(rng.standard_normal((sc.n, 8)).astype(np.float32)).to(DEV)
+    _compare(ctx, Md, G, rhs_space=_lib.MFS_RHS_SOURCE)
+    # the FP64 residual ||b - M^T M x|| / ||b|| of the solver's FP64 iterate
+    # (the FP32 output adds its own rounding, amplified by the conditioning)
+    s = ctx.stats()
+    assert all(s.relres[c] <= 1e-6 for c in range(8))
+
+
+def test_fused_not_used_when_columns_are_not_float4_aligned():
+    sc = scenes.coins(seed=3, n_bodies=3, pts_per_body=161, k=8, box=6.0)  # n = 483
+    ctx, Md, B = _setup(sc)
+    _compare(ctx, Md, B, want_fused=False)
+
+
+def test_fused_full_workload_one_slice_per_cu():
+    """64 ellipsoids x 1024 points, 8 right-hand sides: M is 65,536 x 65,536
+    (17.2 GB), 256 slices; the default is two passes."""
+    sc = scenes.ellipsoids(seed=0, n_bodies=64, k=8)
+    ctx = _lib.Context(0)
+    x = torch.from_numpy(sc.x).to(DEV)
+    nx = torch.from_numpy(sc.nx).to(DEV)
+    y = torch.from_numpy(sc.y).to(DEV)
+    Md = ctx.assemble_dense(x, nx, y)
+    B = torch.from_numpy(sc.b.astype(np.float32)).to(DEV)
+    ctx.plan(sc.y, 3.0)
+    ctx.factor_dense(Md, _lib.MFS_FP32)
+    Xa, la, ita = _solve(ctx, Md, B, FORCE)
+    X2, l2, it2 = _solve(ctx, Md, B, DEFAULT)
+    print(f"fused full: launches {la}, iters {ita} vs {it2}", flush=True)
+    assert la > 0 and l2 == 0
+    assert ctx.normal_residual(Md, B, Xa).max() <= 1e-6
+    assert max(ita) <= max(it2) + 1, (ita, it2)
+    X = torch.randn((sc.n, 8), device=DEV, generator=torch.Generator(DEV).manual_seed(4))
+    ref = _ref_normal(Md, X)
+    for mode in (FORCE, DEFAULT):
+        Q, launches = _apply(ctx, Md, X, mode)
+        assert (launches > 0) == (mode == FORCE)
+        err = ((Q - ref).norm(dim=0) / ref.norm(dim=0)).max().item()
+        print(f"normal apply full mode {mode}: max rel. error {err:.2e}", flush=True)
+        assert err < 2e-6, (mode, err)
+    del Md, ref
+    torch.cuda.empty_cache()
