@@ -730,6 +730,46 @@ void mv_cols_raw(hipStream_t s, const float *M, int64_t m, int64_t n, int32_t ro
   }
 }
 
+}  // namespace
+
+// Blocks of the dense M^T pass resident per CU (the instantiation launched
+// for large grids: 4 rows in flight per thread).
+int cols_resident_blocks(bool fp64, int K, bool vec) {
+  static int cache[2][5][2] = {};
+  const int ki = K == 1 ? 0 : K == 2 ? 1 : K == 4 ? 2 : K == 8 ? 3 : 4;
+  int &c = cache[fp64][ki][vec];
+  if (c > 0) return c;
+  const void *f = nullptr;
+#define MFS_COLS_FN(T, KK)                                                                         \
+  f = vec ? reinterpret_cast<const void *>(k_mv_cols<T, KK, true, 4>)                             \
+          : reinterpret_cast<const void *>(k_mv_cols<T, KK, false, 4>)
+  if (fp64) {
+    switch (K) {
+      case 1: MFS_COLS_FN(double, 1); break;
+      case 2: MFS_COLS_FN(double, 2); break;
+      case 4: MFS_COLS_FN(double, 4); break;
+      case 8: MFS_COLS_FN(double, 8); break;
+      default: MFS_COLS_FN(double, 16); break;
+    }
+  } else {
+    switch (K) {
+      case 1: MFS_COLS_FN(float, 1); break;
+      case 2: MFS_COLS_FN(float, 2); break;
+      case 4: MFS_COLS_FN(float, 4); break;
+      case 8: MFS_COLS_FN(float, 8); break;
+      default: MFS_COLS_FN(float, 16); break;
+    }
+  }
+#undef MFS_COLS_FN
+  int nb = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, f, kThreads, 0) != hipSuccess || nb < 1)
+    nb = 2;
+  c = nb;
+  return c;
+}
+
+namespace {
+
 // part = split sums of M^T Y (a.row_blocks splits in both forms).
 template <typename TA, typename TY, int K>
 void mv_cols(hipStream_t s, const SolveArgs<TA> &a, const TY *Y, const PcgState *st) {

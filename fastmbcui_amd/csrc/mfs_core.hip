@@ -472,11 +472,14 @@ int values_for(mfs_ctx *ctx, int want, const void **lval, const void **ltval) {
   return MFS_OK;
 }
 
-// Row blocks of the M^T pass: about `target` (default 2048) blocks in all, at
-// least 64 rows each.
-int32_t row_blocks_for(int64_t m, int64_t n, int32_t target) {
+// Row blocks of the M^T pass: about `target` blocks in all (default: one
+// full round, every CU holding as many as fit), at least 64 rows each.  At
+// m = n = 65,536 (FP32, 8 columns, 3 resident blocks per CU) one round of
+// 768 blocks reads M at 6.74 TB/s where 2048 (2.7 rounds) reach 6.31 TB/s and
+// 1024 (1.3 rounds) 5.6 TB/s: a partial last round leaves CUs idle.
+int32_t row_blocks_for(int64_t m, int64_t n, int32_t target, int64_t per_round) {
   const int64_t cols = (n + 1023) / 1024;
-  const int64_t want = target > 0 ? target : 2048;
+  const int64_t want = target > 0 ? target : std::max<int64_t>(per_round, 1);
   int64_t rb = (want + cols - 1) / cols;
   rb = std::min<int64_t>(rb, std::max<int64_t>(m / 64, 1));
   return int32_t(std::max<int64_t>(rb, 1));
@@ -532,7 +535,9 @@ int workspace(mfs_ctx *ctx, const float *M, const PointsOp *pts, int64_t m, int6
   a.tcol = ctx->dtcol.as<int32_t>();
   a.lval = static_cast<const TA *>(lv);
   a.ltval = static_cast<const TA *>(ltv);
-  a.row_blocks = pts ? points_splits(n, m, pts->target_blocks) : row_blocks_for(m, n, ctx->cols_blocks);
+  a.row_blocks = pts ? points_splits(n, m, pts->target_blocks)
+                     : row_blocks_for(m, n, ctx->cols_blocks,
+                                      int64_t(ctx->cus) * cols_resident_blocks(sizeof(TA) == 8, K, n % 4 == 0));
   a.pts = pts ? *pts : PointsOp{};
   const size_t nv = size_t(n) * K, mv = size_t(m) * K;
   HIPCHK(ctx->dvec.ensure((7 * nv + mv) * sizeof(TA) + 64), "alloc vectors");
@@ -876,6 +881,17 @@ struct MixedRun {
     a32.lval = static_cast<const float *>(lv);
     a32.ltval = static_cast<const float *>(ltv);
     a32.row_blocks = a64.row_blocks;
+    if (!pts) {
+      // the FP32 passes get their own split count (a different kernel with
+      // its own occupancy) over the same partials buffer
+      a32.row_blocks = row_blocks_for(m, n, ctx->cols_blocks,
+                                      int64_t(ctx->cus) * cols_resident_blocks(false, K, n % 4 == 0));
+      if (a32.row_blocks > a64.row_blocks) {
+        HIPCHK(ctx->dpart.ensure(size_t(a32.row_blocks) * size_t(n) * K * sizeof(double)),
+               "alloc partials");
+        a64.part = ctx->dpart.as<double>();
+      }
+    }
     a32.pts = a64.pts;
     float *base = reinterpret_cast<float *>(a64.x);
     const size_t nv = size_t(n) * K;
@@ -1264,7 +1280,8 @@ int mfs_dense_normal_apply(mfs_ctx *ctx, const float *M, int64_t m, int64_t n, c
   a.m = m;
   a.n = n;
   a.K = K;
-  a.row_blocks = row_blocks_for(m, n, ctx->cols_blocks);
+  a.row_blocks = row_blocks_for(m, n, ctx->cols_blocks,
+                                int64_t(ctx->cus) * cols_resident_blocks(false, K, n % 4 == 0));
   a.pts = PointsOp{};
   const size_t nv = size_t(n) * K, mv = size_t(m) * K;
   HIPCHK(ctx->dvec.ensure((nv + mv) * sizeof(float) + 64), "alloc vectors");

@@ -158,6 +158,10 @@ hipError_t mv_fused8(hipStream_t st, const float *M, int64_t m, int64_t n, const
 hipError_t normal_apply32(hipStream_t st, const SolveArgs<float> &a, int kval, double *Q,
                           void (*track)(void *, int, int), void *track_ctx);
 
+// Blocks of the dense M^T pass (k_mv_cols, TA = double when fp64) resident
+// on one CU; vec: n % 4 == 0.
+int cols_resident_blocks(bool fp64, int K, bool vec);
+
 // thresh_abs (device, K values, may be null) replaces tol^2 ||r0||^2 as the
 // stopping threshold of each column (used by the refinement's inner solves).
 template <typename TA>

@@ -212,7 +212,7 @@ int mfs_single_layer_backward(mfs_ctx *ctx, int precision, const void *S, int64_
  * beyond summation order. */
 enum mfs_param {
     MFS_PARAM_POINTS_BLOCKS = 1,  /* blocks per points pass, owners x splits (4096) */
-    MFS_PARAM_COLS_BLOCKS = 2,    /* blocks of the dense M^T pass (2048) */
+    MFS_PARAM_COLS_BLOCKS = 2,    /* blocks of the dense M^T pass (0: one round, CUs x resident) */
     MFS_PARAM_FUSED = 3           /* one-pass M^T M p for dense FP32 batches of 8:
                                      0 default (two passes), 1 off, 2 whenever the shape
                                      qualifies (n % 4 == 0, 8 <= n / 256 <= CUs) */
