@@ -182,7 +182,9 @@ __global__ __launch_bounds__(kThreads) void k_super_factor(const FactorScene *__
   __shared__ TC W[NS * MB];
   __shared__ int32_t mcol[MB], mc[MB], moff[MB];
 
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  // wave index in a scalar register: loop bounds derived from it stay
+  // uniform branches (and exact wait counts) instead of exec-masked regions
+  const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int32_t s = item.s;
   const SuperDesc D = sd[s];
   const int ns = D.ns;
@@ -282,13 +284,20 @@ __global__ __launch_bounds__(kThreads) void k_super_factor(const FactorScene *__
     // f0 + 4 (l >> 4) + i in both operands, so the four MFMAs sum rows
     // f0 .. f0 + 15 exactly once.  Straight from global memory into registers,
     // with the next group's loads in flight: no LDS staging, no barriers.
-    const float *col[T];
+    // M^T comes from a scene record in memory, so the compiler cannot tell
+    // it is global memory: the address-space cast of the base makes these
+    // global loads (counted by vmcnt only) instead of flat ones (vmcnt and
+    // lgkmcnt, waited for conservatively).
+    typedef const __attribute__((address_space(1))) float gfloat;
+    typedef float fv4 __attribute__((ext_vector_type(4)));
+    typedef const __attribute__((address_space(1))) fv4 gfv4;
+    gfloat *col[T];
     bool ok[T];
 #pragma unroll
     for (int a = 0; a < T; ++a) {
       const int p = a * 16 + (lane & 15);
       ok[a] = p < ns;
-      col[a] = Mt + int64_t(ok[a] ? src[p] : src[0]) * ldmt + 4 * (lane >> 4);
+      col[a] = (gfloat *)Mt + int64_t(ok[a] ? src[p] : src[0]) * ldmt + 4 * (lane >> 4);
     }
     // W float4 per tile and step: each wave reads 16 W consecutive rows (64 W
     // bytes) of every column -- whole 128-byte lines from W = 2 on; the small
@@ -296,47 +305,74 @@ __global__ __launch_bounds__(kThreads) void k_super_factor(const FactorScene *__
     constexpr int W = LARGE ? 1 : (T <= 4 ? 4 : 2);
     // A wave reads slabs [f0, f0 + 16 W) with f0 = kph 16 W + j FSTEP.  16 W
     // divides 128 and ldmt is a multiple of 128, so a slab that starts below
-    // ldmt ends within the row; FSTEP (up to 256) need not divide ldmt, so
-    // every load, the first prefetch included, is issued only for f0 < ldmt.
+    // ldmt ends within the row; FSTEP (up to 256) need not divide ldmt, so no
+    // slab at or past ldmt is ever read: the first load is issued only when
+    // the wave's first slab lies below ldmt, and later ones are clamped to
+    // its last such slab.
     constexpr int FSTEP = 16 * W * KS;
-    auto load = [&](int64_t f0, float4 (&v)[T][W]) {
+    // Every lane loads (padding columns read a real row, src[0]'s, and are
+    // zeroed when used): a select on a loaded value, or loads under a
+    // lane-dependent branch, make the compiler wait for the loads at once,
+    // prefetch included.
+    auto load = [&](int64_t f0, fv4 (&v)[T][W]) {
 #pragma unroll
       for (int a = 0; a < T; ++a)
 #pragma unroll
-        for (int u = 0; u < W; ++u)
-          v[a][u] = ok[a] ? *reinterpret_cast<const float4 *>(col[a] + f0 + 16 * u)
-                          : make_float4(0.f, 0.f, 0.f, 0.f);
+        for (int u = 0; u < W; ++u) v[a][u] = *(gfv4 *)(col[a] + f0 + 16 * u);
     };
-    constexpr bool PREFETCH = !LARGE;  // the large tiles need the registers
-    float4 cur[T][W], nxt[PREFETCH ? T : 1][W];
-    if (PREFETCH && kph * 16 * W < ldmt) load(kph * 16 * W, cur);
-    for (int64_t f0 = kph * 16 * W; f0 < ldmt; f0 += FSTEP) {
-      if constexpr (PREFETCH) {
-        if (f0 + FSTEP < ldmt) load(f0 + FSTEP, nxt);
-      } else {
-        load(f0, cur);
-      }
+    // use[a]: column a's values enter the product (false for padding
+    // columns and for a slab re-read past the end of the row)
+    auto compute = [&](const fv4 (&v)[T][W], bool live) {
+      bool use[T];
+#pragma unroll
+      for (int a = 0; a < T; ++a) use[a] = ok[a] && live;
 #pragma unroll
       for (int u = 0; u < W; ++u) {
         TC fr[T];
 #pragma unroll
-        for (int a = 0; a < T; ++a) fr[a] = TC(cur[a][u].x);
+        for (int a = 0; a < T; ++a) fr[a] = use[a] ? TC(v[a][u].x) : TC(0);
         step(fr);
 #pragma unroll
-        for (int a = 0; a < T; ++a) fr[a] = TC(cur[a][u].y);
+        for (int a = 0; a < T; ++a) fr[a] = use[a] ? TC(v[a][u].y) : TC(0);
         step(fr);
 #pragma unroll
-        for (int a = 0; a < T; ++a) fr[a] = TC(cur[a][u].z);
+        for (int a = 0; a < T; ++a) fr[a] = use[a] ? TC(v[a][u].z) : TC(0);
         step(fr);
 #pragma unroll
-        for (int a = 0; a < T; ++a) fr[a] = TC(cur[a][u].w);
+        for (int a = 0; a < T; ++a) fr[a] = use[a] ? TC(v[a][u].w) : TC(0);
         step(fr);
       }
-      if constexpr (PREFETCH) {
-#pragma unroll
-        for (int a = 0; a < T; ++a)
-#pragma unroll
-          for (int u = 0; u < W; ++u) cur[a][u] = nxt[a][u];
+    };
+    const int64_t fs = kph * 16 * W;  // this wave's first slab
+    if constexpr (!LARGE) {
+      // Two register buffers in turn, each slab requested one slab ahead.
+      // Slabs past the row re-read the wave's last slab (in bounds) and are
+      // multiplied in as zeros, so the loop body has no branch: a buffer
+      // used under a branch has its loads sunk into that branch and waited
+      // for there.
+      const int64_t last = fs < ldmt ? fs + (ldmt - 1 - fs) / FSTEP * FSTEP : fs;
+      auto clampf = [&](int64_t f) { return f <= last ? f : last; };
+      fv4 b0[T][W], b1[T][W];
+      if (fs < ldmt) {
+        load(fs, b0);
+        for (int64_t f0 = fs; f0 < ldmt; f0 += 2 * FSTEP) {
+          // scheduling fences: the machine scheduler would otherwise pull
+          // the register-only MFMAs above the prefetch and wait for it
+          load(clampf(f0 + FSTEP), b1);
+          __builtin_amdgcn_sched_barrier(0);
+          compute(b0, true);
+          __builtin_amdgcn_sched_barrier(0);
+          load(clampf(f0 + 2 * FSTEP), b0);
+          __builtin_amdgcn_sched_barrier(0);
+          compute(b1, f0 + FSTEP < ldmt);
+          __builtin_amdgcn_sched_barrier(0);
+        }
+      }
+    } else {  // the large tiles need the registers: no prefetch
+      fv4 cur[T][W];
+      for (int64_t f0 = fs; f0 < ldmt; f0 += FSTEP) {
+        load(f0, cur);
+        compute(cur, true);
       }
     }
   }
