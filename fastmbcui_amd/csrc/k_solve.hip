@@ -95,6 +95,8 @@ __device__ inline bool batch_done(const PcgState *st) { return st != nullptr && 
 // next chunk is staged while this one is consumed: one barrier per chunk.
 // Optional Bsub gives y = b - M x.
 constexpr int kRowsThreads = 512;
+constexpr int kF64x8Rows = 4;    // k_mv_rows_f64x8: rows per wave
+constexpr int kF64x8Depth = 2;   // k_mv_rows_f64x8: x chunks per group, 2 steps each
 
 // M is streamed once per pass: non-temporal loads keep it from displacing the
 // vectors (x, y, partials) that are reused from L2.
@@ -220,6 +222,142 @@ __global__ __launch_bounds__(kRowsThreads) void k_mv_rows(const float *__restric
           Y[f * K + r] = b - s;
         } else {
           Y[f * K + r] = s;
+        }
+      }
+    }
+  }
+}
+
+// FP64 batches of 8 (the certified residual y = b - M x and its kin).  In
+// k_mv_rows these take two rows per wave: each 16-byte x value read from LDS
+// serves only two rows (8 bytes of LDS per byte of M, the LDS pipe's limit)
+// and a wave has 2 KB of M in flight between barriers (3.4 TB/s at m = n =
+// 65,536).  Here a wave takes RW = 4 rows and keeps D = 2 P steps of 256
+// columns in flight: the loads of step t + D are issued as soon as step t is
+// computed, across the x-chunk barriers too (M loads do not depend on x).
+// Loads complete in order, so waiting for an x fetch also waits for every M
+// load issued before it: the x of chunk h is fetched P - 1 chunks ahead (P
+// register sets), behind M loads that are the next ones needed anyway.  The
+// op sequence is the same on every step (rows clamped to m - 1, columns to
+// n - 4, x fetches to column n - 1, zeroed where used), so the compiler's
+// wait counts are exact.  Steps past the row (the last P chunks may overhang
+// n) add zeros.  Needs n % 4 == 0.  RW = 4, P = 2: 244 VGPRs, 2.78 ms at
+// 65,536^2 (6.2 TB/s); RW = 2 with P = 2..4 stays at 4.0 TB/s.
+template <int RW, int P>
+__global__ __launch_bounds__(kRowsThreads) void k_mv_rows_f64x8(const float *__restrict__ M, int64_t m,
+                                                                int64_t n, const double *__restrict__ X,
+                                                                const float *__restrict__ Bsub,
+                                                                int64_t ldb, int64_t col0, int kval,
+                                                                double *__restrict__ Y,
+                                                                const PcgState *st) {
+  if (batch_done(st)) return;
+  constexpr int K = 8, D = 2 * P;
+  constexpr int CH = RowsCfg<double, K>::CH;
+  static_assert(CH == 512, "two 256-column steps per x chunk");
+  static_assert(P >= 2, "the x of the next chunk is fetched at least one chunk ahead");
+  constexpr int XPT = CH * K / kRowsThreads;
+  __shared__ __align__(16) double xs[2][K * CH];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int64_t row0 = (int64_t(blockIdx.x) * (kRowsThreads / 64) + wave) * RW;
+  const float *rowp[RW];
+  bool rok[RW];
+#pragma unroll
+  for (int i = 0; i < RW; ++i) {
+    rok[i] = row0 + i < m;
+    rowp[i] = M + (rok[i] ? row0 + i : m - 1) * n;
+  }
+  double acc[RW][K];
+#pragma unroll
+  for (int i = 0; i < RW; ++i)
+#pragma unroll
+    for (int r = 0; r < K; ++r) acc[i][r] = 0.0;
+
+  auto loadm = [&](int64_t c, float4 (&mv)[RW]) {  // c: this lane's first column
+    const int64_t cc = c < n ? c : n - 4;
+#pragma unroll
+    for (int i = 0; i < RW; ++i) mv[i] = load_stream4(rowp[i] + cc);
+  };
+  // x in LDS: column c of row r at r CH + (c & 2) CH / 4 + 2 (c >> 2) + (c & 1),
+  // so a lane's pairs (4q, 4q + 1) and (4q + 2, 4q + 3) sit in two 16-byte
+  // slots that are consecutive across lanes: bank-conflict-free 128-bit reads
+  // (the plain [r][c] layout puts lanes 32 bytes apart, two-way conflicts).
+  auto xslot = [](int r, int c) { return r * CH + (c & 2) * (CH / 4) + 2 * (c >> 2) + (c & 1); };
+  auto compute = [&](int64_t c, int cl, const float4 (&mv)[RW], const double *xb) {
+    const bool cok = c < n;
+#pragma unroll
+    for (int r = 0; r < K; ++r) {
+      const double2 xa = *reinterpret_cast<const double2 *>(xb + xslot(r, cl));
+      const double2 xc = *reinterpret_cast<const double2 *>(xb + xslot(r, cl + 2));
+      const double x0 = xa.x, x1 = xa.y, x2 = xc.x, x3 = xc.y;
+#pragma unroll
+      for (int i = 0; i < RW; ++i) {
+        const bool use = rok[i] && cok;
+        double s = acc[i][r];
+        s += double(use ? mv[i].x : 0.f) * x0;
+        s += double(use ? mv[i].y : 0.f) * x1;
+        s += double(use ? mv[i].z : 0.f) * x2;
+        s += double(use ? mv[i].w : 0.f) * x3;
+        acc[i][r] = s;
+      }
+    }
+  };
+  auto fetch = [&](int64_t c0, double (&xq)[XPT]) {
+#pragma unroll
+    for (int j = 0; j < XPT; ++j) {
+      const int idx = threadIdx.x + j * kRowsThreads;
+      const int64_t c = c0 + idx / K;
+      xq[j] = X[(c < n ? c : n - 1) * K + idx % K];
+    }
+  };
+  auto store = [&](int64_t c0, const double (&xq)[XPT], double *dst) {
+#pragma unroll
+    for (int j = 0; j < XPT; ++j) {
+      const int idx = threadIdx.x + j * kRowsThreads;
+      dst[xslot(idx % K, idx / K)] = (c0 + idx / K < n) ? xq[j] : 0.0;
+    }
+  };
+  float4 mv[D][RW];     // step t lives in slot t % D
+  double xq[P][XPT];    // x of chunk h lives in set h % P
+#pragma unroll
+  for (int d = 0; d < D; ++d) loadm(d * 256 + 4 * lane, mv[d]);
+#pragma unroll
+  for (int h = 0; h < P - 1; ++h) fetch(h * CH, xq[h]);
+  store(0, xq[0], xs[0]);
+  __syncthreads();
+  int buf = 0;
+  // scheduling fences keep each phase in place: the machine scheduler would
+  // otherwise hoist the FMAs above the loads and wait for them
+  for (int64_t c0 = 0; c0 < n; c0 += D * 256) {
+#pragma unroll
+    for (int h = 0; h < P; ++h) {
+      const int64_t cc = c0 + h * CH;      // this x chunk: slots 2h, 2h + 1
+      fetch(cc + (P - 1) * CH, xq[(h + P - 1) % P]);   // past the row: clamped, never read
+      __builtin_amdgcn_sched_barrier(0);
+      compute(cc + 4 * lane, 4 * lane, mv[2 * h], xs[buf]);
+      __builtin_amdgcn_sched_barrier(0);
+      loadm(cc + D * 256 + 4 * lane, mv[2 * h]);
+      __builtin_amdgcn_sched_barrier(0);
+      compute(cc + 256 + 4 * lane, 256 + 4 * lane, mv[2 * h + 1], xs[buf]);
+      __builtin_amdgcn_sched_barrier(0);
+      loadm(cc + 256 + D * 256 + 4 * lane, mv[2 * h + 1]);
+      __builtin_amdgcn_sched_barrier(0);
+      store(cc + CH, xq[(h + 1) % P], xs[buf ^ 1]);
+      __syncthreads();
+      buf ^= 1;
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < RW; ++i) {
+    const int64_t f = row0 + i;
+#pragma unroll
+    for (int r = 0; r < K; ++r) {
+      const double sum = wave_sum(acc[i][r]);
+      if (lane == 0 && f < m) {
+        if (Bsub != nullptr) {
+          const double bv = (r < kval) ? double(Bsub[f * ldb + col0 + r]) : 0.0;
+          Y[f * K + r] = bv - sum;
+        } else {
+          Y[f * K + r] = sum;
         }
       }
     }
@@ -699,6 +837,14 @@ void mv_rows(hipStream_t s, const SolveArgs<TA> &a, const TA *X, const float *Bs
   const int64_t rpb = int64_t(waves) * (small ? RWS : RW);
   const unsigned grid = unsigned((a.m + rpb - 1) / rpb);
   const bool vec = a.n % 4 == 0;
+  if constexpr (sizeof(TA) == 8 && K == 8) {
+    if (!small && vec) {
+      const int64_t rows = int64_t(waves) * kF64x8Rows;
+      k_mv_rows_f64x8<kF64x8Rows, kF64x8Depth><<<unsigned((a.m + rows - 1) / rows), kRowsThreads, 0, s>>>(
+          a.M, a.m, a.n, X, Bsub, ldb, col0, kval, Y, st);
+      return;
+    }
+  }
   if (small && vec)
     k_mv_rows<TA, K, RWS, true><<<grid, kRowsThreads, 0, s>>>(a.M, a.m, a.n, X, Bsub, ldb, col0, kval, Y, st);
   else if (small)
