@@ -577,8 +577,9 @@ class Batch:
                                                   _ptr_array(ys), int(precision)), "mfs_batch_factor_points")
         self.m = [int(v) for v in m]
 
-    def solve_points(self, Bs, maxit=1000, tol=1e-6, inner_tol=0.5, chunk=4, use_graph=True):
-        """Bs: list of (m_i, k_i) float32 device tensors, k_i <= 16.  Returns the X_i."""
+    def solve_points(self, Bs, maxit=1000, tol=1e-6, inner_tol=0.5, chunk=4, use_graph=False):
+        """Bs: list of (m_i, k_i) float32 device tensors, k_i <= 16.  Returns the X_i.
+        use_graph=True (graph replay) is switched off in the library: MfsError."""
         import torch
         Bs = [_as_2d_f32(b) for b in Bs]
         Xs = [torch.empty((self.n[i], b.shape[1]), dtype=torch.float32, device=b.device)

@@ -62,13 +62,12 @@ def tune_points_blocks(ctx: _lib.Context, k: int = 8, candidates=(1024, 2048, 40
     return _cache[key]
 
 
-def tune_batch_chunk(batch: _lib.Batch, Bs, candidates=(4, 8), graphs=(True, False), maxit=1000,
+def tune_batch_chunk(batch: _lib.Batch, Bs, candidates=(4, 8), graphs=(False,), maxit=1000,
                      tol=1e-6) -> dict:
-    """Solve the batch once per (chunk length, graph or direct launches) --
-    whole solves, since the chunk trades polls against launches over the whole
-    convergence history, and whether a captured graph beats direct launches on
-    many streams depends on how the runtime schedules the graph's branches;
-    keep the fastest."""
+    """Solve the batch once per chunk length -- whole solves, since the chunk
+    trades polls against launches over the whole convergence history; keep the
+    fastest.  `graphs` lists the launch modes tried (False: direct launches;
+    graph replay is switched off in the library)."""
     import torch
     key = ("batch_chunk", tuple(batch.n), tuple(int(b.shape[1]) for b in Bs))
     if key in _cache:

@@ -250,10 +250,10 @@ typedef struct mfs_batch_stats {
     double plan_ms, factor_ms, solve_ms;
     int32_t factor_launches;    /* merged bucket launches */
     int32_t factor_items;       /* supernodes of all scenes */
-    int32_t graph_nodes;        /* kernels + dependencies captured per replay */
-    int32_t graph_replays;
+    int32_t graph_nodes;        /* 0 (graph replay is switched off) */
+    int32_t graph_replays;      /* rounds of `chunk` iterations launched */
     int32_t sweeps;             /* refinement sweeps (max over scenes) */
-    int32_t graph_captures;     /* graphs captured (the set of running scenes shrinks) */
+    int32_t graph_captures;     /* 0 (graph replay is switched off) */
 } mfs_batch_stats;
 int mfs_batch_create(mfs_batch **out, int device, void *stream, int32_t count);
 void mfs_batch_destroy(mfs_batch *b);
@@ -269,9 +269,10 @@ int mfs_batch_plan(mfs_batch *b, const double *const *xyz, const int64_t *n, dou
 int mfs_batch_factor_points(mfs_batch *b, const double *const *x, const double *const *nx,
                             const int64_t *m, const double *const *y, int precision);
 /* mfs_solve_points for every scene (B[i]: m[i] x k[i], X[i]: n[i] x k[i], FP32
- * device, k[i] <= 16), the PCG iterations of the running scenes replayed from
- * a captured hipGraph of `chunk` iterations per scene, re-captured as scenes
- * converge (use_graph 0: the same launches issued directly). */
+ * device, k[i] <= 16), the PCG iterations of the running scenes launched in
+ * rounds of `chunk` iterations per scene, each scene on its own stream.
+ * use_graph must be 0: replaying the rounds from a captured hipGraph is
+ * switched off (MFS_ERR_ARG otherwise). */
 int mfs_batch_solve_points(mfs_batch *b, const float *const *B, const int64_t *k, float *const *X,
                            int32_t maxit, double tol, double inner_tol, int32_t chunk,
                            int use_graph);

@@ -2,8 +2,14 @@
 
 The batch runs the same per-scene arithmetic as a single context -- the same
 factor kernels (merged launches only regroup the blocks), the same PCG kernels
-(replayed from a graph or launched directly) -- so each scene's strengths must
-equal the single-scene result bit for bit."""
+(launched in rounds on the scenes' streams) -- so each scene's strengths must
+equal the single-scene result bit for bit.
+
+Replaying the rounds from a captured hipGraph is switched off in the library:
+one full GPU test run ended in a host segmentation fault inside
+mfs_batch_solve_points with graphs on (test_batch_solutions_meet_tolerance,
+graph mode), with no cause found in this code.  Its graph-mode cases are kept
+below, skipped; the library refuses use_graph = 1 (checked without a solve)."""
 import numpy as np
 import pytest
 import torch
@@ -33,7 +39,11 @@ def _run_batch(scs, use_graph, chunk=3, tol=1e-6):
     return b, Xs
 
 
-@pytest.mark.parametrize("use_graph", [True, False])
+GRAPH_OFF = pytest.mark.skip(reason="graph replay switched off: a host segmentation fault inside "
+                                     "mfs_batch_solve_points in graph mode, cause not found")
+
+
+@pytest.mark.parametrize("use_graph", [pytest.param(True, marks=GRAPH_OFF), False])
 def test_batch_equals_single_scene_solves(use_graph):
     scs = _scenes()
     print("scene sizes", [sc.n for sc in scs], flush=True)
@@ -50,13 +60,24 @@ def test_batch_equals_single_scene_solves(use_graph):
         assert ctx.stats().iters[0] == b.scene_stats(i).iters[0]
 
 
-def test_batch_solutions_meet_tolerance():
+@pytest.mark.parametrize("use_graph", [pytest.param(True, marks=GRAPH_OFF), False])
+def test_batch_solutions_meet_tolerance(use_graph):
     scs = _scenes(8)
-    b, Xs = _run_batch(scs, True, chunk=2)
+    b, Xs = _run_batch(scs, use_graph, chunk=2)
     for i, sc in enumerate(scs):
         M = scenes.kernel_matrix(sc.x, sc.nx, sc.y)
         res = oracle.normal_residual(M, sc.b.astype(np.float32), Xs[i].cpu().numpy())
         assert res <= 2e-6, f"scene {i}: {res}"
+
+
+def test_batch_refuses_graph_replay():
+    scs = _scenes(2)
+    b = _lib.Batch(2)
+    b.plan([sc.y for sc in scs], 3.0)
+    with pytest.raises(_lib.MfsError) as e:    # refused before any solve work
+        b.solve_points([_dev(sc.b.astype(np.float32)) for sc in scs], use_graph=True)
+    assert e.value.code == _lib.MFS_ERR_ARG
+    assert "graph replay is switched off" in str(e.value)
 
 
 def test_batch_rejects_unplanned_scene():
