@@ -697,8 +697,22 @@ __global__ __launch_bounds__(kThreads) void k_upd_xr(int64_t n, TA *__restrict__
 }
 
 // ------------------------------------------------------------------ sparse L / L^T
-// out[row] = sum_e val[e] * in[idx[e]] for K right-hand sides; 8 lanes per row.
-// With DOT, also dot(out, w) per right-hand side -> st->dot_rz.
+// Lanes per row of the preconditioner products.  Each lane walks the entries
+// of its row one after another (a chain of dependent index -> value loads), so
+// batches of 8 and 16 give every row several lanes per right-hand side, which
+// take every 4th (8th) entry and are summed by shuffles.  At rho = 3 a column
+// of L (a row of L^T r) has 36 entries on average and at most 86 -- 32 lanes
+// per row, L^T r at N = 65,536, K = 8: 0.14 -> 0.025 ms; a row of L (L t, the
+// product with the dot) 36 on average but up to 251, so it takes 64 lanes.
+// The product with the dot keeps the block count of 8 lanes per row (several
+// row groups per block, one after another): one block per 4 rows made its
+// per-block partials and their final gather cost more than the chains saved.
+__host__ __device__ constexpr int spmv_lanes(int K, bool dot) { return K >= 8 ? (dot ? 64 : 32) : 8; }
+__host__ __device__ constexpr int spmv_passes(int K, bool dot) { return dot ? spmv_lanes(K, dot) / 8 : 1; }
+
+// out[row] = sum_e val[e] * in[idx[e]] for K right-hand sides; spmv_lanes(K,
+// DOT) lanes per row.  With DOT, also dot(out, w) per right-hand side ->
+// st->dot_rz.
 template <typename TA, int K, bool DOT>
 __global__ __launch_bounds__(kThreads) void k_spmv(int64_t n, const int32_t *__restrict__ ptr,
                                                    const int32_t *__restrict__ idx,
@@ -707,53 +721,65 @@ __global__ __launch_bounds__(kThreads) void k_spmv(int64_t n, const int32_t *__r
                                                    const TA *__restrict__ w,
                                                    double *__restrict__ dpart, PcgState *st) {
   if (batch_done(st)) return;
-  constexpr int G = 8;                         // lanes per row
+  constexpr int G = spmv_lanes(K, DOT);        // lanes per row
+  constexpr int PASSES = spmv_passes(K, DOT);  // row groups per block, one after another
   constexpr int SPLIT = (K >= G) ? 1 : G / K;  // lanes sharing one right-hand side
   constexpr int RPL = (K > G) ? K / G : 1;     // right-hand sides per lane
   __shared__ double red[5 * K];
-  __shared__ int last;
   const int g = threadIdx.x & (G - 1);
-  const int64_t row = (int64_t(blockIdx.x) * kThreads + threadIdx.x) / G;
   const int rbase = (K >= G) ? g : (g % K);
   const int phase = (K >= G) ? 0 : (g / K);
-  TA acc[RPL];
-#pragma unroll
-  for (int u = 0; u < RPL; ++u) acc[u] = TA(0);
-  if (row < n) {
-    const int32_t b = ptr[row], e = ptr[row + 1];
-    for (int32_t t = b + phase; t < e; t += SPLIT) {
-      const TA v = val[t];
-      const int64_t col = idx[t];
-#pragma unroll
-      for (int u = 0; u < RPL; ++u) acc[u] += v * in[col * K + rbase + u * G];
-    }
-  }
-#pragma unroll
-  for (int off = K; off < G; off <<= 1)
-#pragma unroll
-    for (int u = 0; u < RPL; ++u) acc[u] += __shfl_xor(acc[u], off, 64);
   double d[K];
 #pragma unroll
   for (int k = 0; k < K; ++k) d[k] = 0.0;
-  if (row < n && phase == 0) {
+  for (int pass = 0; pass < PASSES; ++pass) {
+    const int64_t row = (int64_t(blockIdx.x) * PASSES + pass) * (kThreads / G) + threadIdx.x / G;
+    TA acc[RPL];
 #pragma unroll
-    for (int u = 0; u < RPL; ++u) {
-      const int k = rbase + u * G;
-      out[row * K + k] = acc[u];
-      if (DOT) {
+    for (int u = 0; u < RPL; ++u) acc[u] = TA(0);
+    if (row < n) {
+      const int32_t b = ptr[row], e = ptr[row + 1];
+      for (int32_t t = b + phase; t < e; t += SPLIT) {
+        const TA v = val[t];
+        const int64_t col = idx[t];
 #pragma unroll
-        for (int kk = 0; kk < K; ++kk)
-          if (kk == k) d[kk] = double(acc[u]) * double(w[row * K + k]);
+        for (int u = 0; u < RPL; ++u) acc[u] += v * in[col * K + rbase + u * G];
+      }
+    }
+#pragma unroll
+    for (int off = K; off < G; off <<= 1)
+#pragma unroll
+      for (int u = 0; u < RPL; ++u) acc[u] += __shfl_xor(acc[u], off, 64);
+    if (row < n && phase == 0) {
+#pragma unroll
+      for (int u = 0; u < RPL; ++u) {
+        const int k = rbase + u * G;
+        out[row * K + k] = acc[u];
+        if (DOT) {
+#pragma unroll
+          for (int kk = 0; kk < K; ++kk)
+            if (kk == k) d[kk] += double(acc[u]) * double(w[row * K + k]);
+        }
       }
     }
   }
   if (DOT) {
+    // plain stores: k_dot_rz (the next launch) sums the partials, so the
+    // blocks skip the device-scope fence and arrival atomic of
+    // publish_and_arrive (for 2048 blocks those took 0.11 of 0.14 ms)
     block_sum<K>(d, red);
-    if (publish_and_arrive<K>(red + 4 * K, dpart, &st->counter[C_RZ], gridDim.x, &last)) {
-      gather_partials<K>(dpart, gridDim.x, red);
-      if (threadIdx.x < K) st->dot_rz[threadIdx.x] = red[4 * K + threadIdx.x];
-    }
+    if (threadIdx.x < K) dpart[size_t(blockIdx.x) * K + threadIdx.x] = red[4 * K + threadIdx.x];
   }
+}
+
+// st->dot_rz = the sum of the nblocks partials k_spmv<.., true> left in dpart.
+template <int K>
+__global__ __launch_bounds__(kThreads) void k_dot_rz(const double *__restrict__ dpart,
+                                                     uint32_t nblocks, PcgState *st) {
+  if (batch_done(st)) return;
+  __shared__ double red[5 * K];
+  gather_partials<K>(dpart, nblocks, red);
+  if (threadIdx.x < K) st->dot_rz[threadIdx.x] = red[4 * K + threadIdx.x];
 }
 
 // ------------------------------------------------------------------ p update
@@ -878,6 +904,11 @@ void mv_cols_raw(hipStream_t s, const float *M, int64_t m, int64_t n, int32_t ro
 
 }  // namespace
 
+int64_t spmv_grid(int64_t n, int K, bool dot) {
+  const int64_t rows = int64_t(kThreads / spmv_lanes(K, dot)) * spmv_passes(K, dot);  // per block
+  return (n + rows - 1) / rows;
+}
+
 // Blocks of the dense M^T pass resident per CU (the instantiation launched
 // for large grids: 4 rows in flight per thread).
 int cols_resident_blocks(bool fp64, int K, bool vec) {
@@ -947,11 +978,12 @@ hipError_t init_k(hipStream_t s, const SolveArgs<TA> &a, const float *B, int64_t
                                                       a.dpart, a.st, tol, maxit, thresh_abs);
   }
   // z = L (L^T r); p = z; absnew = r^T z.
-  const unsigned gs = unsigned((a.n * 8 + kThreads - 1) / kThreads);
-  k_spmv<TA, K, false><<<gs, kThreads, 0, s>>>(a.n, a.sptr, a.sind, a.lval, a.r, a.t, nullptr,
-                                               nullptr, a.st);
-  k_spmv<TA, K, true><<<gs, kThreads, 0, s>>>(a.n, a.tptr, a.tcol, a.ltval, a.t, a.z, a.r, a.dpart,
+  k_spmv<TA, K, false><<<unsigned(spmv_grid(a.n, K, false)), kThreads, 0, s>>>(
+      a.n, a.sptr, a.sind, a.lval, a.r, a.t, nullptr, nullptr, a.st);
+  const unsigned gz = unsigned(spmv_grid(a.n, K, true));
+  k_spmv<TA, K, true><<<gz, kThreads, 0, s>>>(a.n, a.tptr, a.tcol, a.ltval, a.t, a.z, a.r, a.dpart,
                                               a.st);
+  k_dot_rz<K><<<1, kThreads, 0, s>>>(a.dpart, gz, a.st);
   k_upd_p<TA, K, true><<<gn, kThreads, 0, s>>>(a.n, a.Pinv, a.p, a.z, a.pv, a.st);
   return hipGetLastError();
 }
@@ -989,11 +1021,12 @@ hipError_t iter_k(hipStream_t s, const SolveArgs<TA> &a, TrackFn fn, void *tctx)
   k_mv_reduce<TA, K, RED_Q><<<gn, kThreads, 0, s>>>(a.part, splits, a.n, a.P, a.pv, a.q,
                                                      nullptr, a.dpart, a.st, nullptr, 0.0, K, 0);
   k_upd_xr<TA, K><<<gn, kThreads, 0, s>>>(a.n, a.x, a.r, a.p, a.q, a.dpart, a.st);
-  const unsigned gs = unsigned((a.n * 8 + kThreads - 1) / kThreads);
-  k_spmv<TA, K, false><<<gs, kThreads, 0, s>>>(a.n, a.sptr, a.sind, a.lval, a.r, a.t, nullptr,
-                                               nullptr, a.st);
-  k_spmv<TA, K, true><<<gs, kThreads, 0, s>>>(a.n, a.tptr, a.tcol, a.ltval, a.t, a.z, a.r, a.dpart,
+  k_spmv<TA, K, false><<<unsigned(spmv_grid(a.n, K, false)), kThreads, 0, s>>>(
+      a.n, a.sptr, a.sind, a.lval, a.r, a.t, nullptr, nullptr, a.st);
+  const unsigned gz = unsigned(spmv_grid(a.n, K, true));
+  k_spmv<TA, K, true><<<gz, kThreads, 0, s>>>(a.n, a.tptr, a.tcol, a.ltval, a.t, a.z, a.r, a.dpart,
                                               a.st);
+  k_dot_rz<K><<<1, kThreads, 0, s>>>(a.dpart, gz, a.st);
   k_upd_p<TA, K, false><<<gn, kThreads, 0, s>>>(a.n, a.Pinv, a.p, a.z, a.pv, a.st);
   return hipGetLastError();
 }

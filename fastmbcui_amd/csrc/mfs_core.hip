@@ -554,7 +554,7 @@ int workspace(mfs_ctx *ctx, const float *M, const PointsOp *pts, int64_t m, int6
   if (pts) part_elems = std::max(part_elems, size_t(points_splits(m, n, pts->target_blocks)) * mv);
   HIPCHK(ctx->dpart.ensure(part_elems * sizeof(double)), "alloc partials");
   a.part = ctx->dpart.as<double>();
-  const size_t maxblocks = size_t((n * 8 + kThreads - 1) / kThreads) + size_t((m + kThreads - 1) / kThreads) + 16;
+  const size_t maxblocks = size_t(spmv_grid(n, K, true)) + size_t((m + kThreads - 1) / kThreads) + 16;
   HIPCHK(ctx->ddpart.ensure(maxblocks * K * sizeof(double)), "alloc dot partials");
   a.dpart = ctx->ddpart.as<double>();
   if (ctx->dstate.p == nullptr) {

@@ -161,6 +161,10 @@ hipError_t normal_apply32(hipStream_t st, const SolveArgs<float> &a, int kval, d
 // Blocks of the dense M^T pass (k_mv_cols, TA = double when fp64) resident
 // on one CU; vec: n % 4 == 0.
 int cols_resident_blocks(bool fp64, int K, bool vec);
+// Blocks of the preconditioner products (k_spmv) for n rows and batch width
+// K, with or without the dot: the dot's partials (SolveArgs::dpart) take one
+// slot of K per block.
+int64_t spmv_grid(int64_t n, int K, bool dot);
 
 // thresh_abs (device, K values, may be null) replaces tol^2 ||r0||^2 as the
 // stopping threshold of each column (used by the refinement's inner solves).
