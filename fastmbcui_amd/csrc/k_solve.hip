@@ -702,11 +702,12 @@ __global__ __launch_bounds__(kThreads) void k_upd_xr(int64_t n, TA *__restrict__
 // batches of 8 and 16 give every row several lanes per right-hand side, which
 // take every 4th (8th) entry and are summed by shuffles.  At rho = 3 a column
 // of L (a row of L^T r) has 36 entries on average and at most 86 -- 32 lanes
-// per row, L^T r at N = 65,536, K = 8: 0.14 -> 0.025 ms; a row of L (L t, the
-// product with the dot) 36 on average but up to 251, so it takes 64 lanes.
-// The product with the dot keeps the block count of 8 lanes per row (several
-// row groups per block, one after another): one block per 4 rows made its
-// per-block partials and their final gather cost more than the chains saved.
+// per row; a row of L (L t, the product with the dot) 36 on average but up to
+// 251, so it takes 64 lanes.  The product with the dot keeps the block count
+// of 8 lanes per row (several row groups per block, one after another) to
+// bound its per-block partials.  At N = 65,536, K = 8 (average over the
+// launches): L^T r 0.035 -> 0.025 ms, L t with the dot 0.114 -> 0.075 ms
+// (with the partials summed by k_dot_rz).
 __host__ __device__ constexpr int spmv_lanes(int K, bool dot) { return K >= 8 ? (dot ? 64 : 32) : 8; }
 __host__ __device__ constexpr int spmv_passes(int K, bool dot) { return dot ? spmv_lanes(K, dot) / 8 : 1; }
 
@@ -766,7 +767,7 @@ __global__ __launch_bounds__(kThreads) void k_spmv(int64_t n, const int32_t *__r
   if (DOT) {
     // plain stores: k_dot_rz (the next launch) sums the partials, so the
     // blocks skip the device-scope fence and arrival atomic of
-    // publish_and_arrive (for 2048 blocks those took 0.11 of 0.14 ms)
+    // publish_and_arrive (2048 of them per launch at N = 65,536)
     block_sum<K>(d, red);
     if (threadIdx.x < K) dpart[size_t(blockIdx.x) * K + threadIdx.x] = red[4 * K + threadIdx.x];
   }
