@@ -520,7 +520,8 @@ def _ptr_array(items):
 
 class Batch:
     """Independent scenes solved together (mfs_batch): parallel host plans, one
-    merged factor launch per tile size, graph-replayed PCG iterations."""
+    merged factor launch per tile size, PCG iterations launched in rounds on
+    the scenes' streams."""
 
     def __init__(self, count, device=0):
         import torch

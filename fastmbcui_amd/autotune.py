@@ -7,10 +7,8 @@ hand rather than fixed:
 * MFS_PARAM_POINTS_BLOCKS -- blocks per matrix-free pass (owners x splits of
   the other side): more splits fill the CUs and balance XCDs, fewer keep the
   partial sums small;
-* the batch's PCG chunk -- iterations per replay: longer chunks cost fewer
-  polls, shorter ones launch fewer no-op kernels after a scene converged --
-  and whether the chunk is replayed from a captured graph or launched
-  directly on the scenes' streams.
+* the batch's PCG chunk -- iterations per round: longer chunks cost fewer
+  polls, shorter ones launch fewer no-op kernels after a scene converged.
 
 Each candidate is timed with HIP events on the work it controls; the fastest
 is set and returned.  Results are kept per shape key for the process.
