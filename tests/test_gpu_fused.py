@@ -1,5 +1,7 @@
 """The one-pass normal matvec (k_fused.hip) through the C-ABI.
 
+* Q = M^T (M X) on small-integer data, where every FP32 partial sum is exact:
+  bit-equal to the integer product through both passes and the one-pass kernel;
 * Q = M^T (M X) from mfs_dense_normal_apply against the same product in FP64
   (numpy / torch on FP64 copies of the FP32 inputs, whose products are
   exact): the one-pass kernel and the two-pass matvec both to FP32-level
@@ -98,6 +100,30 @@ def _check_apply(Md, X, want_fused=True, tol=2e-6):
         print(f"normal apply m={Md.shape[0]} n={Md.shape[1]} k={X.shape[1]} mode {mode}: "
               f"one-pass launches {launches}, max rel. error {err:.2e}", flush=True)
         assert err < tol, (mode, err)
+
+
+@pytest.mark.parametrize("m,bodies,per_body,k", [(16411, 4, 501, 8), (16411, 3, 667, 5)])
+def test_normal_apply_exact_on_small_integers(m, bodies, per_body, k):
+    """M, X with entries in -2..2: every FP32 product and partial sum of both
+    passes (rows of M X: |.| <= 4 n; 32-row chunks of M^T y: <= 256 n) is an
+    integer below 2^24, and the chunk sums are carried in FP64, so Q must equal
+    the integer product exactly -- the production passes at m >= 16,384 (n = 2004
+    and n = 2001, not a multiple of 4) and the one-pass kernel where it applies."""
+    sc = scenes.coins(seed=1, n_bodies=bodies, pts_per_body=per_body, k=1, box=8.0)
+    n = sc.n
+    rng = np.random.default_rng(m + n + k)
+    M = rng.integers(-2, 3, (m, n)).astype(np.float32)
+    X = rng.integers(-2, 3, (n, k)).astype(np.float32)
+    ref = M.astype(np.int64).T @ (M.astype(np.int64) @ X.astype(np.int64))
+    ctx = _lib.Context(0)
+    Md = torch.from_numpy(M).to(DEV)
+    Xd = torch.from_numpy(X).to(DEV)
+    for mode in (FORCE, OFF):
+        Q, launches = _apply(ctx, Md, Xd, mode)
+        got = Q.cpu().numpy()
+        print(f"exact normal apply m={m} n={n} k={k} mode {mode}: one-pass launches {launches}, "
+              f"max abs diff {np.abs(got - ref).max():.3g}", flush=True)
+        assert np.array_equal(got, ref.astype(np.float64)), (mode, np.abs(got - ref).max())
 
 
 def test_normal_apply_matches_fp64_ragged_rows():
