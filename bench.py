@@ -666,7 +666,7 @@ def main():
     achieved = (tot_bytes / max(launches, 1)) / (avg_ms * 1e-3) / 1e9 if launches else 0.0
     g_l, g_ms, g_bytes = tracked[_lib.MFS_K_GRAM]
     fused_launches = tracked[_lib.MFS_K_MATVEC_FUSED][0]
-    kernel_ms = {names.get(c, "supernode factor (Gram+Cholesky+L)"): round(tracked[c][1] / args.steps, 3)
+    kernel_ms = {names.get(c, "supernode factor (M^T ranges + Gram + Cholesky + L, overlapped)"): round(tracked[c][1] / args.steps, 3)
                  for c in classes}
 
     traffic = None

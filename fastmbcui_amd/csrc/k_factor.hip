@@ -89,15 +89,16 @@ __device__ inline void gram_step(const TC (&fr)[T], acc_t (&acc)[PPW]) {
 }
 
 // ------------------------------------------------------------------ transpose
-// Mt[c][f] = M[f][c] for f < m, 0 for m <= f < ldmt.  64x64 tiles through LDS,
-// float4 in and out when the row length allows it.
+// Mt[c][f] = M[f][c] for f < m, 0 for m <= f < ldmt, for the columns c from cb
+// (a multiple of 64) on.  64x64 tiles through LDS, float4 in and out when the
+// row length allows it.
 template <bool VEC>
 __global__ __launch_bounds__(kThreads) void k_transpose(const float *__restrict__ M, int64_t m,
                                                         int64_t n, float *__restrict__ Mt,
-                                                        int64_t ldmt) {
+                                                        int64_t ldmt, int64_t cb) {
   __shared__ float tile[64][65];
   const int t = threadIdx.x;
-  const int64_t c0 = int64_t(blockIdx.x) * 64, f0 = int64_t(blockIdx.y) * 64;
+  const int64_t c0 = cb + int64_t(blockIdx.x) * 64, f0 = int64_t(blockIdx.y) * 64;
   const int q = t & 15, r16 = t >> 4;  // 16 float4 per 64-wide row, 16 rows per pass
   for (int r = r16; r < 64; r += 16) {
     const int64_t f = f0 + r, c = c0 + 4 * q;
@@ -524,12 +525,13 @@ hipError_t launch_prec(hipStream_t st, int tiles, const FactorLaunch &L) {
 }  // namespace
 
 hipError_t launch_transpose(hipStream_t st, const float *M, int64_t m, int64_t n, float *Mt,
-                            int64_t ldmt) {
-  dim3 grid(unsigned((n + 63) / 64), unsigned(ldmt / 64));
+                            int64_t ldmt, int64_t c_begin, int64_t c_end) {
+  if (c_end <= c_begin) return hipSuccess;
+  dim3 grid(unsigned((c_end - c_begin + 63) / 64), unsigned(ldmt / 64));
   if (n % 4 == 0)
-    k_transpose<true><<<grid, kThreads, 0, st>>>(M, m, n, Mt, ldmt);
+    k_transpose<true><<<grid, kThreads, 0, st>>>(M, m, n, Mt, ldmt, c_begin);
   else
-    k_transpose<false><<<grid, kThreads, 0, st>>>(M, m, n, Mt, ldmt);
+    k_transpose<false><<<grid, kThreads, 0, st>>>(M, m, n, Mt, ldmt, c_begin);
   return hipGetLastError();
 }
 

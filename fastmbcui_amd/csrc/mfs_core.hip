@@ -53,9 +53,22 @@ struct DevBuf {
   }
 };
 
+// The dense factor transposes M in kFactorPhases column ranges [phase_col(p),
+// phase_col(p + 1)) and starts the supernodes whose columns (original order)
+// all lie below phase_col(p + 1) once range p is done, so the transpose of
+// the later ranges overlaps the Gram of the earlier ones.  On surface scenes
+// a supernode's columns nearly always lie in one body, whose sources are
+// contiguous, so the Gram work splits about evenly over the phases.
+constexpr int kFactorPhases = 4;
+int64_t phase_col(int p, int64_t n) {
+  if (p >= kFactorPhases) return n;
+  return std::min<int64_t>(n, (int64_t(p) * n / kFactorPhases + 63) / 64 * 64);
+}
+
 struct Bucket {
   int tiles;
   int32_t offset, count;
+  int32_t phase_off[kFactorPhases + 1];  // phase p: items offset + [phase_off[p], phase_off[p + 1])
 };
 
 }  // namespace
@@ -113,7 +126,7 @@ struct mfs_ctx {
   // side, forked from and joined back into `stream` with events.
   static constexpr int kSide = 4;
   hipStream_t side[kSide] = {};
-  hipEvent_t fork_ev = nullptr, join_ev[kSide] = {};
+  hipEvent_t fork_ev = nullptr, join_ev[kSide] = {}, tr_ev[kFactorPhases] = {};
   int track_mask = 0;
   struct Track {
     std::vector<std::pair<hipEvent_t, hipEvent_t>> pairs;
@@ -382,24 +395,41 @@ int plan_impl(mfs_ctx *ctx, std::vector<double> &&xyz, int64_t n64, double rho, 
       }
       key[s] = k;
     }
+    // column phase of each supernode (see phase_col)
+    std::vector<int8_t> phase(n_super);
+    for (int32_t s = 0; s < n_super; ++s) {
+      const SuperDesc &d = ctx->sdesc[s];
+      int32_t mc = 0;
+      for (int32_t p = 0; p < d.ns; ++p) mc = std::max(mc, ctx->Pinv[ctx->sind[d.rows_off + p]]);
+      int ph = 0;
+      while (ph + 1 < kFactorPhases && mc >= phase_col(ph + 1, n)) ++ph;
+      phase[s] = int8_t(ph);
+    }
     ctx->list.clear();
     ctx->buckets.clear();
     constexpr int kXcd = 8;
     for (int t : kTileSizes) {
       std::vector<int32_t> &v = by_t[t];
       if (v.empty()) continue;
-      std::stable_sort(v.begin(), v.end(), [&](int32_t a, int32_t b) { return key[a] < key[b]; });
-      const int32_t cnt = int32_t(v.size());
-      const int32_t per = (cnt + kXcd - 1) / kXcd;
-      Bucket b{t, int32_t(ctx->list.size()), cnt};
-      std::vector<int32_t> dealt;
-      dealt.reserve(cnt);
-      for (int32_t i = 0; i < per; ++i)
-        for (int x = 0; x < kXcd; ++x) {
-          const int32_t src = x * per + i;
-          if (src < cnt) dealt.push_back(v[src]);
-        }
-      ctx->list.insert(ctx->list.end(), dealt.begin(), dealt.end());
+      std::stable_sort(v.begin(), v.end(), [&](int32_t a, int32_t b) {
+        return phase[a] != phase[b] ? phase[a] < phase[b] : key[a] < key[b];
+      });
+      Bucket b{t, int32_t(ctx->list.size()), int32_t(v.size()), {}};
+      int32_t lo = 0;
+      for (int ph = 0; ph < kFactorPhases; ++ph) {
+        int32_t hi = lo;
+        while (hi < int32_t(v.size()) && phase[v[hi]] == ph) ++hi;
+        b.phase_off[ph] = lo;
+        const int32_t cnt = hi - lo;
+        const int32_t per = (cnt + kXcd - 1) / kXcd;
+        for (int32_t i = 0; i < per; ++i)
+          for (int x = 0; x < kXcd; ++x) {
+            const int32_t src = x * per + i;
+            if (src < cnt) ctx->list.push_back(v[lo + src]);
+          }
+        lo = hi;
+      }
+      b.phase_off[kFactorPhases] = lo;
       ctx->buckets.push_back(b);
     }
   }
@@ -652,6 +682,8 @@ int ensure_side_streams(mfs_ctx *ctx) {
     HIPCHK(hipStreamCreateWithFlags(&ctx->side[i], hipStreamNonBlocking), "create stream");
     HIPCHK(hipEventCreateWithFlags(&ctx->join_ev[i], hipEventDisableTiming), "create event");
   }
+  for (int i = 0; i < kFactorPhases; ++i)
+    HIPCHK(hipEventCreateWithFlags(&ctx->tr_ev[i], hipEventDisableTiming), "create event");
   HIPCHK(hipEventCreateWithFlags(&ctx->fork_ev, hipEventDisableTiming), "create event");
   return MFS_OK;
 }
@@ -676,8 +708,9 @@ void set_pass_work(mfs_ctx *ctx, const PointsOp *pts, int64_t m, int64_t n, int 
   }
 }
 
-// Factor phase 1: M^T (dense form), value buffers, cleared statuses, and the
-// scene record the factor kernels read (ctx->fscene, ctx->dfscene).
+// Factor phase 1: the M^T buffer (dense form), value buffers, cleared
+// statuses, and the scene record the factor kernels read (ctx->fscene,
+// ctx->dfscene).
 int factor_prepare(mfs_ctx *ctx, const float *M, const PointsOp *pts, int64_t m, int64_t n,
                    int precision) {
   if (precision != MFS_FP64 && precision != MFS_FP32)
@@ -686,10 +719,8 @@ int factor_prepare(mfs_ctx *ctx, const float *M, const PointsOp *pts, int64_t m,
   ctx->has_factor = false;
   ctx->alt_prec = -1;
   const int64_t ldmt = (m + kGramChunk - 1) / kGramChunk * kGramChunk;
-  if (pts == nullptr) {
+  if (pts == nullptr)  // filled by factor_impl, range by range
     HIPCHK(ctx->dMt.ensure(size_t(n) * size_t(ldmt) * sizeof(float)), "alloc M^T");
-    HIPCHK(launch_transpose(ctx->stream, M, m, n, ctx->dMt.as<float>(), ldmt), "transpose");
-  }
   const int64_t nnz = ctx->sptr[ctx->n];
   const size_t w = (precision == MFS_FP64) ? 8 : 4;
   HIPCHK(ctx->dval.ensure(size_t(std::max<int64_t>(nnz, 1)) * w), "alloc factor");
@@ -798,16 +829,44 @@ int factor_impl(mfs_ctx *ctx, const float *M, const PointsOp *pts, int64_t m, in
     scratch_bytes += size_t(ctx->buckets[bi].count) * size_t(factor_scratch_elems(ctx->buckets[bi].tiles)) * 8;
   }
   HIPCHK(ctx->dthg.ensure(std::max<size_t>(scratch_bytes, 8)), "alloc factor scratch");
-  HIPCHK(hipEventRecord(ctx->fork_ev, ctx->stream), "fork");
+  const int prec_code = precision == MFS_FP64 ? 0 : 1;
   const int nside = std::min<int>(mfs_ctx::kSide, int(ctx->buckets.size()));
-  for (int i = 0; i < nside; ++i) HIPCHK(hipStreamWaitEvent(ctx->side[i], ctx->fork_ev, 0), "fork");
-  for (size_t bi = 0; bi < ctx->buckets.size(); ++bi) {
-    const Bucket &b = ctx->buckets[bi];
-    HIPCHK(launch_super_factor(ctx->side[bi % mfs_ctx::kSide], b.tiles, precision == MFS_FP64 ? 0 : 1,
-                               pts != nullptr, ctx->dfscene.as<FactorScene>(),
-                               ctx->dlist.as<FactorItem>() + b.offset, b.count,
-                               ctx->dthg.as<char>() + scratch_off[bi]),
-           "supernode factor");
+  if (pts == nullptr) {
+    // M^T range by range on the context stream; a bucket's side stream runs
+    // its phase-p supernodes once range p is done, phases in turn, while the
+    // later ranges are transposed.
+    const int64_t ldmt = ctx->fscene.ldmt;
+    for (int ph = 0; ph < kFactorPhases; ++ph) {
+      HIPCHK(launch_transpose(ctx->stream, M, m, n, ctx->dMt.as<float>(), ldmt, phase_col(ph, n),
+                              phase_col(ph + 1, n)),
+             "transpose");
+      HIPCHK(hipEventRecord(ctx->tr_ev[ph], ctx->stream), "fork");
+    }
+    for (int ph = 0; ph < kFactorPhases; ++ph) {
+      for (int i = 0; i < nside; ++i) HIPCHK(hipStreamWaitEvent(ctx->side[i], ctx->tr_ev[ph], 0), "fork");
+      for (size_t bi = 0; bi < ctx->buckets.size(); ++bi) {
+        const Bucket &b = ctx->buckets[bi];
+        const int32_t lo = b.phase_off[ph], cnt = b.phase_off[ph + 1] - lo;
+        if (cnt <= 0) continue;
+        HIPCHK(launch_super_factor(ctx->side[bi % mfs_ctx::kSide], b.tiles, prec_code, false,
+                                   ctx->dfscene.as<FactorScene>(),
+                                   ctx->dlist.as<FactorItem>() + b.offset + lo, cnt,
+                                   ctx->dthg.as<char>() + scratch_off[bi] +
+                                       size_t(lo) * size_t(factor_scratch_elems(b.tiles)) * 8),
+               "supernode factor");
+      }
+    }
+  } else {
+    HIPCHK(hipEventRecord(ctx->fork_ev, ctx->stream), "fork");
+    for (int i = 0; i < nside; ++i) HIPCHK(hipStreamWaitEvent(ctx->side[i], ctx->fork_ev, 0), "fork");
+    for (size_t bi = 0; bi < ctx->buckets.size(); ++bi) {
+      const Bucket &b = ctx->buckets[bi];
+      HIPCHK(launch_super_factor(ctx->side[bi % mfs_ctx::kSide], b.tiles, prec_code, true,
+                                 ctx->dfscene.as<FactorScene>(),
+                                 ctx->dlist.as<FactorItem>() + b.offset, b.count,
+                                 ctx->dthg.as<char>() + scratch_off[bi]),
+             "supernode factor");
+    }
   }
   for (int i = 0; i < nside; ++i) {
     HIPCHK(hipEventRecord(ctx->join_ev[i], ctx->side[i]), "join");
@@ -1084,6 +1143,8 @@ void mfs_ctx_destroy(mfs_ctx *ctx) {
     if (ctx->side[i]) (void)hipStreamDestroy(ctx->side[i]);
     if (ctx->join_ev[i]) (void)hipEventDestroy(ctx->join_ev[i]);
   }
+  for (int i = 0; i < kFactorPhases; ++i)
+    if (ctx->tr_ev[i]) (void)hipEventDestroy(ctx->tr_ev[i]);
   if (ctx->fork_ev) (void)hipEventDestroy(ctx->fork_ev);
   for (auto &t : ctx->tr)
     for (auto &pr : t.pairs) {

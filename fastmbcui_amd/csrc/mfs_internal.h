@@ -78,8 +78,9 @@ hipError_t launch_scan(hipStream_t st, const int32_t *cnt, int32_t n, int32_t *p
                        int64_t *total);
 
 // --------------------------------------------------------------- factor kernels
+// Columns c_begin .. c_end of M (c_begin a multiple of 64) into rows of M^T.
 hipError_t launch_transpose(hipStream_t st, const float *M, int64_t m, int64_t n, float *Mt,
-                            int64_t ldmt);
+                            int64_t ldmt, int64_t c_begin, int64_t c_end);
 // precision: 0 = FP64 compute / FP64 values, 1 = FP32 compute / FP32 values,
 // 2 = FP64 compute / FP32 values (retry of FP32 failures).
 // One scene's inputs and outputs of the factor (device pointers).  Dense form:
