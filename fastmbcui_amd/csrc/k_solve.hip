@@ -97,6 +97,8 @@ __device__ inline bool batch_done(const PcgState *st) { return st != nullptr && 
 // next chunk is staged while this one is consumed: one barrier per chunk.
 // Optional Bsub gives y = b - M x.
 constexpr int kRowsThreads = 512;
+constexpr int kF32x8Rows = 4;    // k_mv_rows_f32x8: rows per wave
+constexpr int kF32x8Depth = 2;   // k_mv_rows_f32x8: x chunks per group, 2 steps each
 constexpr int kF64x8Rows = 4;    // k_mv_rows_f64x8: rows per wave
 constexpr int kF64x8Depth = 2;   // k_mv_rows_f64x8: x chunks per group, 2 steps each
 
@@ -357,6 +359,128 @@ __global__ __launch_bounds__(kRowsThreads) void k_mv_rows_f64x8(const float *__r
       if (lane == 0 && f < m) {
         if (Bsub != nullptr) {
           const double bv = (r < kval) ? double(Bsub[f * ldb + col0 + r]) : 0.0;
+          Y[f * K + r] = bv - sum;
+        } else {
+          Y[f * K + r] = sum;
+        }
+      }
+    }
+  }
+}
+
+// FP32 batches of 8 (the PCG's y = M p), n % 4 == 0: k_mv_rows' sums (each
+// lane's columns in the same order, then the wave sum) with the same
+// pipelining as k_mv_rows_f64x8 -- 4 rows per wave, D = 2 P steps of 256
+// columns in flight across the x-chunk barriers, x fetched P - 1 chunks
+// ahead.  FP32 x values sit in LDS as [r][c]: a lane's four columns are one
+// 16-byte read and lanes are 16 bytes apart, so no swizzle is needed.  At
+// m = n = 65,536: 2.44 ms (7.05 TB/s) against 2.55 ms for k_mv_rows with 8
+// rows per wave (which holds 2 KB of M per wave between barriers).
+template <int RW, int P>
+__global__ __launch_bounds__(kRowsThreads) void k_mv_rows_f32x8(const float *__restrict__ M, int64_t m,
+                                                                int64_t n, const float *__restrict__ X,
+                                                                const float *__restrict__ Bsub,
+                                                                int64_t ldb, int64_t col0, int kval,
+                                                                float *__restrict__ Y,
+                                                                const PcgState *st) {
+  if (batch_done(st)) return;
+  constexpr int K = 8, CH = 512, SPC = CH / 256, D = P * SPC;
+  static_assert(P >= 2, "the x of the next chunk is fetched at least one chunk ahead");
+  constexpr int XPT = CH * K / kRowsThreads;
+  __shared__ __align__(16) float xs[2][K * CH];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int64_t row0 = (int64_t(blockIdx.x) * (kRowsThreads / 64) + wave) * RW;
+  const float *rowp[RW];
+  bool rok[RW];
+#pragma unroll
+  for (int i = 0; i < RW; ++i) {
+    rok[i] = row0 + i < m;
+    rowp[i] = M + (rok[i] ? row0 + i : m - 1) * n;
+  }
+  float acc[RW][K];
+#pragma unroll
+  for (int i = 0; i < RW; ++i)
+#pragma unroll
+    for (int r = 0; r < K; ++r) acc[i][r] = 0.f;
+  auto loadm = [&](int64_t c, float4 (&mv)[RW]) {  // c: this lane's first column
+    const int64_t cc = c < n ? c : n - 4;
+#pragma unroll
+    for (int i = 0; i < RW; ++i) mv[i] = load_stream4(rowp[i] + cc);
+  };
+  auto compute = [&](int64_t c, int cl, const float4 (&mv)[RW], const float *xb) {
+    const bool cok = c < n;
+#pragma unroll
+    for (int r = 0; r < K; ++r) {
+      const float4 xv = *reinterpret_cast<const float4 *>(xb + r * CH + cl);
+#pragma unroll
+      for (int i = 0; i < RW; ++i) {
+        const bool use = rok[i] && cok;
+        float sum = acc[i][r];
+        sum += (use ? mv[i].x : 0.f) * xv.x;
+        sum += (use ? mv[i].y : 0.f) * xv.y;
+        sum += (use ? mv[i].z : 0.f) * xv.z;
+        sum += (use ? mv[i].w : 0.f) * xv.w;
+        acc[i][r] = sum;
+      }
+    }
+  };
+  auto fetch = [&](int64_t c0, float (&xq)[XPT]) {
+#pragma unroll
+    for (int j = 0; j < XPT; ++j) {
+      const int idx = threadIdx.x + j * kRowsThreads;
+      const int64_t c = c0 + idx / K;
+      xq[j] = X[(c < n ? c : n - 1) * K + idx % K];
+    }
+  };
+  auto store = [&](int64_t c0, const float (&xq)[XPT], float *dst) {
+#pragma unroll
+    for (int j = 0; j < XPT; ++j) {
+      const int idx = threadIdx.x + j * kRowsThreads;
+      dst[(idx % K) * CH + idx / K] = (c0 + idx / K < n) ? xq[j] : 0.f;
+    }
+  };
+  float4 mv[D][RW];   // step t lives in slot t % D
+  float xq[P][XPT];   // x of chunk h lives in set h % P
+#pragma unroll
+  for (int d = 0; d < D; ++d) loadm(d * 256 + 4 * lane, mv[d]);
+#pragma unroll
+  for (int h = 0; h < P - 1; ++h) fetch(h * CH, xq[h]);
+  store(0, xq[0], xs[0]);
+  __syncthreads();
+  int buf = 0;
+  // scheduling fences keep each phase in place; the empty asm pins each
+  // step's FMAs before the next loads
+  for (int64_t c0 = 0; c0 < n; c0 += D * 256) {
+#pragma unroll
+    for (int h = 0; h < P; ++h) {
+      const int64_t cc = c0 + h * CH;
+      fetch(cc + (P - 1) * CH, xq[(h + P - 1) % P]);   // past the row: clamped, never read
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int t = 0; t < SPC; ++t) {
+        compute(cc + t * 256 + 4 * lane, t * 256 + 4 * lane, mv[h * SPC + t], xs[buf]);
+#pragma unroll
+        for (int i = 0; i < RW; ++i)
+#pragma unroll
+          for (int r = 0; r < K; ++r) asm volatile("" : "+v"(acc[i][r]));
+        __builtin_amdgcn_sched_barrier(0);
+        loadm(cc + t * 256 + D * 256 + 4 * lane, mv[h * SPC + t]);
+        __builtin_amdgcn_sched_barrier(0);
+      }
+      store(cc + CH, xq[(h + 1) % P], xs[buf ^ 1]);
+      __syncthreads();
+      buf ^= 1;
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < RW; ++i) {
+    const int64_t f = row0 + i;
+#pragma unroll
+    for (int r = 0; r < K; ++r) {
+      const float sum = wave_sum(acc[i][r]);
+      if (lane == 0 && f < m) {
+        if (Bsub != nullptr) {
+          const float bv = (r < kval) ? Bsub[f * ldb + col0 + r] : 0.f;
           Y[f * K + r] = bv - sum;
         } else {
           Y[f * K + r] = sum;
@@ -977,6 +1101,14 @@ void mv_rows(hipStream_t s, const SolveArgs<TA> &a, const TA *X, const float *Bs
   const int64_t rpb = int64_t(waves) * (small ? RWS : RW);
   const unsigned grid = unsigned((a.m + rpb - 1) / rpb);
   const bool vec = a.n % 4 == 0;
+  if constexpr (sizeof(TA) == 4 && K == 8) {
+    const int64_t rows = int64_t(waves) * kF32x8Rows;
+    if (vec && (a.m + rows - 1) / rows >= 256) {
+      k_mv_rows_f32x8<kF32x8Rows, kF32x8Depth><<<unsigned((a.m + rows - 1) / rows), kRowsThreads, 0, s>>>(
+          a.M, a.m, a.n, X, Bsub, ldb, col0, kval, Y, st);
+      return;
+    }
+  }
   if constexpr (sizeof(TA) == 8 && K == 8) {
     if (!small && vec) {
       const int64_t rows = int64_t(waves) * kF64x8Rows;
