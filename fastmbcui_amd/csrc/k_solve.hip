@@ -604,6 +604,126 @@ __global__ __launch_bounds__(kThreads) void k_mv_cols(const float *__restrict__ 
   }
 }
 
+// FP32 batches of 8 (the PCG's M^T y), n % 4 == 0: k_mv_cols' sums (FP32
+// over 32-row chunks from the block's first row, carried in FP64) with M kept
+// in flight as in k_mv_cols_f64x8 -- D steps of RU rows per thread, y in
+// 64-row chunks fetched one chunk ahead.  Launched one block per CU (the
+// row split sized for a single resident block): at m = n = 65,536, 64 x 4
+// blocks take 2.43 ms (7.06 TB/s); k_mv_cols at its best split 2.55 ms,
+// this kernel with 2 resident blocks per CU 2.57-2.61 ms.
+template <int RU, int D>
+__global__ __launch_bounds__(kThreads) void k_mv_cols_f32x8(const float *__restrict__ M, int64_t m,
+                                                            int64_t n, const float *__restrict__ Y,
+                                                            int64_t rows_per_block,
+                                                            double *__restrict__ part,
+                                                            const PcgState *st) {
+  if (batch_done(st)) return;
+  constexpr int K = 8, CHR = 64, SPC = CHR / RU;
+  static_assert(D >= 1 && D < SPC && SPC % D == 0, "ring slots repeat within a chunk");
+  static_assert(kRowChunk % RU == 0 && CHR % kRowChunk == 0, "FP32 chunks within y chunks");
+  constexpr int YPT = CHR * K / kThreads;  // y values fetched per thread
+  __shared__ float ys[2][CHR * K];
+  const int64_t i0 = (int64_t(blockIdx.x) * kThreads + threadIdx.x) * 4;
+  const bool col_ok = i0 < n;
+  const int64_t ic = col_ok ? i0 : n - 4;
+  const int64_t fb = int64_t(blockIdx.y) * rows_per_block;
+  const int64_t fe = (fb + rows_per_block < m) ? fb + rows_per_block : m;
+  double acc[4][K];
+  float c[4][K];
+#pragma unroll
+  for (int j = 0; j < 4; ++j)
+#pragma unroll
+    for (int r = 0; r < K; ++r) {
+      acc[j][r] = 0.0;
+      c[j][r] = 0.f;
+    }
+  if (fb >= fe) {  // an empty row block still writes its (zero) partials
+    if (col_ok)
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+#pragma unroll
+        for (int r = 0; r < K; ++r) part[(int64_t(blockIdx.y) * n + i0 + j) * K + r] = 0.0;
+    return;
+  }
+  auto loadm = [&](int64_t f, float4 (&v)[RU]) {  // rows f .. f + RU - 1
+#pragma unroll
+    for (int u = 0; u < RU; ++u) {
+      const int64_t fr = f + u < fe ? f + u : fe - 1;
+      v[u] = load_stream4(M + fr * n + ic);
+    }
+  };
+  float yq[YPT];
+  auto fetch = [&](int64_t fc) {
+#pragma unroll
+    for (int j = 0; j < YPT; ++j) {
+      const int idx = threadIdx.x + j * kThreads;
+      const int64_t f = fc + idx / K;
+      yq[j] = Y[(f < fe ? f : fe - 1) * K + idx % K];
+    }
+  };
+  auto store = [&](int64_t fc, float *dst) {
+#pragma unroll
+    for (int j = 0; j < YPT; ++j) {
+      const int idx = threadIdx.x + j * kThreads;
+      dst[idx] = (fc + idx / K < fe) ? yq[j] : 0.f;
+    }
+  };
+  float4 ring[D][RU];  // step t (rows fb + RU t ..) lives in slot t % D
+#pragma unroll
+  for (int d = 0; d < D; ++d) loadm(fb + d * RU, ring[d]);
+  fetch(fb);
+  store(fb, ys[0]);
+  __syncthreads();
+  int buf = 0;
+  for (int64_t fc = fb; fc < fe; fc += CHR) {
+    fetch(fc + CHR);  // past the range: clamped, stored as zeros
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int t = 0; t < SPC; ++t) {
+      const float *yb = ys[buf] + t * RU * K;
+#pragma unroll
+      for (int u = 0; u < RU; ++u)
+#pragma unroll
+        for (int r = 0; r < K; ++r) {
+          const float yv = yb[u * K + r];
+          c[0][r] += ring[t % D][u].x * yv;
+          c[1][r] += ring[t % D][u].y * yv;
+          c[2][r] += ring[t % D][u].z * yv;
+          c[3][r] += ring[t % D][u].w * yv;
+        }
+      if ((t + 1) % (kRowChunk / RU) == 0) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+          for (int r = 0; r < K; ++r) {
+            acc[j][r] += double(c[j][r]);
+            c[j][r] = 0.f;
+          }
+      }
+      // pin the step's FMAs here: without it the instruction selector
+      // gathers every step's loads first and the FMAs after them
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+#pragma unroll
+        for (int r = 0; r < K; ++r) asm volatile("" : "+v"(c[j][r]));
+      __builtin_amdgcn_sched_barrier(0);
+      loadm(fc + (t + D) * RU, ring[t % D]);
+      __builtin_amdgcn_sched_barrier(0);
+    }
+    store(fc + CHR, ys[buf ^ 1]);
+    __syncthreads();
+    buf ^= 1;
+  }
+  if (col_ok) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      double *dst = part + (int64_t(blockIdx.y) * n + i0 + j) * K;
+#pragma unroll
+      for (int r = 0; r < K; ++r) dst[r] = acc[j][r];
+    }
+  }
+}
+
 // FP64 batches of 8 (the certified residual's M^T r), n % 4 == 0: the same
 // sums as k_mv_cols<double, 8, true, 4> in the same order, with M kept in
 // flight.  k_mv_cols loads 4 rows, computes them and drains at every 32-row
@@ -1141,6 +1261,12 @@ void mv_cols_raw(hipStream_t s, const float *M, int64_t m, int64_t n, int32_t ro
       return;
     }
   }
+  if constexpr (std::is_same<TY, float>::value && K == 8) {
+    if (n % 4 == 0) {
+      k_mv_cols_f32x8<4, 2><<<grid, kThreads, 0, s>>>(M, m, n, Y, rpb, part, st);
+      return;
+    }
+  }
   if (n % 4 == 0) {
     if (small)
       k_mv_cols<TY, K, true, RUS><<<grid, kThreads, 0, s>>>(M, m, n, Y, rpb, part, st);
@@ -1188,7 +1314,13 @@ int cols_resident_blocks(bool fp64, int K, bool vec) {
       case 1: MFS_COLS_FN(float, 1); break;
       case 2: MFS_COLS_FN(float, 2); break;
       case 4: MFS_COLS_FN(float, 4); break;
-      case 8: MFS_COLS_FN(float, 8); break;
+      case 8:
+        if (vec) {  // k_mv_cols_f32x8 is sized for one resident block per CU
+          c = 1;
+          return c;
+        }
+        MFS_COLS_FN(float, 8);
+        break;
       default: MFS_COLS_FN(float, 16); break;
     }
   }
