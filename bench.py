@@ -655,10 +655,10 @@ def main():
          f"outer {outer}, max residual {res.max():.2e}")
 
     # dominant matvec class (largest total event time)
-    names = {_lib.MFS_K_MATVEC_ROWS: "k_mv_rows (y = M p, FP32, 8 RHS)",
-             _lib.MFS_K_MATVEC_COLS: "k_mv_cols (partial M^T y, FP32, 8 RHS)",
-             _lib.MFS_K_RESID_ROWS: "k_mv_rows (y = b - M x, FP64, 8 RHS)",
-             _lib.MFS_K_RESID_COLS: "k_mv_cols (partial M^T y, FP64, 8 RHS)",
+    names = {_lib.MFS_K_MATVEC_ROWS: "k_mv_rows_f32x8 (y = M p, FP32, 8 RHS)",
+             _lib.MFS_K_MATVEC_COLS: "k_mv_cols_f32x8 (partial M^T y, FP32, 8 RHS)",
+             _lib.MFS_K_RESID_ROWS: "k_mv_rows_f64x8 (y = b - M x, FP64, 8 RHS)",
+             _lib.MFS_K_RESID_COLS: "k_mv_cols_f64x8 (partial M^T y, FP64, 8 RHS)",
              _lib.MFS_K_MATVEC_FUSED: "k_mv_fused8 (q = M^T M p in one pass over M, FP32, 8 RHS)"}
     dom = max(names, key=lambda c: tracked[c][1])
     launches, tot_ms, tot_bytes = tracked[dom]
