@@ -1304,8 +1304,11 @@ int cols_resident_blocks(bool fp64, int K, bool vec) {
       case 2: MFS_COLS_FN(double, 2); break;
       case 4: MFS_COLS_FN(double, 4); break;
       case 8:
-        if (vec) f = reinterpret_cast<const void *>(k_mv_cols_f64x8<kF64x8ColsDepth>);
-        else MFS_COLS_FN(double, 8);
+        if (vec) {  // k_mv_cols_f64x8 is sized for one resident block per CU
+          c = 1;
+          return c;
+        }
+        MFS_COLS_FN(double, 8);
         break;
       default: MFS_COLS_FN(double, 16); break;
     }
