@@ -126,6 +126,21 @@ def test_normal_apply_exact_on_small_integers(m, bodies, per_body, k):
         assert np.array_equal(got, ref.astype(np.float64)), (mode, np.abs(got - ref).max())
 
 
+@pytest.mark.parametrize("m,n,k", [(1, 4, 8), (5, 8, 8), (70, 12, 3), (129, 1028, 8)])
+def test_normal_apply_exact_tiny_shapes(m, n, k):
+    """Degenerate shapes through the two passes: a single row, fewer rows than a
+    64-row chunk, a ragged last chunk, a partial 1024-column block; small
+    integers, so Q must equal the integer product exactly."""
+    rng = np.random.default_rng(m * 1000 + n + k)
+    M = rng.integers(-2, 3, (m, n)).astype(np.float32)
+    X = rng.integers(-2, 3, (n, k)).astype(np.float32)
+    ref = M.astype(np.int64).T @ (M.astype(np.int64) @ X.astype(np.int64))
+    ctx = _lib.Context(0)
+    Q, _ = _apply(ctx, torch.from_numpy(M).to(DEV), torch.from_numpy(X).to(DEV), OFF)
+    got = Q.cpu().numpy()
+    assert np.array_equal(got, ref.astype(np.float64)), np.abs(got - ref).max()
+
+
 def test_normal_apply_matches_fp64_ragged_rows():
     sc = scenes.coins(seed=0, n_bodies=8, pts_per_body=512, k=8)
     _, Md, _ = _setup(sc, extra_rows=45)                              # m = 4141: last panel of 13 rows
