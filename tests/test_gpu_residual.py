@@ -11,7 +11,8 @@ batches of 8 that keeps several 256-column steps of M in flight per wave
 counts that end inside a 256-column step and inside the 2048-column group of
 steps in flight, and a batch of 5 padded to 8; m below 16,384 takes the
 load-then-compute row pass, and so do column counts that are not a
-multiple of 4 and batches of 9..16 (padded to 16).
+multiple of 4 and batches of 9..16 (padded to 16); tiny scenes (5 and 70
+rows) check the clamped chunk edges of the M^T r pass.
 """
 import numpy as np
 import pytest
@@ -38,6 +39,8 @@ def _relres_fp64(M, B, X):
     (9001, 4, 501, 8),      # below 16,384 rows: the load-then-compute pass
     (16411, 3, 667, 8),     # n = 2001 (not a multiple of 4): the scalar-load row pass
     (16411, 4, 501, 13),    # a batch of 13 padded to 16
+    (5, 2, 4, 8),           # n = 8, 5 rows: one short row chunk
+    (70, 3, 4, 3),          # n = 12, a ragged row chunk, 3 of 4 columns
 ])
 def test_normal_residual_matches_fp64(m, bodies, per_body, k):
     sc = scenes.coins(seed=1, n_bodies=bodies, pts_per_body=per_body, k=k, box=8.0)
