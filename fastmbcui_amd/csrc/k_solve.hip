@@ -1221,6 +1221,8 @@ void mv_rows(hipStream_t s, const SolveArgs<TA> &a, const TA *X, const float *Bs
   const int64_t rpb = int64_t(waves) * (small ? RWS : RW);
   const unsigned grid = unsigned((a.m + rpb - 1) / rpb);
   const bool vec = a.n % 4 == 0;
+  // batches of 8 with float4 rows: the pipelined passes, once their grid
+  // covers the CUs (fewer rows keep k_mv_rows' shorter blocks)
   if constexpr (sizeof(TA) == 4 && K == 8) {
     const int64_t rows = int64_t(waves) * kF32x8Rows;
     if (vec && (a.m + rows - 1) / rows >= 256) {
