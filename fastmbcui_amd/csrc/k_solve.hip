@@ -1016,12 +1016,22 @@ __global__ __launch_bounds__(kThreads) void k_upd_xr(int64_t n, TA *__restrict__
 #pragma unroll
   for (int k = 0; k < K; ++k) d[k] = 0.0;
   if (c < n) {
+    // every slot's loads first: loads inside the per-slot branches completed
+    // one slot at a time
+    TA xv[K], pk[K], rk[K], qk[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      xv[k] = x[c * K + k];
+      pk[k] = p[c * K + k];
+      rk[k] = r[c * K + k];
+      qk[k] = q[c * K + k];
+    }
 #pragma unroll
     for (int k = 0; k < K; ++k) {
       if (st->active[k]) {
         const TA alpha = TA(st->absnew[k] / st->dot_pq[k]);
-        x[c * K + k] += alpha * p[c * K + k];
-        const TA rv = r[c * K + k] + (-alpha) * q[c * K + k];
+        x[c * K + k] = xv[k] + alpha * pk[k];
+        const TA rv = rk[k] + (-alpha) * qk[k];
         r[c * K + k] = rv;
         d[k] = double(rv) * double(rv);
       }
@@ -1194,16 +1204,23 @@ __global__ __launch_bounds__(kThreads) void k_upd_p(int64_t n, const int32_t *__
   const int64_t c = int64_t(blockIdx.x) * kThreads + threadIdx.x;
   if (c < n) {
     const int64_t o = Pinv[c];
+    // every slot's loads first (see k_upd_xr)
+    TA zk[K], pk[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      zk[k] = z[c * K + k];
+      pk[k] = INIT ? TA(0) : p[c * K + k];
+    }
 #pragma unroll
     for (int k = 0; k < K; ++k) {
       TA v;
       if (INIT) {
-        v = z[c * K + k];
+        v = zk[k];
       } else if (st->active[k]) {
         const TA beta = TA(st->dot_rz[k] / st->absnew[k]);
-        v = beta * p[c * K + k] + z[c * K + k];
+        v = beta * pk[k] + zk[k];
       } else {
-        v = p[c * K + k];
+        v = pk[k];
       }
       p[c * K + k] = v;
       pv[o * K + k] = v;
