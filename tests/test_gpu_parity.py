@@ -197,6 +197,35 @@ def test_pcg_ragged_sizes_match_oracle_trajectory(ppb):
     assert res < 1e-6, res
 
 
+@pytest.mark.parametrize("k", [8, 13])
+def test_mixed_fixed_iterations_match_oracle_trajectory(k):
+    """The batched FP32 PCG of the chosen form (batch of 8; 13 padded to 16):
+    its preconditioner products L^T r and L t (+ dot), vector updates and
+    alpha/beta, against the oracle's FP64 PCG run with the GPU's own FP32
+    factor values, after 1 and 3 iterations from zero (one sweep, no
+    refinement).  Rows of L at rho = 3 hold from 1 to over 100 entries, so
+    the products' four-entries-at-a-time steps and their one-entry remainders
+    both run; a wrong index or a dropped entry moves x by far more than the
+    tolerance, FP32 arithmetic by 3.8e-7 .. 4.6e-7 (measured)."""
+    sc = _ellips(k=k)
+    ctx, Md, M, pl, val, ref = _factor_pair(sc, _lib.MFS_FP32)
+    Mp = oracle.permuted_colmajor(M.astype(np.float64), pl.Pinv)
+    B = torch.from_numpy(sc.b.astype(np.float32)).to(DEV)
+    for iters in (1, 3):
+        X = ctx.solve_mixed(Md, B, maxit=iters, tol=1e-30).cpu().numpy().astype(np.float64)
+        s = ctx.stats()
+        assert s.outer_iters == 1
+        worst = 0.0
+        for c in range(k):
+            b = sc.b[:, c].astype(np.float32).astype(np.float64)
+            xo, it, _ = oracle.pcg(Mp, M.shape[0], pl, val, b, 0, iters, 0.0)
+            assert it == iters and s.iters[c] == iters, (c, it, s.iters[c])
+            xo = xo[pl.P]
+            worst = max(worst, np.linalg.norm(X[:, c] - xo) / np.linalg.norm(xo))
+        print(f"mixed k={k} after {iters} iterations: max rel. diff {worst:.2e}", flush=True)
+        assert worst < 5e-6, (k, iters, worst)
+
+
 def test_pcg_converged_matches_direct_solve():
     sc = _small(k=2)
     ctx, Md, M, pl, val, ref = _factor_pair(sc)
