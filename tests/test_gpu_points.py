@@ -5,7 +5,8 @@
 * mfs_factor_points: the KL factor with the Gram evaluated from the points,
   against the oracle's factor of the same scene;
 * mfs_solve_points: relative normal-equation residual <= tol, recomputed by the
-  oracle on the FP64 kernel matrix; warm start; error paths.
+  oracle on the FP64 kernel matrix; fixed-iteration trajectories against the
+  oracle's PCG with the same factor values; warm start; error paths.
 """
 import numpy as np
 import pytest
@@ -144,6 +145,36 @@ def test_solve_points_reaches_tolerance(maker, k):
     M = scenes.kernel_matrix(sc.x, sc.nx, sc.y)
     res_o = oracle.normal_residual(M, sc.b.astype(np.float32), X.cpu().numpy())
     assert res_o <= 2e-6
+
+
+@pytest.mark.parametrize("k", [3, 8])
+def test_solve_points_fixed_iterations_match_oracle_trajectory(k):
+    """The matrix-free batched FP32 PCG after 1 and 3 iterations from zero
+    (one sweep, no refinement) against the oracle's FP64 PCG on the FP64
+    kernel matrix, run with the GPU's own factor values: the on-the-fly M p
+    and M^T y passes, the preconditioner products and the updates.  Their
+    FP32 evaluation moves x by 5.2e-7 .. 6.1e-7 (measured); a wrong target,
+    source or sign moves it at O(1)."""
+    sc = _ellips(k=k)
+    ctx = _points_ctx(sc)
+    val = ctx.factor_export()
+    M = scenes.kernel_matrix(sc.x, sc.nx, sc.y)
+    pl = oracle.plan(sc.y, 3.0)
+    Mp = oracle.permuted_colmajor(M, pl.Pinv)
+    B = _dev(sc.b.astype(np.float32))
+    for iters in (1, 3):
+        X = ctx.solve_points(B, maxit=iters, tol=1e-30).cpu().numpy().astype(np.float64)
+        s = ctx.stats()
+        assert s.outer_iters == 1
+        worst = 0.0
+        for c in range(k):
+            b = sc.b[:, c].astype(np.float32).astype(np.float64)
+            xo, it, _ = oracle.pcg(Mp, M.shape[0], pl, val, b, 0, iters, 0.0)
+            assert it == iters and s.iters[c] == iters, (c, it, s.iters[c])
+            xo = xo[pl.P]
+            worst = max(worst, np.linalg.norm(X[:, c] - xo) / np.linalg.norm(xo))
+        print(f"points k={k} after {iters} iterations: max rel. diff {worst:.2e}", flush=True)
+        assert worst < 6e-6, (k, iters, worst)
 
 
 def test_solve_points_config1_coins():
