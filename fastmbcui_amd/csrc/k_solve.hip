@@ -1183,24 +1183,37 @@ __global__ __launch_bounds__(kThreads) void k_spmv(int64_t n, const int32_t *__r
 }
 
 // st->dot_rz = the sum of the nblocks partials k_spmv<.., true> left in dpart.
-template <int K>
+// Also beta = dot_rz / absnew and then absnew = dot_rz for the iterating
+// columns (INIT: absnew = dot_rz for all), so k_upd_p, the next launch, reads
+// beta and needs no last-block step of its own (kl_chol.cu:1056-1077).
+template <int K, bool INIT>
 __global__ __launch_bounds__(kThreads) void k_dot_rz(const double *__restrict__ dpart,
                                                      uint32_t nblocks, PcgState *st) {
   if (batch_done(st)) return;
   __shared__ double red[5 * K];
   gather_partials<K>(dpart, nblocks, red);
-  if (threadIdx.x < K) st->dot_rz[threadIdx.x] = red[4 * K + threadIdx.x];
+  if (threadIdx.x < K) {
+    const int k = threadIdx.x;
+    const double rz = red[4 * K + k];
+    st->dot_rz[k] = rz;
+    if (INIT) {
+      st->absnew[k] = rz;
+    } else if (st->active[k]) {
+      st->beta[k] = rz / st->absnew[k];
+      st->absnew[k] = rz;
+    }
+  }
 }
 
 // ------------------------------------------------------------------ p update
-// INIT: p = z, absnew = r^T z (kl_chol.cu:957-982).  Else p = z + beta p
-// (kl_chol.cu:1056-1077).  Also scatters p into original order for the matvec.
+// INIT: p = z (kl_chol.cu:957-982).  Else p = z + beta p with beta from
+// k_dot_rz (kl_chol.cu:1056-1077).  Also scatters p into original order for
+// the matvec.
 template <typename TA, int K, bool INIT>
 __global__ __launch_bounds__(kThreads) void k_upd_p(int64_t n, const int32_t *__restrict__ Pinv,
                                                     TA *__restrict__ p, const TA *__restrict__ z,
-                                                    TA *__restrict__ pv, PcgState *st) {
+                                                    TA *__restrict__ pv, const PcgState *st) {
   if (batch_done(st)) return;
-  __shared__ int last;
   const int64_t c = int64_t(blockIdx.x) * kThreads + threadIdx.x;
   if (c < n) {
     const int64_t o = Pinv[c];
@@ -1217,25 +1230,13 @@ __global__ __launch_bounds__(kThreads) void k_upd_p(int64_t n, const int32_t *__
       if (INIT) {
         v = zk[k];
       } else if (st->active[k]) {
-        const TA beta = TA(st->dot_rz[k] / st->absnew[k]);
+        const TA beta = TA(st->beta[k]);
         v = beta * pk[k] + zk[k];
       } else {
         v = pk[k];
       }
       p[c * K + k] = v;
       pv[o * K + k] = v;
-    }
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    __threadfence();
-    const uint32_t prev = atomicAdd(&st->counter[C_P], 1u);
-    last = (prev == gridDim.x - 1);
-    if (last) {
-      atomicExch(&st->counter[C_P], 0u);
-      __threadfence();
-      for (int k = 0; k < K; ++k)
-        if (INIT || st->active[k]) st->absnew[k] = st->dot_rz[k];
     }
   }
 }
@@ -1434,7 +1435,7 @@ hipError_t init_k(hipStream_t s, const SolveArgs<TA> &a, const float *B, int64_t
   const unsigned gz = unsigned(spmv_grid(a.n, K, true));
   k_spmv<TA, K, true><<<gz, kThreads, 0, s>>>(a.n, a.tptr, a.tcol, a.ltval, a.t, a.z, a.r, a.dpart,
                                               a.st);
-  k_dot_rz<K><<<1, kThreads, 0, s>>>(a.dpart, gz, a.st);
+  k_dot_rz<K, true><<<1, kThreads, 0, s>>>(a.dpart, gz, a.st);
   k_upd_p<TA, K, true><<<gn, kThreads, 0, s>>>(a.n, a.Pinv, a.p, a.z, a.pv, a.st);
   return hipGetLastError();
 }
@@ -1477,7 +1478,7 @@ hipError_t iter_k(hipStream_t s, const SolveArgs<TA> &a, TrackFn fn, void *tctx)
   const unsigned gz = unsigned(spmv_grid(a.n, K, true));
   k_spmv<TA, K, true><<<gz, kThreads, 0, s>>>(a.n, a.tptr, a.tcol, a.ltval, a.t, a.z, a.r, a.dpart,
                                               a.st);
-  k_dot_rz<K><<<1, kThreads, 0, s>>>(a.dpart, gz, a.st);
+  k_dot_rz<K, false><<<1, kThreads, 0, s>>>(a.dpart, gz, a.st);
   k_upd_p<TA, K, false><<<gn, kThreads, 0, s>>>(a.n, a.Pinv, a.p, a.z, a.pv, a.st);
   return hipGetLastError();
 }

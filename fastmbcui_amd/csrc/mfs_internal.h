@@ -44,9 +44,10 @@ struct PcgState {
   uint32_t fdone;
   uint32_t fabort;
   uint32_t fpad;
+  double beta[kMaxRhs];     // r^T z (new) / r^T z (old), set by k_dot_rz for k_upd_p
 };
 
-enum Counter { C_PQ = 0, C_RR = 1, C_RZ = 2, C_P = 3, C_R0 = 4, C_NORM = 5 };
+enum Counter { C_PQ = 0, C_RR = 1, C_R0 = 4, C_NORM = 5 };
 
 // Packed point records of the matrix-free form (layouts in kernel_eval.h).
 // tgt32 == nullptr means the dense form (M is stored).
