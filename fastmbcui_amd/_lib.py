@@ -96,6 +96,7 @@ def lib():
             "mfs_points_residual64": (ci, [vp, vp, i64, vp, vp]),
             "mfs_points_apply": (ci, [vp, ci, ci, vp, i64, vp]),
             "mfs_dense_normal_apply": (ci, [vp, vp, i64, i64, vp, i64, vp]),
+            "mfs_dense_normal_apply_noalloc": (ci, [vp, vp, i64, i64, vp, i64, vp]),
             "mfs_solid_angle": (ci, [vp, ci, vp, i64, vp, i64, vp, i64, vp]),
             "mfs_solid_angle_backward": (ci, [vp, ci, vp, i64, vp, i64, vp, i64, vp, vp, vp]),
             "mfs_single_layer_apply": (ci, [vp, ci, vp, i64, vp, i64, vp, i64, vp]),
@@ -365,6 +366,26 @@ class Context:
         Q = torch.empty((M.shape[1], X.shape[1]), dtype=torch.float64, device=X.device)
         self._check(lib().mfs_dense_normal_apply(self._h, _ptr(M.contiguous()), M.shape[0], M.shape[1],
                                                  _ptr(X), X.shape[1], _ptr(Q)), "mfs_dense_normal_apply")
+        return Q
+
+    def dense_normal_apply_noalloc(self, M, X, Q=None):
+        """dense_normal_apply through the entry point that allocates nothing
+        (static device scratch); asynchronous on the current stream.  X must be
+        a contiguous n x k float32 device tensor, Q (n x k float64) is
+        allocated here when not given."""
+        import torch
+        self._sync_stream()
+        if X.dim() == 1:
+            X = X.unsqueeze(1)
+        if X.dtype != torch.float32 or not X.is_contiguous() or not M.is_contiguous():
+            raise ValueError("M and X must be contiguous float32 device tensors")
+        if M.dim() != 2 or X.shape[0] != M.shape[1]:
+            raise ValueError("X must have as many rows as M has columns")
+        if Q is None:
+            Q = torch.empty((M.shape[1], X.shape[1]), dtype=torch.float64, device=X.device)
+        self._check(lib().mfs_dense_normal_apply_noalloc(self._h, _ptr(M), M.shape[0], M.shape[1],
+                                                         _ptr(X), X.shape[1], _ptr(Q)),
+                    "mfs_dense_normal_apply_noalloc")
         return Q
 
     def points_apply(self, v, trans=False, precision=MFS_FP32):

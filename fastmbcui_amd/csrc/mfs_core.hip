@@ -13,6 +13,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <vector>
@@ -70,6 +71,38 @@ struct Bucket {
   int32_t offset, count;
   int32_t phase_off[kFactorPhases + 1];  // phase p: items offset + [phase_off[p], phase_off[p + 1])
 };
+
+}  // namespace
+
+// ---- scratch of mfs_dense_normal_apply_noalloc.  The only device memory that
+// entry point uses beyond its inputs and outputs: the padded copy of X, y = M X
+// and the split partials of M^T y.  Each GPU holds its own copy from module
+// load until the process ends.  In host code `scratch` names a host-side
+// stand-in, so kernels are given the device address that hipGetSymbolAddress
+// returns (once per device, when the first context on it is created).
+constexpr size_t kScratchBytes = size_t(32) << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+
+constexpr int kScratchDevices = 64;
+struct ScratchSlot {
+  std::once_flag once;
+  void *ptr = nullptr;
+  hipError_t err = hipSuccess;
+};
+ScratchSlot g_scratch[kScratchDevices];
+
+// Called with `device` current.
+void scratch_lookup(int device) {
+  if (device < 0 || device >= kScratchDevices) return;
+  ScratchSlot &s = g_scratch[device];
+  std::call_once(s.once, [&s]() {
+    void *p = nullptr;
+    s.err = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (s.err == hipSuccess) s.ptr = p;
+  });
+}
 
 }  // namespace
 
@@ -645,6 +678,14 @@ __global__ void k_x64_from_f32(int64_t n, const float *X, int64_t ldx, int64_t c
   for (int r = 0; r < K; ++r) X64[i * K + r] = (r < kval && X) ? double(X[i * ldx + col0 + r]) : 0.0;
 }
 
+// pv (n x K, zero beyond k) = X (n x k, row-major), FP32.
+__global__ void k_pack_rhs(int64_t n, const float *X, int64_t k, int K, float *pv) {
+  const int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x;
+  if (i >= n * K) return;
+  const int64_t row = i / K, r = i % K;
+  pv[i] = r < k ? X[row * k + r] : 0.f;
+}
+
 // FP64 strided columns <-> the K-padded FP64 iterate (FP64 warm start / output).
 __global__ void k_x64_from_f64(int64_t n, const double *X, int64_t ldx, int64_t col0, int kval,
                                int K, double *X64) {
@@ -1118,6 +1159,7 @@ int mfs_ctx_create(mfs_ctx **out, int device, void *stream) {
     mfs_ctx_destroy(ctx);
     return MFS_ERR_HIP;
   }
+  scratch_lookup(device);  // the device is current (set above)
   *out = ctx;
   return MFS_OK;
 }
@@ -1372,6 +1414,49 @@ int mfs_dense_normal_apply(mfs_ctx *ctx, const float *M, int64_t m, int64_t n, c
     (void)hipMemcpy(&a.st->fabort, &zero, sizeof(zero), hipMemcpyHostToDevice);
     return ctx->fail(MFS_ERR_HIP, "one-pass matvec: a wait for another block timed out");
   }
+  return MFS_OK;
+}
+
+int mfs_dense_normal_apply_noalloc(mfs_ctx *ctx, const float *M, int64_t m, int64_t n,
+                                   const float *X, int64_t k, double *Q) {
+  if (ctx == nullptr) return MFS_ERR_ARG;
+  if (M == nullptr || X == nullptr || Q == nullptr || m <= 0 || n <= 0 || k <= 0 || k > kMaxRhs)
+    return ctx->fail(MFS_ERR_ARG, "dense_normal_apply_noalloc: bad arguments (1 <= k <= 16)");
+  if (ctx->device < 0 || ctx->device >= kScratchDevices)
+    return ctx->fail(MFS_ERR_ARG, "dense_normal_apply_noalloc: device index above 63");
+  const ScratchSlot &slot = g_scratch[ctx->device];
+  if (slot.ptr == nullptr)
+    return ctx->fail(MFS_ERR_HIP, std::string("dense_normal_apply_noalloc: no scratch address: ") +
+                                      hipGetErrorString(slot.err));
+  const int K = batch_width(k);
+  SolveArgs<float> a{};
+  a.M = M;
+  a.m = m;
+  a.n = n;
+  a.K = K;
+  // the split count of mfs_dense_normal_apply, so both give the same sums
+  a.row_blocks = row_blocks_for(m, n, ctx->cols_blocks,
+                                int64_t(ctx->cus) * cols_resident_blocks(false, K, n % 4 == 0));
+  auto up256 = [](int64_t b) { return (b + 255) / 256 * 256; };
+  const int64_t pv_bytes = up256(n * K * 4), y_bytes = up256(m * K * 4);
+  const int64_t part_bytes = int64_t(a.row_blocks) * n * K * 8;
+  const int64_t need = pv_bytes + y_bytes + part_bytes;
+  if (need > int64_t(kScratchBytes) || (n * K + kThreads - 1) / kThreads > INT32_MAX)
+    return ctx->fail(MFS_ERR_ARG, "dense_normal_apply_noalloc: needs " + std::to_string(need) +
+                                      " bytes of scratch; the static scratch holds " +
+                                      std::to_string(kScratchBytes) + " (m x n about 2^32 / k at most)");
+  HIPCHK(hipSetDevice(ctx->device), "hipSetDevice");
+  char *base = static_cast<char *>(slot.ptr);
+  a.pv = reinterpret_cast<float *>(base);
+  a.y = reinterpret_cast<float *>(base + pv_bytes);
+  a.part = reinterpret_cast<double *>(base + pv_bytes + y_bytes);
+  a.pts = PointsOp{};
+  a.st = nullptr;  // no batch state: the passes always run
+  a.fused = 0;     // the two-pass matvec (the one-pass kernel needs exchange records)
+  k_pack_rhs<<<unsigned((n * K + kThreads - 1) / kThreads), kThreads, 0, ctx->stream>>>(n, X, k, K,
+                                                                                       a.pv);
+  HIPCHK(hipGetLastError(), "pack input");
+  HIPCHK(normal_apply32(ctx->stream, a, int(k), Q, nullptr, nullptr), "normal apply");
   return MFS_OK;
 }
 

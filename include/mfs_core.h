@@ -165,6 +165,19 @@ int mfs_points_apply(mfs_ctx *ctx, int trans, int precision, const void *in, int
 int mfs_dense_normal_apply(mfs_ctx *ctx, const float *M, int64_t m, int64_t n, const float *X,
                            int64_t k, double *Q);
 
+/* mfs_dense_normal_apply without any allocation, for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops).
+ * No counterpart in the reference.  The padded X, y = M X and the split
+ * partials live in a static 32 MiB device array per GPU (held from module
+ * load to process end; about m x n <= 2^32 / k at most -- larger shapes give
+ * MFS_ERR_ARG); the caller passes M, X and Q.  Always the two-pass matvec,
+ * with the same sums as mfs_dense_normal_apply.  Asynchronous on the
+ * context's stream: it returns once the work is queued.  Every call on one
+ * GPU shares that array, so calls that overlap on one GPU (other streams,
+ * other contexts, other threads) race. */
+int mfs_dense_normal_apply_noalloc(mfs_ctx *ctx, const float *M, int64_t m, int64_t n,
+                                   const float *X, int64_t k, double *Q);
+
 /* Dense point-kernel matrix M[i,j] = -(x_i - y_j).n_i / (4 pi |x_i - y_j|^3),
  * evaluated in FP64 and stored FP32.  x, nx: m x 3, y: n x 3, device FP64.
  * Point form of src/kernels.h:24-40 (laplace_nm_3d_sl, up to the factor
