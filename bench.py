@@ -79,32 +79,112 @@ def _log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(tol):
-    """FP64 oracle on the first 8 ellipsoids (N = 8192), scaled by N^2 to 65,536."""
+def _cpu_info():
+    model, sockets = "unknown", set()
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name") and model == "unknown":
+                    model = line.split(":", 1)[1].strip()
+                elif line.startswith("physical id"):
+                    sockets.add(line.split(":", 1)[1].strip())
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "model": model, "sockets": max(len(sockets), 1)}
+
+
+def _median_ms(fn, reps=5, warmup=1):
+    for _ in range(warmup):
+        fn()
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        fn()
+        ts.append((time.perf_counter() - t0) * 1e3)
+    return float(sorted(ts)[len(ts) // 2])
+
+
+def cpu_baseline(tol, iters_c2):
+    """The FP64 CPU oracle (C, OpenMP) on this box's host cores.
+
+    * Config 1 (8 coins, N = 4096, 1 RHS): plan + factor + PCG timed in full,
+      1 warm-up + 5 repeats, median.
+    * Config 2 (the bench scene, N = 65,536, 8 RHS): factor + ms/iteration x
+      measured iterations.  Its dense FP64 M (34 GB, twice with the permuted
+      copy) is not built; the Gram and the matvec cost scale linearly in the
+      row count m for fixed n, so the oracle's factor and PCG iteration are
+      timed on 2048 and 4096 rows (Gaussian values: the cost does not depend
+      on them) with all 65,536 columns and the scene's full plan, and
+      extrapolated linearly to m = 65,536.  Iteration
+      counts are the plain FP64 form's (the same PCG as the oracle's).  The
+      plan is timed in full and reported beside it.
+    """
     import numpy as np
 
     import oracle
     from fastmbcui_amd import scenes
-    sc = scenes.ellipsoids(seed=0, n_bodies=8, k=8)
-    M = scenes.kernel_matrix(sc.x, sc.nx, sc.y).astype(np.float32).astype(np.float64)
-    B = sc.b.astype(np.float32).astype(np.float64)
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    info = _cpu_info()
+
+    # ---- config 1, in full
+    c1 = scenes.coins(seed=0, k=1)
+    M1 = scenes.kernel_matrix(c1.x, c1.nx, c1.y).astype(np.float32).astype(np.float64)
+    B1 = c1.b.astype(np.float32).astype(np.float64)
+    res1 = {}
+
+    def run_c1():
+        res1["r"] = oracle.minvf(M1, B1, c1.y, 3.0, 1000, tol)
+    c1_ms = _median_ms(run_c1)
+
+    # ---- config 2: plan in full, factor and iterations on row samples
+    sc = scenes.ellipsoids(seed=0, k=8)
     t0 = time.perf_counter()
     pl = oracle.plan(sc.y, 3.0)
-    t1 = time.perf_counter()
-    r = oracle.minvf(M, B, sc.y, 3.0, 1000, tol, pl=pl)
-    t2 = time.perf_counter()
-    per_solve = (t2 - t1) * 1e3 / sc.k
-    scale = (65536.0 / sc.n) ** 2
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    plan_ms = (time.perf_counter() - t0) * 1e3
+    rows = (2048, 4096)
+    fac, itr = [], []
+    rng = np.random.default_rng(0)
+    for mr in rows:
+        # The work of both depends on the plan and the row count only, not on
+        # M's values; a few thousand rows of the scene's own M leave supernode
+        # Grams singular (most sources see no nearby target), so the samples
+        # are Gaussian rows (SPD Grams), column-major as the oracle takes them.
+        Mp = rng.standard_normal((sc.n, mr))
+        t1 = time.perf_counter()
+        val = oracle.factor(Mp, mr, pl)
+        fac.append((time.perf_counter() - t1) * 1e3)
+        b = rng.standard_normal(mr)
+        t1 = time.perf_counter()
+        oracle.pcg(Mp, mr, pl, val, b, 0, 3, 0.0)
+        itr.append((time.perf_counter() - t1) * 1e3 / 3.5)   # M^T b (half a matvec) + 3 iterations
+        del Mp
+    m_full = sc.m
+
+    def at_full(t):
+        slope = (t[1] - t[0]) / (rows[1] - rows[0])
+        return t[1] + slope * (m_full - rows[1])
+    factor_ms = at_full(fac)
+    iter_ms = at_full(itr)
+    its = list(iters_c2)
+    per_solve = (factor_ms + sum(i + 0.5 for i in its) * iter_ms) / len(its)
     return {
-        "value": per_solve * scale,
+        "value": round(per_solve, 1),
         "unit": "ms/solve",
         "cores": threads,
         "kind": "port",
-        "sample": (f"oracle (C, FP64, OpenMP) factor+PCG on the first 8 ellipsoids of the same scene "
-                   f"(N={sc.n}, 8 RHS, rho 3, tol {tol:g}): {per_solve:.1f} ms/solve measured, "
-                   f"x{scale:.0f} for N^2 scaling to 65,536; plan {1e3 * (t1 - t0):.0f} ms not included; "
-                   f"iterations {r.iters}"),
+        "sample": (f"oracle (C, FP64, OpenMP, {threads} threads) on the bench scene (N={sc.n}, 8 RHS, "
+                   f"rho 3, tol {tol:g}): factor + ms/iteration x iterations, factor and iteration "
+                   f"timed on 2048 and 4096 Gaussian rows (all {sc.n} columns, the scene's plan) and "
+                   f"extrapolated linearly in m to {m_full}; iterations {its} from the plain FP64 form"),
+        "cpu": info,
+        "config2": {"factor_ms": round(factor_ms, 1), "ms_per_iteration": round(iter_ms, 2),
+                    "iterations": its, "plan_ms": round(plan_ms, 1),
+                    "ms_per_solve_with_plan": round(per_solve + plan_ms / len(its), 1),
+                    "sample_factor_ms": [round(v, 1) for v in fac],
+                    "sample_iteration_ms": [round(v, 2) for v in itr]},
+        "config1": {"workload": "8 coins x 512 points (N=4096), 1 RHS, rho 3, tol %g" % tol,
+                    "ms_per_solve_with_plan": round(c1_ms, 1), "iterations": res1["r"].iters,
+                    "timing": "1 warm-up + 5 repeats, median"},
     }
 
 
@@ -660,10 +740,29 @@ def main():
              _lib.MFS_K_RESID_ROWS: "k_mv_rows_f64x8 (y = b - M x, FP64, 8 RHS)",
              _lib.MFS_K_RESID_COLS: "k_mv_cols_f64x8 (partial M^T y, FP64, 8 RHS)",
              _lib.MFS_K_MATVEC_FUSED: "k_mv_fused8 (q = M^T M p in one pass over M, FP32, 8 RHS)"}
-    dom = max(names, key=lambda c: tracked[c][1])
-    launches, tot_ms, tot_bytes = tracked[dom]
-    avg_ms = tot_ms / max(launches, 1)
-    achieved = (tot_bytes / max(launches, 1)) / (avg_ms * 1e-3) / 1e9 if launches else 0.0
+    # The roofline's unit is the PCG's normal matvec q = M^T (M p), whose
+    # algorithmic traffic is one pass over M (m n 4 bytes): the one-pass kernel
+    # when it runs, else the pair of passes (row pass + column pass, the same
+    # number of launches each), timed live per launch.
+    mv_bytes = float(m) * float(n) * 4.0
+    f_l, f_ms, _ = tracked[_lib.MFS_K_MATVEC_FUSED]
+    r_l, r_ms, _ = tracked[_lib.MFS_K_MATVEC_ROWS]
+    c_l, c_ms, _ = tracked[_lib.MFS_K_MATVEC_COLS]
+    if f_l > 0:
+        dom_name = names[_lib.MFS_K_MATVEC_FUSED]
+        launches, avg_ms = f_l, f_ms / f_l
+        dom_key = "k_mv_fused8"
+    else:
+        dom_name = ("normal matvec q = M^T (M p), FP32, 8 RHS: k_mv_rows_f32x8 + k_mv_cols_f32x8 "
+                    "(two passes over M)")
+        launches = min(r_l, c_l)
+        avg_ms = (r_ms / max(r_l, 1) + c_ms / max(c_l, 1))
+        dom_key = "normal_matvec"
+    achieved = mv_bytes / (avg_ms * 1e-3) / 1e9 if launches else 0.0
+    per_pass = {"k_mv_rows_f32x8_avg_ms": round(r_ms / max(r_l, 1), 4),
+                "k_mv_cols_f32x8_avg_ms": round(c_ms / max(c_l, 1), 4),
+                "pass_stream_gbs": [round(mv_bytes / (r_ms / max(r_l, 1) * 1e-3) / 1e9, 1) if r_l else None,
+                                    round(mv_bytes / (c_ms / max(c_l, 1) * 1e-3) / 1e9, 1) if c_l else None]}
     g_l, g_ms, g_bytes = tracked[_lib.MFS_K_GRAM]
     fused_launches = tracked[_lib.MFS_K_MATVEC_FUSED][0]
     kernel_ms = {names.get(c, "supernode factor (M^T ranges + Gram + Cholesky + L, overlapped)"): round(tracked[c][1] / args.steps, 3)
@@ -673,7 +772,7 @@ def main():
     tpath = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.exists(tpath):
         try:
-            traffic = json.load(open(tpath)).get(names[dom].split(" ")[0])
+            traffic = json.load(open(tpath)).get(dom_key)
         except Exception:
             traffic = None
 
@@ -712,18 +811,20 @@ def main():
             t2 = time.perf_counter()
             ctx.factor_dense(M, _lib.MFS_FP64)
             pf = ctx.stats().factor_ms
-            Xp = ctx.solve_dense(M, B, maxit=1000, tol=args.tol)
+            Xp = ctx.solve_dense64(M, B, maxit=1000, tol=args.tol)
             torch.cuda.synchronize()
             tp.append((time.perf_counter() - t2) * 1e3)
         sp = ctx.stats()
         ptr = {c: ctx.tracked(c) for c in classes}
         ctx.track([])
-        pres = ctx.normal_residual(M, B, Xp)
+        pres = ctx.normal_residual(M, B, Xp.float())
+        err_plain = [float((X[:, c].double() - Xp[:, c]).norm() / Xp[:, c].norm()) for c in range(k)]
         pnames = {_lib.MFS_K_MATVEC_ROWS: "k_mv_rows (y = M p, FP64, 1 RHS)",
                   _lib.MFS_K_MATVEC_COLS: "k_mv_cols (partial M^T y, FP64, 1 RHS)",
                   _lib.MFS_K_GRAM: "supernode factor FP64"}
         plain = dict(ms_per_solve=float(np.mean(tp)) / k, factor_ms=pf,
                      iters=[int(sp.iters[c]) for c in range(k)], residual_max=float(pres.max()),
+                     chosen_rel_err_vs_plain_fp64=max(err_plain),
                      kernel_avg_ms={pnames[c]: round(ptr[c][1] / max(ptr[c][0], 1), 4) for c in pnames})
         _log(f"bench: plain {np.mean(tp):.1f} ms/step, factor {pf:.1f} ms, iters {plain['iters']}, "
              f"max residual {pres.max():.2e}")
@@ -773,8 +874,8 @@ def main():
 
     cpu = None
     if not args.skip_cpu:
-        _log("bench: CPU oracle baseline (bounded sample)")
-        cpu = cpu_baseline(args.tol)
+        _log("bench: CPU oracle baseline (config 1 in full; config 2 by row samples)")
+        cpu = cpu_baseline(args.tol, plain["iters"] if plain else iters)
 
     value = ms_per_step / k
     line = {
@@ -795,8 +896,8 @@ def main():
                    "n": n, "m": m, "k": k},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "kernel": names[dom], "avg_launch_ms": round(avg_ms, 4),
-                     "bytes_per_launch": (tot_bytes / max(launches, 1))},
+                     "kernel": dom_name, "avg_ms": round(avg_ms, 4), "launches": launches,
+                     "algorithmic_bytes": mv_bytes, "per_pass": per_pass},
         "cpu_baseline": cpu,
         "solves_per_s": round(1e3 / value, 2),
         "plan_ms": round(plan_ms, 2),

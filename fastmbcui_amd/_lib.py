@@ -43,7 +43,8 @@ class Stats(ctypes.Structure):
         ("iters", ctypes.c_int32 * 64), ("relres", ctypes.c_double * 64),
         ("outer_iters", ctypes.c_int32), ("retried_supernodes", ctypes.c_int32),
         ("plan_order_ms", ctypes.c_double), ("plan_pattern_ms", ctypes.c_double),
-        ("plan_grid_columns", ctypes.c_int32), ("pad1", ctypes.c_int32),
+        ("plan_grid_columns", ctypes.c_int32), ("fp64_sweeps", ctypes.c_int32),
+        ("reverted_columns", ctypes.c_int32), ("pad1", ctypes.c_int32),
     ]
 
 
@@ -86,6 +87,8 @@ def lib():
             "mfs_factor_dense": (ci, [vp, vp, i64, i64, ci]),
             "mfs_factor_export": (ci, [vp, vp]),
             "mfs_solve_dense": (ci, [vp, vp, i64, i64, vp, i64, ci, vp, i32, f64]),
+            "mfs_solve_dense64": (ci, [vp, vp, i64, i64, vp, i64, ci, vp, i32, f64]),
+            "mfs_order_maximin": (ci, [vp, i64, vp, vp, vp]),
             "mfs_solve_mixed": (ci, [vp, vp, i64, i64, vp, i64, ci, vp, vp, i32, f64, f64]),
             "mfs_normal_residual": (ci, [vp, vp, i64, i64, vp, i64, vp, vp]),
             "mfs_assemble_dense": (ci, [vp, vp, vp, vp, i64, i64, vp]),
@@ -245,6 +248,19 @@ class Context:
         self._check(lib().mfs_solve_dense(self._h, _ptr(M), M.shape[0], M.shape[1], _ptr(B), B.shape[1],
                                           int(rhs_space), _ptr(out), int(maxit), float(tol)),
                     "mfs_solve_dense")
+        return out
+
+    def solve_dense64(self, M, B, rhs_space=MFS_RHS_BOUNDARY, maxit=100, tol=1e-6, out=None):
+        """solve_dense with FP64 strengths (n x k float64 cuda)."""
+        import torch
+        self._sync_stream()
+        _check_dense(M, self.n)
+        B = _as_2d_f32(B)
+        if out is None:
+            out = torch.empty((M.shape[1], B.shape[1]), dtype=torch.float64, device=M.device)
+        self._check(lib().mfs_solve_dense64(self._h, _ptr(M), M.shape[0], M.shape[1], _ptr(B),
+                                            B.shape[1], int(rhs_space), _ptr(out), int(maxit),
+                                            float(tol)), "mfs_solve_dense64")
         return out
 
     def solve_mixed(self, M, B, maxit=1000, tol=1e-6, inner_tol=0.5, x0=None, out=None,
@@ -503,6 +519,22 @@ class Context:
         self._check(lib().mfs_ctx_tracked(self._h, int(kernel_class), ctypes.byref(n), ctypes.byref(ms),
                                           ctypes.byref(by)), "mfs_ctx_tracked")
         return n.value, ms.value, by.value
+
+
+def order_maximin(xyz):
+    """Host maximin ordering of n x 3 points: (P, Pinv, scale), as the plan
+    computes it (no device work)."""
+    xyz = np.ascontiguousarray(xyz, dtype=np.float64)
+    if xyz.ndim != 2 or xyz.shape[1] != 3:
+        raise ValueError("points must be n x 3")
+    n = xyz.shape[0]
+    P = np.empty(n, np.int32)
+    Pinv = np.empty(n, np.int32)
+    scale = np.empty(n, np.float64)
+    code = lib().mfs_order_maximin(_ptr(xyz), n, _ptr(P), _ptr(Pinv), _ptr(scale))
+    if code != MFS_OK:
+        raise MfsError(code, "mfs_order_maximin")
+    return P, Pinv, scale
 
 
 def _check_xyz(t, name):

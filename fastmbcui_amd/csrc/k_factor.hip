@@ -13,8 +13,9 @@
 //     add their accumulators at the end; stored reversed, TH[ns-1-p][ns-1-q],
 //     as the reference stores it (kl_chol.cu:216-220).
 //  2. Upper Cholesky TH = U^T U, right-looking, one barrier per column.  A
-//     non-positive pivot marks the supernode (status = 1) instead of producing
-//     NaN (the reference does not check, kl_chol.cu:33).
+//     non-positive pivot (FP32: one within the Gram's rounding noise) marks
+//     the supernode (status = 1) instead of producing NaN or noise (the
+//     reference does not check, kl_chol.cu:33).
 //  3. For each member column j with nb rows: x = U[0:nb,0:nb]^{-1} e_{nb-1},
 //     written in ascending row order -- the KL column
 //     Theta_j^{-1} e_1 / sqrt(e_1^T Theta_j^{-1} e_1) (kl_chol.cu:378-386).
@@ -409,6 +410,18 @@ __global__ __launch_bounds__(kThreads) void k_super_factor(const FactorScene *__
   static_for<NPASS>(gram_pass);
 
   // ---- 2. Upper Cholesky TH = U^T U ----------------------------------------
+  // FP32: a pivot below the FP32 Gram's rounding noise relative to its
+  // diagonal (8 sqrt(m) eps, kept within [2^-20, 2^-10]) has no correct digits
+  // left and is marked like a non-positive one, for the FP64 retry.  At rho = 3
+  // on surface scenes every pivot keeps over a fifth of its diagonal.
+  __shared__ TC d0[sizeof(TC) == 4 ? NS : 1];
+  TC floor_rel = TC(0);
+  if constexpr (sizeof(TC) == 4) {
+    for (int p = tid; p < ns; p += kThreads) d0[p] = TH[p * RS + p];
+    floor_rel = TC(8.0 * sqrt(double(m)) * 0x1p-24);
+    floor_rel = floor_rel < TC(0x1p-20) ? TC(0x1p-20) : (floor_rel > TC(0x1p-10) ? TC(0x1p-10) : floor_rel);
+    __syncthreads();
+  }
   for (int j = 0; j < ns; ++j) {
     if (j > 0) {  // finish row j-1 (disjoint from the update of step j)
       const int jr = j - 1;
@@ -416,7 +429,9 @@ __global__ __launch_bounds__(kThreads) void k_super_factor(const FactorScene *__
         TH[jr * RS + t] = (t == jr) ? dg[jr] : TH[jr * RS + t] * dinv[jr];
     }
     const TC d = TH[j * RS + j];
-    if (!(d > TC(0))) {
+    bool bad = !(d > TC(0));
+    if constexpr (sizeof(TC) == 4) bad = bad || !(d > floor_rel * d0[j]);
+    if (bad) {
       if (tid == 0) status[s] = 1;
       return;  // uniform: every thread read the same pivot
     }

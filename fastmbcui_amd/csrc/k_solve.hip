@@ -1263,6 +1263,17 @@ __global__ __launch_bounds__(kThreads) void k_finish(int64_t n, const int32_t *_
   }
 }
 
+template <int K>
+__global__ __launch_bounds__(kThreads) void k_finish64(int64_t n, const int32_t *__restrict__ P,
+                                                       const double *__restrict__ x,
+                                                       double *__restrict__ X, int64_t ldx,
+                                                       int64_t col0, int kval) {
+  const int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x;
+  if (i >= n) return;
+  const int64_t s = int64_t(P[i]) * K;
+  for (int k = 0; k < kval && k < K; ++k) X[i * ldx + col0 + k] = x[s + k];
+}
+
 // ------------------------------------------------------------------ launch helpers
 inline unsigned blocks_for(int64_t count) { return unsigned((count + kThreads - 1) / kThreads); }
 
@@ -1627,6 +1638,13 @@ hipError_t inner_thresholds(hipStream_t s, const double *nb2, double tol_frac, i
   k_inner_thresholds<<<1, 64, 0, s>>>(nb2, tol_frac, K, thr);
   return hipGetLastError();
 }
+hipError_t pcg_finish64(hipStream_t s, const SolveArgs<double> &a, double *X, int64_t ldx,
+                        int64_t col0, int kval) {
+  MFS_DISPATCH_K(a.K, (k_finish64<KK><<<blocks_for(a.n), kThreads, 0, s>>>(a.n, a.P, a.x, X, ldx, col0,
+                                                                          kval),
+                       hipGetLastError()));
+}
+
 template hipError_t pcg_iteration<double>(hipStream_t, const SolveArgs<double> &, TrackFn, void *);
 template hipError_t pcg_iteration<float>(hipStream_t, const SolveArgs<float> &, TrackFn, void *);
 template hipError_t pcg_finish<double>(hipStream_t, const SolveArgs<double> &, float *, int64_t,
@@ -1635,22 +1653,23 @@ template hipError_t pcg_finish<float>(hipStream_t, const SolveArgs<float> &, flo
                                       int64_t, int, double *);
 
 // ------------------------------------------------------------------ kernel assembly
-// M[i,j] = -(x_i - y_j).n_i / (4 pi r^3) in FP64, stored FP32; thread per (i, 4 j).
+// M[i,j] = -(x_i - y_j).n_i / (4 pi r^3) in FP64, stored FP32; thread per
+// (row, column), rows strided over gridDim.y (at most 65,535 row blocks).
 __global__ __launch_bounds__(kThreads) void k_assemble(const double *__restrict__ x,
                                                        const double *__restrict__ nx,
                                                        const double *__restrict__ y, int64_t m,
                                                        int64_t n, float *__restrict__ M) {
-  const int64_t i = blockIdx.y;
   const int64_t j = int64_t(blockIdx.x) * kThreads + threadIdx.x;
-  if (i >= m || j >= n) return;
-  M[i * n + j] = float(kernel64(x[3 * i] - y[3 * j], x[3 * i + 1] - y[3 * j + 1],
-                                x[3 * i + 2] - y[3 * j + 2], nx[3 * i], nx[3 * i + 1],
-                                nx[3 * i + 2]));
+  if (j >= n) return;
+  for (int64_t i = blockIdx.y; i < m; i += gridDim.y)
+    M[i * n + j] = float(kernel64(x[3 * i] - y[3 * j], x[3 * i + 1] - y[3 * j + 1],
+                                  x[3 * i + 2] - y[3 * j + 2], nx[3 * i], nx[3 * i + 1],
+                                  nx[3 * i + 2]));
 }
 
 hipError_t launch_assemble(hipStream_t s, const double *x, const double *nx, const double *y,
                            int64_t m, int64_t n, float *M) {
-  dim3 grid(blocks_for(n), unsigned(m));
+  dim3 grid(blocks_for(n), unsigned(m < 65535 ? m : 65535));
   k_assemble<<<grid, kThreads, 0, s>>>(x, nx, y, m, n, M);
   return hipGetLastError();
 }

@@ -686,6 +686,14 @@ __global__ void k_pack_rhs(int64_t n, const float *X, int64_t k, int K, float *p
   pv[i] = r < k ? X[row * k + r] : 0.f;
 }
 
+// Columns r with bit r of `mask` set: X64 = backup (a sweep that made them worse).
+__global__ void k_restore_cols(int64_t n, int K, uint32_t mask, const double *backup, double *X64) {
+  const int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x;
+  if (i >= n) return;
+  for (int r = 0; r < K; ++r)
+    if (mask & (1u << r)) X64[i * K + r] = backup[i * K + r];
+}
+
 // FP64 strided columns <-> the K-padded FP64 iterate (FP64 warm start / output).
 __global__ void k_x64_from_f64(int64_t n, const double *X, int64_t ldx, int64_t col0, int kval,
                                int K, double *X64) {
@@ -941,9 +949,22 @@ struct MixedRun {
   SolveArgs<double> a64{};
   SolveArgs<float> a32{};
   double *X64 = nullptr, *R64 = nullptr, *nb2 = nullptr, *res2 = nullptr, *thr = nullptr;
+  double *X64b = nullptr;       // the iterate before the running sweep
   int32_t inner_used = 0;
   int outer = 0;
   std::vector<int32_t> iters;
+  // Safeguard of the FP32 sweeps: a column whose FP64 residual a sweep made
+  // worse is restored from the backup, and once a sweep gains less than 4x on
+  // a column that is not yet done the remaining sweeps run in FP64 (the plain
+  // form's PCG on the same residual system).  FP32 sweeps stall like this
+  // only on blocks near singularity, where the FP32 recurrences lose the
+  // directions the preconditioner amplifies.
+  bool fp64_inner = false;
+  std::vector<double> prev_rel;
+  int reverted = 0;             // columns restored (all sweeps)
+  int fp64_sweeps = 0;          // sweeps run in FP64
+  int fault = MFS_OK;           // set when decide() could not restore columns
+  bool swept = false;           // a sweep ended since the last decide()
 
   // FP64 residual of the right-hand-side space: M^T (b - M x) or b - M^T M x.
   hipError_t residual(const double *X, double *out2) {
@@ -962,10 +983,17 @@ struct MixedRun {
     int rc = workspace<double>(ctx, M, pts, m, n, K, a64);
     if (rc != MFS_OK) return rc;
     if (pts) set_pass_work(ctx, pts, m, n, K);
-    HIPCHK(ctx->dx64.ensure(size_t(n) * K * 8), "alloc x64");
+    HIPCHK(ctx->dx64.ensure(size_t(n) * K * 8 * 2), "alloc x64");
     HIPCHK(ctx->dr64.ensure(size_t(n) * K * 8), "alloc r64");
     HIPCHK(ctx->dnorm.ensure(4 * kMaxRhs * 8), "alloc norms");
     X64 = ctx->dx64.as<double>();
+    X64b = X64 + size_t(n) * K;
+    fp64_inner = false;
+    prev_rel.assign(kval, -1.0);
+    reverted = 0;
+    fp64_sweeps = 0;
+    swept = false;
+    fault = MFS_OK;
     R64 = ctx->dr64.as<double>();
     nb2 = ctx->dnorm.as<double>();
     res2 = nb2 + kMaxRhs;
@@ -1017,14 +1045,45 @@ struct MixedRun {
     return MFS_OK;
   }
 
-  // After ctx->stream has been synchronized.
+  double rel_of(int r) const {
+    const double b2 = ctx->hnorm[r], e2 = ctx->hnorm[kMaxRhs + r];
+    return b2 > 0 ? std::sqrt(e2 / b2) : 0.0;
+  }
+
+  // After ctx->stream has been synchronized.  Returns whether another sweep
+  // is due; may restore columns (and then recompute and re-read the residual).
   bool decide() {
+    if (swept) {
+      swept = false;
+      uint32_t worse = 0;
+      bool stalled = false;
+      for (int r = 0; r < kval; ++r) {
+        if (!(ctx->hnorm[r] >= 1e-16) || !(prev_rel[r] >= 0)) continue;
+        const double rel = rel_of(r);
+        if (!(rel <= prev_rel[r])) worse |= 1u << r;   // also NaN
+        if (!(rel <= tol) && !(rel <= 0.25 * prev_rel[r])) stalled = true;
+      }
+      if (stalled) fp64_inner = true;
+      if (worse != 0u) {
+        const unsigned gn = unsigned((n + kThreads - 1) / kThreads);
+        k_restore_cols<<<gn, kThreads, 0, ctx->stream>>>(n, K, worse, X64b, X64);
+        for (int r = 0; r < kval; ++r) reverted += (worse >> r) & 1u;
+        hipError_t e = residual(X64, res2);
+        if (e == hipSuccess)
+          e = hipMemcpyAsync(ctx->hnorm, nb2, 2 * kMaxRhs * 8, hipMemcpyDeviceToHost, ctx->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+        if (e != hipSuccess) {
+          fault = ctx->hip_fail(e, "mixed solve: restore columns");
+          return false;
+        }
+      }
+    }
     bool done = true;
     for (int r = 0; r < kval; ++r) {
-      const double b2 = ctx->hnorm[r], e2 = ctx->hnorm[kMaxRhs + r];
-      const double rel = b2 > 0 ? std::sqrt(e2 / b2) : 0.0;
+      const double rel = rel_of(r);
+      prev_rel[r] = rel;
       if (col0 + r < 64) ctx->stats.relres[col0 + r] = rel;
-      if (b2 >= 1e-16 && !(rel <= tol)) done = false;
+      if (ctx->hnorm[r] >= 1e-16 && !(rel <= tol)) done = false;
     }
     return !(done || inner_used >= maxit || outer >= 30);
   }
@@ -1034,14 +1093,36 @@ struct MixedRun {
     // one sweep normally meets the tolerance and the FP64 residual check
     // only confirms it; FP32 round-off left above it is removed by the next.
     HIPCHK(inner_thresholds(ctx->stream, nb2, inner_tol * tol, K, thr), "inner thresholds");
-    HIPCHK(pcg_init<float>(ctx->stream, a32, nullptr, 0, 0, 1, K, 0.0, maxit - inner_used, R64, thr),
-           "inner init");
+    HIPCHK(hipMemcpyAsync(X64b, X64, size_t(n) * K * 8, hipMemcpyDeviceToDevice, ctx->stream),
+           "backup");
+    if (fp64_inner) {
+      HIPCHK(pcg_init<double>(ctx->stream, a64, nullptr, 0, 0, 1, K, 0.0, maxit - inner_used, R64, thr),
+             "inner init");
+    } else {
+      // An FP32 sweep gets at most a quarter of the budget (at least 64), so a
+      // stall leaves iterations for the FP64 sweeps.
+      const int32_t cap = std::min(maxit - inner_used, std::max<int32_t>(64, maxit / 4));
+      HIPCHK(pcg_init<float>(ctx->stream, a32, nullptr, 0, 0, 1, K, 0.0, cap, R64, thr), "inner init");
+    }
     return MFS_OK;
+  }
+
+  // One PCG iteration of the running sweep (FP32, or FP64 after a stall).
+  hipError_t step(hipStream_t st, void (*fn)(void *, int, int), void *tctx) {
+    return fp64_inner ? pcg_iteration<double>(st, a64, fn, tctx) : pcg_iteration<float>(st, a32, fn, tctx);
+  }
+  // Runs the sweep's iterations to the end (polling the state).
+  int run_inner() {
+    return fp64_inner ? iterate<double>(ctx, a64, maxit - inner_used)
+                      : iterate<float>(ctx, a32, maxit - inner_used);
   }
 
   // After ctx->hstate holds the finished inner PCG's state.
   int end() {
-    HIPCHK(pcg_finish<float>(ctx->stream, a32, nullptr, 0, 0, K, X64), "inner finish");
+    if (fp64_inner)
+      HIPCHK(pcg_finish<double>(ctx->stream, a64, nullptr, 0, 0, K, X64), "inner finish");
+    else
+      HIPCHK(pcg_finish<float>(ctx->stream, a32, nullptr, 0, 0, K, X64), "inner finish");
     int32_t used = 0;
     for (int r = 0; r < kval; ++r) {
       iters[r] += ctx->hstate->iters[r];
@@ -1049,6 +1130,8 @@ struct MixedRun {
     }
     inner_used += std::max(used, 1);
     ++outer;
+    if (fp64_inner) ++fp64_sweeps;
+    swept = true;
     HIPCHK(residual(X64, res2), "residual");
     HIPCHK(hipMemcpyAsync(ctx->hnorm, nb2, 2 * kMaxRhs * 8, hipMemcpyDeviceToHost, ctx->stream), "copy");
     return MFS_OK;
@@ -1082,6 +1165,8 @@ int solve_mixed_impl(mfs_ctx *ctx, const float *M, const PointsOp *pts, int64_t 
   phase_begin(ctx);
   set_pass_work(ctx, pts, m, n);
   int outer_total = 0;
+  ctx->stats.fp64_sweeps = 0;
+  ctx->stats.reverted_columns = 0;
   for (int64_t col0 = 0; col0 < k; col0 += kMaxRhs) {
     const int kval = int(std::min<int64_t>(kMaxRhs, k - col0));
     MixedRun run;
@@ -1089,13 +1174,18 @@ int solve_mixed_impl(mfs_ctx *ctx, const float *M, const PointsOp *pts, int64_t 
     if (rc != MFS_OK) return rc;
     for (;;) {
       HIPCHK(hipStreamSynchronize(ctx->stream), "sync");
-      if (!run.decide()) break;
+      if (!run.decide()) {
+        if (run.fault != MFS_OK) return run.fault;
+        break;
+      }
       if ((rc = run.start()) != MFS_OK) return rc;
-      if ((rc = iterate<float>(ctx, run.a32, maxit - run.inner_used)) != MFS_OK) return rc;
+      if ((rc = run.run_inner()) != MFS_OK) return rc;
       if ((rc = run.end()) != MFS_OK) return rc;
     }
     if ((rc = run.finish(X, Xd)) != MFS_OK) return rc;
     outer_total = std::max(outer_total, run.outer);
+    ctx->stats.fp64_sweeps = std::max(ctx->stats.fp64_sweeps, run.fp64_sweeps);
+    ctx->stats.reverted_columns += run.reverted;
   }
   ctx->stats.outer_iters = outer_total;
   ctx->stats.solve_ms = phase_end(ctx);
@@ -1227,6 +1317,19 @@ int mfs_plan_f64(mfs_ctx *ctx, const double *src_xyz_host, int64_t n, double rho
   return plan_impl(ctx, std::move(xyz), n, rho, lambda, max_super);
 }
 
+int mfs_order_maximin(const double *xyz, int64_t n, int32_t *P, int32_t *Pinv, double *scale) {
+  if (xyz == nullptr || n <= 0 || n > (int64_t(1) << 30)) return MFS_ERR_ARG;
+  std::vector<int32_t> p, pinv;
+  std::vector<double> sc;
+  const int rc = order_maximin(int32_t(n), xyz, p, pinv, sc);
+  if (rc == PLAN_ERR_DUPLICATE) return MFS_ERR_DUPLICATE;
+  if (rc != PLAN_OK) return MFS_ERR_ARG;
+  if (P != nullptr) std::memcpy(P, p.data(), size_t(n) * 4);
+  if (Pinv != nullptr) std::memcpy(Pinv, pinv.data(), size_t(n) * 4);
+  if (scale != nullptr) std::memcpy(scale, sc.data(), size_t(n) * 8);
+  return MFS_OK;
+}
+
 int mfs_plan_export(const mfs_ctx *ctx, int32_t *P, int32_t *Pinv, double *scale, int32_t *ptr,
                     int32_t *ind, int32_t *sup_ptr, int32_t *sup_ind, int32_t *sup_parent,
                     int32_t *sptr, int32_t *sind) {
@@ -1262,13 +1365,17 @@ int mfs_factor_export(const mfs_ctx *ctx_c, double *val) {
   return MFS_OK;
 }
 
-int mfs_solve_dense(mfs_ctx *ctx, const float *M, int64_t m, int64_t n, const float *B, int64_t k,
-                    int rhs_space, float *X, int32_t maxit, double tol) {
-  if (ctx == nullptr) return MFS_ERR_ARG;
+}  // extern "C"
+
+namespace {
+
+// The plain form's column loop; the strengths go to X (FP32) or Xd (FP64).
+int solve_dense_impl(mfs_ctx *ctx, const float *M, int64_t m, int64_t n, const float *B, int64_t k,
+                     int rhs_space, float *X, double *Xd, int32_t maxit, double tol) {
   int rc = check_dense_args(ctx, M, m, n);
   if (rc != MFS_OK) return rc;
   if (!ctx->has_factor) return ctx->fail(MFS_ERR_STATE, "no factor: call mfs_factor_dense first");
-  if (B == nullptr || X == nullptr || k <= 0 || (rhs_space != 0 && rhs_space != 1))
+  if (B == nullptr || (X == nullptr && Xd == nullptr) || k <= 0 || (rhs_space != 0 && rhs_space != 1))
     return ctx->fail(MFS_ERR_ARG, "solve: bad right-hand side arguments");
   HIPCHK(hipSetDevice(ctx->device), "hipSetDevice");
   SolveArgs<double> a{};
@@ -1280,13 +1387,34 @@ int mfs_solve_dense(mfs_ctx *ctx, const float *M, int64_t m, int64_t n, const fl
     HIPCHK(pcg_init<double>(ctx->stream, a, B, k, c, rhs_space, 1, tol, maxit, nullptr), "pcg init");
     rc = iterate<double>(ctx, a, maxit);
     if (rc != MFS_OK) return rc;
-    HIPCHK(pcg_finish<double>(ctx->stream, a, X, k, c, 1, nullptr), "pcg finish");
+    if (Xd != nullptr)
+      HIPCHK(pcg_finish64(ctx->stream, a, Xd, k, c, 1), "pcg finish");
+    else
+      HIPCHK(pcg_finish<double>(ctx->stream, a, X, k, c, 1, nullptr), "pcg finish");
     record_batch_stats(ctx, c, 1);
   }
   ctx->stats.outer_iters = 0;
   ctx->stats.solve_ms = phase_end(ctx);
   track_fold(ctx);
   return MFS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mfs_solve_dense(mfs_ctx *ctx, const float *M, int64_t m, int64_t n, const float *B, int64_t k,
+                    int rhs_space, float *X, int32_t maxit, double tol) {
+  if (ctx == nullptr) return MFS_ERR_ARG;
+  if (X == nullptr) return ctx->fail(MFS_ERR_ARG, "solve: X is null");
+  return solve_dense_impl(ctx, M, m, n, B, k, rhs_space, X, nullptr, maxit, tol);
+}
+
+int mfs_solve_dense64(mfs_ctx *ctx, const float *M, int64_t m, int64_t n, const float *B, int64_t k,
+                      int rhs_space, double *X, int32_t maxit, double tol) {
+  if (ctx == nullptr) return MFS_ERR_ARG;
+  if (X == nullptr) return ctx->fail(MFS_ERR_ARG, "solve: X is null");
+  return solve_dense_impl(ctx, M, m, n, B, k, rhs_space, nullptr, X, maxit, tol);
 }
 
 int mfs_factor_dense(mfs_ctx *ctx, const float *M, int64_t m, int64_t n, int precision) {
@@ -1997,7 +2125,7 @@ int mfs_batch_solve_points(mfs_batch *b, const float *const *B, const int64_t *k
     if (fork_scenes(b) != MFS_OK) return MFS_ERR_HIP;
     for (size_t i : set)
       for (int32_t c = 0; c < chunk; ++c)
-        BHIPCHK(pcg_iteration<float>(b->streams[i], runs[i].a32, nullptr, nullptr), "pcg iteration");
+        BHIPCHK(runs[i].step(b->streams[i], nullptr, nullptr), "pcg iteration");
     if (join_scenes(b) != MFS_OK) return MFS_ERR_HIP;
     k_gather_active<<<unsigned((count + kThreads - 1) / kThreads), kThreads, 0, b->root>>>(
         b->dstates.as<PcgState *>(), int32_t(count), b->dactive.as<int32_t>());
@@ -2011,11 +2139,15 @@ int mfs_batch_solve_points(mfs_batch *b, const float *const *B, const int64_t *k
   for (;;) {
     for (size_t i = 0; i < count; ++i) BHIPCHK(hipStreamSynchronize(b->streams[i]), "sync");
     std::vector<size_t> active;
-    for (size_t i = 0; i < count; ++i)
+    for (size_t i = 0; i < count; ++i) {
       if (runs[i].decide()) {
         if ((rc = runs[i].start()) != MFS_OK) break;
         active.push_back(i);
+      } else if (runs[i].fault != MFS_OK) {
+        rc = scene_error(b, i, runs[i].fault);
+        break;
       }
+    }
     if (rc != MFS_OK || active.empty()) break;
     ++sweeps;
     int32_t budget = 0;
@@ -2053,6 +2185,8 @@ int mfs_batch_solve_points(mfs_batch *b, const float *const *B, const int64_t *k
   for (size_t i = 0; i < count; ++i) {
     if ((rc = runs[i].finish(X[i], nullptr)) != MFS_OK) return scene_error(b, i, rc);
     b->scenes[i]->stats.outer_iters = runs[i].outer;
+    b->scenes[i]->stats.fp64_sweeps = runs[i].fp64_sweeps;
+    b->scenes[i]->stats.reverted_columns = runs[i].reverted;
   }
   for (size_t i = 0; i < count; ++i) BHIPCHK(hipStreamSynchronize(b->streams[i]), "sync");
   b->stats.solve_ms = ms_since(t0);

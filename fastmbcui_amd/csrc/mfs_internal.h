@@ -181,6 +181,9 @@ hipError_t pcg_iteration(hipStream_t st, const SolveArgs<TA> &a, void (*track)(v
 template <typename TA>
 hipError_t pcg_finish(hipStream_t st, const SolveArgs<TA> &a, float *X, int64_t ldx, int64_t col0,
                       int k_valid, double *X64);
+// The FP64 iterate (new order) to X (n x ldx, original order, columns col0 ..).
+hipError_t pcg_finish64(hipStream_t st, const SolveArgs<double> &a, double *X, int64_t ldx,
+                        int64_t col0, int k_valid);
 
 // FP64 normal-equation residual R = M^T (B - M X) in original order, plus
 // per-column ||R||^2 into out2[0..K) and (if with_rhs) ||M^T B||^2 into rhs2.

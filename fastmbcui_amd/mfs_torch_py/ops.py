@@ -66,7 +66,9 @@ def _solve(M, B, Vs_s, rho, pcg_max_iter, tol_r, timing_verbose, solve_verbose, 
     out = torch.empty((n, k), dtype=torch.float32, device=M.device)
     if k > 0:
         if mode == "mixed":
-            ctx.solve_mixed(Mc, Bc, maxit=max(int(pcg_max_iter), 1) * 4, tol=float(tol_r), out=out,
+            # pcg_max_iter caps the inner iterations of all refinement sweeps
+            # together, as it caps each column's PCG in the plain form
+            ctx.solve_mixed(Mc, Bc, maxit=int(pcg_max_iter), tol=float(tol_r), out=out,
                             rhs_space=_lib.MFS_RHS_BOUNDARY if forward else _lib.MFS_RHS_SOURCE)
         else:
             ctx.solve_dense(Mc, Bc, _lib.MFS_RHS_BOUNDARY if forward else _lib.MFS_RHS_SOURCE,

@@ -9,11 +9,13 @@ keys they were built from:
 
 * plan key   = hash of the source points (float32 bytes) and rho;
 * factor key = plan key + identity of M (address, shape, strides, in-place
-  version counter) + precision.
+  version counter) + a fingerprint of M's contents + precision.
 
-A call whose keys match reuses the plan and/or the factor -- same inputs, same
-result -- and any mismatch rebuilds, so a stale factor is never applied to a
-different geometry.
+The fingerprint is a weighted FP64 sum of 65,536 entries of M spread evenly
+over the whole matrix (one small gather on the device): a new M that torch's
+caching allocator places at a recycled address, with a fresh version counter,
+still differs in its sampled values.  A call whose keys match reuses the plan
+and/or the factor -- same inputs, same result -- and any mismatch rebuilds.
 """
 from __future__ import annotations
 
@@ -88,6 +90,22 @@ def geometry_key(Vs_np: np.ndarray, rho: float) -> bytes:
     return h.digest()
 
 
+_FP_SAMPLES = 65536
+
+
+def content_fingerprint(M) -> float:
+    """Weighted FP64 sum of up to 65,536 evenly spaced entries of M (synchronises)."""
+    import torch
+    flat = M.reshape(-1)
+    count = flat.numel()
+    if count == 0:
+        return 0.0
+    take = min(count, _FP_SAMPLES)
+    idx = torch.linspace(0, count - 1, take, device=M.device, dtype=torch.float64).round().to(torch.int64)
+    w = torch.arange(1, take + 1, device=M.device, dtype=torch.float64).mul_(0.6180339887498949).frac_()
+    return float((flat[idx].to(torch.float64) * (w + 0.5)).sum().item())
+
+
 def _slot(device):
     import torch
     idx = device.index if device.index is not None else torch.cuda.current_device()
@@ -121,7 +139,8 @@ def prepared(M, Vs_cpu, rho, precision):
         info.plan_ms = s.ctx.stats().plan_ms
     else:
         info.plan_reused = True
-    fk = (gk, M.data_ptr(), tuple(M.shape), tuple(M.stride()), M._version, precision)
+    fk = (gk, M.data_ptr(), tuple(M.shape), tuple(M.stride()), M._version, content_fingerprint(M),
+          precision)
     if s.factor_key != fk:
         s.factor_key = None
         s.ctx.factor_dense(M, precision)

@@ -57,6 +57,8 @@ typedef struct mfs_stats {
     double plan_order_ms;       /* host maximin ordering */
     double plan_pattern_ms;     /* neighbour grids + device pattern search, to host */
     int32_t plan_grid_columns;  /* columns found by the grid search (rest: scan) */
+    int32_t fp64_sweeps;        /* mixed solves: sweeps run in FP64 after FP32 sweeps stalled */
+    int32_t reverted_columns;   /* mixed solves: columns restored after a sweep made them worse */
     int32_t pad1;
 } mfs_stats;
 
@@ -78,6 +80,13 @@ int mfs_plan(mfs_ctx *ctx, const float *src_xyz_host, int64_t n, double rho,
 int mfs_plan_f64(mfs_ctx *ctx, const double *src_xyz_host, int64_t n, double rho,
                  double lambda, int32_t max_super);
 
+/* The plan's maximin ordering alone, on the host (no device work): xyz is
+ * n x 3 row-major FP64; P[orig] = new, Pinv[new] = orig, scale[new] ascending
+ * (any output may be NULL).  Replaces fps_sampler::fast_fps
+ * (src/preprocess.h:782-831, create_ordering_3d src/maxmin.h:588-665).
+ * MFS_ERR_DUPLICATE for coincident points. */
+int mfs_order_maximin(const double *xyz, int64_t n, int32_t *P, int32_t *Pinv, double *scale);
+
 /* Host copies of the plan (any pointer may be NULL).  Sizes: P, Pinv, scale,
  * sup_parent: n; ptr, sptr: n+1; ind: nnz_pattern; sind: nnz_factor;
  * sup_ptr: n_super+1; sup_ind: n. */
@@ -98,6 +107,12 @@ int mfs_factor_export(const mfs_ctx *ctx, double *val);
 int mfs_solve_dense(mfs_ctx *ctx, const float *M, int64_t m, int64_t n,
                     const float *B, int64_t k, int rhs_space, float *X,
                     int32_t maxit, double tol);
+/* mfs_solve_dense with the strengths written in FP64 (X: n x k row-major
+ * double, original order): the plain form's own FP64 iterate, without the
+ * FP32 rounding of the op's output. */
+int mfs_solve_dense64(mfs_ctx *ctx, const float *M, int64_t m, int64_t n,
+                      const float *B, int64_t k, int rhs_space, double *X,
+                      int32_t maxit, double tol);
 
 /* Chosen (mixed) form: all k right-hand sides batched through one FP32 PCG
  * whose matvec reads M once per pass for the whole batch, wrapped in FP64

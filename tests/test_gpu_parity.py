@@ -409,6 +409,44 @@ def test_op_reuses_plan_and_factor_for_identical_inputs():
     assert not mfs.last_solve_info().factor_reused
 
 
+def test_op_factor_cache_sees_new_contents_at_a_recycled_address():
+    """A new M of the same shape, placed by torch's caching allocator where a
+    freed M was (fresh version counter), must not reuse the old factor."""
+    import fastmbcui_amd.mfs_torch_py as mfs
+    runtime.reset()
+    sc = _small(k=1)
+    M32 = _M32(sc)
+    F = torch.from_numpy(sc.b.astype(np.float32)).to(DEV)
+    Vs = torch.from_numpy(sc.y.astype(np.float32))
+    M1 = torch.from_numpy(M32).to(DEV)
+    addr = M1.data_ptr()
+    mfs.ops.MinvF_float_fastGrad(M1, F, Vs, 3.0, 75, 1e-6)
+    del M1
+    M2 = torch.from_numpy(M32 * np.float32(1.5)).to(DEV)
+    print(f"recycled address: {M2.data_ptr() == addr}", flush=True)
+    out = mfs.ops.MinvF_float_fastGrad(M2, F, Vs, 3.0, 500, 1e-10)
+    assert not mfs.last_solve_info().factor_reused
+    r = oracle.minvf((M32 * np.float32(1.5)).astype(np.float64), sc.b.astype(np.float32),
+                     sc.y, 3.0, 500, 1e-10)
+    assert np.linalg.norm(out.cpu().numpy() - r.x) / np.linalg.norm(r.x) < 1e-5
+
+
+def test_assemble_tall_matrix_beyond_the_grid_row_limit():
+    """m > 65,535 rows (more than one grid row per target would allow)."""
+    rng = np.random.default_rng(4)
+    m, n = 70001, 5
+    x = rng.uniform(-1, 1, (m, 3))
+    nx = rng.standard_normal((m, 3))
+    nx /= np.linalg.norm(nx, axis=1, keepdims=True)
+    y = rng.uniform(-1, 1, (n, 3)) * 0.3 + 3.0
+    ctx = _ctx()
+    M = ctx.assemble_dense(torch.from_numpy(x).to(DEV), torch.from_numpy(nx).to(DEV),
+                           torch.from_numpy(y).to(DEV)).cpu().numpy()
+    ref = scenes.kernel_matrix(x, nx, y)
+    # FP32 rounding per entry; an absolute floor for entries where (x - y).n cancels
+    assert np.all(np.abs(M - ref) <= 1e-6 * np.abs(ref) + 1e-12 * np.abs(ref).max())
+
+
 def test_op_rejects_coincident_sources():
     import fastmbcui_amd.mfs_torch_py as mfs
     runtime.reset()
