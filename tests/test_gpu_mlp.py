@@ -128,4 +128,4 @@ def test_trained_network_guess_is_judged_by_the_residual():
         print(f"held-out step {step}: zero start {zero} iterations, learned {learned} (guess residual "
               f"{ctx.points_residual(B, g).max():.2e}), previous step {pv}; residual {res:.2e}", flush=True)
         assert res <= 1e-6
-        assert learned <= zero + 3
+        assert learned <= zero + 1
