@@ -1,4 +1,5 @@
-"""Builds libmfs_core.so (HIP for gfx950 + host plan code) in-tree.
+"""Builds libmfs_core.so (HIP for gfx950 + host plan code) in-tree, and the
+torch operator library libmfs_torch.so over it.
 
     python -m fastmbcui_amd.build
 
@@ -38,6 +39,32 @@ def _run(cmd):
     return r
 
 
+ROOT = os.path.dirname(HERE)
+# The reference's op package loads ./build/mfs_torch/csrc/libmfs_torch.so
+# relative to the working directory (core/mfs_torch_py/__init__.py:7-8).
+SHIM = os.path.join(ROOT, "build", "mfs_torch", "csrc", "libmfs_torch.so")
+
+
+def build_torch_shim(verbose: bool = False) -> str:
+    """libmfs_torch.so: the mfs_torch operator library (host C++, torch headers)
+    over libmfs_core.so, at the path the reference's op package loads."""
+    import torch
+    tdir = os.path.dirname(torch.__file__)
+    abi = int(bool(torch._C._GLIBCXX_USE_CXX11_ABI))
+    os.makedirs(os.path.dirname(SHIM), exist_ok=True)
+    tmp = SHIM + ".tmp"
+    r = _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-D__HIP_PLATFORM_AMD__=1", "-DUSE_ROCM=1",
+              f"-D_GLIBCXX_USE_CXX11_ABI={abi}", f"-I{tdir}/include",
+              f"-I{tdir}/include/torch/csrc/api/include", "-I/opt/rocm/include",
+              os.path.join(CSRC, "torch_shim.cpp"), "-o", tmp, f"-L{tdir}/lib", "-lc10", "-lc10_hip",
+              "-ltorch", "-ltorch_cpu", "-ltorch_hip", f"-L{LIB_DIR}", "-lmfs_core",
+              "-Wl,-rpath,$ORIGIN/../../../fastmbcui_amd/lib", f"-Wl,-rpath,{tdir}/lib"])
+    if verbose and (r.stdout or r.stderr):
+        sys.stderr.write(r.stdout + r.stderr)
+    os.replace(tmp, SHIM)
+    return SHIM
+
+
 def build(verbose: bool = False) -> str:
     os.makedirs(OBJ, exist_ok=True)
     os.makedirs(LIB_DIR, exist_ok=True)
@@ -56,6 +83,7 @@ def build(verbose: bool = False) -> str:
     tmp = LIB + ".tmp"
     _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp, *objs])
     os.replace(tmp, LIB)
+    build_torch_shim(verbose)
     return LIB
 
 

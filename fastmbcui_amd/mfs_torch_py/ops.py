@@ -9,6 +9,8 @@ current stream.
 """
 from __future__ import annotations
 
+import time
+
 import torch
 from torch import Tensor
 
@@ -46,22 +48,36 @@ def _check_inputs(M, B, Vs_s, name_b, forward):
     torch._check(Vs_s.device.type == "cpu", lambda: "Vs_s must be a CPU tensor")
 
 
-def _report(info, timing_verbose, solve_verbose, ctx_stats):
+def _report(info, timing_verbose, solve_verbose, total_ms):
     if timing_verbose:
-        print(f"TIME(plan): {0.0 if info.plan_reused else info.plan_ms} ms")
-        print(f"TIME(compute): {0.0 if info.factor_reused else info.factor_ms} ms")
-        print(f"TIME(pcg): {info.solve_ms} ms")
+        # the reference's phases (mfs_torch_ops.cu:48-193); permutation, the
+        # covariance copy, the cache copy and the back-permutation are fused
+        # into the kernels here and take no time of their own, and a reused
+        # plan or factor costs nothing
+        plan = 0.0 if info.plan_reused else info.plan_ms
+        order = 0.0 if info.plan_reused else info.plan_order_ms
+        pattern = 0.0 if info.plan_reused else info.plan_pattern_ms
+        phases = [("prep", info.prep_ms), ("fps_sampler", order), ("permutation", 0.0),
+                  ("build_cov_mat", 0.0), ("pattern", pattern),
+                  ("setup", max(plan - order - pattern, 0.0)),
+                  ("compute", 0.0 if info.factor_reused else info.factor_ms), ("cache", 0.0),
+                  ("pcg", info.solve_ms), ("perm_back", 0.0), ("total", total_ms)]
+        for name, ms in phases:
+            print(f"TIME({name}): {ms} ms")
     if solve_verbose:
         for c, (it, rr) in enumerate(zip(info.iters, info.relres)):
             print(f"pcg column {c}: iterations {it}, relative residual {rr:.3e}")
 
 
 def _solve(M, B, Vs_s, rho, pcg_max_iter, tol_r, timing_verbose, solve_verbose, forward):
+    t0 = time.perf_counter()
     Mc = M.contiguous()
     Bc = B.contiguous()
     mode = runtime.solver_mode()
     prec = _lib.MFS_FP32 if mode == "mixed" else _lib.MFS_FP64
+    prep_ms = (time.perf_counter() - t0) * 1e3
     ctx, info = runtime.prepared(Mc, Vs_s, rho, prec)
+    info.prep_ms = prep_ms
     n, k = Mc.shape[1], Bc.shape[1]
     out = torch.empty((n, k), dtype=torch.float32, device=M.device)
     if k > 0:
@@ -79,7 +95,9 @@ def _solve(M, B, Vs_s, rho, pcg_max_iter, tol_r, timing_verbose, solve_verbose, 
         info.solve_ms = s.solve_ms
     info.mode = mode
     runtime.last_info = info
-    _report(info, timing_verbose, solve_verbose, None)
+    if timing_verbose:
+        torch.cuda.synchronize(M.device)
+    _report(info, timing_verbose, solve_verbose, (time.perf_counter() - t0) * 1e3)
     return out
 
 

@@ -36,7 +36,10 @@ class SolveInfo:
     """What the reference computes and drops (mfs_torch_ops.cu:169-170)."""
     iters: list = field(default_factory=list)
     relres: list = field(default_factory=list)
+    prep_ms: float = 0.0
     plan_ms: float = 0.0
+    plan_order_ms: float = 0.0
+    plan_pattern_ms: float = 0.0
     factor_ms: float = 0.0
     solve_ms: float = 0.0
     plan_reused: bool = False
@@ -136,7 +139,10 @@ def prepared(M, Vs_cpu, rho, precision):
         s.plan_key = None
         s.ctx.plan(Vs_np, rho, LAMBDA, MAX_SUPER)
         s.plan_key = gk
-        info.plan_ms = s.ctx.stats().plan_ms
+        st = s.ctx.stats()
+        info.plan_ms = st.plan_ms
+        info.plan_order_ms = st.plan_order_ms
+        info.plan_pattern_ms = st.plan_pattern_ms
     else:
         info.plan_reused = True
     fk = (gk, M.data_ptr(), tuple(M.shape), tuple(M.stride()), M._version, content_fingerprint(M),

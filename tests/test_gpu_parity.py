@@ -447,6 +447,23 @@ def test_assemble_tall_matrix_beyond_the_grid_row_limit():
     assert np.all(np.abs(M - ref) <= 1e-6 * np.abs(ref) + 1e-12 * np.abs(ref).max())
 
 
+def test_op_prints_the_reference_phase_timers(capsys):
+    """timing_verbose prints the reference's eleven TIME(...) phases
+    (mfs_torch_ops.cu:48-193), in its order."""
+    import fastmbcui_amd.mfs_torch_py as mfs
+    runtime.reset()
+    sc = _small(k=1)
+    M = torch.from_numpy(_M32(sc)).to(DEV)
+    F = torch.from_numpy(sc.b.astype(np.float32)).to(DEV)
+    Vs = torch.from_numpy(sc.y.astype(np.float32))
+    mfs.ops.MinvF_float_fastGrad(M, F, Vs, 3.0, 75, 1e-6, True, True)
+    out = capsys.readouterr().out.splitlines()
+    phases = [ln.split("(")[1].split(")")[0] for ln in out if ln.startswith("TIME(")]
+    assert phases == ["prep", "fps_sampler", "permutation", "build_cov_mat", "pattern", "setup",
+                      "compute", "cache", "pcg", "perm_back", "total"]
+    assert any(ln.startswith("pcg column 0: iterations") for ln in out)
+
+
 def test_op_rejects_coincident_sources():
     import fastmbcui_amd.mfs_torch_py as mfs
     runtime.reset()
