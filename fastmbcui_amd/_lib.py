@@ -631,9 +631,10 @@ class Batch:
                                                   _ptr_array(ys), int(precision)), "mfs_batch_factor_points")
         self.m = [int(v) for v in m]
 
-    def solve_points(self, Bs, maxit=1000, tol=1e-6, inner_tol=0.5, chunk=4, use_graph=False):
+    def solve_points(self, Bs, maxit=1000, tol=1e-6, inner_tol=0.5, chunk=4, use_graph=0):
         """Bs: list of (m_i, k_i) float32 device tensors, k_i <= 16.  Returns the X_i.
-        use_graph=True (graph replay) is switched off in the library: MfsError."""
+        use_graph: 0 direct launches, 2 per-scene graphs of `chunk` iterations;
+        1 (True; one multi-stream graph) is switched off in the library: MfsError."""
         import torch
         Bs = [_as_2d_f32(b) for b in Bs]
         Xs = [torch.empty((self.n[i], b.shape[1]), dtype=torch.float32, device=b.device)
@@ -642,7 +643,7 @@ class Batch:
         self._sync()
         self._check(lib().mfs_batch_solve_points(self._h, _ptr_array(Bs), k.ctypes.data, _ptr_array(Xs),
                                                  int(maxit), float(tol), float(inner_tol), int(chunk),
-                                                 int(bool(use_graph))), "mfs_batch_solve_points")
+                                                 int(use_graph)), "mfs_batch_solve_points")
         return Xs
 
     def stats(self):

@@ -60,12 +60,12 @@ def tune_points_blocks(ctx: _lib.Context, k: int = 8, candidates=(1024, 2048, 40
     return _cache[key]
 
 
-def tune_batch_chunk(batch: _lib.Batch, Bs, candidates=(4, 8), graphs=(False,), maxit=1000,
+def tune_batch_chunk(batch: _lib.Batch, Bs, candidates=(4, 8), graphs=(0, 2), maxit=1000,
                      tol=1e-6) -> dict:
-    """Solve the batch once per chunk length -- whole solves, since the chunk
-    trades polls against launches over the whole convergence history; keep the
-    fastest.  `graphs` lists the launch modes tried (False: direct launches;
-    graph replay is switched off in the library)."""
+    """Solve the batch once per (chunk length, launch mode) -- whole solves,
+    since the chunk trades polls against launches over the whole convergence
+    history; keep the fastest.  `graphs` lists the launch modes tried (0:
+    direct launches; 2: per-scene graphs of `chunk` iterations)."""
     import torch
     key = ("batch_chunk", tuple(batch.n), tuple(int(b.shape[1]) for b in Bs))
     if key in _cache:
@@ -78,6 +78,6 @@ def tune_batch_chunk(batch: _lib.Batch, Bs, candidates=(4, 8), graphs=(False,), 
             batch.solve_points(Bs, maxit=maxit, tol=tol, chunk=c, use_graph=g)
             times[(c, g)] = (time.perf_counter() - t0) * 1e3
     best = min(times, key=times.get)
-    _cache[key] = {"best": best, "ms": {f"chunk {c}, {'graph' if g else 'direct'}": t
+    _cache[key] = {"best": best, "ms": {f"chunk {c}, {'graphs' if g else 'direct'}": t
                                         for (c, g), t in times.items()}}
     return _cache[key]

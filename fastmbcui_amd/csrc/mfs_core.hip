@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <atomic>
 #include <chrono>
 #include <cmath>
@@ -1099,9 +1100,11 @@ struct MixedRun {
       HIPCHK(pcg_init<double>(ctx->stream, a64, nullptr, 0, 0, 1, K, 0.0, maxit - inner_used, R64, thr),
              "inner init");
     } else {
-      // An FP32 sweep gets at most a quarter of the budget (at least 64), so a
-      // stall leaves iterations for the FP64 sweeps.
-      const int32_t cap = std::min(maxit - inner_used, std::max<int32_t>(64, maxit / 4));
+      // An FP32 sweep leaves a quarter of the remaining budget (it gets at
+      // least 64), so a stall still leaves iterations for FP64 sweeps; a
+      // sweep cut short restarts the Krylov space, so the cap is kept high.
+      const int32_t left = maxit - inner_used;
+      const int32_t cap = std::min(left, std::max<int32_t>(64, left - left / 4));
       HIPCHK(pcg_init<float>(ctx->stream, a32, nullptr, 0, 0, 1, K, 0.0, cap, R64, thr), "inner init");
     }
     return MFS_OK;
@@ -2089,12 +2092,17 @@ int mfs_batch_solve_points(mfs_batch *b, const float *const *B, const int64_t *k
                            int32_t maxit, double tol, double inner_tol, int32_t chunk,
                            int use_graph) {
   if (b == nullptr || B == nullptr || k == nullptr || X == nullptr || chunk <= 0) return MFS_ERR_ARG;
-  // Replaying the iterations from a captured hipGraph (re-captured as scenes
-  // converged) is switched off: a full test run once ended in a host
-  // segmentation fault inside this call with graphs on, with no cause found
-  // in this code.  Direct launches are the same kernels in the same order.
-  if (use_graph != 0)
-    return b->fail(MFS_ERR_ARG, "batch solve: graph replay is switched off; pass use_graph = 0");
+  // use_graph 1 -- one multi-stream capture of all scenes' rounds, forked and
+  // joined with events and re-captured as scenes converged -- is switched
+  // off: a full test run once ended in a host segmentation fault inside this
+  // call in that mode, with no cause found in this code.  use_graph 2 replays
+  // per-scene graphs: each scene's `chunk` iterations are captured from its
+  // own stream alone (no event is recorded or waited on inside a capture),
+  // instantiated once per sweep precision, launched on that stream every
+  // round and destroyed only after every stream has been synchronized.
+  if (use_graph == 1)
+    return b->fail(MFS_ERR_ARG, "batch solve: graph replay is switched off; pass use_graph = 0 or 2");
+  if (use_graph != 0 && use_graph != 2) return b->fail(MFS_ERR_ARG, "batch solve: use_graph is 0 or 2");
   if (!(inner_tol > 0) || !(inner_tol <= 1)) inner_tol = 0.5;
   BHIPCHK(hipSetDevice(b->device), "hipSetDevice");
   BHIPCHK(hipStreamSynchronize(b->caller), "sync caller");
@@ -2119,13 +2127,54 @@ int mfs_batch_solve_points(mfs_batch *b, const float *const *B, const int64_t *k
   BHIPCHK(hipMemcpyAsync(b->dstates.p, b->states.data(), count * sizeof(PcgState *),
                          hipMemcpyHostToDevice, b->root),
           "upload");
-  // Iterations: `chunk` per scene on the scene's stream, for the scenes in
-  // `set`, then a gather of every scene's active count on root.
+  // Per-scene graphs of `chunk` iterations (use_graph 2): [scene][precision],
+  // precision 0 for the FP32 sweeps, 1 for the FP64 ones.  Kernel arguments
+  // are fixed for the whole call (device state carries the iteration).
+  std::vector<std::array<hipGraphExec_t, 2>> execs(count, {nullptr, nullptr});
+  auto destroy_graphs = [&]() {
+    for (size_t i = 0; i < count; ++i) (void)hipStreamSynchronize(b->streams[i]);
+    for (auto &pair : execs)
+      for (hipGraphExec_t &e : pair)
+        if (e != nullptr) {
+          (void)hipGraphExecDestroy(e);
+          e = nullptr;
+        }
+  };
+  auto scene_graph = [&](size_t i, hipGraphExec_t *out) -> int {
+    hipStream_t s = b->streams[i];
+    BHIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "begin capture");
+    hipError_t e = hipSuccess;
+    for (int32_t c = 0; c < chunk && e == hipSuccess; ++c) e = runs[i].step(s, nullptr, nullptr);
+    hipGraph_t g = nullptr;
+    const hipError_t ec = hipStreamEndCapture(s, &g);
+    if (e == hipSuccess) e = ec;
+    size_t nodes = 0;
+    if (e == hipSuccess) e = hipGraphGetNodes(g, nullptr, &nodes);
+    if (e == hipSuccess) e = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
+    if (g != nullptr) (void)hipGraphDestroy(g);
+    BHIPCHK(e, "capture scene iterations");
+    ++b->stats.graph_captures;
+    b->stats.graph_nodes = std::max<int32_t>(b->stats.graph_nodes, int32_t(nodes));
+    return MFS_OK;
+  };
+  // Iterations: `chunk` per scene on the scene's stream (direct launches or
+  // the scene's graph), for the scenes in `set`, then a gather of every
+  // scene's active count on root.
   auto launch_chunk = [&](const std::vector<size_t> &set) -> int {
     if (fork_scenes(b) != MFS_OK) return MFS_ERR_HIP;
-    for (size_t i : set)
-      for (int32_t c = 0; c < chunk; ++c)
-        BHIPCHK(runs[i].step(b->streams[i], nullptr, nullptr), "pcg iteration");
+    for (size_t i : set) {
+      if (use_graph == 2) {
+        hipGraphExec_t &ex = execs[i][runs[i].fp64_inner ? 1 : 0];
+        if (ex == nullptr) {
+          const int rc = scene_graph(i, &ex);
+          if (rc != MFS_OK) return rc;
+        }
+        BHIPCHK(hipGraphLaunch(ex, b->streams[i]), "graph launch");
+      } else {
+        for (int32_t c = 0; c < chunk; ++c)
+          BHIPCHK(runs[i].step(b->streams[i], nullptr, nullptr), "pcg iteration");
+      }
+    }
     if (join_scenes(b) != MFS_OK) return MFS_ERR_HIP;
     k_gather_active<<<unsigned((count + kThreads - 1) / kThreads), kThreads, 0, b->root>>>(
         b->dstates.as<PcgState *>(), int32_t(count), b->dactive.as<int32_t>());
@@ -2181,6 +2230,7 @@ int mfs_batch_solve_points(mfs_batch *b, const float *const *B, const int64_t *k
     }
     if (rc != MFS_OK) break;
   }
+  destroy_graphs();
   if (rc != MFS_OK) return rc;
   for (size_t i = 0; i < count; ++i) {
     if ((rc = runs[i].finish(X[i], nullptr)) != MFS_OK) return scene_error(b, i, rc);

@@ -278,10 +278,10 @@ typedef struct mfs_batch_stats {
     double plan_ms, factor_ms, solve_ms;
     int32_t factor_launches;    /* merged bucket launches */
     int32_t factor_items;       /* supernodes of all scenes */
-    int32_t graph_nodes;        /* 0 (graph replay is switched off) */
+    int32_t graph_nodes;        /* nodes of the largest per-scene graph (use_graph 2) */
     int32_t graph_replays;      /* rounds of `chunk` iterations launched */
     int32_t sweeps;             /* refinement sweeps (max over scenes) */
-    int32_t graph_captures;     /* 0 (graph replay is switched off) */
+    int32_t graph_captures;     /* per-scene graphs captured (use_graph 2) */
 } mfs_batch_stats;
 int mfs_batch_create(mfs_batch **out, int device, void *stream, int32_t count);
 void mfs_batch_destroy(mfs_batch *b);
@@ -299,8 +299,11 @@ int mfs_batch_factor_points(mfs_batch *b, const double *const *x, const double *
 /* mfs_solve_points for every scene (B[i]: m[i] x k[i], X[i]: n[i] x k[i], FP32
  * device, k[i] <= 16), the PCG iterations of the running scenes launched in
  * rounds of `chunk` iterations per scene, each scene on its own stream.
- * use_graph must be 0: replaying the rounds from a captured hipGraph is
- * switched off (MFS_ERR_ARG otherwise). */
+ * use_graph 0: direct launches; 2: each scene's `chunk` iterations captured
+ * from its own stream into a graph (once per sweep precision) and replayed
+ * every round -- the same kernels in the same order, so the results are
+ * identical.  use_graph 1 (one multi-stream capture forked and joined with
+ * events) is switched off: MFS_ERR_ARG. */
 int mfs_batch_solve_points(mfs_batch *b, const float *const *B, const int64_t *k, float *const *X,
                            int32_t maxit, double tol, double inner_tol, int32_t chunk,
                            int use_graph);

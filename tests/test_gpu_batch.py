@@ -5,11 +5,15 @@ factor kernels (merged launches only regroup the blocks), the same PCG kernels
 (launched in rounds on the scenes' streams) -- so each scene's strengths must
 equal the single-scene result bit for bit.
 
-Replaying the rounds from a captured hipGraph is switched off in the library:
-one full GPU test run ended in a host segmentation fault inside
-mfs_batch_solve_points with graphs on (test_batch_solutions_meet_tolerance,
-graph mode), with no cause found in this code.  Its graph-mode cases are kept
-below, skipped; the library refuses use_graph = 1 (checked without a solve)."""
+Replaying the rounds from one multi-stream captured hipGraph (use_graph 1)
+is switched off in the library: one full GPU test run ended in a host
+segmentation fault inside mfs_batch_solve_points in that mode
+(test_batch_solutions_meet_tolerance, graph mode), with no cause found in this
+code.  Its cases are kept below, skipped; the library refuses use_graph = 1
+(checked without a solve).  use_graph 2 replays per-scene graphs (each
+scene's chunk of iterations captured from its own stream, no events inside a
+capture); it must give the direct launches' results bit for bit, over many
+replays with eager work and synchronisations between them."""
 import numpy as np
 import pytest
 import torch
@@ -86,3 +90,35 @@ def test_batch_rejects_unplanned_scene():
     with pytest.raises(_lib.MfsError) as e:
         b.factor_points([_dev(sc.x) for sc in scs], [_dev(sc.nx) for sc in scs], [_dev(sc.y) for sc in scs])
     assert e.value.code == _lib.MFS_ERR_STATE
+
+
+@pytest.mark.parametrize("chunk", [1, 3])
+def test_batch_per_scene_graphs_equal_direct_launches(chunk):
+    """Every round replays each running scene's graph once, with eager
+    kernels (fork, join, the active-count gather) and a host poll between
+    rounds; two solves in a row on the same batch."""
+    scs = _scenes(6)
+    b = _lib.Batch(len(scs))
+    b.plan([sc.y for sc in scs], 3.0)
+    b.factor_points([_dev(sc.x) for sc in scs], [_dev(sc.nx) for sc in scs], [_dev(sc.y) for sc in scs])
+    Bs = [_dev(sc.b.astype(np.float32)) for sc in scs]
+    Xd = b.solve_points(Bs, tol=1e-6, chunk=chunk, use_graph=0)
+    it_d = [b.scene_stats(i).iters[0] for i in range(len(scs))]
+    for rep in range(2):
+        Xg = b.solve_points(Bs, tol=1e-6, chunk=chunk, use_graph=2)
+        st = b.stats()
+        print(f"chunk {chunk} solve {rep}: {st.graph_captures} graphs of up to {st.graph_nodes} nodes, "
+              f"{st.graph_replays} rounds", flush=True)
+        assert st.graph_captures >= len(scs) and st.graph_nodes > 0
+        for i in range(len(scs)):
+            assert torch.equal(Xg[i], Xd[i]), (chunk, rep, i)
+            assert b.scene_stats(i).iters[0] == it_d[i]
+
+
+def test_batch_rejects_unknown_launch_mode():
+    scs = _scenes(2)
+    b = _lib.Batch(2)
+    b.plan([sc.y for sc in scs], 3.0)
+    with pytest.raises(_lib.MfsError) as e:
+        b.solve_points([_dev(sc.b.astype(np.float32)) for sc in scs], use_graph=3)
+    assert e.value.code == _lib.MFS_ERR_ARG
