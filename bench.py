@@ -46,6 +46,8 @@ def _parse():
     ap.add_argument("--one-pass-steps", type=int, default=1,
                     help="steps of the chosen form with the opt-in one-pass matvec, for comparison")
     ap.add_argument("--skip-plain", action="store_true")
+    ap.add_argument("--tune-factor", type=int, default=1,
+                    help="autotune the factor's launch shape before the timed steps (0: defaults)")
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--inner-tol", type=float, default=0.5)
@@ -702,6 +704,16 @@ def main():
     _log(f"bench: plan {plan_ms:.1f} ms, supernodes {st.n_super}, max local {st.max_local}, "
          f"nnz(L) {st.nnz_factor}")
 
+    # factor launch shape (tile-count buckets, transpose ranges, streams),
+    # tuned on this problem before the timed steps; the context is re-planned
+    # with the winner (the plan itself is not part of the metric)
+    from fastmbcui_amd import autotune
+    ftune = None
+    if args.tune_factor:
+        ftune = autotune.tune_factor(ctx, M, sc.y, 3.0, _lib.MFS_FP32)
+        _log(f"bench: factor tuning best {ftune['best']}: " +
+             ", ".join(f"{c}: {t:.2f} ms" for c, t in sorted(ftune["ms"].items(), key=lambda kv: kv[1])))
+
     # ---------------------------------------------------------------- chosen form
     def chosen():
         ctx.factor_dense(M, _lib.MFS_FP32)
@@ -903,6 +915,10 @@ def main():
         "plan_ms": round(plan_ms, 2),
         "factor_ms": round(float(np.mean(factor_ms)), 3),
         "factor_retried_supernodes": retried,
+        "factor_tuning": ({"best": {"tile_set": hex(ftune["best"][0]), "column_ranges": ftune["best"][1],
+                                    "streams": ftune["best"][2]},
+                           "ms": {f"{hex(c[0])}/{c[1]}/{c[2]}": round(t, 3) for c, t in ftune["ms"].items()}}
+                          if ftune else None),
         "pcg_inner_iters": iters,
         "refinement_sweeps": outer,
         "residual_max": float(res.max()),

@@ -8,7 +8,12 @@ hand rather than fixed:
   the other side): more splits fill the CUs and balance XCDs, fewer keep the
   partial sums small;
 * the batch's PCG chunk -- iterations per round: longer chunks cost fewer
-  polls, shorter ones launch fewer no-op kernels after a scene converged.
+  polls, shorter ones launch fewer no-op kernels after a scene converged;
+* the supernode factor's launch shape (MFS_PARAM_TILE_SET, _FACTOR_PHASES,
+  _FACTOR_STREAMS) -- which tile counts form buckets (a supernode of ns rows
+  runs in the smallest enabled 16 T >= ns: fewer buckets mean fewer, fuller
+  launches but more zero padding), how many column ranges of the dense
+  transpose the Gram overlaps, and on how many streams the buckets run.
 
 Each candidate is timed with HIP events on the work it controls; the fastest
 is set and returned.  Results are kept per shape key for the process.
@@ -80,4 +85,48 @@ def tune_batch_chunk(batch: _lib.Batch, Bs, candidates=(4, 8), graphs=(0, 2), ma
     best = min(times, key=times.get)
     _cache[key] = {"best": best, "ms": {f"chunk {c}, {'graphs' if g else 'direct'}": t
                                         for (c, g), t in times.items()}}
+    return _cache[key]
+
+
+# Tile sets: all counts; without 3, 6 and 12 (powers of two only); without 1
+# and 3 (small supernodes padded into 2 and 4 tiles).
+FACTOR_TILE_SETS = (0xff, 0xd5, 0x5f)
+
+
+def factor_candidates(tile_sets=FACTOR_TILE_SETS, phases=(2, 4, 8), streams=(2, 4)):
+    return [(t, p, s) for t in tile_sets for p in phases for s in streams]
+
+
+def tune_factor(ctx: _lib.Context, M, xyz, rho: float, precision=_lib.MFS_FP32, candidates=None,
+                reps: int = 2, points=None) -> dict:
+    """Time the numeric factor of M (dense form; or from `points` = (x, nx, y)
+    device tensors) for each (tile set, column ranges, streams) candidate --
+    re-planning for each, since the plan's buckets depend on the first two --
+    and keep the fastest (the context is left planned with it).  Returns
+    {"best": (tile_set, phases, streams), "ms": {candidate: factor ms}} with the
+    best of `reps` timed factorisations per candidate (the context's own HIP
+    event timing of the factor)."""
+    key = ("factor", tuple(M.shape) if M is not None else points[0].shape[0], ctx.n, float(rho), precision)
+    cands = candidates or factor_candidates()
+    times = {}
+    for (tset, ph, st) in cands:
+        ctx.set_param(_lib.MFS_PARAM_TILE_SET, tset)
+        ctx.set_param(_lib.MFS_PARAM_FACTOR_PHASES, ph)
+        ctx.set_param(_lib.MFS_PARAM_FACTOR_STREAMS, st)
+        ctx.plan(xyz, rho)
+        best = float("inf")
+        for r in range(reps + 1):   # the first factorisation warms buffers
+            if points is None:
+                ctx.factor_dense(M, precision)
+            else:
+                ctx.factor_points(*points, precision)
+            if r:
+                best = min(best, ctx.stats().factor_ms)
+        times[(tset, ph, st)] = best
+    win = min(times, key=times.get)
+    ctx.set_param(_lib.MFS_PARAM_TILE_SET, win[0])
+    ctx.set_param(_lib.MFS_PARAM_FACTOR_PHASES, win[1])
+    ctx.set_param(_lib.MFS_PARAM_FACTOR_STREAMS, win[2])
+    ctx.plan(xyz, rho)
+    _cache[key] = {"best": win, "ms": times}
     return _cache[key]

@@ -55,23 +55,32 @@ struct DevBuf {
   }
 };
 
-// The dense factor transposes M in kFactorPhases column ranges [phase_col(p),
-// phase_col(p + 1)) and starts the supernodes whose columns (original order)
-// all lie below phase_col(p + 1) once range p is done, so the transpose of
-// the later ranges overlaps the Gram of the earlier ones.  On surface scenes
-// a supernode's columns nearly always lie in one body, whose sources are
-// contiguous, so the Gram work splits about evenly over the phases.
-constexpr int kFactorPhases = 4;
-int64_t phase_col(int p, int64_t n) {
-  if (p >= kFactorPhases) return n;
-  return std::min<int64_t>(n, (int64_t(p) * n / kFactorPhases + 63) / 64 * 64);
+// The dense factor transposes M in `phases` column ranges [phase_col(p),
+// phase_col(p + 1)) (4 by default, at most kMaxPhases) and starts the
+// supernodes whose columns (original order) all lie below phase_col(p + 1)
+// once range p is done, so the transpose of the later ranges overlaps the
+// Gram of the earlier ones.  On surface scenes a supernode's columns nearly
+// always lie in one body, whose sources are contiguous, so the Gram work
+// splits about evenly over the phases.
+constexpr int kMaxPhases = 8;
+constexpr int kDefaultPhases = 4;
+int64_t phase_col(int p, int64_t n, int phases) {
+  if (p >= phases) return n;
+  return std::min<int64_t>(n, (int64_t(p) * n / phases + 63) / 64 * 64);
 }
 
 struct Bucket {
   int tiles;
   int32_t offset, count;
-  int32_t phase_off[kFactorPhases + 1];  // phase p: items offset + [phase_off[p], phase_off[p + 1])
+  int32_t phase_off[kMaxPhases + 1];  // phase p: items offset + [phase_off[p], phase_off[p + 1])
 };
+
+// Tile counts of the factor buckets, launch order (largest first); bit i of
+// a tile-set mask enables kTileSizes[i].  A supernode of local size ns takes
+// the smallest enabled count T with 16 T >= ns (zero-padded rows); 16 is
+// always enabled, so every supernode up to kMaxLocal rows has one.
+constexpr int kTileSizes[] = {16, 12, 8, 6, 4, 3, 2, 1};
+constexpr uint32_t kAllTiles = 0xffu;
 
 }  // namespace
 
@@ -160,7 +169,14 @@ struct mfs_ctx {
   // side, forked from and joined back into `stream` with events.
   static constexpr int kSide = 4;
   hipStream_t side[kSide] = {};
-  hipEvent_t fork_ev = nullptr, join_ev[kSide] = {}, tr_ev[kFactorPhases] = {};
+  hipEvent_t fork_ev = nullptr, join_ev[kSide] = {}, tr_ev[kMaxPhases] = {};
+  // factor launch shape (autotuned; the plan's buckets depend on tile_mask
+  // and phases, so changing them takes effect at the next plan)
+  uint32_t tile_mask = kAllTiles;
+  int phases = kDefaultPhases;
+  int side_streams = kSide;
+  uint32_t plan_mask = kAllTiles;  // what the current plan's buckets were built with
+  int plan_phases = kDefaultPhases;
   int track_mask = 0;
   struct Track {
     std::vector<std::pair<hipEvent_t, hipEvent_t>> pairs;
@@ -197,17 +213,14 @@ hipError_t upload(mfs_ctx *ctx, DevBuf &b, const std::vector<T> &v) {
   return hipMemcpyAsync(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, ctx->stream);
 }
 
-int tiles_for(int ns) {
-  const int t = (ns + 15) / 16;
-  if (t <= 4) return std::max(t, 1);
-  if (t <= 6) return 6;
-  if (t <= 8) return 8;
-  if (t <= 12) return 12;
-  if (t <= 16) return 16;
-  return -1;
+int tiles_for(int ns, uint32_t mask) {
+  const int need = std::max((ns + 15) / 16, 1);
+  int best = -1;
+  for (int i = 0; i < 8; ++i)
+    if (((mask | 1u) >> i) & 1u)   // bit 0 (16 tiles) is always on
+      if (kTileSizes[i] >= need) best = kTileSizes[i];
+  return best;
 }
-
-constexpr int kTileSizes[] = {16, 12, 8, 6, 4, 3, 2, 1};  // launch order: largest first
 
 // ---- kernel-class timing (events bracket each launch of an enabled class)
 void track_cb(void *p, int cls, int phase) {
@@ -372,6 +385,8 @@ int plan_impl(mfs_ctx *ctx, std::vector<double> &&xyz, int64_t n64, double rho, 
   transpose_pattern(n, ctx->sptr, ctx->sind, ctx->tptr, ctx->tcol, ctx->tmap);
 
   // 4. supernode descriptors, bucketed by tile count (largest first in each)
+  ctx->plan_mask = ctx->tile_mask;
+  ctx->plan_phases = ctx->phases;
   ctx->sdesc.resize(n_super);
   ctx->max_local = 0;
   double s2 = 0, s3 = 0;
@@ -389,7 +404,7 @@ int plan_impl(mfs_ctx *ctx, std::vector<double> &&xyz, int64_t n64, double rho, 
     s2 += double(d.ns) * d.ns;
     s3 += double(d.ns) * d.ns * d.ns;
     theta += int64_t(d.ns) * d.ns;
-    const int t = tiles_for(d.ns);
+    const int t = tiles_for(d.ns, ctx->plan_mask);
     if (t < 0)
       return ctx->fail(MFS_ERR_ARG, "plan: supernode local size " + std::to_string(d.ns) +
                                         " exceeds the supported maximum of " +
@@ -436,7 +451,7 @@ int plan_impl(mfs_ctx *ctx, std::vector<double> &&xyz, int64_t n64, double rho, 
       int32_t mc = 0;
       for (int32_t p = 0; p < d.ns; ++p) mc = std::max(mc, ctx->Pinv[ctx->sind[d.rows_off + p]]);
       int ph = 0;
-      while (ph + 1 < kFactorPhases && mc >= phase_col(ph + 1, n)) ++ph;
+      while (ph + 1 < ctx->plan_phases && mc >= phase_col(ph + 1, n, ctx->plan_phases)) ++ph;
       phase[s] = int8_t(ph);
     }
     ctx->list.clear();
@@ -450,7 +465,7 @@ int plan_impl(mfs_ctx *ctx, std::vector<double> &&xyz, int64_t n64, double rho, 
       });
       Bucket b{t, int32_t(ctx->list.size()), int32_t(v.size()), {}};
       int32_t lo = 0;
-      for (int ph = 0; ph < kFactorPhases; ++ph) {
+      for (int ph = 0; ph < ctx->plan_phases; ++ph) {
         int32_t hi = lo;
         while (hi < int32_t(v.size()) && phase[v[hi]] == ph) ++hi;
         b.phase_off[ph] = lo;
@@ -463,7 +478,7 @@ int plan_impl(mfs_ctx *ctx, std::vector<double> &&xyz, int64_t n64, double rho, 
           }
         lo = hi;
       }
-      b.phase_off[kFactorPhases] = lo;
+      for (int ph = ctx->plan_phases; ph <= kMaxPhases; ++ph) b.phase_off[ph] = lo;
       ctx->buckets.push_back(b);
     }
   }
@@ -732,7 +747,7 @@ int ensure_side_streams(mfs_ctx *ctx) {
     HIPCHK(hipStreamCreateWithFlags(&ctx->side[i], hipStreamNonBlocking), "create stream");
     HIPCHK(hipEventCreateWithFlags(&ctx->join_ev[i], hipEventDisableTiming), "create event");
   }
-  for (int i = 0; i < kFactorPhases; ++i)
+  for (int i = 0; i < kMaxPhases; ++i)
     HIPCHK(hipEventCreateWithFlags(&ctx->tr_ev[i], hipEventDisableTiming), "create event");
   HIPCHK(hipEventCreateWithFlags(&ctx->fork_ev, hipEventDisableTiming), "create event");
   return MFS_OK;
@@ -818,7 +833,7 @@ int factor_finish(mfs_ctx *ctx, bool points, int precision) {
   if (!failed.empty() && precision == MFS_FP32) {
     ctx->stats.retried_supernodes = int32_t(failed.size());
     std::vector<std::vector<FactorItem>> by_t(17);
-    for (int32_t s : failed) by_t[tiles_for(ctx->sdesc[s].ns)].push_back({0, s});
+    for (int32_t s : failed) by_t[tiles_for(ctx->sdesc[s].ns, ctx->plan_mask)].push_back({0, s});
     std::vector<FactorItem> flist;
     std::vector<Bucket> fb;
     size_t scratch_bytes = 8;
@@ -880,25 +895,27 @@ int factor_impl(mfs_ctx *ctx, const float *M, const PointsOp *pts, int64_t m, in
   }
   HIPCHK(ctx->dthg.ensure(std::max<size_t>(scratch_bytes, 8)), "alloc factor scratch");
   const int prec_code = precision == MFS_FP64 ? 0 : 1;
-  const int nside = std::min<int>(mfs_ctx::kSide, int(ctx->buckets.size()));
+  const int nside = std::min<int>(ctx->side_streams, int(ctx->buckets.size()));
   if (pts == nullptr) {
     // M^T range by range on the context stream; a bucket's side stream runs
     // its phase-p supernodes once range p is done, phases in turn, while the
     // later ranges are transposed.
     const int64_t ldmt = ctx->fscene.ldmt;
-    for (int ph = 0; ph < kFactorPhases; ++ph) {
-      HIPCHK(launch_transpose(ctx->stream, M, m, n, ctx->dMt.as<float>(), ldmt, phase_col(ph, n),
-                              phase_col(ph + 1, n)),
+    // the plan's buckets were laid out for plan_phases ranges
+    const int nph = ctx->plan_phases;
+    for (int ph = 0; ph < nph; ++ph) {
+      HIPCHK(launch_transpose(ctx->stream, M, m, n, ctx->dMt.as<float>(), ldmt, phase_col(ph, n, nph),
+                              phase_col(ph + 1, n, nph)),
              "transpose");
       HIPCHK(hipEventRecord(ctx->tr_ev[ph], ctx->stream), "fork");
     }
-    for (int ph = 0; ph < kFactorPhases; ++ph) {
+    for (int ph = 0; ph < nph; ++ph) {
       for (int i = 0; i < nside; ++i) HIPCHK(hipStreamWaitEvent(ctx->side[i], ctx->tr_ev[ph], 0), "fork");
       for (size_t bi = 0; bi < ctx->buckets.size(); ++bi) {
         const Bucket &b = ctx->buckets[bi];
         const int32_t lo = b.phase_off[ph], cnt = b.phase_off[ph + 1] - lo;
         if (cnt <= 0) continue;
-        HIPCHK(launch_super_factor(ctx->side[bi % mfs_ctx::kSide], b.tiles, prec_code, false,
+        HIPCHK(launch_super_factor(ctx->side[bi % nside], b.tiles, prec_code, false,
                                    ctx->dfscene.as<FactorScene>(),
                                    ctx->dlist.as<FactorItem>() + b.offset + lo, cnt,
                                    ctx->dthg.as<char>() + scratch_off[bi] +
@@ -911,7 +928,7 @@ int factor_impl(mfs_ctx *ctx, const float *M, const PointsOp *pts, int64_t m, in
     for (int i = 0; i < nside; ++i) HIPCHK(hipStreamWaitEvent(ctx->side[i], ctx->fork_ev, 0), "fork");
     for (size_t bi = 0; bi < ctx->buckets.size(); ++bi) {
       const Bucket &b = ctx->buckets[bi];
-      HIPCHK(launch_super_factor(ctx->side[bi % mfs_ctx::kSide], b.tiles, prec_code, true,
+      HIPCHK(launch_super_factor(ctx->side[bi % nside], b.tiles, prec_code, true,
                                  ctx->dfscene.as<FactorScene>(),
                                  ctx->dlist.as<FactorItem>() + b.offset, b.count,
                                  ctx->dthg.as<char>() + scratch_off[bi]),
@@ -1278,7 +1295,7 @@ void mfs_ctx_destroy(mfs_ctx *ctx) {
     if (ctx->side[i]) (void)hipStreamDestroy(ctx->side[i]);
     if (ctx->join_ev[i]) (void)hipEventDestroy(ctx->join_ev[i]);
   }
-  for (int i = 0; i < kFactorPhases; ++i)
+  for (int i = 0; i < kMaxPhases; ++i)
     if (ctx->tr_ev[i]) (void)hipEventDestroy(ctx->tr_ev[i]);
   if (ctx->fork_ev) (void)hipEventDestroy(ctx->fork_ev);
   for (auto &t : ctx->tr)
@@ -1790,6 +1807,18 @@ int mfs_ctx_set_param(mfs_ctx *ctx, int param, int64_t value) {
     case MFS_PARAM_FUSED:
       if (value > 2) return ctx->fail(MFS_ERR_ARG, "set_param: fused mode is 0, 1 or 2");
       ctx->fused_mode = int32_t(value);
+      return MFS_OK;
+    case MFS_PARAM_TILE_SET:
+      if (value > 0xff) return ctx->fail(MFS_ERR_ARG, "set_param: tile set is an 8-bit mask");
+      ctx->tile_mask = value == 0 ? kAllTiles : uint32_t(value) | 1u;
+      return MFS_OK;
+    case MFS_PARAM_FACTOR_PHASES:
+      if (value > kMaxPhases) return ctx->fail(MFS_ERR_ARG, "set_param: 1 to 8 factor column ranges");
+      ctx->phases = value == 0 ? kDefaultPhases : int(value);
+      return MFS_OK;
+    case MFS_PARAM_FACTOR_STREAMS:
+      if (value > mfs_ctx::kSide) return ctx->fail(MFS_ERR_ARG, "set_param: 1 to 4 factor streams");
+      ctx->side_streams = value == 0 ? mfs_ctx::kSide : int(value);
       return MFS_OK;
     default:
       return ctx->fail(MFS_ERR_ARG, "set_param: unknown parameter");

@@ -241,9 +241,16 @@ int mfs_single_layer_backward(mfs_ctx *ctx, int precision, const void *S, int64_
 enum mfs_param {
     MFS_PARAM_POINTS_BLOCKS = 1,  /* blocks per points pass, owners x splits (4096) */
     MFS_PARAM_COLS_BLOCKS = 2,    /* blocks of the dense M^T pass (0: one round, CUs x resident) */
-    MFS_PARAM_FUSED = 3           /* one-pass M^T M p for dense FP32 batches of 8:
+    MFS_PARAM_FUSED = 3,          /* one-pass M^T M p for dense FP32 batches of 8:
                                      0 default (two passes), 1 off, 2 whenever the shape
                                      qualifies (n % 4 == 0, 8 <= n / 256 <= CUs) */
+    MFS_PARAM_TILE_SET = 4,       /* factor buckets: bit i enables tile count
+                                     {16, 12, 8, 6, 4, 3, 2, 1}[i] (16 always on); a
+                                     supernode takes the smallest enabled count covering
+                                     its rows.  Applies from the next plan (0: all) */
+    MFS_PARAM_FACTOR_PHASES = 5,  /* column ranges of the dense factor's transpose that
+                                     the Gram overlaps, 1..8; from the next plan (0: 4) */
+    MFS_PARAM_FACTOR_STREAMS = 6  /* streams the factor's buckets run on, 1..4 (0: 4) */
 };
 int mfs_ctx_set_param(mfs_ctx *ctx, int param, int64_t value);
 
