@@ -711,8 +711,9 @@ def main():
     ftune = None
     if args.tune_factor:
         ftune = autotune.tune_factor(ctx, M, sc.y, 3.0, _lib.MFS_FP32)
-        _log(f"bench: factor tuning best {ftune['best']}: " +
-             ", ".join(f"{c}: {t:.2f} ms" for c, t in sorted(ftune["ms"].items(), key=lambda kv: kv[1])))
+        _log(f"bench: factor tuning best {ftune['best']}, splits {ftune['splits']}: " +
+             ", ".join(f"{c}: {t:.2f} ms" for c, t in sorted(ftune["ms"].items(), key=lambda kv: kv[1])) +
+             f"; splits {ftune['split_ms']}")
 
     # ---------------------------------------------------------------- chosen form
     def chosen():
@@ -916,8 +917,9 @@ def main():
         "factor_ms": round(float(np.mean(factor_ms)), 3),
         "factor_retried_supernodes": retried,
         "factor_tuning": ({"best": {"tile_set": hex(ftune["best"][0]), "column_ranges": ftune["best"][1],
-                                    "streams": ftune["best"][2]},
-                           "ms": {f"{hex(c[0])}/{c[1]}/{c[2]}": round(t, 3) for c, t in ftune["ms"].items()}}
+                                    "streams": ftune["best"][2], "row_splits": ftune["splits"]},
+                           "ms": {f"{hex(c[0])}/{c[1]}/{c[2]}": round(t, 3) for c, t in ftune["ms"].items()},
+                           "row_split_ms": {str(k): round(v, 3) for k, v in ftune["split_ms"].items()}}
                           if ftune else None),
         "pcg_inner_iters": iters,
         "refinement_sweeps": outer,

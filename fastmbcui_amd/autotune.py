@@ -98,14 +98,16 @@ def factor_candidates(tile_sets=FACTOR_TILE_SETS, phases=(2, 4, 8), streams=(2, 
 
 
 def tune_factor(ctx: _lib.Context, M, xyz, rho: float, precision=_lib.MFS_FP32, candidates=None,
-                reps: int = 2, points=None) -> dict:
+                reps: int = 2, points=None, splits=(1, 2, 4, 8)) -> dict:
     """Time the numeric factor of M (dense form; or from `points` = (x, nx, y)
     device tensors) for each (tile set, column ranges, streams) candidate --
     re-planning for each, since the plan's buckets depend on the first two --
-    and keep the fastest (the context is left planned with it).  Returns
-    {"best": (tile_set, phases, streams), "ms": {candidate: factor ms}} with the
-    best of `reps` timed factorisations per candidate (the context's own HIP
-    event timing of the factor)."""
+    and keep the fastest (the context is left planned with it); then time the
+    row splits of small launches (MFS_PARAM_FACTOR_SPLITS) on that shape.
+    Returns {"best": (tile_set, phases, streams), "ms": {candidate: factor
+    ms}, "splits": best split count, "split_ms": {splits: factor ms}}, each the
+    best of `reps` timed factorisations (the context's own HIP event timing
+    of the factor)."""
     key = ("factor", tuple(M.shape) if M is not None else points[0].shape[0], ctx.n, float(rho), precision)
     cands = candidates or factor_candidates()
     times = {}
@@ -128,5 +130,20 @@ def tune_factor(ctx: _lib.Context, M, xyz, rho: float, precision=_lib.MFS_FP32, 
     ctx.set_param(_lib.MFS_PARAM_FACTOR_PHASES, win[1])
     ctx.set_param(_lib.MFS_PARAM_FACTOR_STREAMS, win[2])
     ctx.plan(xyz, rho)
-    _cache[key] = {"best": win, "ms": times}
+    # then the row splits of the small launches, on the winning shape
+    split_ms = {}
+    for sp in splits:
+        ctx.set_param(_lib.MFS_PARAM_FACTOR_SPLITS, sp)
+        best = float("inf")
+        for r in range(reps + 1):
+            if points is None:
+                ctx.factor_dense(M, precision)
+            else:
+                ctx.factor_points(*points, precision)
+            if r:
+                best = min(best, ctx.stats().factor_ms)
+        split_ms[sp] = best
+    best_split = min(split_ms, key=split_ms.get)
+    ctx.set_param(_lib.MFS_PARAM_FACTOR_SPLITS, best_split)
+    _cache[key] = {"best": win, "ms": times, "splits": best_split, "split_ms": split_ms}
     return _cache[key]

@@ -141,9 +141,14 @@ __global__ __launch_bounds__(kThreads) void k_transpose(const float *__restrict_
 template <int T, typename TC, typename TV, bool PTS>
 __global__ __launch_bounds__(kThreads) void k_super_factor(const FactorScene *__restrict__ scenes,
                                                            const FactorItem *__restrict__ items,
-                                                           TC *__restrict__ th_scratch) {
-  // This block's supernode and its scene (one launch may mix scenes).
-  const FactorItem item = items[blockIdx.x];
+                                                           TC *__restrict__ th_scratch,
+                                                           FactorSplit sp) {
+  // This block's supernode and its scene (one launch may mix scenes); with
+  // row splits, blocks item * S .. item * S + S - 1 share one supernode.
+  const int32_t S = sp.splits;
+  const int32_t bi = S > 1 ? int32_t(blockIdx.x) / S : int32_t(blockIdx.x);
+  const int32_t split = S > 1 ? int32_t(blockIdx.x) % S : 0;
+  const FactorItem item = items[bi];
   const FactorScene &FS = scenes[item.scene];
   const float *__restrict__ Mt = FS.Mt;
   const int64_t ldmt = FS.ldmt;
@@ -183,6 +188,7 @@ __global__ __launch_bounds__(kThreads) void k_super_factor(const FactorScene *__
   __shared__ TC dg[NS], dinv[NS];
   __shared__ TC W[NS * MB];
   __shared__ int32_t mcol[MB], mc[MB], moff[MB];
+  __shared__ int last_split;
 
   // wave index in a scalar register: loop bounds derived from it stay
   // uniform branches (and exact wait counts) instead of exec-masked regions
@@ -195,6 +201,12 @@ __global__ __launch_bounds__(kThreads) void k_super_factor(const FactorScene *__
   __syncthreads();
 
   // ---- 1. Gram on MFMA ------------------------------------------------------
+  // Rows [r0, r1) of this block's split (all rows without splits); chunks are
+  // multiples of 256 rows, so every wave's slabs and row quads stay inside.
+  const int64_t rows_total = PTS ? m : ldmt;
+  const int64_t chunk = S > 1 ? factor_chunk(rows_total, S) : rows_total;
+  const int64_t r0 = int64_t(split) * chunk;
+  const int64_t r1 = (S > 1 && r0 + chunk < rows_total) ? r0 + chunk : rows_total;
   const int grp = wave % NG, kph = wave / NG;
   TC *TH = LARGE ? th_scratch + size_t(blockIdx.x) * NS * RS : reinterpret_cast<TC *>(big_raw);
   // Pass P accumulates the pairs grp + NG (P PPW + u), u < PPW, over all rows.
@@ -262,9 +274,9 @@ __global__ __launch_bounds__(kThreads) void k_super_factor(const FactorScene *__
     constexpr int FSTEP = 4 * KS;  // rows per k-step over the waves sharing a pair group
     const float scale = float(-kInv4Pi);
     float4 cur[3], nxt[3];
-    int64_t f = kph * 4 + (lane >> 4);
+    int64_t f = r0 + kph * 4 + (lane >> 4);
     load_target(f, cur);
-    for (int64_t f0 = kph * 4; f0 < m; f0 += FSTEP, f += FSTEP) {
+    for (int64_t f0 = r0 + kph * 4; f0 < r1; f0 += FSTEP, f += FSTEP) {
       load_target(f + FSTEP, nxt);
       const float sn0 = scale * cur[1].z, sn1 = scale * cur[1].w, sn2 = scale * cur[2].x;
       TC fr[T];
@@ -345,19 +357,19 @@ __global__ __launch_bounds__(kThreads) void k_super_factor(const FactorScene *__
         step(fr);
       }
     };
-    const int64_t fs = kph * 16 * W;  // this wave's first slab
+    const int64_t fs = r0 + kph * 16 * W;  // this wave's first slab
     if constexpr (!LARGE) {
       // Two register buffers in turn, each slab requested one slab ahead.
-      // Slabs past the row re-read the wave's last slab (in bounds) and are
+      // Slabs past the range re-read the wave's last slab (in bounds) and are
       // multiplied in as zeros, so the loop body has no branch: a buffer
       // used under a branch has its loads sunk into that branch and waited
       // for there.
-      const int64_t last = fs < ldmt ? fs + (ldmt - 1 - fs) / FSTEP * FSTEP : fs;
+      const int64_t last = fs < r1 ? fs + (r1 - 1 - fs) / FSTEP * FSTEP : fs;
       auto clampf = [&](int64_t f) { return f <= last ? f : last; };
       fv4 b0[T][W], b1[T][W];
-      if (fs < ldmt) {
+      if (fs < r1) {
         load(fs, b0);
-        for (int64_t f0 = fs; f0 < ldmt; f0 += 2 * FSTEP) {
+        for (int64_t f0 = fs; f0 < r1; f0 += 2 * FSTEP) {
           // scheduling fences: the machine scheduler would otherwise pull
           // the register-only MFMAs above the prefetch and wait for it
           load(clampf(f0 + FSTEP), b1);
@@ -366,13 +378,13 @@ __global__ __launch_bounds__(kThreads) void k_super_factor(const FactorScene *__
           __builtin_amdgcn_sched_barrier(0);
           load(clampf(f0 + 2 * FSTEP), b0);
           __builtin_amdgcn_sched_barrier(0);
-          compute(b1, f0 + FSTEP < ldmt);
+          compute(b1, f0 + FSTEP < r1);
           __builtin_amdgcn_sched_barrier(0);
         }
       }
     } else {  // the large tiles need the registers: no prefetch
       fv4 cur[T][W];
-      for (int64_t f0 = fs; f0 < ldmt; f0 += FSTEP) {
+      for (int64_t f0 = fs; f0 < r1; f0 += FSTEP) {
         load(f0, cur);
         compute(cur, true);
       }
@@ -408,6 +420,40 @@ __global__ __launch_bounds__(kThreads) void k_super_factor(const FactorScene *__
   __syncthreads();
   };
   static_for<NPASS>(gram_pass);
+
+  // Row splits: every block publishes its partial Gram (LARGE: its TH in
+  // global scratch already); the last of the S blocks to arrive sums all S
+  // in split order (deterministic) into its TH and carries on alone.  It
+  // re-arms the counter for the next launch.
+  if (S > 1) {
+    TC *part = static_cast<TC *>(sp.partial);
+    if constexpr (!LARGE) {
+      for (int idx = tid; idx < ns * ns; idx += kThreads) {
+        const int r = idx / ns, c = idx % ns;
+        part[size_t(blockIdx.x) * NS * NS + r * NS + c] = TH[r * RS + c];
+      }
+    }
+    __threadfence();
+    __syncthreads();
+    if (tid == 0) {
+      const int32_t prev = atomicAdd(sp.counters + bi, 1);
+      last_split = (prev == S - 1);
+      if (last_split) atomicExch(sp.counters + bi, 0);
+    }
+    __syncthreads();
+    if (!last_split) return;
+    __threadfence();
+    for (int idx = tid; idx < ns * ns; idx += kThreads) {
+      const int r = idx / ns, c = idx % ns;
+      TC sum = TC(0);
+      for (int q = 0; q < S; ++q) {
+        const size_t blk = size_t(bi) * S + q;
+        sum += LARGE ? th_scratch[blk * NS * RS + r * RS + c] : part[blk * NS * NS + r * NS + c];
+      }
+      TH[r * RS + c] = sum;
+    }
+    __syncthreads();
+  }
 
   // ---- 2. Upper Cholesky TH = U^T U ----------------------------------------
   // FP32: a pivot below the FP32 Gram's rounding noise relative to its
@@ -508,17 +554,19 @@ struct FactorLaunch {
   int32_t count;
   bool points;
   void *th_scratch;
+  FactorSplit sp;
 };
 
 template <int T, typename TC, typename TV>
 hipError_t launch_t(hipStream_t st, const FactorLaunch &L) {
   if (L.count <= 0) return hipSuccess;
+  const unsigned grid = unsigned(int64_t(L.count) * (L.sp.splits > 1 ? L.sp.splits : 1));
   if (L.points)
-    k_super_factor<T, TC, TV, true><<<unsigned(L.count), kThreads, 0, st>>>(
-        L.scenes, L.items, static_cast<TC *>(L.th_scratch));
+    k_super_factor<T, TC, TV, true><<<grid, kThreads, 0, st>>>(
+        L.scenes, L.items, static_cast<TC *>(L.th_scratch), L.sp);
   else
-    k_super_factor<T, TC, TV, false><<<unsigned(L.count), kThreads, 0, st>>>(
-        L.scenes, L.items, static_cast<TC *>(L.th_scratch));
+    k_super_factor<T, TC, TV, false><<<grid, kThreads, 0, st>>>(
+        L.scenes, L.items, static_cast<TC *>(L.th_scratch), L.sp);
   return hipGetLastError();
 }
 
@@ -552,8 +600,10 @@ hipError_t launch_transpose(hipStream_t st, const float *M, int64_t m, int64_t n
 
 hipError_t launch_super_factor(hipStream_t st, int tiles, int precision, bool points,
                                const FactorScene *scenes, const FactorItem *items, int32_t count,
-                               void *th_scratch) {
-  const FactorLaunch L{scenes, items, count, points, th_scratch};
+                               void *th_scratch, const FactorSplit &sp) {
+  if (sp.splits > 1 && ((sp.partial == nullptr && tiles <= 8) || sp.counters == nullptr))
+    return hipErrorInvalidValue;
+  const FactorLaunch L{scenes, items, count, points, th_scratch, sp};
   if (precision == 0) return launch_prec<double, double>(st, tiles, L);
   if (precision == 1) return launch_prec<float, float>(st, tiles, L);
   return launch_prec<double, float>(st, tiles, L);

@@ -140,7 +140,7 @@ struct mfs_ctx {
   // factor
   bool has_factor = false;
   int prec = MFS_FP64;
-  DevBuf dval, dtval, dMt, dval_alt, dtval_alt, dthg, dfscene;
+  DevBuf dval, dtval, dMt, dval_alt, dtval_alt, dthg, dctr, dfscene;
   FactorScene fscene{};  // what the factor kernels read for this context
   int alt_prec = -1;
 
@@ -177,6 +177,11 @@ struct mfs_ctx {
   int side_streams = kSide;
   uint32_t plan_mask = kAllTiles;  // what the current plan's buckets were built with
   int plan_phases = kDefaultPhases;
+  // Row splits of the factor's Gram per supernode, 1..8.  Off by default:
+  // the buckets already run side by side on several streams, and at
+  // m = n = 65,536 splitting the small launches made the factor slower
+  // (26.7 ms against 23.1 ms); the autotuner tries it.
+  int max_splits = 1;
   int track_mask = 0;
   struct Track {
     std::vector<std::pair<hipEvent_t, hipEvent_t>> pairs;
@@ -886,54 +891,85 @@ int factor_impl(mfs_ctx *ctx, const float *M, const PointsOp *pts, int64_t m, in
   // smaller ones instead of leaving CUs idle between launches.
   rc = ensure_side_streams(ctx);
   if (rc != MFS_OK) return rc;
-  // global TH scratch of the large buckets (sized for FP64, the widest compute type)
-  std::vector<size_t> scratch_off(ctx->buckets.size(), 0);
-  size_t scratch_bytes = 0;
-  for (size_t bi = 0; bi < ctx->buckets.size(); ++bi) {
-    scratch_off[bi] = scratch_bytes;
-    scratch_bytes += size_t(ctx->buckets[bi].count) * size_t(factor_scratch_elems(ctx->buckets[bi].tiles)) * 8;
-  }
-  HIPCHK(ctx->dthg.ensure(std::max<size_t>(scratch_bytes, 8)), "alloc factor scratch");
+  // The launches: one per (bucket, column range) in the dense form, one per
+  // bucket from the points.  Launches of few supernodes split each Gram's
+  // rows over several blocks (factor_splits) so the chip stays busy; their
+  // partial Grams, the large buckets' TH and the arrival counters get
+  // regions of their own (sized for FP64, the widest compute type), since
+  // launches on different streams run at once.
   const int prec_code = precision == MFS_FP64 ? 0 : 1;
   const int nside = std::min<int>(ctx->side_streams, int(ctx->buckets.size()));
+  struct FLaunch {
+    size_t bucket;
+    int phase;
+    int32_t lo, cnt;
+    FactorSplit sp;
+    size_t off;      // bytes into dthg: TH (large tiles) or partial Grams
+    size_t ctr;      // counters offset (int32 elements)
+  };
+  const int64_t rows = pts ? m : ctx->fscene.ldmt;
+  const int64_t target = int64_t(ctx->cus) * 4;
+  const int nph = pts ? 1 : ctx->plan_phases;
+  std::vector<FLaunch> fl;
+  size_t bytes = 8, ctrs = 1;
+  for (int ph = 0; ph < nph; ++ph)
+    for (size_t bi = 0; bi < ctx->buckets.size(); ++bi) {
+      const Bucket &b = ctx->buckets[bi];
+      const int32_t lo = pts ? 0 : b.phase_off[ph];
+      const int32_t cnt = pts ? b.count : b.phase_off[ph + 1] - lo;
+      if (cnt <= 0) continue;
+      FLaunch L{bi, ph, lo, cnt, FactorSplit{}, bytes, ctrs};
+      L.sp.splits = std::min(ctx->max_splits, factor_splits(cnt, rows, target));
+      const size_t ns = size_t(16) * b.tiles;
+      const size_t blocks = size_t(cnt) * L.sp.splits;
+      if (b.tiles > 8) bytes += blocks * size_t(factor_scratch_elems(b.tiles)) * 8;
+      else if (L.sp.splits > 1) bytes += blocks * ns * ns * 8;
+      bytes = (bytes + 255) / 256 * 256;
+      if (L.sp.splits > 1) ctrs += size_t(cnt);
+      fl.push_back(L);
+    }
+  HIPCHK(ctx->dthg.ensure(bytes), "alloc factor scratch");
+  HIPCHK(ctx->dctr.ensure(ctrs * 4), "alloc factor counters");
+  HIPCHK(hipMemsetAsync(ctx->dctr.p, 0, ctrs * 4, ctx->stream), "clear factor counters");
+  for (FLaunch &L : fl) {
+    const Bucket &b = ctx->buckets[L.bucket];
+    if (L.sp.splits > 1) {
+      L.sp.counters = ctx->dctr.as<int32_t>() + L.ctr;
+      if (b.tiles <= 8) L.sp.partial = ctx->dthg.as<char>() + L.off;
+    }
+  }
+  auto launch = [&](const FLaunch &L) -> int {
+    const Bucket &b = ctx->buckets[L.bucket];
+    HIPCHK(launch_super_factor(ctx->side[L.bucket % nside], b.tiles, prec_code, pts != nullptr,
+                               ctx->dfscene.as<FactorScene>(),
+                               ctx->dlist.as<FactorItem>() + b.offset + L.lo, L.cnt,
+                               ctx->dthg.as<char>() + L.off, L.sp),
+           "supernode factor");
+    return MFS_OK;
+  };
   if (pts == nullptr) {
     // M^T range by range on the context stream; a bucket's side stream runs
     // its phase-p supernodes once range p is done, phases in turn, while the
     // later ranges are transposed.
     const int64_t ldmt = ctx->fscene.ldmt;
-    // the plan's buckets were laid out for plan_phases ranges
-    const int nph = ctx->plan_phases;
     for (int ph = 0; ph < nph; ++ph) {
       HIPCHK(launch_transpose(ctx->stream, M, m, n, ctx->dMt.as<float>(), ldmt, phase_col(ph, n, nph),
                               phase_col(ph + 1, n, nph)),
              "transpose");
       HIPCHK(hipEventRecord(ctx->tr_ev[ph], ctx->stream), "fork");
     }
+    size_t li = 0;
     for (int ph = 0; ph < nph; ++ph) {
       for (int i = 0; i < nside; ++i) HIPCHK(hipStreamWaitEvent(ctx->side[i], ctx->tr_ev[ph], 0), "fork");
-      for (size_t bi = 0; bi < ctx->buckets.size(); ++bi) {
-        const Bucket &b = ctx->buckets[bi];
-        const int32_t lo = b.phase_off[ph], cnt = b.phase_off[ph + 1] - lo;
-        if (cnt <= 0) continue;
-        HIPCHK(launch_super_factor(ctx->side[bi % nside], b.tiles, prec_code, false,
-                                   ctx->dfscene.as<FactorScene>(),
-                                   ctx->dlist.as<FactorItem>() + b.offset + lo, cnt,
-                                   ctx->dthg.as<char>() + scratch_off[bi] +
-                                       size_t(lo) * size_t(factor_scratch_elems(b.tiles)) * 8),
-               "supernode factor");
-      }
+      // this range's launches (fl holds them range by range, in bucket order)
+      for (; li < fl.size() && fl[li].phase == ph; ++li)
+        if ((rc = launch(fl[li])) != MFS_OK) return rc;
     }
   } else {
     HIPCHK(hipEventRecord(ctx->fork_ev, ctx->stream), "fork");
     for (int i = 0; i < nside; ++i) HIPCHK(hipStreamWaitEvent(ctx->side[i], ctx->fork_ev, 0), "fork");
-    for (size_t bi = 0; bi < ctx->buckets.size(); ++bi) {
-      const Bucket &b = ctx->buckets[bi];
-      HIPCHK(launch_super_factor(ctx->side[bi % nside], b.tiles, prec_code, true,
-                                 ctx->dfscene.as<FactorScene>(),
-                                 ctx->dlist.as<FactorItem>() + b.offset, b.count,
-                                 ctx->dthg.as<char>() + scratch_off[bi]),
-             "supernode factor");
-    }
+    for (const FLaunch &L : fl)
+      if ((rc = launch(L)) != MFS_OK) return rc;
   }
   for (int i = 0; i < nside; ++i) {
     HIPCHK(hipEventRecord(ctx->join_ev[i], ctx->side[i]), "join");
@@ -1283,7 +1319,7 @@ void mfs_ctx_destroy(mfs_ctx *ctx) {
                     &ctx->dtmap, &ctx->dsupind, &ctx->dsd, &ctx->dlist, &ctx->dstatus, &ctx->dpx,
                     &ctx->dpy, &ctx->dpz, &ctx->dscale, &ctx->dcnt, &ctx->dptr, &ctx->dind,
                     &ctx->dtotal, &ctx->dbands, &ctx->dcells, &ctx->dcpts, &ctx->dlong, &ctx->dval, &ctx->dtval, &ctx->dMt, &ctx->dval_alt,
-                    &ctx->dtval_alt, &ctx->dthg, &ctx->dfscene, &ctx->dvec, &ctx->dpart, &ctx->ddpart, &ctx->dstate,
+                    &ctx->dtval_alt, &ctx->dthg, &ctx->dctr, &ctx->dfscene, &ctx->dvec, &ctx->dpart, &ctx->ddpart, &ctx->dstate,
                     &ctx->dx64, &ctx->dr64, &ctx->dnorm, &ctx->dt32, &ctx->dt64, &ctx->ds32,
                     &ctx->ds64, &ctx->dapin, &ctx->dapout, &ctx->dcaller, &ctx->dflag, &ctx->dfrec})
     b->release();
@@ -1816,6 +1852,10 @@ int mfs_ctx_set_param(mfs_ctx *ctx, int param, int64_t value) {
       if (value > kMaxPhases) return ctx->fail(MFS_ERR_ARG, "set_param: 1 to 8 factor column ranges");
       ctx->phases = value == 0 ? kDefaultPhases : int(value);
       return MFS_OK;
+    case MFS_PARAM_FACTOR_SPLITS:
+      if (value > 8) return ctx->fail(MFS_ERR_ARG, "set_param: 1 to 8 row splits");
+      ctx->max_splits = value == 0 ? 1 : int(value);
+      return MFS_OK;
     case MFS_PARAM_FACTOR_STREAMS:
       if (value > mfs_ctx::kSide) return ctx->fail(MFS_ERR_ARG, "set_param: 1 to 4 factor streams");
       ctx->side_streams = value == 0 ? mfs_ctx::kSide : int(value);
@@ -1886,7 +1926,7 @@ struct mfs_batch {
   hipEvent_t root_ev = nullptr;
   hipStream_t side[mfs_ctx::kSide] = {};
   hipEvent_t side_ev[mfs_ctx::kSide] = {};
-  DevBuf dtable, ditems, dthg, dstates, dactive;
+  DevBuf dtable, ditems, dthg, dctr, dstates, dactive;
   std::vector<FactorScene> table;
   std::vector<FactorItem> items;
   std::vector<PcgState *> states;
@@ -1979,7 +2019,7 @@ void mfs_batch_destroy(mfs_batch *b) {
   if (b->root_ev) (void)hipEventDestroy(b->root_ev);
   if (b->root) (void)hipStreamDestroy(b->root);
   if (b->hactive) (void)hipHostFree(b->hactive);
-  for (DevBuf *d : {&b->dtable, &b->ditems, &b->dthg, &b->dstates, &b->dactive}) d->release();
+  for (DevBuf *d : {&b->dtable, &b->ditems, &b->dthg, &b->dctr, &b->dstates, &b->dactive}) d->release();
   delete b;
 }
 
@@ -2062,29 +2102,51 @@ int mfs_batch_factor_points(mfs_batch *b, const double *const *x, const double *
     if (rc != MFS_OK) return scene_error(b, i, rc);
   }
   // 2. merged buckets: every scene's supernodes of one tile size in one launch
+  // per row-split count.  A scene's supernodes get the split count they get
+  // when the scene is factored alone (factor_splits of its bucket size and
+  // target rows), so the batch's factor equals the single-scene one bit for
+  // bit.
   b->table.resize(count);
   for (size_t i = 0; i < count; ++i) b->table[i] = b->scenes[i]->fscene;
   b->items.clear();
-  std::vector<Bucket> buckets;
-  size_t scratch_bytes = 8;
-  std::vector<size_t> scratch_off;
+  struct BLaunch {
+    int tiles;
+    int32_t offset, count;
+    FactorSplit sp;
+    size_t off, ctr;
+  };
+  std::vector<BLaunch> buckets;
+  size_t scratch_bytes = 8, ctrs = 1;
+  const int64_t target = int64_t(b->scenes[0]->cus) * 4;
   for (int t : kTileSizes) {
-    const int32_t off = int32_t(b->items.size());
-    for (size_t i = 0; i < count; ++i)
-      for (const Bucket &bk : b->scenes[i]->buckets)
-        if (bk.tiles == t)
+    for (int32_t S = 1; S <= 8; ++S) {
+      const int32_t off = int32_t(b->items.size());
+      for (size_t i = 0; i < count; ++i)
+        for (const Bucket &bk : b->scenes[i]->buckets) {
+          if (bk.tiles != t) continue;
+          const int32_t si = std::min(b->scenes[i]->max_splits, factor_splits(bk.count, m[i], target));
+          if (si != S) continue;
           for (int32_t q = 0; q < bk.count; ++q)
             b->items.push_back({int32_t(i), b->scenes[i]->list[size_t(bk.offset + q)]});
-    const int32_t cnt = int32_t(b->items.size()) - off;
-    if (cnt == 0) continue;
-    buckets.push_back({t, off, cnt});
-    scratch_off.push_back(scratch_bytes);
-    scratch_bytes += size_t(cnt) * size_t(factor_scratch_elems(t)) * 8;
+        }
+      const int32_t cnt = int32_t(b->items.size()) - off;
+      if (cnt == 0) continue;
+      BLaunch L{t, off, cnt, FactorSplit{}, scratch_bytes, ctrs};
+      L.sp.splits = S;
+      const size_t ns = size_t(16) * t, blocks = size_t(cnt) * S;
+      if (t > 8) scratch_bytes += blocks * size_t(factor_scratch_elems(t)) * 8;
+      else if (S > 1) scratch_bytes += blocks * ns * ns * 8;
+      scratch_bytes = (scratch_bytes + 255) / 256 * 256;
+      if (S > 1) ctrs += size_t(cnt);
+      buckets.push_back(L);
+    }
   }
   BHIPCHK(b->dtable.ensure(count * sizeof(FactorScene)), "alloc");
   BHIPCHK(b->ditems.ensure(std::max<size_t>(b->items.size(), 1) * sizeof(FactorItem)), "alloc");
   BHIPCHK(b->dthg.ensure(scratch_bytes), "alloc factor scratch");
+  BHIPCHK(b->dctr.ensure(ctrs * 4), "alloc factor counters");
   if (join_scenes(b) != MFS_OK) return MFS_ERR_HIP;
+  BHIPCHK(hipMemsetAsync(b->dctr.p, 0, ctrs * 4, b->root), "clear factor counters");
   BHIPCHK(hipMemcpyAsync(b->dtable.p, b->table.data(), count * sizeof(FactorScene),
                          hipMemcpyHostToDevice, b->root),
           "upload");
@@ -2095,12 +2157,17 @@ int mfs_batch_factor_points(mfs_batch *b, const double *const *x, const double *
   BHIPCHK(hipEventRecord(b->root_ev, b->root), "fork");
   const int nside = std::min<int>(mfs_ctx::kSide, int(buckets.size()));
   for (int i = 0; i < nside; ++i) BHIPCHK(hipStreamWaitEvent(b->side[i], b->root_ev, 0), "fork");
-  for (size_t bi = 0; bi < buckets.size(); ++bi)
-    BHIPCHK(launch_super_factor(b->side[bi % mfs_ctx::kSide], buckets[bi].tiles,
-                                precision == MFS_FP64 ? 0 : 1, true, b->dtable.as<FactorScene>(),
-                                b->ditems.as<FactorItem>() + buckets[bi].offset, buckets[bi].count,
-                                b->dthg.as<char>() + scratch_off[bi]),
+  for (size_t bi = 0; bi < buckets.size(); ++bi) {
+    BLaunch &L = buckets[bi];
+    if (L.sp.splits > 1) {
+      L.sp.counters = b->dctr.as<int32_t>() + L.ctr;
+      if (L.tiles <= 8) L.sp.partial = b->dthg.as<char>() + L.off;
+    }
+    BHIPCHK(launch_super_factor(b->side[bi % nside], L.tiles, precision == MFS_FP64 ? 0 : 1, true,
+                                b->dtable.as<FactorScene>(), b->ditems.as<FactorItem>() + L.offset,
+                                L.count, b->dthg.as<char>() + L.off, L.sp),
             "supernode factor");
+  }
   for (int i = 0; i < nside; ++i) {
     BHIPCHK(hipEventRecord(b->side_ev[i], b->side[i]), "join");
     BHIPCHK(hipStreamWaitEvent(b->root, b->side_ev[i], 0), "join");

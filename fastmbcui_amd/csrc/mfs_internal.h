@@ -102,16 +102,43 @@ struct FactorItem {
   int32_t scene, s;
 };
 
+// Row splits of a factor launch: each supernode's Gram is summed over
+// `splits` blocks of factor_chunk(rows, splits) rows (rows: the scene's M^T
+// rows, or targets in the points form) whose partial Grams meet in `partial`
+// (tiles <= 8: count * splits * (16 tiles)^2 elements of the compute type;
+// larger tiles use th_scratch, sized count * splits blocks); `counters`
+// (count int32, zero before the first launch, left zero) counts arrivals.
+// splits = 1: one block per supernode, the other fields unused.
+struct FactorSplit {
+  int32_t splits = 1;
+  void *partial = nullptr;
+  int32_t *counters = nullptr;
+};
+
 // Factors the supernodes items[0..count) (all of `tiles` tiles) of the scenes
 // in `scenes` (device table); points = the scenes are in the points form (the
 // Gram's operands are evaluated from the records).  tiles in {1, 2, 3, 4, 6,
-// 8, 12, 16}; for 12 and 16 th_scratch holds count * factor_scratch_elems(tiles)
-// elements of the compute type (8 bytes for precision 0 and 2, 4 for 1).
+// 8, 12, 16}; for 12 and 16 th_scratch holds count * splits *
+// factor_scratch_elems(tiles) elements of the compute type (8 bytes for
+// precision 0 and 2, 4 for 1).
 hipError_t launch_super_factor(hipStream_t st, int tiles, int precision, bool points,
                                const FactorScene *scenes, const FactorItem *items, int32_t count,
-                               void *th_scratch);
+                               void *th_scratch, const FactorSplit &sp = FactorSplit{});
 inline int64_t factor_scratch_elems(int tiles) {
   return tiles > 8 ? int64_t(16 * tiles) * (16 * tiles + 1) : 0;
+}
+// Row splits for a launch of `count` supernodes over `rows` rows (M^T rows,
+// or targets in the points form): enough blocks for `target` in all, at most
+// 8 splits, at least 256 rows each.
+inline int32_t factor_splits(int32_t count, int64_t rows, int64_t target) {
+  if (count <= 0 || count >= target) return 1;
+  int64_t s = (target + count - 1) / count;
+  if (s > 8) s = 8;
+  while (s > 1 && rows / s < 256) --s;
+  return int32_t(s);
+}
+__host__ __device__ inline int64_t factor_chunk(int64_t rows, int32_t splits) {
+  return ((rows + splits - 1) / splits + 255) / 256 * 256;
 }
 hipError_t launch_gather_values(hipStream_t st, int precision, int32_t nnz, const int32_t *tmap,
                                 const void *val, void *tval);

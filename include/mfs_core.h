@@ -250,7 +250,11 @@ enum mfs_param {
                                      its rows.  Applies from the next plan (0: all) */
     MFS_PARAM_FACTOR_PHASES = 5,  /* column ranges of the dense factor's transpose that
                                      the Gram overlaps, 1..8; from the next plan (0: 4) */
-    MFS_PARAM_FACTOR_STREAMS = 6  /* streams the factor's buckets run on, 1..4 (0: 4) */
+    MFS_PARAM_FACTOR_STREAMS = 6, /* streams the factor's buckets run on, 1..4 (0: 4) */
+    MFS_PARAM_FACTOR_SPLITS = 7   /* most row splits of a supernode's Gram, 1..8: launches of
+                                     fewer supernodes than 4 x CUs spread each Gram's rows
+                                     over several blocks and sum their partial Grams (0: 1,
+                                     no splits) */
 };
 int mfs_ctx_set_param(mfs_ctx *ctx, int param, int64_t value);
 

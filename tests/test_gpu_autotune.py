@@ -79,3 +79,32 @@ def test_factor_shape_parameters_are_checked():
                        (_lib.MFS_PARAM_FACTOR_STREAMS, 5)):
         with pytest.raises(_lib.MfsError):
             ctx.set_param(param, bad)
+
+
+@pytest.mark.parametrize("points", [False, True])
+def test_factor_row_splits_keep_the_factor(scene, points):
+    """Gram rows split over up to 8 blocks per supernode (partial Grams summed
+    by the last block to arrive) against one block per supernode: the same
+    factor up to summation order, FP64 and FP32."""
+    sc, M, pl, ref = scene
+    Md = torch.from_numpy(M).to(DEV)
+    pts = [torch.from_numpy(a).to(DEV) for a in (sc.x, sc.nx, sc.y)]
+    vals = {}
+    for splits in (1, 8):
+        ctx = _lib.Context(0)
+        ctx.set_param(_lib.MFS_PARAM_FACTOR_SPLITS, splits)
+        ctx.plan(sc.y, 3.0)
+        for prec in (_lib.MFS_FP64, _lib.MFS_FP32):
+            if points:
+                ctx.factor_points(*pts, prec)
+            else:
+                ctx.factor_dense(Md, prec)
+            vals[(splits, prec)] = ctx.factor_export()
+    d64 = _col_rel_err(pl, vals[(8, _lib.MFS_FP64)], vals[(1, _lib.MFS_FP64)])
+    d32 = _col_rel_err(pl, vals[(8, _lib.MFS_FP32)], vals[(1, _lib.MFS_FP32)])
+    e64 = _col_rel_err(pl, vals[(8, _lib.MFS_FP64)], ref)
+    print(f"points={points}: split vs unsplit FP64 {d64:.2e}, FP32 {d32:.2e}; split vs oracle {e64:.2e}",
+          flush=True)
+    assert d64 < 1e-10
+    assert d32 < 5e-2
+    assert e64 < (2e-6 if points else 1e-9)

@@ -122,3 +122,23 @@ def test_batch_rejects_unknown_launch_mode():
     with pytest.raises(_lib.MfsError) as e:
         b.solve_points([_dev(sc.b.astype(np.float32)) for sc in scs], use_graph=3)
     assert e.value.code == _lib.MFS_ERR_ARG
+
+
+def test_batch_with_row_splits_equals_single_scene_solves():
+    """Row-split factor launches (MFS_PARAM_FACTOR_SPLITS): a scene's
+    supernodes get the same split count in the batch's merged launches as in
+    its own factorisation, so the results stay bit-identical."""
+    scs = _scenes(5)
+    b = _lib.Batch(len(scs))
+    b.set_param(_lib.MFS_PARAM_FACTOR_SPLITS, 8)
+    b.plan([sc.y for sc in scs], 3.0)
+    b.factor_points([_dev(sc.x) for sc in scs], [_dev(sc.nx) for sc in scs], [_dev(sc.y) for sc in scs])
+    Xs = b.solve_points([_dev(sc.b.astype(np.float32)) for sc in scs], tol=1e-6, chunk=3)
+    print(f"batch factor launches with splits: {b.stats().factor_launches}", flush=True)
+    for i, sc in enumerate(scs):
+        ctx = _lib.Context(0)
+        ctx.set_param(_lib.MFS_PARAM_FACTOR_SPLITS, 8)
+        ctx.plan(sc.y, 3.0)
+        ctx.factor_points(_dev(sc.x), _dev(sc.nx), _dev(sc.y), _lib.MFS_FP32)
+        X = ctx.solve_points(_dev(sc.b.astype(np.float32)), tol=1e-6)
+        assert torch.equal(X, Xs[i]), f"scene {i}"
