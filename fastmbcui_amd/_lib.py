@@ -102,6 +102,7 @@ def lib():
             "mfs_points_apply": (ci, [vp, ci, ci, vp, i64, vp]),
             "mfs_dense_normal_apply": (ci, [vp, vp, i64, i64, vp, i64, vp]),
             "mfs_dense_normal_apply_noalloc": (ci, [vp, vp, i64, i64, vp, i64, vp]),
+            "mfs_dense_grad": (ci, [vp, vp, i64, i64, vp, vp, vp, i64, vp, vp]),
             "mfs_solid_angle": (ci, [vp, ci, vp, i64, vp, i64, vp, i64, vp]),
             "mfs_solid_angle_backward": (ci, [vp, ci, vp, i64, vp, i64, vp, i64, vp, vp, vp]),
             "mfs_single_layer_apply": (ci, [vp, ci, vp, i64, vp, i64, vp, i64, vp]),
@@ -405,6 +406,24 @@ class Context:
                                                          _ptr(X), X.shape[1], _ptr(Q)),
                     "mfs_dense_normal_apply_noalloc")
         return Q
+
+    def dense_grad(self, M, F, A, V, need_m=True, need_f=True):
+        """(grad_M, grad_F) of the solve's backward: grad_F = M V,
+        grad_M = (F - M A) V^T - (M V) A^T; FP32 device tensors, k <= 16."""
+        import torch
+        self._sync_stream()
+        F, A, V = (_as_2d_f32(t) for t in (F, A, V))
+        if not M.is_contiguous() or M.dtype != torch.float32:
+            raise ValueError("M must be a contiguous float32 device tensor")
+        m, n = M.shape
+        k = F.shape[1]
+        if F.shape[0] != m or A.shape != (n, k) or V.shape != (n, k):
+            raise ValueError("F must be m x k, A and V n x k")
+        gM = torch.empty((m, n), dtype=torch.float32, device=M.device) if need_m else None
+        gF = torch.empty((m, k), dtype=torch.float32, device=M.device) if need_f else None
+        self._check(lib().mfs_dense_grad(self._h, _ptr(M), m, n, _ptr(F), _ptr(A), _ptr(V), k,
+                                         _ptr(gM), _ptr(gF)), "mfs_dense_grad")
+        return gM, gF
 
     def points_apply(self, v, trans=False, precision=MFS_FP32):
         """M v (v: n x k) or M^T v (trans, v: m x k) from the packed points."""

@@ -1652,6 +1652,98 @@ template hipError_t pcg_finish<double>(hipStream_t, const SolveArgs<double> &, f
 template hipError_t pcg_finish<float>(hipStream_t, const SolveArgs<float> &, float *, int64_t,
                                       int64_t, int, double *);
 
+// ------------------------------------------------------------------ backward products
+// Y (m x K, FP32) = M X (X: n x K FP32, K padded) through the PCG's row pass.
+hipError_t dense_rows32(hipStream_t s, const float *M, int64_t m, int64_t n, const float *X, int K,
+                        float *Y) {
+  SolveArgs<float> a{};
+  a.M = M;
+  a.m = m;
+  a.n = n;
+  a.K = K;
+  a.pts = PointsOp{};
+  switch (K) {
+    case 1: mv_rows<float, 1>(s, a, X, nullptr, 0, 0, K, Y, nullptr); break;
+    case 2: mv_rows<float, 2>(s, a, X, nullptr, 0, 0, K, Y, nullptr); break;
+    case 4: mv_rows<float, 4>(s, a, X, nullptr, 0, 0, K, Y, nullptr); break;
+    case 8: mv_rows<float, 8>(s, a, X, nullptr, 0, 0, K, Y, nullptr); break;
+    case 16: mv_rows<float, 16>(s, a, X, nullptr, 0, 0, K, Y, nullptr); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+namespace {
+
+// G (m x n, FP32) = U W^T, U: m x K, W: n x K (FP32, row-major, K padded).
+// Block: 32 rows x 1024 columns; a thread owns 4 adjacent columns and keeps
+// their K values of W in registers; the block walks its rows together, so a
+// row of U is the same for every lane and arrives by scalar loads as SGPR
+// operands of the FMAs; each row of G is written as whole float4 lines.  FP32
+// products summed over r in order.
+constexpr int kOuterRows = 32;
+template <int K>
+__global__ __launch_bounds__(kThreads) void k_outer(int64_t m, int64_t n, const float *__restrict__ U,
+                                                    const float *__restrict__ W, float *__restrict__ G) {
+  const int64_t c0 = (int64_t(blockIdx.x) * kThreads + threadIdx.x) * 4;
+  const int64_t f0 = int64_t(blockIdx.y) * kOuterRows;
+  if (c0 >= n) return;
+  float w[4][K];
+#pragma unroll
+  for (int j = 0; j < 4; ++j)
+#pragma unroll
+    for (int r = 0; r < K; ++r) w[j][r] = (c0 + j < n) ? W[(c0 + j) * K + r] : 0.f;
+  const bool vec = (n % 4 == 0);
+  const int rows = int(m - f0 < kOuterRows ? m - f0 : kOuterRows);
+  // rows in groups of RG: the group's U rows are loaded together (one wait per
+  // group instead of one per row); rows past the block's end read its first
+  // row and are not stored
+  constexpr int RG = K <= 8 ? 8 : 4;
+  for (int i0 = 0; i0 < rows; i0 += RG) {
+    float u[RG][K];
+#pragma unroll
+    for (int g = 0; g < RG; ++g) {
+      const float *__restrict__ up = U + (f0 + (i0 + g < rows ? i0 + g : 0)) * K;  // uniform
+#pragma unroll
+      for (int r = 0; r < K; ++r) u[g][r] = up[r];
+    }
+#pragma unroll
+    for (int g = 0; g < RG; ++g) {
+      if (i0 + g >= rows) break;
+      float o[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int r = 0; r < K; ++r)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) o[j] += u[g][r] * w[j][r];
+      float *dst = G + (f0 + i0 + g) * n + c0;
+      if (vec) {
+        *reinterpret_cast<float4 *>(dst) = make_float4(o[0], o[1], o[2], o[3]);
+      } else {
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+          if (c0 + j < n) dst[j] = o[j];
+      }
+    }
+  }
+}
+
+}  // namespace
+
+hipError_t outer_update32(hipStream_t s, int64_t m, int64_t n, int K, const float *U, const float *W,
+                          float *G) {
+  const dim3 grid(unsigned((n + 4 * kThreads - 1) / (4 * kThreads)),
+                  unsigned((m + kOuterRows - 1) / kOuterRows));
+  switch (K) {
+    case 2: k_outer<2><<<grid, kThreads, 0, s>>>(m, n, U, W, G); break;
+    case 4: k_outer<4><<<grid, kThreads, 0, s>>>(m, n, U, W, G); break;
+    case 8: k_outer<8><<<grid, kThreads, 0, s>>>(m, n, U, W, G); break;
+    case 16: k_outer<16><<<grid, kThreads, 0, s>>>(m, n, U, W, G); break;
+    case 32: k_outer<32><<<grid, kThreads, 0, s>>>(m, n, U, W, G); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
 // ------------------------------------------------------------------ kernel assembly
 // M[i,j] = -(x_i - y_j).n_i / (4 pi r^3) in FP64, stored FP32; thread per
 // (row, column), rows strided over gridDim.y (at most 65,535 row blocks).

@@ -156,6 +156,7 @@ struct mfs_ctx {
 
   // caller steps (solid angles, single layer): scratch and the index flag
   DevBuf dcaller, dflag;
+  DevBuf dgrad;  // the backward's products (mfs_dense_grad)
 
   // solve workspaces
   DevBuf dvec, dpart, ddpart, dstate, dx64, dr64, dnorm;
@@ -713,6 +714,42 @@ __global__ void k_restore_cols(int64_t n, int K, uint32_t mask, const double *ba
   if (i >= n) return;
   for (int r = 0; r < K; ++r)
     if (mask & (1u << r)) X64[i * K + r] = backup[i * K + r];
+}
+
+// X (n x Kx) = [A | V] padded with zeros (A, V: n x k); with `which` = 1 only
+// A, 2 only V (each padded to Kx).
+__global__ void k_pack_av(int64_t n, const float *A, const float *V, int64_t k, int Kx, int which,
+                          float *X) {
+  const int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x;
+  if (i >= n * Kx) return;
+  const int64_t j = i / Kx, r = i % Kx;
+  float v = 0.f;
+  if (which == 0) v = r < k ? A[j * k + r] : (r < 2 * k ? V[j * k + r - k] : 0.f);
+  else v = r < k ? (which == 1 ? A : V)[j * k + r] : 0.f;
+  X[i] = v;
+}
+
+// The backward's factors: U (m x Ku) = [F - MA | -MV], W (n x Ku) = [V | A]
+// (zero padded), grad_F = MV when asked.  MA, MV: columns ca / cv of Y
+// (m x Ky).
+__global__ void k_grad_uw(int64_t m, int64_t n, int64_t k, const float *F, const float *Y, int Ky,
+                          int ca, int cv, const float *YV, int Kyv, const float *A, const float *V,
+                          int Ku, float *U, float *W, float *grad_F) {
+  const int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x;
+  if (i < m) {
+    for (int r = 0; r < Ku; ++r) {
+      float u = 0.f;
+      if (r < k) u = F[i * k + r] - Y[i * Ky + ca + r];
+      else if (r < 2 * k) u = -YV[i * Kyv + cv + r - k];
+      U[i * Ku + r] = u;
+    }
+    if (grad_F != nullptr)
+      for (int r = 0; r < k; ++r) grad_F[i * k + r] = YV[i * Kyv + cv + r];
+  }
+  if (i < n) {
+    for (int r = 0; r < Ku; ++r)
+      W[i * Ku + r] = r < k ? V[i * k + r] : (r < 2 * k ? A[i * k + r - k] : 0.f);
+  }
 }
 
 // FP64 strided columns <-> the K-padded FP64 iterate (FP64 warm start / output).
@@ -1321,7 +1358,8 @@ void mfs_ctx_destroy(mfs_ctx *ctx) {
                     &ctx->dtotal, &ctx->dbands, &ctx->dcells, &ctx->dcpts, &ctx->dlong, &ctx->dval, &ctx->dtval, &ctx->dMt, &ctx->dval_alt,
                     &ctx->dtval_alt, &ctx->dthg, &ctx->dctr, &ctx->dfscene, &ctx->dvec, &ctx->dpart, &ctx->ddpart, &ctx->dstate,
                     &ctx->dx64, &ctx->dr64, &ctx->dnorm, &ctx->dt32, &ctx->dt64, &ctx->ds32,
-                    &ctx->ds64, &ctx->dapin, &ctx->dapout, &ctx->dcaller, &ctx->dflag, &ctx->dfrec})
+                    &ctx->ds64, &ctx->dapin, &ctx->dapout, &ctx->dcaller, &ctx->dflag, &ctx->dfrec,
+                    &ctx->dgrad})
     b->release();
   if (ctx->hstate) (void)hipHostFree(ctx->hstate);
   if (ctx->hnorm) (void)hipHostFree(ctx->hnorm);
@@ -1641,6 +1679,44 @@ int mfs_dense_normal_apply_noalloc(mfs_ctx *ctx, const float *M, int64_t m, int6
                                                                                        a.pv);
   HIPCHK(hipGetLastError(), "pack input");
   HIPCHK(normal_apply32(ctx->stream, a, int(k), Q, nullptr, nullptr), "normal apply");
+  return MFS_OK;
+}
+
+int mfs_dense_grad(mfs_ctx *ctx, const float *M, int64_t m, int64_t n, const float *F,
+                   const float *A, const float *V, int64_t k, float *grad_M, float *grad_F) {
+  if (ctx == nullptr) return MFS_ERR_ARG;
+  if (M == nullptr || F == nullptr || A == nullptr || V == nullptr || m <= 0 || n <= 0 || k <= 0 ||
+      k > kMaxRhs || m > int64_t(65535) * 32 || n > INT32_MAX)
+    return ctx->fail(MFS_ERR_ARG, "dense_grad: bad arguments (1 <= k <= 16, m <= 2,097,120)");
+  if (grad_M == nullptr && grad_F == nullptr) return MFS_OK;
+  HIPCHK(hipSetDevice(ctx->device), "hipSetDevice");
+  // One row pass over M for [MA | MV] when 2k <= 8, else one each (the row
+  // pass of batches of 8 runs at 7 TB/s, the one of batches of 16 at a third
+  // of that: two passes of 8 are faster than one of 16).
+  const bool one = 2 * k <= 8;
+  const int Kx = one ? batch_width(2 * k) : batch_width(k);
+  const int Ku = 2 * k <= 2 ? 2 : 2 * k <= 4 ? 4 : 2 * k <= 8 ? 8 : 2 * k <= 16 ? 16 : 32;
+  const size_t xs = size_t(n) * Kx, ys = size_t(m) * Kx;
+  const size_t floats = (one ? xs + ys : 2 * (xs + ys)) + size_t(m) * Ku + size_t(n) * Ku;
+  HIPCHK(ctx->dgrad.ensure(floats * sizeof(float) + 256), "alloc gradient workspace");
+  float *X = ctx->dgrad.as<float>(), *Y = X + xs;
+  float *X2 = one ? X : Y + ys, *Y2 = one ? Y : X2 + xs;
+  float *U = Y2 + ys, *W = U + size_t(m) * Ku;
+  const unsigned gx = unsigned((n * Kx + kThreads - 1) / kThreads);
+  if (one) {
+    k_pack_av<<<gx, kThreads, 0, ctx->stream>>>(n, A, V, k, Kx, 0, X);
+    HIPCHK(dense_rows32(ctx->stream, M, m, n, X, Kx, Y), "row pass");
+  } else {
+    k_pack_av<<<gx, kThreads, 0, ctx->stream>>>(n, A, V, k, Kx, 1, X);
+    HIPCHK(dense_rows32(ctx->stream, M, m, n, X, Kx, Y), "row pass");
+    k_pack_av<<<gx, kThreads, 0, ctx->stream>>>(n, A, V, k, Kx, 2, X2);
+    HIPCHK(dense_rows32(ctx->stream, M, m, n, X2, Kx, Y2), "row pass");
+  }
+  const int64_t rows = std::max(m, n);
+  k_grad_uw<<<unsigned((rows + kThreads - 1) / kThreads), kThreads, 0, ctx->stream>>>(
+      m, n, k, F, Y, Kx, 0, one ? int(k) : 0, Y2, Kx, A, V, Ku, U, W, grad_F);
+  HIPCHK(hipGetLastError(), "gradient factors");
+  if (grad_M != nullptr) HIPCHK(outer_update32(ctx->stream, m, n, Ku, U, W, grad_M), "outer product");
   return MFS_OK;
 }
 

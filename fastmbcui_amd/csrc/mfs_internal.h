@@ -243,6 +243,13 @@ template <typename TA>
 hipError_t points_cols_reduce(hipStream_t st, const PointsOp &op, int K, const double *part,
                               TA *out);
 
+// Y (m x K) = M X for FP32 X (n x K, K in {1, 2, 4, 8, 16}) through the row pass.
+hipError_t dense_rows32(hipStream_t st, const float *M, int64_t m, int64_t n, const float *X, int K,
+                        float *Y);
+// G (m x n) = U W^T, U m x K, W n x K, K in {2, 4, 8, 16, 32}, FP32.
+hipError_t outer_update32(hipStream_t st, int64_t m, int64_t n, int K, const float *U, const float *W,
+                          float *G);
+
 // Kernel assembly of M from FP64 points.
 hipError_t launch_assemble(hipStream_t st, const double *x, const double *nx, const double *y,
                            int64_t m, int64_t n, float *M);

@@ -162,7 +162,9 @@ def _(M, pL, Vs_s, rho, pcg_max_iter, tol_r, timing_verbose=False, solve_verbose
 #   dL/dF = M V,   dL/dM = F V^T - M (A V^T + V A^T)
 # (core/mfs_torch_py/ops.py:42-58).  The reference forms W = A V^T (n x n) and
 # M (W + W^T) (m n^2 flops); here the same quantity is evaluated as
-# (F - M A) V^T - (M V) A^T, two rank-k products, and M V is shared with dL/dF.
+# (F - M A) V^T - (M V) A^T, M V shared with dL/dF: mfs_dense_grad reads M for
+# M A and M V (once when 2k <= 8, twice above) and writes dL/dM once, as one
+# rank-2k outer product.
 def _setup_context(ctx, inputs, output):
     M, F, Vs_s, rho, pcg_max_iter, tol_r, timing_verbose, solve_verbose = inputs
     if ctx.needs_input_grad[0] or ctx.needs_input_grad[1]:
@@ -184,11 +186,17 @@ def _backward(ctx, grad):
         VpL = torch.ops.mfs_torch.VpL_fast_float.default(M, grad.contiguous(), Vs_s, ctx.rho,
                                                          ctx.pcg_max_iter, ctx.tol_r,
                                                          ctx.timing_verbose, ctx.solve_verbose)
-        MV = M @ VpL
-        if ctx.needs_input_grad[0]:
-            grad_M = (F - M @ A) @ VpL.T - MV @ A.T
-        if ctx.needs_input_grad[1]:
-            grad_F = MV
+        if M.dtype == torch.float32 and M.is_cuda and VpL.shape[1] <= 16:
+            # row passes over M for M A and M V, one pass writing grad_M
+            dctx = runtime.caller_context(M.device)
+            grad_M, grad_F = dctx.dense_grad(M.contiguous(), F, A, VpL, ctx.needs_input_grad[0],
+                                             ctx.needs_input_grad[1])
+        else:
+            MV = M @ VpL
+            if ctx.needs_input_grad[0]:
+                grad_M = (F - M @ A) @ VpL.T - MV @ A.T
+            if ctx.needs_input_grad[1]:
+                grad_F = MV
     return grad_M, grad_F, None, None, None, None, None, None
 
 

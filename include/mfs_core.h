@@ -193,6 +193,18 @@ int mfs_dense_normal_apply(mfs_ctx *ctx, const float *M, int64_t m, int64_t n, c
 int mfs_dense_normal_apply_noalloc(mfs_ctx *ctx, const float *M, int64_t m, int64_t n,
                                    const float *X, int64_t k, double *Q);
 
+/* The backward of MinvF_float_fastGrad without the n x n matrix
+ * W = A V^T the reference forms (core/mfs_torch_py/ops.py:42-58):
+ *   grad_F = M V,   grad_M = (F - M A) V^T - (M V) A^T
+ * (= F V^T - M (W + W^T)), with V = (M^T M)^{-1} G the VpL solve of the
+ * incoming gradient and A the forward's output.  One pass over M gives M A and
+ * M V together (two when 2k > 8), one pass writes grad_M (a rank-2k outer
+ * product).  M: m x n, F: m x k, A and V: n x k, grad_M: m x n, grad_F: m x k;
+ * row-major FP32 device arrays, 1 <= k <= 16; grad_M or grad_F may be NULL.
+ * Asynchronous on the context's stream. */
+int mfs_dense_grad(mfs_ctx *ctx, const float *M, int64_t m, int64_t n, const float *F,
+                   const float *A, const float *V, int64_t k, float *grad_M, float *grad_F);
+
 /* Dense point-kernel matrix M[i,j] = -(x_i - y_j).n_i / (4 pi |x_i - y_j|^3),
  * evaluated in FP64 and stored FP32.  x, nx: m x 3, y: n x 3, device FP64.
  * Point form of src/kernels.h:24-40 (laplace_nm_3d_sl, up to the factor

@@ -545,3 +545,26 @@ def test_supernode_above_256_rows_is_rejected():
     with pytest.raises(_lib.MfsError) as e:
         ctx.plan(sc.y, 8.0)
     assert e.value.code == _lib.MFS_ERR_ARG and "256" in str(e.value)
+
+
+@pytest.mark.parametrize("m,n,k", [(700, 512, 3), (1301, 1027, 8), (517, 300, 11), (64, 40, 16)])
+def test_dense_grad_matches_fp64_formula(m, n, k):
+    """mfs_dense_grad (one pass over M for M A and M V, one rank-2k write of
+    grad_M) against the backward formula in FP64: grad_F = M V,
+    grad_M = F V^T - M (A V^T + V A^T) (core/mfs_torch_py/ops.py:42-58);
+    2k <= 16 and 2k > 16 (two row passes), ragged n."""
+    g = torch.Generator().manual_seed(m + n + k)
+    M = torch.randn(m, n, generator=g)
+    F = torch.randn(m, k, generator=g)
+    A = torch.randn(n, k, generator=g)
+    V = torch.randn(n, k, generator=g)
+    ctx = _ctx()
+    gM, gF = ctx.dense_grad(M.to(DEV), F.to(DEV), A.to(DEV), V.to(DEV))
+    M64, F64, A64, V64 = (t.double() for t in (M, F, A, V))
+    W = A64 @ V64.T
+    gM_ref = F64 @ V64.T - M64 @ (W + W.T)
+    gF_ref = M64 @ V64
+    assert ((gF.cpu().double() - gF_ref).norm() / gF_ref.norm()).item() < 1e-6
+    assert ((gM.cpu().double() - gM_ref).norm() / gM_ref.norm()).item() < 1e-5
+    gM2, gF2 = ctx.dense_grad(M.to(DEV), F.to(DEV), A.to(DEV), V.to(DEV), need_m=False)
+    assert gM2 is None and torch.equal(gF2, gF)
