@@ -9,16 +9,18 @@
 //     R^T w_b, the boundary datum b_j, and the max over the 16 nearest sources
 //     of the same body of (r_k - r_j, n_k, b_k)        (kNN max-pool, 7)
 //   MLP: 20 -> 128 -> 128 -> 128 -> 6, ReLU; BF16 weights (FP32 master copy
-//     for Adam), FP32 activations and accumulation
+//     for Adam), FP32 activations rounded to BF16 as matrix-product operands,
+//     FP32 accumulation
 //   head: x_j = sum_d out_jd q_jd,  q_j = (R^T v_b, R^T w_b)
 // The head is linear in the velocities as the strengths are; in the body
 // frame every body of one shape looks alike, so the network learns one
 // response per shape instead of every orientation.
 //
-// Kernels: feature assembly, a tiled GEMM with fused bias/ReLU (forward,
-// input gradients, weight gradients with split-K), bias gradients, the
-// head + loss, Adam, and a fused inference kernel (whole MLP per 64 points,
-// weights in LDS).
+// Kernels: feature assembly; BF16 matrix-core GEMMs with fused bias/ReLU
+// (forward, input gradients) and the weight and bias gradients as fixed-size
+// point-chunk partials summed in chunk order (deterministic: no atomics);
+// the head + loss (block partials, summed in a fixed order); Adam; and a
+// fused inference kernel (whole MLP per 64 points on the matrix cores).
 #include "mfs_internal.h"
 
 namespace mfs {
@@ -109,81 +111,185 @@ __global__ __launch_bounds__(kThreads) void k_features(
   for (int e = 0; e < 7; ++e) f[13 + e] = found ? pool[e] : 0.f;
 }
 
-// ------------------------------------------------------------------ GEMM
-// C[M x N] = op(A) op(B) (+ bias) (ReLU), or C += op(A) op(B) with split-K
-// (gridDim.z > 1, atomic adds).  op(A) is M x K: A row-major M x K (lda), or
-// its transpose stored K x M when TA.  op(B) is K x N: B row-major K x N, or
-// stored N x K when TB.  A is FP32; B is FP32 or BF16 (uint16_t).
-template <bool TA, bool TB, typename BT>
-__global__ __launch_bounds__(kThreads) void k_gemm(int64_t M, int64_t N, int64_t K,
-                                                   const float *__restrict__ A, int64_t lda,
-                                                   const BT *__restrict__ B, int64_t ldb,
-                                                   float *__restrict__ C, int64_t ldc,
-                                                   const float *__restrict__ bias, int relu,
-                                                   int64_t kchunk) {
-  constexpr int TM = 64, TN = 64, TK = 16;
-  __shared__ float As[TK][TM + 4];
-  __shared__ float Bs[TK][TN + 4];
-  const int tid = threadIdx.x;
-  const int tx = tid % 16, ty = tid / 16;  // 4 x 4 outputs per thread
-  const int64_t m0 = int64_t(blockIdx.y) * TM, n0 = int64_t(blockIdx.x) * TN;
-  const int64_t kb = int64_t(blockIdx.z) * kchunk;
-  const int64_t ke = (kb + kchunk < K) ? kb + kchunk : K;
-  float acc[4][4] = {};
-  for (int64_t k0 = kb; k0 < ke; k0 += TK) {
-    for (int idx = tid; idx < TK * TM; idx += kThreads) {
-      const int kk = idx / TM, mm = idx % TM;
-      const int64_t gm = m0 + mm, gk = k0 + kk;
-      float v = 0.f;
-      if (gm < M && gk < ke) v = TA ? A[gk * lda + gm] : A[gm * lda + gk];
-      As[kk][mm] = v;
+// ------------------------------------------------------------------ GEMMs
+// BF16 matrix-core products with FP32 accumulation: both operands are BF16
+// (the weights as stored, the FP32 activations and gradients rounded to
+// nearest even as they are loaded), so every product is exact in FP32.
+//
+// v_mfma_f32_16x16x16bf16_1k, K = 16 in four groups of four per lane:
+//   A[i][k] in lane i + 16 (k / 4), element k % 4
+//   B[k][j] in lane j + 16 (k / 4), element k % 4
+//   D[i][j] in lane j + 16 (i / 4), element i % 4
+typedef short s4 __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+__device__ inline v4f mfma_bf16(s4 a, s4 b, v4f c) {
+  return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+}
+__device__ inline s4 pack_bf16(float a, float b, float c, float d) {
+  return s4{short(f32_to_bf16(a)), short(f32_to_bf16(b)), short(f32_to_bf16(c)), short(f32_to_bf16(d))};
+}
+
+constexpr int kGM = 64;     // rows of C per block (16 per wave)
+constexpr int kMaxN = 128;  // widest layer
+constexpr int kMaxK = 128;
+constexpr int kChunk = 256; // points per weight-gradient partial (fixed: the sum order never
+                            // depends on the device)
+constexpr int kSub = 32;    // points staged in LDS per round
+
+// C[M x N] = bf(A)[M x K] Bop (+ bias) (ReLU), N, K <= 128.  Bop[k][j] is the
+// BF16 weight B[k ldb + j] (forward: W is K x N) or, with TB, B[j ldb + k]
+// (input gradient G W^T: W is N x K).  One block per 64 rows; Bop sits in
+// LDS as Bs[j][k] so that a lane's four k values are one 8-byte read.
+template <bool TB>
+__global__ __launch_bounds__(kThreads) void k_gemm_bf16(int64_t M, int N, int K,
+                                                        const float *__restrict__ A, int64_t lda,
+                                                        const uint16_t *__restrict__ B, int64_t ldb,
+                                                        float *__restrict__ C, int64_t ldc,
+                                                        const float *__restrict__ bias, int relu) {
+  constexpr int kLd = kMaxK + 4;  // 264-byte rows
+  __shared__ __align__(16) uint16_t Bs[kMaxN][kLd];
+  const int Kp = (K + 15) / 16 * 16, Np = (N + 15) / 16 * 16;
+  for (int idx = threadIdx.x; idx < Np * Kp; idx += kThreads) {
+    const int j = TB ? idx / Kp : idx % Np;  // consecutive threads read consecutive addresses
+    const int k = TB ? idx % Kp : idx / Np;
+    uint16_t v = 0;
+    if (j < N && k < K) v = TB ? B[j * ldb + k] : B[k * ldb + j];
+    Bs[j][k] = v;
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int li = lane & 15, lk = (lane >> 4) * 4;
+  const int64_t r0 = int64_t(blockIdx.x) * kGM + w * 16;
+  const int64_t row = r0 + li;
+  const float *arow = A + (row < M ? row : 0) * lda;
+  const int NT = Np / 16;
+  v4f acc[kMaxN / 16];
+#pragma unroll
+  for (int t = 0; t < kMaxN / 16; ++t) acc[t] = v4f{0.f, 0.f, 0.f, 0.f};
+  for (int k0 = 0; k0 < Kp; k0 += 16) {
+    float a[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int k = k0 + lk + v;
+      a[v] = (row < M && k < K) ? arow[k] : 0.f;
     }
-    for (int idx = tid; idx < TK * TN; idx += kThreads) {
-      const int kk = idx / TN, nn = idx % TN;
-      const int64_t gn = n0 + nn, gk = k0 + kk;
-      float v = 0.f;
-      if (gn < N && gk < ke) {
-        const BT e = TB ? B[gn * ldb + gk] : B[gk * ldb + gn];
-        if constexpr (sizeof(BT) == 2)
-          v = bf16_to_f32(e);
-        else
-          v = e;
-      }
-      Bs[kk][nn] = v;
+    const s4 af = pack_bf16(a[0], a[1], a[2], a[3]);
+#pragma unroll
+    for (int t = 0; t < kMaxN / 16; ++t)
+      if (t < NT) acc[t] = mfma_bf16(af, *reinterpret_cast<const s4 *>(&Bs[t * 16 + li][k0 + lk]), acc[t]);
+  }
+#pragma unroll
+  for (int t = 0; t < kMaxN / 16; ++t) {
+    if (t >= NT) continue;
+    const int c = t * 16 + li;
+    if (c >= N) continue;
+    const float bc = bias ? bias[c] : 0.f;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int64_t r = r0 + lk + v;
+      if (r >= M) continue;
+      float x = acc[t][v] + bc;
+      if (relu) x = fmaxf(x, 0.f);
+      C[r * ldc + c] = x;
+    }
+  }
+}
+
+// Weight and bias gradients of one layer, chunk c = blockIdx.x of kChunk
+// points:  part[c][i N + j] = sum_p bf(A[p][i]) bf(G[p][j])  (K x N <= 128 x 128)
+//          partb[c][j]     = sum_p G[p][j]                  (FP32, in point order)
+// Points are staged kSub at a time in LDS, transposed (As[i][p], Gs[j][p]) so
+// that a lane's four points are one 8-byte read; wave w owns output tiles
+// w, w + 4, ... of the (K / 16) x (N / 16) tile grid.
+__global__ __launch_bounds__(kThreads) void k_wgrad_bf16(int64_t P, int K, int N,
+                                                         const float *__restrict__ A, int64_t lda,
+                                                         const float *__restrict__ G, int64_t ldg,
+                                                         float *__restrict__ part,
+                                                         float *__restrict__ partb) {
+  constexpr int kLd = kSub + 4;  // 72-byte rows
+  __shared__ __align__(16) uint16_t As[kMaxK][kLd];
+  __shared__ __align__(16) uint16_t Gs[kMaxN][kLd];
+  const int Kp = (K + 15) / 16 * 16, Np = (N + 15) / 16 * 16;
+  const int NT = Np / 16, tiles = (Kp / 16) * NT;
+  const int64_t p0 = int64_t(blockIdx.x) * kChunk;
+  const int64_t p1 = (p0 + kChunk < P) ? p0 + kChunk : P;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int li = lane & 15, lk = (lane >> 4) * 4;
+  constexpr int kTilesPerWave = (kMaxK / 16) * (kMaxN / 16) / 4;
+  v4f acc[kTilesPerWave];
+#pragma unroll
+  for (int s = 0; s < kTilesPerWave; ++s) acc[s] = v4f{0.f, 0.f, 0.f, 0.f};
+  for (int64_t s0 = p0; s0 < p1; s0 += kSub) {
+    __syncthreads();  // the previous round's reads are done
+    for (int idx = threadIdx.x; idx < kSub * Kp; idx += kThreads) {
+      const int pp = idx / Kp, i = idx % Kp;
+      const int64_t p = s0 + pp;
+      As[i][pp] = (p < p1 && i < K) ? f32_to_bf16(A[p * lda + i]) : uint16_t(0);
+    }
+    for (int idx = threadIdx.x; idx < kSub * Np; idx += kThreads) {
+      const int pp = idx / Np, j = idx % Np;
+      const int64_t p = s0 + pp;
+      Gs[j][pp] = (p < p1 && j < N) ? f32_to_bf16(G[p * ldg + j]) : uint16_t(0);
     }
     __syncthreads();
 #pragma unroll
-    for (int kk = 0; kk < TK; ++kk) {
-      float a[4], bb[4];
+    for (int kk = 0; kk < kSub; kk += 16) {
 #pragma unroll
-      for (int i = 0; i < 4; ++i) a[i] = As[kk][ty * 4 + i];
-#pragma unroll
-      for (int jj = 0; jj < 4; ++jj) bb[jj] = Bs[kk][tx * 4 + jj];
-#pragma unroll
-      for (int i = 0; i < 4; ++i)
-#pragma unroll
-        for (int jj = 0; jj < 4; ++jj) acc[i][jj] += a[i] * bb[jj];
-    }
-    __syncthreads();
-  }
-  const bool split = gridDim.z > 1;
-#pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    const int64_t gm = m0 + ty * 4 + i;
-    if (gm >= M) continue;
-#pragma unroll
-    for (int jj = 0; jj < 4; ++jj) {
-      const int64_t gn = n0 + tx * 4 + jj;
-      if (gn >= N) continue;
-      if (split) {
-        atomicAdd(C + gm * ldc + gn, acc[i][jj]);
-      } else {
-        float v = acc[i][jj] + (bias ? bias[gn] : 0.f);
-        if (relu) v = fmaxf(v, 0.f);
-        C[gm * ldc + gn] = v;
+      for (int s = 0; s < kTilesPerWave; ++s) {
+        const int t = w + 4 * s;
+        if (t < tiles) {
+          const int ti = t / NT, tj = t - ti * NT;
+          const s4 af = *reinterpret_cast<const s4 *>(&As[ti * 16 + li][kk + lk]);
+          const s4 bf = *reinterpret_cast<const s4 *>(&Gs[tj * 16 + li][kk + lk]);
+          acc[s] = mfma_bf16(af, bf, acc[s]);
+        }
       }
     }
   }
+  float *pc = part + int64_t(blockIdx.x) * K * N;
+#pragma unroll
+  for (int s = 0; s < kTilesPerWave; ++s) {
+    const int t = w + 4 * s;
+    if (t >= tiles) continue;
+    const int ti = t / NT, tj = t - ti * NT;
+    const int j = tj * 16 + li;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int i = ti * 16 + lk + v;
+      if (i < K && j < N) pc[i * N + j] = acc[s][v];
+    }
+  }
+  if (int(threadIdx.x) < N) {
+    float sb = 0.f;
+    for (int64_t p = p0; p < p1; ++p) sb += G[p * ldg + threadIdx.x];
+    partb[int64_t(blockIdx.x) * N + threadIdx.x] = sb;
+  }
+}
+
+// out[e] = sum over chunks c (in order) of part[c][e], e < count.
+__global__ void k_sum_chunks(int64_t chunks, int64_t count, const float *__restrict__ part,
+                             float *__restrict__ out) {
+  const int64_t e = int64_t(blockIdx.x) * kThreads + threadIdx.x;
+  if (e >= count) return;
+  float s = 0.f;
+  for (int64_t c = 0; c < chunks; ++c) s += part[c * count + e];
+  out[e] = s;
+}
+
+// *acc += sum of partial[0 .. count) in a fixed order (one block).
+__global__ void k_sum_partials(int64_t count, const double *__restrict__ partial,
+                               double *__restrict__ acc) {
+  __shared__ double red[kThreads];
+  double s = 0.0;
+  for (int64_t i = threadIdx.x; i < count; i += kThreads) s += partial[i];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int o = kThreads / 2; o > 0; o >>= 1) {
+    if (int(threadIdx.x) < o) red[threadIdx.x] += red[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *acc += red[0];
 }
 
 // G[i] *= (A[i] > 0): ReLU backward through the stored activation.
@@ -192,23 +298,12 @@ __global__ void k_relu_mask(int64_t count, const float *__restrict__ A, float *_
   if (i < count && !(A[i] > 0.f)) G[i] = 0.f;
 }
 
-// db[c] += sum_p G[p][c] (one block per 256 columns x row slice, atomics).
-__global__ void k_bias_grad(int64_t P, int64_t N, const float *__restrict__ G, int64_t rows_per,
-                            float *__restrict__ db) {
-  const int64_t c = int64_t(blockIdx.x) * kThreads + threadIdx.x;
-  if (c >= N) return;
-  const int64_t p0 = int64_t(blockIdx.y) * rows_per;
-  const int64_t p1 = (p0 + rows_per < P) ? p0 + rows_per : P;
-  float s = 0.f;
-  for (int64_t p = p0; p < p1; ++p) s += G[p * N + c];
-  atomicAdd(db + c, s);
-}
-
-// Head and loss: x_j = out_j . q_j; with a target, loss += (x_j - t_j)^2 * w
-// and dout_j = 2 w (x_j - t_j) q_j.
+// Head and loss: x_j = out_j . q_j; with a target, block b's share of the
+// loss, sum over its points of (x_j - t_j)^2 w, goes to lpart[b] (summed in
+// a fixed order afterwards), and dout_j = 2 w (x_j - t_j) q_j.
 __global__ void k_head(int64_t P, const float *__restrict__ out, const float *__restrict__ q,
                        const float *__restrict__ target, float w, float *__restrict__ x,
-                       float *__restrict__ dout, double *__restrict__ loss) {
+                       float *__restrict__ dout, double *__restrict__ lpart) {
   __shared__ double red[kThreads];
   const int64_t j = int64_t(blockIdx.x) * kThreads + threadIdx.x;
   double l = 0.0;
@@ -228,7 +323,7 @@ __global__ void k_head(int64_t P, const float *__restrict__ out, const float *__
     if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
     __syncthreads();
   }
-  if (threadIdx.x == 0 && loss) atomicAdd(loss, red[0]);
+  if (threadIdx.x == 0 && lpart) lpart[blockIdx.x] = red[0];
 }
 
 // Adam on the FP32 master weights, refreshing the BF16 copy.
@@ -254,52 +349,101 @@ __global__ void k_to_bf16(int64_t count, const float *__restrict__ w, uint16_t *
   if (i < count) wb[i] = f32_to_bf16(w[i]);
 }
 
-// Fused inference: 64 points per block, the whole MLP with BF16 weights in
-// LDS (as FP32 after conversion per use), activations in LDS, and the head.
-// Dimensions fixed to the network above (20 -> 128 x 3 -> 6).
+// Fused inference: the whole MLP and the head for 64 points per block (16
+// per wave) on the matrix cores.  The BF16 weights are staged once per block
+// in LDS, transposed (Wt[j][k]) so that a lane's four k values are one 8-byte
+// read; each wave keeps its points' layer inputs in LDS as BF16 -- the same
+// rounding, products and summation order as the layer-by-layer GEMMs, so
+// both paths give the same values.  Dimensions fixed to the network above
+// (20 -> 128 x 3 -> 6); layer 0's 20 inputs are padded to 32 with zeros.
 constexpr int kIH = 128, kIP = 64;
+constexpr int kLdI = kIH + 4;         // activation / weight row, BF16 elements (264 B)
+constexpr int kLdI0 = 32 + 4;         // layer-0 weight row (72 B)
+constexpr size_t kOffW0 = 0;
+constexpr size_t kOffW1 = kOffW0 + size_t(kIH) * kLdI0 * 2;
+constexpr size_t kOffW2 = kOffW1 + size_t(kIH) * kLdI * 2;
+constexpr size_t kOffW3 = kOffW2 + size_t(kIH) * kLdI * 2;
+constexpr size_t kOffH = kOffW3 + size_t(16) * kLdI * 2;
+constexpr size_t kOffO = kOffH + size_t(4) * 2 * 16 * kLdI * 2;
+constexpr size_t kInferLds = kOffO + size_t(4) * 16 * 8 * sizeof(float);
 
 __global__ __launch_bounds__(kThreads) void k_infer(int64_t P, const float *__restrict__ feat,
                                                     const float *__restrict__ q,
                                                     const uint16_t *__restrict__ wb,
                                                     const float *__restrict__ bias_all,
                                                     float *__restrict__ x) {
-  extern __shared__ unsigned char smem[];
-  uint16_t *W = reinterpret_cast<uint16_t *>(smem);  // all weights (bf16)
-  constexpr int nW = kFeat * kIH + 2 * kIH * kIH + kIH * 6;
-  float *H0 = reinterpret_cast<float *>(smem + ((nW * 2 + 15) / 16) * 16);
-  float *H1 = H0 + kIP * kIH;
-  for (int i = threadIdx.x; i < nW; i += kThreads) W[i] = wb[i];
-  const int64_t p0 = int64_t(blockIdx.x) * kIP;
-  for (int i = threadIdx.x; i < kIP * kFeat; i += kThreads) {
-    const int64_t p = p0 + i / kFeat;
-    H1[i] = (p < P) ? feat[p * kFeat + i % kFeat] : 0.f;
+  extern __shared__ __align__(16) unsigned char smem[];
+  uint16_t *Wt[4] = {reinterpret_cast<uint16_t *>(smem + kOffW0), reinterpret_cast<uint16_t *>(smem + kOffW1),
+                     reinterpret_cast<uint16_t *>(smem + kOffW2), reinterpret_cast<uint16_t *>(smem + kOffW3)};
+  const int din[4] = {kFeat, kIH, kIH, kIH};
+  const int dn[4] = {kIH, kIH, kIH, 6};
+  const int kp[4] = {32, kIH, kIH, kIH};
+  const int np[4] = {kIH, kIH, kIH, 16};
+  const int ldw[4] = {kLdI0, kLdI, kLdI, kLdI};
+  {
+    int64_t off = 0;
+#pragma unroll
+    for (int l = 0; l < 4; ++l) {
+      for (int idx = threadIdx.x; idx < np[l] * kp[l]; idx += kThreads) {
+        const int j = idx % np[l], k = idx / np[l];
+        Wt[l][j * ldw[l] + k] = (j < dn[l] && k < din[l]) ? wb[off + int64_t(k) * dn[l] + j] : uint16_t(0);
+      }
+      off += int64_t(din[l]) * dn[l];
+    }
+  }
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int li = lane & 15, lk = (lane >> 4) * 4;
+  const int64_t pb = int64_t(blockIdx.x) * kIP + w * 16;
+  uint16_t *Hin = reinterpret_cast<uint16_t *>(smem + kOffH) + w * 2 * 16 * kLdI;
+  uint16_t *Hout = Hin + 16 * kLdI;
+  float *O = reinterpret_cast<float *>(smem + kOffO) + w * 16 * 8;
+  for (int e = lane; e < 16 * 32; e += 64) {
+    const int pt = e / 32, k = e % 32;
+    const int64_t gp = pb + pt;
+    Hin[pt * kLdI + k] = (gp < P && k < kFeat) ? f32_to_bf16(feat[gp * kFeat + k]) : uint16_t(0);
   }
   __syncthreads();
-  const uint16_t *Wl[4] = {W, W + kFeat * kIH, W + kFeat * kIH + kIH * kIH,
-                           W + kFeat * kIH + 2 * kIH * kIH};
-  const float *bl[4] = {bias_all, bias_all + kIH, bias_all + 2 * kIH, bias_all + 3 * kIH};
-  const int din[4] = {kFeat, kIH, kIH, kIH};
-  const int dout[4] = {kIH, kIH, kIH, 6};
-  float *in = H1, *out = H0;
+  const float *bl = bias_all;
+#pragma unroll
   for (int l = 0; l < 4; ++l) {
-    for (int idx = threadIdx.x; idx < kIP * dout[l]; idx += kThreads) {
-      const int p = idx / dout[l], o = idx % dout[l];
-      float s = bl[l][o];
-      for (int i = 0; i < din[l]; ++i) s += in[p * din[l] + i] * bf16_to_f32(Wl[l][i * dout[l] + o]);
-      out[p * dout[l] + o] = (l < 3) ? fmaxf(s, 0.f) : s;
+    const int NT = np[l] / 16;
+    v4f acc[kIH / 16];
+#pragma unroll
+    for (int t = 0; t < kIH / 16; ++t) acc[t] = v4f{0.f, 0.f, 0.f, 0.f};
+    for (int k0 = 0; k0 < kp[l]; k0 += 16) {
+      const s4 af = *reinterpret_cast<const s4 *>(Hin + li * kLdI + k0 + lk);
+#pragma unroll
+      for (int t = 0; t < kIH / 16; ++t)
+        if (t < NT)
+          acc[t] = mfma_bf16(af, *reinterpret_cast<const s4 *>(Wt[l] + (t * 16 + li) * ldw[l] + k0 + lk), acc[t]);
     }
+#pragma unroll
+    for (int t = 0; t < kIH / 16; ++t) {
+      if (t >= NT) continue;
+      const int j = t * 16 + li;
+      if (j >= dn[l]) continue;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const float z = acc[t][v] + bl[j];
+        if (l < 3)
+          Hout[(lk + v) * kLdI + j] = f32_to_bf16(fmaxf(z, 0.f));
+        else
+          O[(lk + v) * 8 + j] = z;
+      }
+    }
+    bl += dn[l];
     __syncthreads();
-    float *t = in;
-    in = out;
-    out = t;
+    uint16_t *t = Hin;
+    Hin = Hout;
+    Hout = t;
   }
-  for (int p = threadIdx.x; p < kIP; p += kThreads) {
-    const int64_t gp = p0 + p;
-    if (gp >= P) continue;
-    float s = 0.f;
-    for (int d = 0; d < 6; ++d) s += in[p * 6 + d] * q[gp * 6 + d];
-    x[gp] = s;
+  if (lane < 16) {
+    const int64_t gp = pb + lane;
+    if (gp < P) {
+      float s = 0.f;
+      for (int d = 0; d < 6; ++d) s += O[lane * 8 + d] * q[gp * 6 + d];
+      x[gp] = s;
+    }
   }
 }
 
@@ -308,7 +452,8 @@ __global__ void k_scale(int64_t P, float s, float *__restrict__ x) {
   if (i < P) x[i] *= s;
 }
 
-__global__ void k_sqnorm(int64_t P, const float *__restrict__ r, float w, double *__restrict__ acc) {
+// lpart[b] = w * sum of r_i^2 over block b's entries.
+__global__ void k_sqnorm(int64_t P, const float *__restrict__ r, float w, double *__restrict__ lpart) {
   __shared__ double red[kThreads];
   const int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x;
   const double v = (i < P) ? double(r[i]) : 0.0;
@@ -318,7 +463,7 @@ __global__ void k_sqnorm(int64_t P, const float *__restrict__ r, float w, double
     if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
     __syncthreads();
   }
-  if (threadIdx.x == 0) atomicAdd(acc, red[0]);
+  if (threadIdx.x == 0) lpart[blockIdx.x] = red[0];
 }
 
 __global__ void k_fit_dout(int64_t P, const float *__restrict__ g, const float *__restrict__ q,
@@ -338,8 +483,10 @@ hipError_t mlp_scale(hipStream_t st, int64_t P, float s, float *x) {
   return hipGetLastError();
 }
 
-hipError_t mlp_sqnorm(hipStream_t st, int64_t P, const float *r, float w, double *acc) {
-  k_sqnorm<<<blocks_for(P), kThreads, 0, st>>>(P, r, w, acc);
+hipError_t mlp_sqnorm(hipStream_t st, int64_t P, const float *r, float w, double *lpart,
+                      double *acc) {
+  k_sqnorm<<<blocks_for(P), kThreads, 0, st>>>(P, r, w, lpart);
+  k_sum_partials<<<1, kThreads, 0, st>>>(int64_t(blocks_for(P)), lpart, acc);
   return hipGetLastError();
 }
 
@@ -360,31 +507,34 @@ hipError_t mlp_features(hipStream_t st, int64_t n, const double *y, const double
   return hipGetLastError();
 }
 
-// op: 0 = C = A B (+bias, relu), A P x K FP32, B K x N BF16
-//     1 = C += A^T G (split-K), A P x M FP32, G P x N FP32, C M x N FP32
-//     2 = C = G B^T, G P x N FP32, B K x N BF16 -> C P x K
+// op 0: C = bf(A) B (+bias, ReLU), A M x K FP32 (lda), B K x N BF16 (ldb)
+// op 2: C = bf(A) B^T, A M x K FP32, B N x K BF16 (ldb) -> C M x N
 hipError_t mlp_gemm(hipStream_t st, int op, int64_t M, int64_t N, int64_t K, const float *A,
-                    int64_t lda, const void *B, int64_t ldb, float *C, int64_t ldc,
+                    int64_t lda, const uint16_t *B, int64_t ldb, float *C, int64_t ldc,
                     const float *bias, int relu) {
-  const dim3 tiles(unsigned((N + 63) / 64), unsigned((M + 63) / 64), 1);
-  if (op == 0) {
-    k_gemm<false, false, uint16_t><<<tiles, kThreads, 0, st>>>(
-        M, N, K, A, lda, static_cast<const uint16_t *>(B), ldb, C, ldc, bias, relu, K);
-  } else if (op == 1) {
-    // K is the point count: split it so the chip is busy (about 1024 blocks)
-    const int64_t nt = int64_t(tiles.x) * tiles.y;
-    int64_t splits = (1024 + nt - 1) / nt;
-    int64_t kchunk = (K + splits - 1) / splits;
-    kchunk = (kchunk + 15) / 16 * 16;
-    splits = (K + kchunk - 1) / kchunk;
-    const dim3 g(tiles.x, tiles.y, unsigned(splits < 1 ? 1 : splits));
-    k_gemm<true, false, float><<<g, kThreads, 0, st>>>(M, N, K, A, lda,
-                                                       static_cast<const float *>(B), ldb, C, ldc,
-                                                       nullptr, 0, kchunk);
-  } else {
-    k_gemm<false, true, uint16_t><<<tiles, kThreads, 0, st>>>(
-        M, N, K, A, lda, static_cast<const uint16_t *>(B), ldb, C, ldc, nullptr, 0, K);
-  }
+  if (N < 1 || N > kMaxN || K < 1 || K > kMaxK || M < 1 || (op != 0 && op != 2))
+    return hipErrorInvalidValue;
+  const unsigned blocks = unsigned((M + kGM - 1) / kGM);
+  if (op == 0)
+    k_gemm_bf16<false><<<blocks, kThreads, 0, st>>>(M, int(N), int(K), A, lda, B, ldb, C, ldc, bias, relu);
+  else
+    k_gemm_bf16<true><<<blocks, kThreads, 0, st>>>(M, int(N), int(K), A, lda, B, ldb, C, ldc, bias, relu);
+  return hipGetLastError();
+}
+
+int64_t mlp_wgrad_chunks(int64_t P) { return (P + kChunk - 1) / kChunk; }
+
+// dW[K x N] = bf(A)^T bf(G) and db[N] = column sums of G over P points, in a
+// fixed order: kChunk-point partials (part: chunks x (K N + N) floats), then
+// a sum over chunks in chunk order.
+hipError_t mlp_wgrad(hipStream_t st, int64_t P, int64_t K, int64_t N, const float *A, int64_t lda,
+                     const float *G, int64_t ldg, float *part, float *dW, float *db) {
+  if (N < 1 || N > kMaxN || K < 1 || K > kMaxK || P < 1) return hipErrorInvalidValue;
+  const int64_t chunks = mlp_wgrad_chunks(P);
+  float *partb = part + chunks * K * N;
+  k_wgrad_bf16<<<unsigned(chunks), kThreads, 0, st>>>(P, int(K), int(N), A, lda, G, ldg, part, partb);
+  k_sum_chunks<<<blocks_for(K * N), kThreads, 0, st>>>(chunks, K * N, part, dW);
+  k_sum_chunks<<<blocks_for(N), kThreads, 0, st>>>(chunks, N, partb, db);
   return hipGetLastError();
 }
 
@@ -393,16 +543,11 @@ hipError_t mlp_relu_mask(hipStream_t st, int64_t count, const float *A, float *G
   return hipGetLastError();
 }
 
-hipError_t mlp_bias_grad(hipStream_t st, int64_t P, int64_t N, const float *G, float *db) {
-  const int64_t rows = 512;
-  const dim3 g(blocks_for(N), unsigned((P + rows - 1) / rows));
-  k_bias_grad<<<g, kThreads, 0, st>>>(P, N, G, rows, db);
-  return hipGetLastError();
-}
-
 hipError_t mlp_head(hipStream_t st, int64_t P, const float *out, const float *q,
-                    const float *target, float w, float *x, float *dout, double *loss) {
-  k_head<<<blocks_for(P), kThreads, 0, st>>>(P, out, q, target, w, x, dout, loss);
+                    const float *target, float w, float *x, float *dout, double *lpart,
+                    double *loss) {
+  k_head<<<blocks_for(P), kThreads, 0, st>>>(P, out, q, target, w, x, dout, loss ? lpart : nullptr);
+  if (loss) k_sum_partials<<<1, kThreads, 0, st>>>(int64_t(blocks_for(P)), lpart, loss);
   return hipGetLastError();
 }
 
@@ -420,10 +565,7 @@ hipError_t mlp_to_bf16(hipStream_t st, int64_t count, const float *w, uint16_t *
   return hipGetLastError();
 }
 
-size_t mlp_infer_lds() {
-  constexpr int nW = kFeat * kIH + 2 * kIH * kIH + kIH * 6;
-  return size_t((nW * 2 + 15) / 16 * 16) + 2 * size_t(kIP) * kIH * sizeof(float);
-}
+size_t mlp_infer_lds() { return kInferLds; }
 
 hipError_t mlp_infer(hipStream_t st, int64_t P, const float *feat, const float *q,
                      const uint16_t *wb, const float *bias_all, float *x) {

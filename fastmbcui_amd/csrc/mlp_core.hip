@@ -26,12 +26,15 @@ hipError_t mlp_features(hipStream_t st, int64_t n, const double *y, const double
                         const double *rot, const double *ctr, const double *vw, float *feat,
                         float *q);
 hipError_t mlp_gemm(hipStream_t st, int op, int64_t M, int64_t N, int64_t K, const float *A,
-                    int64_t lda, const void *B, int64_t ldb, float *C, int64_t ldc,
+                    int64_t lda, const uint16_t *B, int64_t ldb, float *C, int64_t ldc,
                     const float *bias, int relu);
+int64_t mlp_wgrad_chunks(int64_t P);
+hipError_t mlp_wgrad(hipStream_t st, int64_t P, int64_t K, int64_t N, const float *A, int64_t lda,
+                     const float *G, int64_t ldg, float *part, float *dW, float *db);
 hipError_t mlp_relu_mask(hipStream_t st, int64_t count, const float *A, float *G);
-hipError_t mlp_bias_grad(hipStream_t st, int64_t P, int64_t N, const float *G, float *db);
 hipError_t mlp_head(hipStream_t st, int64_t P, const float *out, const float *q,
-                    const float *target, float w, float *x, float *dout, double *loss);
+                    const float *target, float w, float *x, float *dout, double *lpart,
+                    double *loss);
 hipError_t mlp_adam(hipStream_t st, int64_t count, int64_t count_w, float *w, uint16_t *wb,
                     const float *g, float *m, float *v, float lr, float b1, float b2, float eps,
                     int32_t step);
@@ -39,7 +42,8 @@ hipError_t mlp_to_bf16(hipStream_t st, int64_t count, const float *w, uint16_t *
 hipError_t mlp_infer(hipStream_t st, int64_t P, const float *feat, const float *q,
                      const uint16_t *wb, const float *bias_all, float *x);
 hipError_t mlp_scale(hipStream_t st, int64_t P, float s, float *x);
-hipError_t mlp_sqnorm(hipStream_t st, int64_t P, const float *r, float w, double *acc);
+hipError_t mlp_sqnorm(hipStream_t st, int64_t P, const float *r, float w, double *lpart,
+                      double *acc);
 hipError_t mlp_fit_dout(hipStream_t st, int64_t P, const float *g, const float *q, float c, float *dout);
 }  // namespace mfs
 
@@ -81,7 +85,7 @@ struct mfs_mlp {
   std::string err;
   int64_t w_off[kLayers] = {}, b_off[kLayers] = {};
   int64_t n_w = 0, n_all = 0;
-  Buf w, wb, g, m, v, act, gbuf0, gbuf1, out, dout, loss, fit;
+  Buf w, wb, g, m, v, act, gbuf0, gbuf1, out, dout, loss, fit, wpart, lpart;
   int64_t cap_points = 0;
   int32_t step = 0;
   double *hloss = nullptr;  // pinned
@@ -100,6 +104,11 @@ struct mfs_mlp {
 
 namespace {
 
+// Block partials of the loss sums (one double per 256 entries).
+hipError_t ensure_partials(mfs_mlp *net, int64_t count) {
+  return net->lpart.ensure(size_t((count + kThreads - 1) / kThreads) * sizeof(double));
+}
+
 int ensure_points(mfs_mlp *net, int64_t P) {
   if (P <= net->cap_points) return MFS_OK;
   const size_t hid = size_t(P) * 128 * sizeof(float);
@@ -108,6 +117,9 @@ int ensure_points(mfs_mlp *net, int64_t P) {
   MHIPCHK(net->gbuf1.ensure(hid), "alloc");
   MHIPCHK(net->out.ensure(size_t(P) * 6 * sizeof(float)), "alloc");
   MHIPCHK(net->dout.ensure(size_t(P) * 6 * sizeof(float)), "alloc");
+  // weight-gradient partials: chunks x (largest K N + N) floats
+  MHIPCHK(net->wpart.ensure(size_t(mlp_wgrad_chunks(P)) * (128 * 128 + 128) * sizeof(float)), "alloc");
+  MHIPCHK(ensure_partials(net, P), "alloc");
   net->cap_points = P;
   return MFS_OK;
 }
@@ -127,18 +139,16 @@ int forward(mfs_mlp *net, const float *feat, int64_t P) {
   return MFS_OK;
 }
 
-// Gradients (accumulated into net->g, cleared first) from dout.
+// Gradients (written to net->g, every entry) from dout.
 int backward(mfs_mlp *net, const float *feat, int64_t P) {
-  MHIPCHK(hipMemsetAsync(net->g.p, 0, size_t(net->n_all) * 4, net->stream), "clear grads");
   const float *G = net->dout.as<float>();
   float *bufs[2] = {net->gbuf0.as<float>(), net->gbuf1.as<float>()};
   for (int l = kLayers - 1; l >= 0; --l) {
     const int64_t di = kDims[l], dn = kDims[l + 1];
     const float *A = (l == 0) ? feat : net->act.as<float>() + size_t(l - 1) * P * 128;
-    MHIPCHK(mlp_gemm(net->stream, 1, di, dn, P, A, di, G, dn, net->g.as<float>() + net->w_off[l], dn,
-                     nullptr, 0),
+    MHIPCHK(mlp_wgrad(net->stream, P, di, dn, A, di, G, dn, net->wpart.as<float>(),
+                      net->g.as<float>() + net->w_off[l], net->g.as<float>() + net->b_off[l]),
             "weight gradient");
-    MHIPCHK(mlp_bias_grad(net->stream, P, dn, G, net->g.as<float>() + net->b_off[l]), "bias gradient");
     if (l > 0) {
       float *Gp = bufs[l % 2];
       MHIPCHK(mlp_gemm(net->stream, 2, P, di, dn, G, dn, net->wb.as<uint16_t>() + net->w_off[l], dn, Gp,
@@ -204,7 +214,7 @@ void mfs_mlp_destroy(mfs_mlp *net) {
   (void)hipSetDevice(net->device);
   (void)hipStreamSynchronize(net->stream);
   for (Buf *b : {&net->w, &net->wb, &net->g, &net->m, &net->v, &net->act, &net->gbuf0, &net->gbuf1,
-                 &net->out, &net->dout, &net->loss, &net->fit})
+                 &net->out, &net->dout, &net->loss, &net->fit, &net->wpart, &net->lpart})
     b->release();
   if (net->hloss) (void)hipHostFree(net->hloss);
   delete net;
@@ -266,7 +276,7 @@ int mfs_mlp_loss_grad(mfs_mlp *net, const float *feat, const float *q, const flo
   if ((rc = forward(net, feat, P)) != MFS_OK) return rc;
   MHIPCHK(hipMemsetAsync(net->loss.p, 0, 8, net->stream), "clear loss");
   MHIPCHK(mlp_head(net->stream, P, net->out.as<float>(), q, target, weight, nullptr, net->dout.as<float>(),
-                   net->loss.as<double>()),
+                   net->lpart.as<double>(), net->loss.as<double>()),
           "head");
   if ((rc = backward(net, feat, P)) != MFS_OK) return rc;
   if (loss != nullptr) {
@@ -325,7 +335,9 @@ int mfs_mlp_fit_step(mfs_mlp *net, const float *feat, const float *q, int32_t sa
   float *h = g + P;
   float *r = h + P;
   double *partial = reinterpret_cast<double *>(r + ((max_m + 1) / 2) * 2);
-  MHIPCHK(mlp_head(net->stream, P, net->out.as<float>(), q, nullptr, 0.f, x, nullptr, nullptr), "head");
+  MHIPCHK(ensure_partials(net, std::max(P, max_m)), "alloc");
+  MHIPCHK(mlp_head(net->stream, P, net->out.as<float>(), q, nullptr, 0.f, x, nullptr, nullptr, nullptr),
+          "head");
   MHIPCHK(mlp_scale(net->stream, P, scale, x), "scale");
   MHIPCHK(hipMemsetAsync(net->loss.p, 0, 8, net->stream), "clear loss");
   for (int32_t s = 0; s < samples; ++s) {
@@ -337,13 +349,17 @@ int mfs_mlp_fit_step(mfs_mlp *net, const float *feat, const float *q, int32_t sa
     const float *xs = x + offsets[s];
     // r = b - M x (rows pass with the boundary data subtracted from)
     MHIPCHK(points_rows<float>(net->stream, op, 1, xs, b[s], 1, 0, 1, r, partial, nullptr), "M x");
-    if (!normal) MHIPCHK(mlp_sqnorm(net->stream, op.m, r, weight, net->loss.as<double>()), "loss");
+    if (!normal)
+      MHIPCHK(mlp_sqnorm(net->stream, op.m, r, weight, net->lpart.as<double>(), net->loss.as<double>()),
+              "loss");
     // g = M^T r
     MHIPCHK(points_cols<float>(net->stream, op, 1, r, partial, nullptr), "M^T r");
     MHIPCHK(points_cols_reduce<float>(net->stream, op, 1, partial, g + offsets[s]), "M^T r");
     if (normal) {
       // loss w |M^T r|^2, dL/dx = -2 w M^T M (M^T r): h = M g, g <- M^T h
-      MHIPCHK(mlp_sqnorm(net->stream, op.n, g + offsets[s], weight, net->loss.as<double>()), "loss");
+      MHIPCHK(mlp_sqnorm(net->stream, op.n, g + offsets[s], weight, net->lpart.as<double>(),
+                         net->loss.as<double>()),
+              "loss");
       MHIPCHK(points_rows<float>(net->stream, op, 1, g + offsets[s], nullptr, 1, 0, 1, h, partial, nullptr),
               "M g");
       MHIPCHK(points_cols<float>(net->stream, op, 1, h, partial, nullptr), "M^T M g");
@@ -367,7 +383,8 @@ int mfs_mlp_predict(mfs_mlp *net, const float *feat, const float *q, int64_t P, 
   int rc = ensure_points(net, P);
   if (rc != MFS_OK) return rc;
   if ((rc = forward(net, feat, P)) != MFS_OK) return rc;
-  MHIPCHK(mlp_head(net->stream, P, net->out.as<float>(), q, nullptr, 0.f, x, nullptr, nullptr), "head");
+  MHIPCHK(mlp_head(net->stream, P, net->out.as<float>(), q, nullptr, 0.f, x, nullptr, nullptr, nullptr),
+          "head");
   return MFS_OK;
 }
 

@@ -11,7 +11,14 @@ against their definition on hand-built bodies, Adam against its closed form.
 
 Parameters are one flat FP32 vector [W0 W1 W2 W3 b0 b1 b2 b3], W_l (in x out).
 The forward pass uses the weights rounded to BF16 (round to nearest even), as
-the GPU does; gradients are with respect to those rounded weights.
+the GPU does; gradients are with respect to those rounded weights.  With
+bf16_ops (the default) every matrix product also takes its other operand
+rounded to BF16 from FP32, as the GPU's BF16 matrix-core products do: the
+layer inputs in the forward pass, and the back-propagated gradient G in both
+backward products (dW = bf(A)^T bf(G), dA = bf(G) bf(W)^T); the bias
+gradients and ReLU masks use the unrounded values.  The products are then
+approximations of the exact gradient, which the finite-difference check pins
+with bf16_ops=False.
 """
 from __future__ import annotations
 
@@ -80,23 +87,28 @@ def features(y, nrm, bval, body, starts, rot, ctr, vw):
     return feat, q
 
 
-def forward(p, feat, round_weights=True):
+def _op(a, bf16_ops):
+    """A matrix-product operand: FP32, then BF16 when bf16_ops."""
+    return bf16_round(np.asarray(a, np.float32)).astype(np.float64) if bf16_ops else a
+
+
+def forward(p, feat, round_weights=True, bf16_ops=True):
     """Returns (out n x 6, activations [A0 .. A3])."""
     W, b = unpack(p, round_weights)
     A = [np.asarray(feat, np.float64)]
     h = A[0]
     for l in range(4):
-        z = h @ W[l] + b[l]
+        z = _op(h, bf16_ops) @ W[l] + b[l]
         h = np.maximum(z, 0.0) if l < 3 else z
         if l < 3:
             A.append(h)
     return h, A
 
 
-def loss_grad(p, feat, q, target, weight, round_weights=True):
+def loss_grad(p, feat, q, target, weight, round_weights=True, bf16_ops=True):
     """(loss, flat gradient) of weight * sum_j (out_j . q_j - t_j)^2."""
     W, b = unpack(p, round_weights)
-    out, A = forward(p, feat, round_weights)
+    out, A = forward(p, feat, round_weights, bf16_ops)
     x = np.sum(out * q, axis=1)
     e = x - target
     loss = weight * float(np.sum(e * e))
@@ -104,10 +116,11 @@ def loss_grad(p, feat, q, target, weight, round_weights=True):
     w_off, b_off, _, n_all = layout()
     g = np.zeros(n_all)
     for l in range(3, -1, -1):
-        g[w_off[l]:w_off[l] + W[l].size] = (A[l].T @ G).ravel()
+        Gb = _op(G, bf16_ops)
+        g[w_off[l]:w_off[l] + W[l].size] = (_op(A[l], bf16_ops).T @ Gb).ravel()
         g[b_off[l]:b_off[l] + b[l].size] = G.sum(axis=0)
         if l > 0:
-            G = (G @ W[l].T) * (A[l] > 0)
+            G = (Gb @ W[l].T) * (A[l] > 0)
     return loss, g
 
 

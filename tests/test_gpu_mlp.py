@@ -33,6 +33,15 @@ def test_features_match_oracle():
     assert err_f < 1e-5 and err_q < 1e-6
 
 
+# The GPU's layer inputs are FP32 sums in the matrix cores' order, the
+# oracle's FP64 ones; a few of them fall on opposite sides of a BF16 rounding
+# boundary and round to neighbouring BF16 values (one BF16 unit, 2^-8
+# relative, on that operand).  Measured with an FP32 emulation of the same
+# products on these inputs: 7e-5 norm-wise, 3.4e-4 at worst.  Dropping the
+# operand rounding anywhere moves the outputs by 3e-3 (norm-wise 1e-3).
+FWD_NORM, FWD_MAX = 2e-4, 1.5e-3
+
+
 @pytest.mark.parametrize("fused", [True, False])
 def test_forward_matches_oracle(fused):
     rng = np.random.default_rng(2)
@@ -44,15 +53,45 @@ def test_forward_matches_oracle(fused):
     out, _ = mo.forward(p, feat)
     ref = np.sum(out * q, axis=1)
     err = np.abs(x - ref).max() / np.abs(ref).max()
-    print(f"forward fused={fused}: max rel err {err:.2e}", flush=True)
-    assert err < 1e-5
+    nerr = np.linalg.norm(x - ref) / np.linalg.norm(ref)
+    print(f"forward fused={fused}: max rel err {err:.2e}, norm-wise {nerr:.2e}", flush=True)
+    assert err < FWD_MAX and nerr < FWD_NORM
+
+
+def test_fused_inference_equals_layer_by_layer_products():
+    """The fused kernel runs the same BF16 products in the same order as the
+    layer-by-layer GEMMs: the two agree to FP32 rounding of the head."""
+    rng = np.random.default_rng(8)
+    net = warmstart.Network(seed=9)
+    feat = torch.from_numpy(rng.standard_normal((1000, 20)).astype(np.float32)).to(DEV)
+    q = torch.from_numpy(rng.standard_normal((1000, 6)).astype(np.float32)).to(DEV)
+    a = net.predict(feat, q, fused=True)
+    b = net.predict(feat, q, fused=False)
+    d = ((a - b).abs().max() / b.abs().max()).item()
+    print(f"fused vs layered: max rel diff {d:.2e}, bit-identical {torch.equal(a, b)}", flush=True)
+    assert d < 1e-6
+
+
+def test_gradients_are_deterministic():
+    """Weight and bias gradients are chunk partials summed in chunk order and
+    the loss a fixed-order sum of block partials: repeated calls agree bit
+    for bit."""
+    rng = np.random.default_rng(10)
+    net = warmstart.Network(seed=11)
+    P = 5000
+    args = [torch.from_numpy(rng.standard_normal(s).astype(np.float32)).to(DEV) for s in ((P, 20), (P, 6), (P,))]
+    l1 = net.loss_grad(*args, 1.0 / P)
+    g1 = net.grads()
+    l2 = net.loss_grad(*args, 1.0 / P)
+    g2 = net.grads()
+    assert l1 == l2 and np.array_equal(g1, g2)
 
 
 def test_gradients_match_oracle():
     rng = np.random.default_rng(4)
     net = warmstart.Network(seed=5)
     p = net.params()
-    P = 2000   # several split-K chunks in the weight gradients
+    P = 2000   # eight 256-point chunks in the weight gradients, the last one partial
     feat = rng.standard_normal((P, 20)).astype(np.float32)
     q = rng.standard_normal((P, 6)).astype(np.float32)
     t = rng.standard_normal(P).astype(np.float32)
@@ -67,6 +106,8 @@ def test_gradients_match_oracle():
             a, b = g[off:off + size], g_ref[off:off + size]
             err = np.linalg.norm(a - b) / np.linalg.norm(b)
             print(f"layer {l} block at {off}: grad rel err {err:.2e}", flush=True)
+            # FP32 emulation of the same products: <= 1.7e-5; without the
+            # BF16 operand rounding: 2.3e-4 .. 4e-2
             assert err < 1e-4
 
 

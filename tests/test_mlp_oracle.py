@@ -15,7 +15,7 @@ def test_gradient_matches_central_differences():
     feat = rng.standard_normal((7, 20))
     q = rng.standard_normal((7, 6))
     t = rng.standard_normal(7)
-    loss, g = mo.loss_grad(p, feat, q, t, 0.37, round_weights=False)
+    loss, g = mo.loss_grad(p, feat, q, t, 0.37, round_weights=False, bf16_ops=False)
     w_off, b_off, _, n_all = mo.layout()
     idx = [w_off[0] + 5, w_off[1] + 1000, w_off[2] + 77, w_off[3] + 100, b_off[0] + 3, b_off[3] + 2,
            b_off[2] + 50, w_off[3] + 767]
@@ -25,8 +25,8 @@ def test_gradient_matches_central_differences():
         pp, pm = p64.copy(), p64.copy()
         pp[i] += h
         pm[i] -= h
-        lp, _ = mo.loss_grad(pp, feat, q, t, 0.37, round_weights=False)
-        lm, _ = mo.loss_grad(pm, feat, q, t, 0.37, round_weights=False)
+        lp, _ = mo.loss_grad(pp, feat, q, t, 0.37, round_weights=False, bf16_ops=False)
+        lm, _ = mo.loss_grad(pm, feat, q, t, 0.37, round_weights=False, bf16_ops=False)
         fd = (lp - lm) / (2 * h)
         assert abs(fd - g[i]) <= 1e-6 * max(1.0, abs(fd)), (i, fd, g[i])
 
@@ -72,3 +72,43 @@ def test_adam_first_step_moves_by_lr_times_sign():
     g = np.array([0.3, -4.0, 0.0])
     p2, m, v = mo.adam(p, g, np.zeros(3), np.zeros(3), 1, 1e-3)
     assert np.allclose(p2, p - 1e-3 * np.sign(g), atol=1e-9)
+
+
+def test_bf16_operands_leave_bf16_exact_values_alone():
+    """With sparse weights of +-0.5, features and velocities in {-1, 0, 1}
+    and targets equal to the outputs except at one point (a residual of 1
+    there), every product operand -- the activations forward, the one-point
+    back-propagated gradients backward -- is a short binary fraction, hence a
+    BF16 value, so rounding the operands (bf16_ops) must change nothing:
+    rounding happens at the operands of the products only, never to their
+    results (the weight gradients themselves need not be BF16 values)."""
+    rng = np.random.default_rng(5)
+    n_all = mo.layout()[3]
+    p = (rng.integers(-1, 2, n_all) * 0.5).astype(np.float32)
+    p[rng.random(n_all) < 0.95] = 0.0         # sparse: sums stay short binary fractions
+    feat = rng.integers(-1, 2, (9, 20)).astype(np.float64)
+    q = rng.integers(-1, 2, (9, 6)).astype(np.float64)
+    o1, A1 = mo.forward(p, feat, bf16_ops=True)
+    o0, A0 = mo.forward(p, feat, bf16_ops=False)
+    exact = lambda a: np.array_equal(mo.bf16_round(np.asarray(a, np.float32)), a)
+    assert all(exact(a) for a in A0)          # the premise: every operand is a BF16 value
+    assert np.array_equal(o1, o0)
+    t = np.sum(o0 * q, axis=1)
+    t[3] -= 1.0
+    l1, g1 = mo.loss_grad(p, feat, q, t, 0.25, bf16_ops=True)
+    l0, g0 = mo.loss_grad(p, feat, q, t, 0.25, bf16_ops=False)
+    assert l1 == l0 == 0.25 and np.array_equal(g1, g0)
+    assert np.count_nonzero(g0) > 100
+
+
+def test_bf16_operands_move_results_by_about_the_bf16_unit():
+    """On generic data the rounded-operand network differs from the exact
+    one by about BF16's unit roundoff (2^-9 relative per operand), far more
+    than FP32 rounding and far less than O(1)."""
+    rng = np.random.default_rng(6)
+    p = _params(seed=2, scale=0.15)
+    feat = rng.standard_normal((50, 20))
+    o1, _ = mo.forward(p, feat, bf16_ops=True)
+    o0, _ = mo.forward(p, feat, bf16_ops=False)
+    rel = np.linalg.norm(o1 - o0) / np.linalg.norm(o0)
+    assert 1e-4 < rel < 2e-2, rel
