@@ -142,7 +142,7 @@ def test_learned_warm_start_recipe_against_the_oracle_and_its_gain():
     on average."""
     from fastmbcui_amd import warmstart
     from oracle import mlp_oracle as mo
-    from test_gpu_mlp import FWD_MAX, FWD_NORM
+    from test_gpu_mlp import check_forward
     net = warmstart.Network(seed=0)
     data = warmstart.generate(net, seeds=list(range(1000, 1010)), steps=200, every=20)
     mse = warmstart.train(net, data, epochs=10, batch_samples=8, lr=1e-3)
@@ -165,12 +165,8 @@ def test_learned_warm_start_recipe_against_the_oracle_and_its_gain():
                 got = net.predict(feat, q).cpu().numpy()
                 out, _ = mo.forward(net.params(), feat.cpu().numpy())
                 ref = np.sum(out * q.cpu().numpy(), axis=1)
-                err = np.abs(got - ref).max() / np.abs(ref).max()
-                nerr = np.linalg.norm(got - ref) / np.linalg.norm(ref)
-                print(f"trained forward vs oracle: max rel err {err:.2e}, norm-wise {nerr:.2e}", flush=True)
                 # BF16 operands: neighbouring roundings of a few layer inputs
-                # (tests/test_gpu_mlp.py)
-                assert err < FWD_MAX and nerr < FWD_NORM
+                check_forward(got, ref, "trained forward vs oracle")
             ctx.plan(sc.y, 3.0)
             ctx.factor_points(_dev(sc.x), _dev(sc.nx), _dev(sc.y), _lib.MFS_FP32)
             B = _dev(sc.b.astype(np.float32))

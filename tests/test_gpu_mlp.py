@@ -36,10 +36,22 @@ def test_features_match_oracle():
 # The GPU's layer inputs are FP32 sums in the matrix cores' order, the
 # oracle's FP64 ones; a few of them fall on opposite sides of a BF16 rounding
 # boundary and round to neighbouring BF16 values (one BF16 unit, 2^-8
-# relative, on that operand).  Measured with an FP32 emulation of the same
-# products on these inputs: 7e-5 norm-wise, 3.4e-4 at worst.  Dropping the
-# operand rounding anywhere moves the outputs by 3e-3 (norm-wise 1e-3).
-FWD_NORM, FWD_MAX = 2e-4, 1.5e-3
+# relative, on that operand), which moves the few points downstream of them.
+# Measured: random inputs 7e-7 norm-wise (no flips), a trained network on coin
+# features 1.2e-4 norm-wise with its worst point at 2.2e-3.  Dropping the
+# operand rounding anywhere moves the outputs by 1e-3 norm-wise and most
+# points by more than 1e-4, so the checks are norm-wise and on the share of
+# points beyond 1e-4 (of the largest output).
+FWD_NORM, FWD_FRAC, FWD_MAX = 2e-4, 0.02, 1e-2
+
+
+def check_forward(got, ref, what):
+    scale = np.abs(ref).max()
+    err = np.abs(got - ref) / scale
+    nerr = np.linalg.norm(got - ref) / np.linalg.norm(ref)
+    frac = float(np.mean(err > 1e-4))
+    print(f"{what}: norm-wise {nerr:.2e}, points beyond 1e-4 {frac:.3%}, max {err.max():.2e}", flush=True)
+    assert nerr < FWD_NORM and frac <= FWD_FRAC and err.max() < FWD_MAX
 
 
 @pytest.mark.parametrize("fused", [True, False])
@@ -52,10 +64,7 @@ def test_forward_matches_oracle(fused):
     x = net.predict(torch.from_numpy(feat).to(DEV), torch.from_numpy(q).to(DEV), fused=fused).cpu().numpy()
     out, _ = mo.forward(p, feat)
     ref = np.sum(out * q, axis=1)
-    err = np.abs(x - ref).max() / np.abs(ref).max()
-    nerr = np.linalg.norm(x - ref) / np.linalg.norm(ref)
-    print(f"forward fused={fused}: max rel err {err:.2e}, norm-wise {nerr:.2e}", flush=True)
-    assert err < FWD_MAX and nerr < FWD_NORM
+    check_forward(x, ref, f"forward fused={fused}")
 
 
 def test_fused_inference_equals_layer_by_layer_products():
