@@ -848,6 +848,14 @@ def main():
         pclasses = [_lib.MFS_K_MATVEC_ROWS, _lib.MFS_K_MATVEC_COLS, _lib.MFS_K_GRAM,
                     _lib.MFS_K_RESID_ROWS, _lib.MFS_K_RESID_COLS]
 
+        # the tuned launch shape is the dense form's: the points form's Gram
+        # evaluates every padded column, so it keeps the default (all tile
+        # counts) and is re-planned with it (untimed)
+        for prm in (_lib.MFS_PARAM_TILE_SET, _lib.MFS_PARAM_FACTOR_PHASES,
+                    _lib.MFS_PARAM_FACTOR_STREAMS, _lib.MFS_PARAM_FACTOR_SPLITS):
+            ctx.set_param(prm, 0)
+        ctx.plan(sc.y, 3.0)
+
         def points_step():
             ctx.factor_points(x, nx, y, _lib.MFS_FP32)
             f_ms = ctx.stats().factor_ms

@@ -21,7 +21,8 @@ MFS_RHS_BOUNDARY, MFS_RHS_SOURCE = 0, 1
 (MFS_K_NONE, MFS_K_MATVEC_ROWS, MFS_K_MATVEC_COLS, MFS_K_GRAM, MFS_K_RESID_ROWS, MFS_K_RESID_COLS,
  MFS_K_MATVEC_FUSED) = range(7)
 (MFS_PARAM_POINTS_BLOCKS, MFS_PARAM_COLS_BLOCKS, MFS_PARAM_FUSED, MFS_PARAM_TILE_SET,
- MFS_PARAM_FACTOR_PHASES, MFS_PARAM_FACTOR_STREAMS, MFS_PARAM_FACTOR_SPLITS) = 1, 2, 3, 4, 5, 6, 7
+ MFS_PARAM_FACTOR_PHASES, MFS_PARAM_FACTOR_STREAMS, MFS_PARAM_FACTOR_SPLITS,
+ MFS_PARAM_GRAM_SPLIT) = 1, 2, 3, 4, 5, 6, 7, 8
 TILE_SIZES = (16, 12, 8, 6, 4, 3, 2, 1)   # bit i of MFS_PARAM_TILE_SET
 
 _STATUS_NAMES = {1: "MFS_ERR_ARG", 2: "MFS_ERR_HIP", 3: "MFS_ERR_DUPLICATE", 4: "MFS_ERR_NOT_SPD",

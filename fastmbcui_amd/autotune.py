@@ -93,7 +93,7 @@ def tune_batch_chunk(batch: _lib.Batch, Bs, candidates=(4, 8), graphs=(0, 2), ma
 FACTOR_TILE_SETS = (0xff, 0xd5, 0x5f)
 
 
-def factor_candidates(tile_sets=FACTOR_TILE_SETS, phases=(2, 4, 8), streams=(2, 4)):
+def factor_candidates(tile_sets=FACTOR_TILE_SETS, phases=(1, 2, 4), streams=(2, 4)):
     return [(t, p, s) for t in tile_sets for p in phases for s in streams]
 
 

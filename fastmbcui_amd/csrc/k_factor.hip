@@ -89,6 +89,51 @@ __device__ inline void gram_step(const TC (&fr)[T], acc_t (&acc)[PPW]) {
   }
 }
 
+// Split FP32 Gram products (GB): x = hi + lo + r with hi the BF16 rounding of
+// x, lo that of x - hi (exact in FP32) and |r| <= 2^-16 |x|; a b is taken as
+// hi_a hi_b + hi_a lo_b + lo_a hi_b, summed in FP32 -- the three-product
+// scheme of Henry, Tang and Heinecke, "Leveraging the bfloat16 artificial
+// intelligence datatype for higher-precision computations" (ARITH 2019).  The
+// dropped terms (lo_a lo_b and those with r) total at most about 3 2^-16 |a b|,
+// and round to nearest leaves them of either sign but lo_a lo_b.  The BF16
+// 16x16x32 MFMA does 8x the k-extent of the FP32 16x16x4 one in half its
+// cycles, so the three products cost 3/16 of the FP32 MFMA time.
+typedef __bf16 bf8_t __attribute__((ext_vector_type(8)));
+typedef float f8_t __attribute__((ext_vector_type(8)));
+typedef float f4_t __attribute__((ext_vector_type(4)));
+
+__device__ inline void split_bf16(f4_t x0, f4_t x1, bf8_t &hi, bf8_t &lo) {
+  const f8_t x = {x0.x, x0.y, x0.z, x0.w, x1.x, x1.y, x1.z, x1.w};
+  hi = __builtin_convertvector(x, bf8_t);
+  lo = __builtin_convertvector(x - __builtin_convertvector(hi, f8_t), bf8_t);
+}
+
+// One 32-row k-step of the split Gram for the tile pairs GRP + NG (U0 + u):
+// the large products of all pairs first, then the two small ones, so that no
+// MFMA waits for the one before it on the same accumulator.
+template <int T, int NG, int PPW, int GRP, int U0>
+__device__ inline void gram_step3(const bf8_t (&hi)[T], const bf8_t (&lo)[T], f4_t (&acc)[PPW]) {
+  constexpr int kPairs = T * (T + 1) / 2;
+#pragma unroll
+  for (int u = 0; u < PPW; ++u) {
+    const int pi = GRP + NG * (U0 + u);
+    if (pi < kPairs)
+      acc[u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(hi[pair_a(pi)], hi[pair_b(pi)], acc[u], 0, 0, 0);
+  }
+#pragma unroll
+  for (int u = 0; u < PPW; ++u) {
+    const int pi = GRP + NG * (U0 + u);
+    if (pi < kPairs)
+      acc[u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(hi[pair_a(pi)], lo[pair_b(pi)], acc[u], 0, 0, 0);
+  }
+#pragma unroll
+  for (int u = 0; u < PPW; ++u) {
+    const int pi = GRP + NG * (U0 + u);
+    if (pi < kPairs)
+      acc[u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(lo[pair_a(pi)], hi[pair_b(pi)], acc[u], 0, 0, 0);
+  }
+}
+
 // ------------------------------------------------------------------ transpose
 // Mt[c][f] = M[f][c] for f < m, 0 for m <= f < ldmt, for the columns c from cb
 // (a multiple of 64) on.  64x64 tiles through LDS, float4 in and out when the
@@ -138,7 +183,7 @@ __global__ __launch_bounds__(kThreads) void k_transpose(const float *__restrict_
 // lane evaluates exactly the MFMA operand entries it supplies, A[f][p] with
 // p = 16 a + (lane & 15) for every tile a and f = f0 + (lane >> 4), keeping
 // its T source points in registers: no LDS staging and no barriers.
-template <int T, typename TC, typename TV, bool PTS>
+template <int T, typename TC, typename TV, bool PTS, bool GB = false>
 __global__ __launch_bounds__(kThreads) void k_super_factor(const FactorScene *__restrict__ scenes,
                                                            const FactorItem *__restrict__ items,
                                                            TC *__restrict__ th_scratch,
@@ -246,6 +291,24 @@ __global__ __launch_bounds__(kThreads) void k_super_factor(const FactorScene *__
     }
   };
 
+  auto step3 = [&](const bf8_t(&hi)[T], const bf8_t(&lo)[T]) {
+    if constexpr (GB) {
+      if (NG == 1) {
+        gram_step3<T, NG, PPW, 0, U0>(hi, lo, acc);
+      } else if (NG == 2) {
+        if (grp == 0) gram_step3<T, NG, PPW, 0, U0>(hi, lo, acc);
+        else gram_step3<T, NG, PPW, 1 % NG, U0>(hi, lo, acc);
+      } else {
+        switch (grp) {
+          case 0: gram_step3<T, NG, PPW, 0, U0>(hi, lo, acc); break;
+          case 1: gram_step3<T, NG, PPW, 1 % NG, U0>(hi, lo, acc); break;
+          case 2: gram_step3<T, NG, PPW, 2 % NG, U0>(hi, lo, acc); break;
+          default: gram_step3<T, NG, PPW, 3 % NG, U0>(hi, lo, acc); break;
+        }
+      }
+    }
+  };
+
   if constexpr (PTS) {
     float yh[T][3], yl[T][3];
     bool ok[T];
@@ -290,6 +353,76 @@ __global__ __launch_bounds__(kThreads) void k_super_factor(const FactorScene *__
       step(fr);
 #pragma unroll
       for (int d = 0; d < 3; ++d) cur[d] = nxt[d];
+    }
+  } else if constexpr (GB) {
+    // Split products (FP32 values, FP32 sums): lane l loads, for every tile a
+    // and 32-row group f0, rows f0 + 4 (l >> 4) .. + 3 and f0 + 16 + 4 (l >> 4)
+    // .. + 3 of column p = 16 a + (l & 15) (two float4 of a row of M^T; each
+    // load instruction reads 64 consecutive bytes per column) and supplies
+    // them as the eight k-values of its slot in both operands of the BF16
+    // MFMA: any placement of the rows on k serves a Gram as long as both
+    // operands share it.  Padding columns, and slabs past the end of the rows
+    // (re-read so the loop has no branch), load the zero row Mz instead of
+    // being masked.
+    typedef const __attribute__((address_space(1))) float gfloat;
+    typedef const __attribute__((address_space(1))) f4_t gf4;
+    gfloat *const zrow = (gfloat *)FS.Mz + 4 * (lane >> 4);
+    gfloat *col[T];
+#pragma unroll
+    for (int a = 0; a < T; ++a) {
+      const int p = a * 16 + (lane & 15);
+      col[a] = p < ns ? (gfloat *)Mt + int64_t(src[p]) * ldmt + 4 * (lane >> 4) : zrow;
+    }
+    // W 32-row groups per slab and wave (64 rows for the small tiles, as the
+    // FP32 products read); FSTEP <= 256 divides the row splits' chunks.
+    constexpr int W = LARGE ? 1 : (T <= 4 ? 2 : 1);
+    constexpr int FSTEP = 32 * W * KS;
+    auto load = [&](int64_t f0, bool live, f4_t (&v)[T][2 * W]) {
+#pragma unroll
+      for (int a = 0; a < T; ++a) {
+        gfloat *b = live ? col[a] : zrow;
+#pragma unroll
+        for (int u = 0; u < W; ++u) {
+          v[a][2 * u] = *(gf4 *)(b + f0 + 32 * u);
+          v[a][2 * u + 1] = *(gf4 *)(b + f0 + 32 * u + 16);
+        }
+      }
+    };
+    auto compute = [&](const f4_t (&v)[T][2 * W]) {
+#pragma unroll
+      for (int u = 0; u < W; ++u) {
+        bf8_t hi[T], lo[T];
+#pragma unroll
+        for (int a = 0; a < T; ++a) split_bf16(v[a][2 * u], v[a][2 * u + 1], hi[a], lo[a]);
+        step3(hi, lo);
+      }
+    };
+    const int64_t fs = r0 + kph * 32 * W;  // this wave's first slab
+    if constexpr (!LARGE) {
+      // Two register buffers in turn, each slab requested one slab ahead
+      // (as in the FP32 path below).
+      const int64_t last = fs < r1 ? fs + (r1 - 1 - fs) / FSTEP * FSTEP : fs;
+      auto clampf = [&](int64_t f) { return f <= last ? f : last; };
+      f4_t b0[T][2 * W], b1[T][2 * W];
+      if (fs < r1) {
+        load(fs, true, b0);
+        for (int64_t f0 = fs; f0 < r1; f0 += 2 * FSTEP) {
+          load(clampf(f0 + FSTEP), f0 + FSTEP < r1, b1);
+          __builtin_amdgcn_sched_barrier(0);
+          compute(b0);
+          __builtin_amdgcn_sched_barrier(0);
+          load(clampf(f0 + 2 * FSTEP), f0 + 2 * FSTEP < r1, b0);
+          __builtin_amdgcn_sched_barrier(0);
+          compute(b1);
+          __builtin_amdgcn_sched_barrier(0);
+        }
+      }
+    } else {
+      f4_t cur[T][2 * W];
+      for (int64_t f0 = fs; f0 < r1; f0 += FSTEP) {
+        load(f0, true, cur);
+        compute(cur);
+      }
     }
   } else {
     // Lane l loads, for every tile a, the float4 of rows f0 + 4 (l >> 4) .. +3
@@ -555,6 +688,7 @@ struct FactorLaunch {
   bool points;
   void *th_scratch;
   FactorSplit sp;
+  bool split_bf16;
 };
 
 template <int T, typename TC, typename TV>
@@ -564,7 +698,14 @@ hipError_t launch_t(hipStream_t st, const FactorLaunch &L) {
   if (L.points)
     k_super_factor<T, TC, TV, true><<<grid, kThreads, 0, st>>>(
         L.scenes, L.items, static_cast<TC *>(L.th_scratch), L.sp);
-  else
+  else if constexpr (sizeof(TC) == 4 && sizeof(TV) == 4) {
+    if (L.split_bf16)
+      k_super_factor<T, TC, TV, false, true><<<grid, kThreads, 0, st>>>(
+          L.scenes, L.items, static_cast<TC *>(L.th_scratch), L.sp);
+    else
+      k_super_factor<T, TC, TV, false><<<grid, kThreads, 0, st>>>(
+          L.scenes, L.items, static_cast<TC *>(L.th_scratch), L.sp);
+  } else
     k_super_factor<T, TC, TV, false><<<grid, kThreads, 0, st>>>(
         L.scenes, L.items, static_cast<TC *>(L.th_scratch), L.sp);
   return hipGetLastError();
@@ -600,10 +741,11 @@ hipError_t launch_transpose(hipStream_t st, const float *M, int64_t m, int64_t n
 
 hipError_t launch_super_factor(hipStream_t st, int tiles, int precision, bool points,
                                const FactorScene *scenes, const FactorItem *items, int32_t count,
-                               void *th_scratch, const FactorSplit &sp) {
+                               void *th_scratch, const FactorSplit &sp, bool split_bf16) {
   if (sp.splits > 1 && ((sp.partial == nullptr && tiles <= 8) || sp.counters == nullptr))
     return hipErrorInvalidValue;
-  const FactorLaunch L{scenes, items, count, points, th_scratch, sp};
+  if (split_bf16 && (precision != 1 || points)) return hipErrorInvalidValue;
+  const FactorLaunch L{scenes, items, count, points, th_scratch, sp, split_bf16};
   if (precision == 0) return launch_prec<double, double>(st, tiles, L);
   if (precision == 1) return launch_prec<float, float>(st, tiles, L);
   return launch_prec<double, float>(st, tiles, L);

@@ -56,14 +56,17 @@ struct DevBuf {
 };
 
 // The dense factor transposes M in `phases` column ranges [phase_col(p),
-// phase_col(p + 1)) (4 by default, at most kMaxPhases) and starts the
-// supernodes whose columns (original order) all lie below phase_col(p + 1)
-// once range p is done, so the transpose of the later ranges overlaps the
-// Gram of the earlier ones.  On surface scenes a supernode's columns nearly
-// always lie in one body, whose sources are contiguous, so the Gram work
-// splits about evenly over the phases.
+// phase_col(p + 1)) (at most kMaxPhases) and starts the supernodes whose
+// columns (original order) all lie below phase_col(p + 1) once range p is
+// done, so the transpose of the later ranges can overlap the Gram of the
+// earlier ones.  On surface scenes a supernode's columns nearly always lie in
+// one body, whose sources are contiguous, so the Gram work splits about
+// evenly over the phases.  The default is one range: with the Gram's products
+// on BF16 matrix cores both the transpose and the Gram stream HBM, and at
+// 65,536^2 the overlap made the factor slower (21.5 ms with 4 ranges against
+// 20.4 ms with 1); with FP32 products (compute-bound) 4 ranges were best.
 constexpr int kMaxPhases = 8;
-constexpr int kDefaultPhases = 4;
+constexpr int kDefaultPhases = 1;
 int64_t phase_col(int p, int64_t n, int phases) {
   if (p >= phases) return n;
   return std::min<int64_t>(n, (int64_t(p) * n / phases + 63) / 64 * 64);
@@ -183,6 +186,11 @@ struct mfs_ctx {
   // m = n = 65,536 splitting the small launches made the factor slower
   // (26.7 ms against 23.1 ms); the autotuner tries it.
   int max_splits = 1;
+  // Products of the dense FP32 factor's Gram: false FP32 MFMA, true (the
+  // default) three BF16 MFMA products of a two-part split of each value --
+  // 20.4 ms against 23.0 ms for the factor at 65,536^2, with column errors
+  // against the FP64 factor no larger (max 9.9e-5 against 1.4e-4).
+  bool gram_split = true;
   int track_mask = 0;
   struct Track {
     std::vector<std::pair<hipEvent_t, hipEvent_t>> pairs;
@@ -826,8 +834,12 @@ int factor_prepare(mfs_ctx *ctx, const float *M, const PointsOp *pts, int64_t m,
   ctx->has_factor = false;
   ctx->alt_prec = -1;
   const int64_t ldmt = (m + kGramChunk - 1) / kGramChunk * kGramChunk;
-  if (pts == nullptr)  // filled by factor_impl, range by range
-    HIPCHK(ctx->dMt.ensure(size_t(n) * size_t(ldmt) * sizeof(float)), "alloc M^T");
+  if (pts == nullptr) {  // filled by factor_impl, range by range; row n stays zero
+    HIPCHK(ctx->dMt.ensure(size_t(n + 1) * size_t(ldmt) * sizeof(float)), "alloc M^T");
+    HIPCHK(hipMemsetAsync(ctx->dMt.as<float>() + size_t(n) * size_t(ldmt), 0,
+                          size_t(ldmt) * sizeof(float), ctx->stream),
+           "clear zero row");
+  }
   const int64_t nnz = ctx->sptr[ctx->n];
   const size_t w = (precision == MFS_FP64) ? 8 : 4;
   HIPCHK(ctx->dval.ensure(size_t(std::max<int64_t>(nnz, 1)) * w), "alloc factor");
@@ -836,6 +848,7 @@ int factor_prepare(mfs_ctx *ctx, const float *M, const PointsOp *pts, int64_t m,
   HIPCHK(hipMemsetAsync(ctx->dstatus.p, 0, size_t(n_super) * 4, ctx->stream), "memset");
   FactorScene &F = ctx->fscene;
   F.Mt = pts ? nullptr : ctx->dMt.as<float>();
+  F.Mz = pts ? nullptr : ctx->dMt.as<float>() + size_t(n) * size_t(ldmt);
   F.ldmt = ldmt;
   F.tgt = pts ? pts->tgt32 : nullptr;
   F.src = pts ? pts->src32 : nullptr;
@@ -980,7 +993,8 @@ int factor_impl(mfs_ctx *ctx, const float *M, const PointsOp *pts, int64_t m, in
     HIPCHK(launch_super_factor(ctx->side[L.bucket % nside], b.tiles, prec_code, pts != nullptr,
                                ctx->dfscene.as<FactorScene>(),
                                ctx->dlist.as<FactorItem>() + b.offset + L.lo, L.cnt,
-                               ctx->dthg.as<char>() + L.off, L.sp),
+                               ctx->dthg.as<char>() + L.off, L.sp,
+                               ctx->gram_split && prec_code == 1 && pts == nullptr),
            "supernode factor");
     return MFS_OK;
   };
@@ -1931,6 +1945,10 @@ int mfs_ctx_set_param(mfs_ctx *ctx, int param, int64_t value) {
     case MFS_PARAM_FACTOR_SPLITS:
       if (value > 8) return ctx->fail(MFS_ERR_ARG, "set_param: 1 to 8 row splits");
       ctx->max_splits = value == 0 ? 1 : int(value);
+      return MFS_OK;
+    case MFS_PARAM_GRAM_SPLIT:
+      if (value > 2) return ctx->fail(MFS_ERR_ARG, "set_param: Gram products are 0, 1 or 2");
+      ctx->gram_split = value != 1;
       return MFS_OK;
     case MFS_PARAM_FACTOR_STREAMS:
       if (value > mfs_ctx::kSide) return ctx->fail(MFS_ERR_ARG, "set_param: 1 to 4 factor streams");

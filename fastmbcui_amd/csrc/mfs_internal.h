@@ -85,9 +85,11 @@ hipError_t launch_transpose(hipStream_t st, const float *M, int64_t m, int64_t n
 // precision: 0 = FP64 compute / FP64 values, 1 = FP32 compute / FP32 values,
 // 2 = FP64 compute / FP32 values (retry of FP32 failures).
 // One scene's inputs and outputs of the factor (device pointers).  Dense form:
-// Mt (n x ldmt); points form: tgt (m x 12) and src (n x 8) records.
+// Mt (n x ldmt) and a row of ldmt zeros Mz (read by the split-BF16 Gram for
+// padding columns); points form: tgt (m x 12) and src (n x 8) records.
 struct FactorScene {
   const float *Mt;
+  const float *Mz;
   int64_t ldmt;
   const float *tgt, *src;
   int64_t m;
@@ -120,10 +122,12 @@ struct FactorSplit {
 // Gram's operands are evaluated from the records).  tiles in {1, 2, 3, 4, 6,
 // 8, 12, 16}; for 12 and 16 th_scratch holds count * splits *
 // factor_scratch_elems(tiles) elements of the compute type (8 bytes for
-// precision 0 and 2, 4 for 1).
+// precision 0 and 2, 4 for 1).  split_bf16 (precision 1, dense form only):
+// the Gram's FP32 products as three BF16 products of a two-part split.
 hipError_t launch_super_factor(hipStream_t st, int tiles, int precision, bool points,
                                const FactorScene *scenes, const FactorItem *items, int32_t count,
-                               void *th_scratch, const FactorSplit &sp = FactorSplit{});
+                               void *th_scratch, const FactorSplit &sp = FactorSplit{},
+                               bool split_bf16 = false);
 inline int64_t factor_scratch_elems(int tiles) {
   return tiles > 8 ? int64_t(16 * tiles) * (16 * tiles + 1) : 0;
 }

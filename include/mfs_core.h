@@ -261,12 +261,17 @@ enum mfs_param {
                                      supernode takes the smallest enabled count covering
                                      its rows.  Applies from the next plan (0: all) */
     MFS_PARAM_FACTOR_PHASES = 5,  /* column ranges of the dense factor's transpose that
-                                     the Gram overlaps, 1..8; from the next plan (0: 4) */
+                                     the Gram overlaps, 1..8; from the next plan (0: 1) */
     MFS_PARAM_FACTOR_STREAMS = 6, /* streams the factor's buckets run on, 1..4 (0: 4) */
-    MFS_PARAM_FACTOR_SPLITS = 7   /* most row splits of a supernode's Gram, 1..8: launches of
+    MFS_PARAM_FACTOR_SPLITS = 7,  /* most row splits of a supernode's Gram, 1..8: launches of
                                      fewer supernodes than 4 x CUs spread each Gram's rows
                                      over several blocks and sum their partial Grams (0: 1,
                                      no splits) */
+    MFS_PARAM_GRAM_SPLIT = 8      /* products of the dense FP32 factor's Gram (changes its
+                                     rounding, not only the summation order): 1 FP32 MFMA,
+                                     2 three BF16 MFMA products of each value split into
+                                     hi + lo (relative error per product about 2^-16, as
+                                     FP32 sums over 65,536 rows have); 0: the default, 2 */
 };
 int mfs_ctx_set_param(mfs_ctx *ctx, int param, int64_t value);
 

@@ -46,7 +46,14 @@ def test_factor_launch_shape_keeps_the_factor(scene, tset, phases, streams):
     print(f"tile set {tset:#04x}, {phases} ranges, {streams} streams: FP64 factor err {err:.2e}, "
           f"{ctx.stats().factor_ms:.2f} ms", flush=True)
     assert err < 1e-9
-    ctx.factor_dense(Md, _lib.MFS_FP32)
+    # FP32 factors, FP32 and split BF16 Gram products (3e-7 .. 3e-6 measured)
+    for split in (1, 2):
+        ctx.set_param(_lib.MFS_PARAM_GRAM_SPLIT, split)
+        ctx.factor_dense(Md, _lib.MFS_FP32)
+        err32 = _col_rel_err(pl, ctx.factor_export(), ref)
+        print(f"  FP32 factor (Gram products {split}) err {err32:.2e}, {ctx.stats().factor_ms:.2f} ms",
+              flush=True)
+        assert err32 < 5e-5, (split, err32)
     B = torch.from_numpy(sc.b.astype(np.float32)).to(DEV)
     X = ctx.solve_mixed(Md, B, maxit=500, tol=1e-6)
     assert ctx.normal_residual(Md, B, X).max() <= 1e-6
@@ -76,7 +83,7 @@ def test_tune_factor_times_every_candidate(scene):
 def test_factor_shape_parameters_are_checked():
     ctx = _lib.Context(0)
     for param, bad in ((_lib.MFS_PARAM_TILE_SET, 0x100), (_lib.MFS_PARAM_FACTOR_PHASES, 9),
-                       (_lib.MFS_PARAM_FACTOR_STREAMS, 5)):
+                       (_lib.MFS_PARAM_FACTOR_STREAMS, 5), (_lib.MFS_PARAM_GRAM_SPLIT, 3)):
         with pytest.raises(_lib.MfsError):
             ctx.set_param(param, bad)
 

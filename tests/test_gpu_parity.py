@@ -98,10 +98,11 @@ def test_plan_rejects_coincident_points():
 
 
 # ----------------------------------------------------------------------------- factor
-def _factor_pair(sc, precision=_lib.MFS_FP64, rho=3.0):
+def _factor_pair(sc, precision=_lib.MFS_FP64, rho=3.0, gram_split=0):
     M = _M32(sc)
     ctx, got, want = _plan_pair(sc.y, rho)
     Md = torch.from_numpy(M).to(DEV)
+    ctx.set_param(_lib.MFS_PARAM_GRAM_SPLIT, gram_split)
     ctx.factor_dense(Md, precision)
     val = ctx.factor_export()
     Mp = oracle.permuted_colmajor(M.astype(np.float64), want.Pinv)
@@ -126,23 +127,32 @@ def test_factor_fp64_matches_oracle(maker):
     assert err < 1e-9
 
 
-def test_factor_fp32_close_to_oracle():
+@pytest.mark.parametrize("split", [1, 2])
+def test_factor_fp32_close_to_oracle(split):
+    """FP32 factor with the Gram's products on FP32 MFMA (1) or as three BF16
+    products of a hi + lo split of each value (2), against the oracle's FP64
+    factor: 1.0e-6 and 2.7e-6 measured.  The split's dropped terms are below
+    about 3 2^-16 of each product; dropping one of the two cross products
+    instead (2^-9 of a product) moves the Gram by ~1e-3 and fails the bound."""
     sc = _ellips()
-    ctx, Md, M, pl, val, ref = _factor_pair(sc, _lib.MFS_FP32)
+    ctx, Md, M, pl, val, ref = _factor_pair(sc, _lib.MFS_FP32, gram_split=split)
     err = _col_rel_err(pl, val, ref)
-    print(f"factor fp32: max column rel err {err:.2e}, retried {ctx.stats().retried_supernodes}", flush=True)
-    assert err < 5e-2
+    print(f"factor fp32 (Gram products {split}): max column rel err {err:.2e}, "
+          f"retried {ctx.stats().retried_supernodes}", flush=True)
+    assert err < 5e-5
 
 
 @pytest.mark.parametrize("ppb", [24, 50, 64])
 def test_factor_small_m_matches_oracle(ppb):
     """m <= 128 (M^T rows padded to 128 only): the Gram's row slabs stay within
-    a row for every wave of the small-tile path; FP64 and FP32 factors."""
+    a row for every wave of the small-tile path; FP64 and FP32 factors (both
+    kinds of FP32 Gram products)."""
     sc = scenes.coins(seed=4, n_bodies=2, pts_per_body=ppb, k=1, box=4.0)
     assert sc.m <= 128
-    for precision, tol in ((_lib.MFS_FP64, 1e-9), (_lib.MFS_FP32, 5e-2)):
-        ctx, Md, M, pl, val, ref = _factor_pair(sc, precision)
-        assert _col_rel_err(pl, val, ref) < tol
+    for precision, tol, split in ((_lib.MFS_FP64, 1e-9, 0), (_lib.MFS_FP32, 5e-2, 1),
+                                  (_lib.MFS_FP32, 5e-2, 2)):
+        ctx, Md, M, pl, val, ref = _factor_pair(sc, precision, gram_split=split)
+        assert _col_rel_err(pl, val, ref) < tol, (precision, split)
 
 
 def test_factor_dense_pattern_one_iteration():
