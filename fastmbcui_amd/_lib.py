@@ -47,7 +47,7 @@ class Stats(ctypes.Structure):
         ("outer_iters", ctypes.c_int32), ("retried_supernodes", ctypes.c_int32),
         ("plan_order_ms", ctypes.c_double), ("plan_pattern_ms", ctypes.c_double),
         ("plan_grid_columns", ctypes.c_int32), ("fp64_sweeps", ctypes.c_int32),
-        ("reverted_columns", ctypes.c_int32), ("pad1", ctypes.c_int32),
+        ("reverted_columns", ctypes.c_int32), ("fused_stale_polls", ctypes.c_int32),
     ]
 
 

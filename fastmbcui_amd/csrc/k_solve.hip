@@ -1457,11 +1457,11 @@ template <typename TA, int K>
 hipError_t normal_matvec(hipStream_t s, const SolveArgs<TA> &a, TrackFn fn, void *tctx,
                          int32_t *splits) {
   if constexpr (sizeof(TA) == 4 && K == 8) {
-    if (a.fused > 0 && a.pts.tgt32 == nullptr) {
+    if (a.fused.slices > 0 && a.pts.tgt32 == nullptr) {
       track(fn, tctx, 6, 0);
       const hipError_t e = mv_fused8(s, a.M, a.m, a.n, a.pv, a.part, a.frec, a.st, a.fused);
       track(fn, tctx, 6, 1);
-      *splits = 1;
+      *splits = a.fused.groups;
       return e;
     }
   }

@@ -587,23 +587,26 @@ int batch_width(int64_t k) {
 }
 
 // One pass over M per FP32 iteration for dense batches of 8, when asked for
-// (MFS_PARAM_FUSED = 2) and the shape qualifies.  Not the default: on MI355X
-// at m = n = 65,536 the two passes take 4.9 ms and the one pass 5.3 ms, its
-// cross-block exchange of y being latency-bound.
-int setup_fused(mfs_ctx *ctx, const PointsOp *pts, int64_t m, int64_t n, int K, int32_t &blocks,
-                uint64_t *&records) {
-  blocks = 0;
+// (MFS_PARAM_FUSED = 2) and the shape qualifies: exchange records, and
+// partials for every row group of the grid (part is re-pointed if the buffer
+// grows).
+int setup_fused(mfs_ctx *ctx, const PointsOp *pts, int64_t m, int64_t n, int K, FusedShape &shape,
+                uint64_t *&records, double *&part) {
+  shape = FusedShape{};
   records = nullptr;
   if (pts != nullptr || ctx->fused_mode != 2) return MFS_OK;
-  const int32_t G = fused_blocks(m, n, K, ctx->cus);
-  if (G <= 0) return MFS_OK;
-  const size_t bytes = fused_records(G) * sizeof(uint64_t);
+  const FusedShape f = fused_shape(m, n, K, ctx->cus);
+  if (f.slices <= 0) return MFS_OK;
+  const size_t bytes = fused_records(f) * sizeof(uint64_t);
   if (ctx->dfrec.cap < bytes || ctx->dfrec.p == nullptr) {
     HIPCHK(ctx->dfrec.ensure(bytes), "alloc exchange records");
     // records of an earlier owner of this memory must not carry live tags
     HIPCHK(hipMemsetAsync(ctx->dfrec.p, 0, ctx->dfrec.cap, ctx->stream), "clear records");
   }
-  blocks = G;
+  HIPCHK(ctx->dpart.ensure(size_t(f.groups) * size_t(n) * size_t(K) * sizeof(double)),
+         "alloc partials");
+  part = ctx->dpart.as<double>();
+  shape = f;
   records = ctx->dfrec.as<uint64_t>();
   return MFS_OK;
 }
@@ -655,9 +658,9 @@ int workspace(mfs_ctx *ctx, const float *M, const PointsOp *pts, int64_t m, int6
     HIPCHK(hipMemsetAsync(ctx->dstate.p, 0, sizeof(PcgState), ctx->stream), "memset state");
   }
   a.st = ctx->dstate.as<PcgState>();
-  a.fused = 0;
+  a.fused = FusedShape{};
   a.frec = nullptr;
-  if (sizeof(TA) == 4) return setup_fused(ctx, pts, m, n, K, a.fused, a.frec);
+  if (sizeof(TA) == 4) return setup_fused(ctx, pts, m, n, K, a.fused, a.frec, a.part);
   return MFS_OK;
 }
 
@@ -679,6 +682,7 @@ int iterate(mfs_ctx *ctx, const SolveArgs<TA> &a, int32_t maxit) {
     chunk = std::min(chunk * 2, 16);
   }
   const int dead = launched - ctx->hstate->iter;
+  ctx->stats.fused_stale_polls = int32_t(ctx->hstate->fstale);
   track_discard(ctx, MFS_K_MATVEC_ROWS, dead);
   track_discard(ctx, MFS_K_MATVEC_COLS, dead);
   track_discard(ctx, MFS_K_MATVEC_FUSED, dead);
@@ -1131,8 +1135,9 @@ struct MixedRun {
     a32.x = base; a32.r = base + nv; a32.p = base + 2 * nv; a32.q = base + 3 * nv;
     a32.z = base + 4 * nv; a32.t = base + 5 * nv; a32.pv = base + 6 * nv; a32.y = base + 7 * nv;
     a32.part = a64.part; a32.dpart = a64.dpart; a32.st = a64.st;
-    rc = setup_fused(ctx, pts, m, n, K, a32.fused, a32.frec);
+    rc = setup_fused(ctx, pts, m, n, K, a32.fused, a32.frec, a32.part);
     if (rc != MFS_OK) return rc;
+    a64.part = a32.part;
 
     const unsigned gn = unsigned((n + kThreads - 1) / kThreads);
     if (X0d != nullptr)
@@ -1633,7 +1638,7 @@ int mfs_dense_normal_apply(mfs_ctx *ctx, const float *M, int64_t m, int64_t n, c
     HIPCHK(hipMemsetAsync(ctx->dstate.p, 0, sizeof(PcgState), ctx->stream), "memset state");
   }
   a.st = ctx->dstate.as<PcgState>();
-  int rc = setup_fused(ctx, nullptr, m, n, K, a.fused, a.frec);
+  int rc = setup_fused(ctx, nullptr, m, n, K, a.fused, a.frec, a.part);
   if (rc != MFS_OK) return rc;
   const int32_t one = 1;
   HIPCHK(hipMemcpyAsync(&a.st->n_active, &one, sizeof(one), hipMemcpyHostToDevice, ctx->stream),
@@ -1645,6 +1650,7 @@ int mfs_dense_normal_apply(mfs_ctx *ctx, const float *M, int64_t m, int64_t n, c
   HIPCHK(normal_apply32(ctx->stream, a, int(k), Q, track_cb, ctx), "normal apply");
   HIPCHK(hipStreamSynchronize(ctx->stream), "sync");
   HIPCHK(hipMemcpy(ctx->hstate, a.st, sizeof(PcgState), hipMemcpyDeviceToHost), "read state");
+  ctx->stats.fused_stale_polls = int32_t(ctx->hstate->fstale);
   if (ctx->hstate->fabort != 0u) {
     const uint32_t zero = 0;
     (void)hipMemcpy(&a.st->fabort, &zero, sizeof(zero), hipMemcpyHostToDevice);
@@ -1688,7 +1694,7 @@ int mfs_dense_normal_apply_noalloc(mfs_ctx *ctx, const float *M, int64_t m, int6
   a.part = reinterpret_cast<double *>(base + pv_bytes + y_bytes);
   a.pts = PointsOp{};
   a.st = nullptr;  // no batch state: the passes always run
-  a.fused = 0;     // the two-pass matvec (the one-pass kernel needs exchange records)
+  a.fused = FusedShape{};  // the two-pass matvec (the one-pass kernel needs exchange records)
   k_pack_rhs<<<unsigned((n * K + kThreads - 1) / kThreads), kThreads, 0, ctx->stream>>>(n, X, k, K,
                                                                                        a.pv);
   HIPCHK(hipGetLastError(), "pack input");

@@ -43,7 +43,7 @@ struct PcgState {
   uint32_t fepoch;
   uint32_t fdone;
   uint32_t fabort;
-  uint32_t fpad;
+  uint32_t fstale;          // its record polls that found a stale record (all launches)
   double beta[kMaxRhs];     // r^T z (new) / r^T z (old), set by k_dot_rz for k_upd_p
 };
 
@@ -147,6 +147,13 @@ __host__ __device__ inline int64_t factor_chunk(int64_t rows, int32_t splits) {
 hipError_t launch_gather_values(hipStream_t st, int precision, int32_t nnz, const int32_t *tmap,
                                 const void *val, void *tval);
 
+// Grid of the one-pass normal matvec (k_fused.hip): `slices` column slices
+// (a power of two, `width` columns each) times `groups` row groups of
+// `panels` panels of 16 rows; slices == 0 when the shape does not qualify.
+struct FusedShape {
+  int32_t slices = 0, lg_slices = 0, groups = 0, width = 0, panels = 0;
+};
+
 // ---------------------------------------------------------------- solve kernels
 // TA = double (plain / residual) or float (mixed inner PCG).
 template <typename TA>
@@ -166,23 +173,19 @@ struct SolveArgs {
   PcgState *st;
   int32_t row_blocks;          // number of row blocks (splits) of the M^T pass
   PointsOp pts;                // points form when pts.tgt32 != nullptr (M unused)
-  int32_t fused = 0;           // blocks of the one-pass normal matvec (0: two passes)
+  FusedShape fused;            // grid of the one-pass normal matvec (slices == 0: two passes)
   uint64_t *frec = nullptr;    // its exchange records (fused_records(fused) of them)
 };
 
-// One HBM pass over M for q = M^T (M p), dense FP32 batches of 8 (k_fused.hip).
-// Panels of 16 rows; y of a panel is used kFusedDefer panels later, and the
-// exchange records live in rings of kFusedSlots panels.
-constexpr int kFusedDefer = 8;
-constexpr int kFusedSlots = 16;
-// Blocks (one per 256-column slice, all resident) or 0 when the shape does not
-// qualify (K != 8, n % 4 != 0, fewer than 8 or more slices than CUs, 65,535
-// panels or more).
-int32_t fused_blocks(int64_t m, int64_t n, int K, int cus);
-size_t fused_records(int32_t G);
-// part[c][r] = (M^T M X)[c][r] in FP64, original column order (X: n x 8).
+// One HBM pass over M for q = M^T (M p), dense FP32 batches of 8 (k_fused.hip):
+// K == 8, n even, n <= 65,536 (at most 64 slices of 1024 columns), the
+// panels of a row group below 65,535; one block per CU.
+FusedShape fused_shape(int64_t m, int64_t n, int K, int cus);
+size_t fused_records(const FusedShape &f);
+// part[g][c][r] (FP64, original column order, g < f.groups) = the row group
+// g's share of (M^T M X)[c][r] (X: n x 8 FP32, original order).
 hipError_t mv_fused8(hipStream_t st, const float *M, int64_t m, int64_t n, const float *X,
-                     double *part, uint64_t *records, PcgState *state, int32_t G);
+                     double *part, uint64_t *records, PcgState *state, const FusedShape &f);
 
 // Q (n x kval, FP64, original column order) = M^T (M X) for the FP32 batch in
 // a.pv (n x a.K, original order, zero beyond kval), through the iteration's

@@ -59,7 +59,8 @@ typedef struct mfs_stats {
     int32_t plan_grid_columns;  /* columns found by the grid search (rest: scan) */
     int32_t fp64_sweeps;        /* mixed solves: sweeps run in FP64 after FP32 sweeps stalled */
     int32_t reverted_columns;   /* mixed solves: columns restored after a sweep made them worse */
-    int32_t pad1;
+    int32_t fused_stale_polls;  /* one-pass matvec: exchange polls that found a stale record
+                                   (summed over this context's launches) */
 } mfs_stats;
 
 /* Context lifecycle.  `stream` is a hipStream_t (NULL = legacy default). */

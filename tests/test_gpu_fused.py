@@ -6,15 +6,15 @@
   (numpy / torch on FP64 copies of the FP32 inputs, whose products are
   exact): the one-pass kernel and the two-pass matvec both to FP32-level
   relative error, at ragged row panels (m not a multiple of 16), a partial
-  last column slice (n not a multiple of 256), a batch of 5 padded to 8 and
-  the full 65,536 x 65,536 workload (256 slices, one per CU);
+  last column slice, a batch of 5 padded to 8 and the full 65,536 x 65,536
+  workload (64 slices of 1024 columns x 4 row groups, one block per CU);
 * the mixed solve, whose answer is certified by the FP64 two-pass residual
   M^T (b - M x), reaches its tolerance with the one-pass matvec, boundary and
   source-space right-hand sides, and the batch (which runs until its slowest
   column converges) takes no more iterations than with two passes, up to one
   (per column the counts scatter both ways with the FP32 rounding);
-* shapes the kernel does not take (n % 4 != 0, k = 4, fewer than 8 column
-  slices) use two passes.
+* shapes the kernel does not take (n % 4 != 0, batches wider than 8 or of 4
+  and fewer, n above 65,536) use two passes.
 """
 import numpy as np
 import pytest
@@ -126,19 +126,39 @@ def test_normal_apply_exact_on_small_integers(m, bodies, per_body, k):
         assert np.array_equal(got, ref.astype(np.float64)), (mode, np.abs(got - ref).max())
 
 
-@pytest.mark.parametrize("m,n,k", [(1, 4, 8), (5, 8, 8), (70, 12, 3), (129, 1028, 8)])
+@pytest.mark.parametrize("m,n,k", [(1, 4, 8), (5, 8, 8), (70, 12, 3), (129, 1028, 8), (40, 4096, 8),
+                                   (300, 2052, 8)])
 def test_normal_apply_exact_tiny_shapes(m, n, k):
-    """Degenerate shapes through the two passes: a single row, fewer rows than a
-    64-row chunk, a ragged last chunk, a partial 1024-column block; small
-    integers, so Q must equal the integer product exactly."""
+    """Degenerate shapes through both matvecs: a single row, fewer rows than a
+    panel or a 64-row chunk, ragged last panels and chunks, partial column
+    slices and blocks, row groups of one or two panels (the one-pass kernel's
+    last panels, whose reduce cannot wait for a later step); small integers,
+    so Q must equal the integer product exactly."""
     rng = np.random.default_rng(m * 1000 + n + k)
     M = rng.integers(-2, 3, (m, n)).astype(np.float32)
     X = rng.integers(-2, 3, (n, k)).astype(np.float32)
     ref = M.astype(np.int64).T @ (M.astype(np.int64) @ X.astype(np.int64))
     ctx = _lib.Context(0)
-    Q, _ = _apply(ctx, torch.from_numpy(M).to(DEV), torch.from_numpy(X).to(DEV), OFF)
-    got = Q.cpu().numpy()
-    assert np.array_equal(got, ref.astype(np.float64)), np.abs(got - ref).max()
+    Md, Xd = torch.from_numpy(M).to(DEV), torch.from_numpy(X).to(DEV)
+    eligible = k == 8 and n % 4 == 0 and m >= 2
+    for mode in (OFF, FORCE):
+        Q, launches = _apply(ctx, Md, Xd, mode)
+        assert (launches > 0) == (mode == FORCE and eligible), (mode, launches)
+        got = Q.cpu().numpy()
+        assert np.array_equal(got, ref.astype(np.float64)), (mode, np.abs(got - ref).max())
+
+
+def test_normal_apply_wider_than_the_one_pass_grid():
+    """More than 64 slices of 1024 columns (n > 65,536): two passes."""
+    rng = np.random.default_rng(11)
+    m, n = 40, 65540
+    M = rng.integers(-2, 3, (m, n)).astype(np.float32)
+    X = rng.integers(-2, 3, (n, 8)).astype(np.float32)
+    ref = M.astype(np.int64).T @ (M.astype(np.int64) @ X.astype(np.int64))
+    ctx = _lib.Context(0)
+    Q, launches = _apply(ctx, torch.from_numpy(M).to(DEV), torch.from_numpy(X).to(DEV), FORCE)
+    assert launches == 0
+    assert np.array_equal(Q.cpu().numpy(), ref.astype(np.float64))
 
 
 def test_normal_apply_matches_fp64_ragged_rows():
@@ -160,12 +180,12 @@ def test_normal_apply_other_widths_use_two_passes():
     _, Md, _ = _setup(sc)
     X = torch.randn((sc.n, 8), device=DEV, generator=torch.Generator(DEV).manual_seed(2))
     _check_apply(Md, X, want_fused=False)
-    sc = scenes.coins(seed=3, n_bodies=4, pts_per_body=256, k=4)      # n = 1024: 4 slices
+    sc = scenes.coins(seed=3, n_bodies=4, pts_per_body=256, k=4)      # n = 1024: 2 slices
     _, Md, _ = _setup(sc)
     X = torch.randn((sc.n, 4), device=DEV, generator=torch.Generator(DEV).manual_seed(3))
-    _check_apply(Md, X, want_fused=False)
+    _check_apply(Md, X, want_fused=False)                             # batch of 4: two passes
     X = torch.randn((sc.n, 8), device=DEV, generator=torch.Generator(DEV).manual_seed(5))
-    _check_apply(Md, X, want_fused=False)
+    _check_apply(Md, X, want_fused=True)                              # batch of 8: one pass
 
 
 def test_fused_matches_two_pass_coins_ragged_rows():
@@ -200,7 +220,7 @@ def test_fused_not_used_when_columns_are_not_float4_aligned():
 
 def test_fused_full_workload_one_slice_per_cu():
     """64 ellipsoids x 1024 points, 8 right-hand sides: M is 65,536 x 65,536
-    (17.2 GB), 256 slices; the default is two passes."""
+    (17.2 GB), 64 slices x 4 row groups; forced and default modes."""
     sc = scenes.ellipsoids(seed=0, n_bodies=64, k=8)
     ctx = _lib.Context(0)
     x = torch.from_numpy(sc.x).to(DEV)
