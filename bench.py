@@ -43,8 +43,9 @@ def _parse():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--bodies", type=int, default=64, help="ellipsoids (1024 points each)")
     ap.add_argument("--plain-steps", type=int, default=1)
-    ap.add_argument("--one-pass-steps", type=int, default=1,
-                    help="steps of the chosen form with the opt-in one-pass matvec, for comparison")
+    ap.add_argument("--other-form-steps", type=int, default=1,
+                    help="steps of the chosen form with the other normal-matvec form (two passes "
+                         "when the default ran one, else one), for comparison")
     ap.add_argument("--skip-plain", action="store_true")
     ap.add_argument("--tune-factor", type=int, default=1,
                     help="autotune the factor's launch shape before the timed steps (0: defaults)")
@@ -772,10 +773,12 @@ def main():
         avg_ms = (r_ms / max(r_l, 1) + c_ms / max(c_l, 1))
         dom_key = "normal_matvec"
     achieved = mv_bytes / (avg_ms * 1e-3) / 1e9 if launches else 0.0
-    per_pass = {"k_mv_rows_f32x8_avg_ms": round(r_ms / max(r_l, 1), 4),
-                "k_mv_cols_f32x8_avg_ms": round(c_ms / max(c_l, 1), 4),
-                "pass_stream_gbs": [round(mv_bytes / (r_ms / max(r_l, 1) * 1e-3) / 1e9, 1) if r_l else None,
-                                    round(mv_bytes / (c_ms / max(c_l, 1) * 1e-3) / 1e9, 1) if c_l else None]}
+    per_pass = None
+    if f_l == 0 and r_l and c_l:
+        per_pass = {"k_mv_rows_f32x8_avg_ms": round(r_ms / r_l, 4),
+                    "k_mv_cols_f32x8_avg_ms": round(c_ms / c_l, 4),
+                    "pass_stream_gbs": [round(mv_bytes / (r_ms / r_l * 1e-3) / 1e9, 1),
+                                        round(mv_bytes / (c_ms / c_l * 1e-3) / 1e9, 1)]}
     g_l, g_ms, g_bytes = tracked[_lib.MFS_K_GRAM]
     fused_launches = tracked[_lib.MFS_K_MATVEC_FUSED][0]
     kernel_ms = {names.get(c, "supernode factor (M^T ranges + Gram + Cholesky + L, overlapped)"): round(tracked[c][1] / args.steps, 3)
@@ -789,29 +792,41 @@ def main():
         except Exception:
             traffic = None
 
-    # The same step with the one-pass matvec (k_fused.hip, opt-in), for comparison.
-    one_pass = None
-    if fused_launches == 0 and args.one_pass_steps > 0:
-        ctx.set_param(_lib.MFS_PARAM_FUSED, 2)
+    # The same step with the other form of the normal matvec, for comparison:
+    # two passes when the default ran the one-pass kernel (k_fused.hip), else
+    # the one-pass kernel forced.
+    other_form = None
+    if args.other_form_steps > 0:
+        ran_fused = fused_launches > 0
+        ctx.set_param(_lib.MFS_PARAM_FUSED, 1 if ran_fused else 2)
         chosen()
         torch.cuda.synchronize()
-        ctx.track([_lib.MFS_K_MATVEC_FUSED])
+        oc = [_lib.MFS_K_MATVEC_ROWS, _lib.MFS_K_MATVEC_COLS, _lib.MFS_K_MATVEC_FUSED]
+        ctx.track(oc)
         t2 = time.perf_counter()
-        for i in range(args.one_pass_steps):
+        for i in range(args.other_form_steps):
             X2, _ = chosen()
         torch.cuda.synchronize()
-        one_ms = (time.perf_counter() - t2) * 1e3 / args.one_pass_steps
-        fl, fms, _ = ctx.tracked(_lib.MFS_K_MATVEC_FUSED)
+        other_ms = (time.perf_counter() - t2) * 1e3 / args.other_form_steps
+        ot = {c: ctx.tracked(c) for c in oc}
         ctx.track([])
         s2 = ctx.stats()
         res2 = ctx.normal_residual(M, B, X2)
         ctx.set_param(_lib.MFS_PARAM_FUSED, 0)
-        one_pass = dict(ms_per_solve=round(one_ms / k, 4), iters=[int(s2.iters[c]) for c in range(k)],
-                        residual_max=float(res2.max()),
-                        k_mv_fused8_avg_ms=round(fms / max(fl, 1), 4),
-                        max_rel_diff_x=float((X2.double() - X.double()).norm() / X.double().norm()))
-        _log(f"bench: one-pass matvec {one_ms:.2f} ms/step, k_mv_fused8 {fms / max(fl, 1):.3f} ms, "
-             f"iters {one_pass['iters']}")
+        if ran_fused:
+            (rl, rms, _), (cl, cms, _) = ot[oc[0]], ot[oc[1]]
+            mv = {"k_mv_rows_f32x8_avg_ms": round(rms / max(rl, 1), 4),
+                  "k_mv_cols_f32x8_avg_ms": round(cms / max(cl, 1), 4)}
+        else:
+            fl, fms, _ = ot[oc[2]]
+            mv = {"k_mv_fused8_avg_ms": round(fms / max(fl, 1), 4)}
+        other_form = dict(form=("two passes (k_mv_rows_f32x8 + k_mv_cols_f32x8)" if ran_fused
+                                else "one pass (k_mv_fused8)"),
+                          ms_per_solve=round(other_ms / k, 4), iters=[int(s2.iters[c]) for c in range(k)],
+                          residual_max=float(res2.max()),
+                          max_rel_diff_x=float((X2.double() - X.double()).norm() / X.double().norm()), **mv)
+        _log(f"bench: other matvec form ({other_form['form']}) {other_ms:.2f} ms/step, {mv}, "
+             f"iters {other_form['iters']}")
 
     # ---------------------------------------------------------------- plain form
     plain = None
@@ -934,7 +949,7 @@ def main():
         "residual_max": float(res.max()),
         "kernel_ms_per_step": kernel_ms,
         "gram_gb_gathered_per_step": round(g_bytes / max(args.steps, 1) / 1e9, 2),
-        "one_pass": one_pass,
+        "other_matvec_form": other_form,
         "plain": plain,
         "speedup_vs_plain": (round(plain["ms_per_solve"] / value, 2) if plain else None),
         "points": points,

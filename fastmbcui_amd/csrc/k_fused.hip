@@ -486,6 +486,11 @@ FusedShape fused_shape(int64_t m, int64_t n, int K, int cus) {
   if (NG < 1) return f;
   const int64_t NPg = (NP + NG - 1) / NG;
   if (NPg >= 65535) return f;
+  // every block of the grid must be resident at once (cooperative launch)
+  int per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_mv_fused8, kThreadsB, 0) != hipSuccess ||
+      int64_t(per_cu) * cus < S * NG)
+    return f;
   // width: columns per slice, a multiple of 4 (16-byte loads stay aligned)
   const int64_t width = ((n + S - 1) / S + 3) / 4 * 4;
   f.slices = int32_t(S);

@@ -14,7 +14,8 @@
   column converges) takes no more iterations than with two passes, up to one
   (per column the counts scatter both ways with the FP32 rounding);
 * shapes the kernel does not take (n % 4 != 0, batches wider than 8 or of 4
-  and fewer, n above 65,536) use two passes.
+  and fewer, n above 65,536) use two passes, and so does the default mode
+  below 2^27 entries of M.
 """
 import numpy as np
 import pytest
@@ -141,7 +142,7 @@ def test_normal_apply_exact_tiny_shapes(m, n, k):
     ctx = _lib.Context(0)
     Md, Xd = torch.from_numpy(M).to(DEV), torch.from_numpy(X).to(DEV)
     eligible = k == 8 and n % 4 == 0 and m >= 2
-    for mode in (OFF, FORCE):
+    for mode in (OFF, FORCE, DEFAULT):      # the default keeps two passes at these sizes
         Q, launches = _apply(ctx, Md, Xd, mode)
         assert (launches > 0) == (mode == FORCE and eligible), (mode, launches)
         got = Q.cpu().numpy()
@@ -159,6 +160,27 @@ def test_normal_apply_wider_than_the_one_pass_grid():
     Q, launches = _apply(ctx, torch.from_numpy(M).to(DEV), torch.from_numpy(X).to(DEV), FORCE)
     assert launches == 0
     assert np.array_equal(Q.cpu().numpy(), ref.astype(np.float64))
+
+
+@pytest.mark.parametrize("m,want_fused", [(4096, True), (4095, False)])
+def test_default_takes_the_one_pass_from_2_27_entries(m, want_fused):
+    """The default mode runs the one-pass kernel once M holds 2^27 entries
+    (here 4096 x 32,768) and two passes below; entries in -1..1, so every FP32
+    sum is an integer below 2^24 and Q must equal the FP64 product exactly."""
+    n = 32768
+    g = torch.Generator(DEV).manual_seed(m)
+    Md = torch.randint(-1, 2, (m, n), device=DEV, generator=g).float()
+    X = torch.randint(-1, 2, (n, 8), device=DEV, generator=g).float()
+    M64 = Md.double()
+    ref = M64.t() @ (M64 @ X.double())
+    del M64
+    ctx = _lib.Context(0)
+    Q, launches = _apply(ctx, Md, X, DEFAULT)
+    print(f"default mode m={m} n={n}: one-pass launches {launches}", flush=True)
+    assert (launches > 0) == want_fused
+    assert torch.equal(Q.double(), ref), (Q.double() - ref).abs().max().item()
+    del Md, ref, Q
+    torch.cuda.empty_cache()
 
 
 def test_normal_apply_matches_fp64_ragged_rows():
@@ -189,7 +211,7 @@ def test_normal_apply_other_widths_use_two_passes():
 
 
 def test_fused_matches_two_pass_coins_ragged_rows():
-    sc = scenes.coins(seed=0, n_bodies=8, pts_per_body=512, k=8)    # n = 4096: 16 slices
+    sc = scenes.coins(seed=0, n_bodies=8, pts_per_body=512, k=8)    # n = 4096: 4 slices
     ctx, Md, B = _setup(sc, extra_rows=45)                           # m = 4141
     _compare(ctx, Md, B)
 
@@ -201,7 +223,7 @@ def test_fused_matches_two_pass_partial_slice_and_padded_batch():
 
 
 def test_fused_source_space_right_hand_sides():
-    sc = scenes.coins(seed=2, n_bodies=8, pts_per_body=256, k=1)     # n = 2048: 8 slices
+    sc = scenes.coins(seed=2, n_bodies=8, pts_per_body=256, k=1)     # n = 2048: 2 slices
     ctx, Md, _ = _setup(sc)
     rng = np.random.default_rng(7)
     G = torch.from_numpy(rng.standard_normal((sc.n, 8)).astype(np.float32)).to(DEV)
@@ -220,7 +242,7 @@ def test_fused_not_used_when_columns_are_not_float4_aligned():
 
 def test_fused_full_workload_one_slice_per_cu():
     """64 ellipsoids x 1024 points, 8 right-hand sides: M is 65,536 x 65,536
-    (17.2 GB), 64 slices x 4 row groups; forced and default modes."""
+    (17.2 GB), 64 slices x 4 row groups: the default takes the one pass here."""
     sc = scenes.ellipsoids(seed=0, n_bodies=64, k=8)
     ctx = _lib.Context(0)
     x = torch.from_numpy(sc.x).to(DEV)
@@ -230,17 +252,17 @@ def test_fused_full_workload_one_slice_per_cu():
     B = torch.from_numpy(sc.b.astype(np.float32)).to(DEV)
     ctx.plan(sc.y, 3.0)
     ctx.factor_dense(Md, _lib.MFS_FP32)
-    Xa, la, ita = _solve(ctx, Md, B, FORCE)
-    X2, l2, it2 = _solve(ctx, Md, B, DEFAULT)
+    Xa, la, ita = _solve(ctx, Md, B, DEFAULT)
+    X2, l2, it2 = _solve(ctx, Md, B, OFF)
     print(f"fused full: launches {la}, iters {ita} vs {it2}", flush=True)
     assert la > 0 and l2 == 0
     assert ctx.normal_residual(Md, B, Xa).max() <= 1e-6
     assert max(ita) <= max(it2) + 1, (ita, it2)
     X = torch.randn((sc.n, 8), device=DEV, generator=torch.Generator(DEV).manual_seed(4))
     ref = _ref_normal(Md, X)
-    for mode in (FORCE, DEFAULT):
+    for mode in (DEFAULT, FORCE, OFF):
         Q, launches = _apply(ctx, Md, X, mode)
-        assert (launches > 0) == (mode == FORCE)
+        assert (launches > 0) == (mode != OFF)
         err = ((Q - ref).norm(dim=0) / ref.norm(dim=0)).max().item()
         print(f"normal apply full mode {mode}: max rel. error {err:.2e}", flush=True)
         assert err < 2e-6, (mode, err)
