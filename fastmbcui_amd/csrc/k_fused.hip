@@ -7,32 +7,30 @@
 // until y = M p of its row is complete over all columns, then multiplies it by
 // y for its column of M^T y.
 //
-// Grid.  The columns are cut into S slices (S a power of two, at most 512
-// columns each); the rows into NG groups of NPg panels of 16 rows.  Block
-// (g, s) = blockIdx.x (s = blockIdx.x mod S) streams the panels of group g
-// over slice s; all S x NG blocks are resident (one per CU, cooperative
-// launch).  A block is 8 waves, two per SIMD; wave w owns 64 columns of the
-// slice.  One 8-byte load per lane covers two rows of the wave's panel
-// (lanes 0-31 the even row, 32-63 the odd one), so a lane holds two columns
-// of 8 rows of every panel.
+// Grid.  The columns are cut into S slices (S a power of two, at least 2, at
+// most 1024 columns each); the rows into NG groups of NPg panels of 16 rows.
+// Block (g, s) = blockIdx.x (s = blockIdx.x mod S) streams the panels of
+// group g over slice s.  A block is 8 waves, two per SIMD; wave w owns 128
+// columns of the slice.  One 16-byte load per lane covers two rows of the
+// wave's panel (lanes 0-31 the even row, 32-63 the odd one, four columns
+// each), so a lane holds four columns of 8 rows of every panel.
 //
 // Step j of a wave (register ring of kTiles panels: j - kDefer .. j + 1):
 //   (1) its exchange duty for step j, if any, issues its record loads;
 //   (2) the loads of panel j + 1 are issued (non-temporal);
 //   (3) panel j arrives; the duty consumes its records;
-//   (4) phase A: the 16 x 64 tile goes through a wave-private LDS transpose
-//       (rows padded to 68 floats: conflict-free both ways) into 16
-//       v_mfma_f32_16x16x4f32 with p as the B operand (rows x 8 right-hand
-//       sides, padded to 16); the wave's partial y is staged in LDS and the
-//       last wave of the block to stage sums the 8 partials in wave order
-//       and publishes them;
+//   (4) phase A: the 16 x 128 tile goes, one 64-column half at a time,
+//       through a wave-private LDS transpose (rows padded to 68 floats:
+//       conflict-free both ways) into 16 v_mfma_f32_16x16x4f32 per half with
+//       p as the B operand (rows x 8 right-hand sides, padded to 16); the
+//       wave's partial y is staged in LDS and the last wave of the block to
+//       stage sums the 8 partials in wave order and publishes them;
 //   (5) phase B on panel j - kDefer: y of its rows from the LDS y ring
-//       (broadcast reads) times the wave's registers, 128 VALU FMAs per lane
+//       (broadcast reads) times the wave's registers, 256 VALU FMAs per lane
 //       into FP32 sums that are added into FP64 (LDS) every 8 panels.
 // Loads of a wave complete in order, so every record load is issued before
 // the prefetch of panel j + 1: waiting for the records never waits for M.
-// FP32 MFMA and VALU work of one SIMD do not overlap on this GPU; the two
-// phases take about four fifths of a step's streaming time.
+// FP32 MFMA and VALU work of one SIMD do not overlap on this GPU.
 //
 // Cross-block exchange (tagged 8-byte records, device-coherent stores and
 // loads; tag = (launch epoch << 16) | (panel + 1)):
@@ -47,16 +45,29 @@
 //            writes y[g][j mod kRing][v];
 //   fetch    panel j, at step j + kDefer: the 128 y records -> LDS y ring
 //            (zero for rows past m), flag for the waves.
-// Each hop has about four steps (a step is 32 KB of M per block, about
-// 1.3 us) to cover the record latency and the drift between the blocks of a
-// group.  A fetch whose records are still stale polls again (that wait also
-// drains the wave's prefetch); the state counts these polls and the reduce
-// attempts left to a later step.  Every wait is
-// bounded: after kSpinLimit polls the wave sets PcgState::fabort and stops
-// waiting, the host reports the failure.  Panel j's records are rewritten at
-// panel j + kRing; by then every block has reduced and fetched panel j (a
-// block at step j + kRing used y(j + kRing - kDefer - 1), which needed every
-// block's partials of that panel, so every block is past step j + kDefer).
+// A step is 64 KB of M per block (about 4 us at 65,536^2).  A fetch whose
+// records are still stale polls again (that wait also drains the wave's
+// prefetch); the state counts these polls and the reduce attempts left to a
+// later step.  Every wait is bounded: after kSpinLimit polls the wave sets
+// PcgState::fabort and stops waiting, the host reports the failure.  Panel
+// j's records are rewritten at panel j + kRing; by then every block has
+// reduced and fetched panel j (a block at step j + kRing used
+// y(j + kRing - kDefer - 1), which needed every block's partials of that
+// panel, so every block is past step j + kDefer).
+//
+// Residency.  The blocks wait for each other, so all S x NG of them must be
+// on the GPU at once: the grid is at most one block per CU (the LDS holds one
+// block per CU) and fused_shape checks the occupancy.  The launch is an
+// ordinary one, and the one-pass launches of this process on a device are
+// ordered one after another whatever stream they are queued on, so two such
+// grids never share the CUs while each waits for blocks of its own.
+// Measured at m = n = 65,536: 4.19-4.21 ms per matvec (4.1 TB/s of M), the
+// two passes 2 x 2.45 ms; streaming M alone takes about 2.5 ms, the
+// exchange's waits (y of a panel needs the slowest block of the group) most
+// of the rest.
+#include <map>
+#include <mutex>
+
 #include "mfs_internal.h"
 
 namespace mfs {
@@ -505,16 +516,42 @@ size_t fused_records(const FusedShape &f) {
   return size_t(f.groups) * kRing * 128 * (size_t(f.slices) + 1);
 }
 
+// One-pass launches of this process on one device, in launch order across
+// streams: each waits on the previous one's completion event.
+struct FusedOrder {
+  std::mutex mu;
+  std::map<int, hipEvent_t> last;  // per device; kept until the process ends
+};
+FusedOrder &fused_order() {
+  static FusedOrder *o = new FusedOrder();
+  return *o;
+}
+
 hipError_t mv_fused8(hipStream_t s, const float *M, int64_t m, int64_t n, const float *X,
                      double *part, uint64_t *records, PcgState *st, const FusedShape &f) {
   uint64_t *prec = records;
   uint64_t *yrec = records + size_t(f.groups) * kRing * 128 * size_t(f.slices);
   int32_t lg = f.lg_slices, width = f.width, NPg = f.panels;
-  void *args[] = {(void *)&M,    (void *)&m,    (void *)&n,  (void *)&X,     (void *)&part,
-                  (void *)&prec, (void *)&yrec, (void *)&st, (void *)&lg, (void *)&width,
-                  (void *)&NPg};
-  return hipLaunchCooperativeKernel(reinterpret_cast<const void *>(k_mv_fused8),
-                                    dim3(unsigned(f.slices * f.groups)), dim3(kThreadsB), args, 0, s);
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  FusedOrder &o = fused_order();
+  std::lock_guard<std::mutex> lock(o.mu);
+  auto it = o.last.find(dev);
+  if (it == o.last.end()) {
+    hipEvent_t ev = nullptr;
+    e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+    if (e != hipSuccess) return e;
+    it = o.last.emplace(dev, ev).first;
+  } else {
+    e = hipStreamWaitEvent(s, it->second, 0);
+    if (e != hipSuccess) return e;
+  }
+  k_mv_fused8<<<dim3(unsigned(f.slices * f.groups)), dim3(kThreadsB), 0, s>>>(
+      M, m, n, X, part, prec, yrec, st, lg, width, NPg);
+  e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  return hipEventRecord(it->second, s);
 }
 
 }  // namespace mfs

@@ -153,6 +153,7 @@ struct mfs_ctx {
   int32_t points_blocks = 0;  // tuning: blocks per points pass (0 = default)
   int32_t cols_blocks = 0;    // tuning: blocks of the dense M^T pass (0 = default)
   int32_t fused_mode = 0;     // one-pass normal matvec: 0 for M of 2^27 entries up, 1 off, 2 always
+  FusedShape frec_shape;      // the grid the exchange records were last set up for
   int cus = 0;                // compute units of the device
   DevBuf dfrec;               // exchange records of the one-pass matvec
   DevBuf dt32, dt64, ds32, ds64, dapin, dapout;
@@ -603,10 +604,16 @@ int setup_fused(mfs_ctx *ctx, const PointsOp *pts, int64_t m, int64_t n, int K, 
   const FusedShape f = fused_shape(m, n, K, ctx->cus);
   if (f.slices <= 0) return MFS_OK;
   const size_t bytes = fused_records(f) * sizeof(uint64_t);
-  if (ctx->dfrec.cap < bytes || ctx->dfrec.p == nullptr) {
+  const FusedShape &o = ctx->frec_shape;
+  const bool same = o.slices == f.slices && o.groups == f.groups && o.width == f.width &&
+                    o.panels == f.panels;
+  if (ctx->dfrec.cap < bytes || ctx->dfrec.p == nullptr || !same) {
     HIPCHK(ctx->dfrec.ensure(bytes), "alloc exchange records");
-    // records of an earlier owner of this memory must not carry live tags
+    // A record's tag holds 16 bits of the launch count: launches of one grid
+    // rewrite every record they read, but a record left by another grid (or an
+    // earlier owner of this memory) could match a tag 65,536 launches later.
     HIPCHK(hipMemsetAsync(ctx->dfrec.p, 0, ctx->dfrec.cap, ctx->stream), "clear records");
+    ctx->frec_shape = f;
   }
   HIPCHK(ctx->dpart.ensure(size_t(f.groups) * size_t(n) * size_t(K) * sizeof(double)),
          "alloc partials");

@@ -183,6 +183,51 @@ def test_default_takes_the_one_pass_from_2_27_entries(m, want_fused):
     torch.cuda.empty_cache()
 
 
+def test_one_pass_calls_on_two_streams_at_once():
+    """Two contexts on two streams, each called from a host thread of its own,
+    each forcing the one-pass kernel on a grid of one block per CU (n = 16,384:
+    16 slices x 16 row groups): the library orders such launches one after
+    another across streams (the blocks of one grid wait for each other, so two
+    grids must not split the CUs); every result is exact (entries -1..1)."""
+    import threading
+    m, n, reps = 2048, 16384, 6
+    g = torch.Generator(DEV).manual_seed(21)
+    Md = torch.randint(-1, 2, (m, n), device=DEV, generator=g).float()
+    Xd = [torch.randint(-1, 2, (n, 8), device=DEV, generator=g).float() for _ in range(2)]
+    M64 = Md.double()
+    refs = [M64.t() @ (M64 @ X.double()) for X in Xd]
+    del M64
+    torch.cuda.synchronize()
+    streams = [torch.cuda.Stream(DEV) for _ in range(2)]
+    ctxs = [_lib.Context(0) for _ in range(2)]
+    outs, launches, errs = [[], []], [0, 0], []
+
+    def run(i):
+        try:
+            with torch.cuda.stream(streams[i]):
+                ctxs[i].set_param(_lib.MFS_PARAM_FUSED, FORCE)
+                ctxs[i].track([_lib.MFS_K_MATVEC_FUSED])
+                for _ in range(reps):
+                    outs[i].append(ctxs[i].dense_normal_apply(Md, Xd[i]))
+                launches[i] = ctxs[i].tracked(_lib.MFS_K_MATVEC_FUSED)[0]
+                streams[i].synchronize()
+        except Exception as e:  # noqa: BLE001 - reported below
+            errs.append(e)
+
+    threads = [threading.Thread(target=run, args=(i,)) for i in range(2)]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    torch.cuda.synchronize()
+    print(f"two streams: one-pass launches {launches}, errors {errs}", flush=True)
+    assert not errs, errs
+    assert launches == [reps, reps]
+    for i in range(2):
+        for Q in outs[i]:
+            assert torch.equal(Q, refs[i]), (i, (Q - refs[i]).abs().max().item())
+
+
 def test_normal_apply_matches_fp64_ragged_rows():
     sc = scenes.coins(seed=0, n_bodies=8, pts_per_body=512, k=8)
     _, Md, _ = _setup(sc, extra_rows=45)                              # m = 4141: last panel of 13 rows
