@@ -22,7 +22,7 @@ MFS_RHS_BOUNDARY, MFS_RHS_SOURCE = 0, 1
  MFS_K_MATVEC_FUSED) = range(7)
 (MFS_PARAM_POINTS_BLOCKS, MFS_PARAM_COLS_BLOCKS, MFS_PARAM_FUSED, MFS_PARAM_TILE_SET,
  MFS_PARAM_FACTOR_PHASES, MFS_PARAM_FACTOR_STREAMS, MFS_PARAM_FACTOR_SPLITS,
- MFS_PARAM_GRAM_SPLIT) = 1, 2, 3, 4, 5, 6, 7, 8
+ MFS_PARAM_GRAM_SPLIT, MFS_PARAM_FACTOR_BANDS) = 1, 2, 3, 4, 5, 6, 7, 8, 9
 TILE_SIZES = (16, 12, 8, 6, 4, 3, 2, 1)   # bit i of MFS_PARAM_TILE_SET
 
 _STATUS_NAMES = {1: "MFS_ERR_ARG", 2: "MFS_ERR_HIP", 3: "MFS_ERR_DUPLICATE", 4: "MFS_ERR_NOT_SPD",
@@ -38,7 +38,7 @@ class MfsError(RuntimeError):
 class Stats(ctypes.Structure):
     _fields_ = [
         ("n", ctypes.c_int32), ("n_super", ctypes.c_int32), ("max_local", ctypes.c_int32),
-        ("pad0", ctypes.c_int32),
+        ("band_supernodes", ctypes.c_int32),
         ("nnz_pattern", ctypes.c_int64), ("nnz_factor", ctypes.c_int64),
         ("theta_entries", ctypes.c_int64),
         ("sum_ns2", ctypes.c_double), ("sum_ns3", ctypes.c_double),

@@ -102,10 +102,21 @@ typedef __bf16 bf8_t __attribute__((ext_vector_type(8)));
 typedef float f8_t __attribute__((ext_vector_type(8)));
 typedef float f4_t __attribute__((ext_vector_type(4)));
 
+typedef uint32_t u4_t __attribute__((ext_vector_type(4)));
+
+// hi goes back to FP32 from its packed words (element 2i is the low half of
+// word i): a shift and a mask per pair, not a conversion per value.
 __device__ inline void split_bf16(f4_t x0, f4_t x1, bf8_t &hi, bf8_t &lo) {
   const f8_t x = {x0.x, x0.y, x0.z, x0.w, x1.x, x1.y, x1.z, x1.w};
   hi = __builtin_convertvector(x, bf8_t);
-  lo = __builtin_convertvector(x - __builtin_convertvector(hi, f8_t), bf8_t);
+  const u4_t h = __builtin_bit_cast(u4_t, hi);
+  f8_t hf;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    hf[2 * i] = __uint_as_float(h[i] << 16);
+    hf[2 * i + 1] = __uint_as_float(h[i] & 0xffff0000u);
+  }
+  lo = __builtin_convertvector(x - hf, bf8_t);
 }
 
 // One 32-row k-step of the split Gram for the tile pairs GRP + NG (U0 + u):
@@ -177,13 +188,219 @@ __global__ __launch_bounds__(kThreads) void k_transpose(const float *__restrict_
   }
 }
 
+// Mt[i][f] = M[f][cols[i]]: the compact M^T of the columns that supernodes
+// outside the bands read (scattered reads along a row of M, 16-byte writes
+// along a row of Mt), zero for m <= f < ldmt.
+__global__ __launch_bounds__(kThreads) void k_gather_transpose(const float *__restrict__ M, int64_t m,
+                                                               int64_t n, const int32_t *__restrict__ cols,
+                                                               int32_t ncols, float *__restrict__ Mt,
+                                                               int64_t ldmt) {
+  __shared__ float tile[64][65];
+  const int t = threadIdx.x;
+  const int64_t i0 = int64_t(blockIdx.x) * 64, f0 = int64_t(blockIdx.y) * 64;
+  const int cc = t & 63;
+  const bool have = i0 + cc < ncols;
+  const int64_t col = have ? cols[i0 + cc] : 0;
+  for (int r = t >> 6; r < 64; r += kThreads / 64) {
+    const int64_t f = f0 + r;
+    tile[r][cc] = (have && f < m) ? M[f * n + col] : 0.f;
+  }
+  __syncthreads();
+  const int q = t & 15;
+  for (int ci = t >> 4; ci < 64; ci += kThreads / 16) {
+    const int64_t i = i0 + ci;
+    if (i >= ncols) continue;
+    *reinterpret_cast<float4 *>(Mt + i * ldmt + f0 + 4 * q) =
+        make_float4(tile[4 * q][ci], tile[4 * q + 1][ci], tile[4 * q + 2][ci], tile[4 * q + 3][ci]);
+  }
+}
+
+// ------------------------------------------------------------------ band Gram
+// The Grams of a band's supernodes straight from row-major M, with no M^T: a
+// block stages 32 rows x kBandCols columns of M in LDS, column-major with 36
+// floats per column (a column's 32 rows contiguous), while the next chunk's
+// loads are in flight in registers: 8 bytes (two columns) per lane and row,
+// 32 loads per thread, within the wave's count of loads in flight.  Each wave owns whole supernodes of one tile count
+// T: lane l supplies, for tile a, chunk rows 8 (l >> 4) .. + 7 of column
+// p = 16 a + (l & 15) (two 16-byte LDS reads) as the eight k-values of its
+// slot in both operands of the BF16 MFMA, split into hi + lo (three products
+// per FP32 product, as in the Gram from M^T), and keeps its supernodes'
+// accumulators in registers over all chunks of the block's row part.  LDS
+// column kBandCols stays zero: padding columns of a last tile read it.  The
+// partial Grams go to gram (16 T x 16 T per supernode, both triangles).
+constexpr int kBandRows = 32;
+constexpr int kBandStride = kBandRows + 4;
+constexpr int kBandThreads = kBandWaves * 64;
+constexpr int kBandLds = (kBandCols + 1) * kBandStride;  // floats
+static_assert(2 * kBandThreads == kBandCols, "the loads cover a band in two columns per thread");
+static_assert(kBandLds * 4 <= 160 * 1024, "LDS");
+
+template <int T>
+__device__ inline void band_wave(const float *__restrict__ M, int64_t m, int64_t n, int64_t c0,
+                                 int64_t r0, int64_t r1, float *chunk, const BandWave W,
+                                 const int32_t *__restrict__ items, const SuperDesc *__restrict__ sd,
+                                 const int32_t *__restrict__ sind, const int32_t *__restrict__ Pinv,
+                                 const int64_t *__restrict__ goff, float *__restrict__ gp) {
+  constexpr int C = band_capacity(T);
+  constexpr int P = T * (T + 1) / 2;
+  // supernodes whose MFMAs interleave (an accumulator's next product is at
+  // least P * GS instructions later)
+  constexpr int GS = T == 1 ? 6 : (T == 2 ? 2 : 1);
+  static_assert(C % GS == 0, "groups");
+  const int tid = threadIdx.x, lane = tid & 63;
+  // LDS offsets (floats) of the lane's column of every (supernode, tile);
+  // columns outside the band (none, by construction) read the zero column
+  int off[C][T];
+#pragma unroll
+  for (int j = 0; j < C; ++j) {
+    int32_t ns = 0, ro = 0;
+    if (j < W.count) {
+      const SuperDesc D = sd[items[W.first + j]];
+      ns = D.ns;
+      ro = D.rows_off;
+    }
+#pragma unroll
+    for (int a = 0; a < T; ++a) {
+      const int p = 16 * a + (lane & 15);
+      int64_t c = p < ns ? int64_t(Pinv[sind[ro + p]]) - c0 : int64_t(kBandCols);
+      if (c < 0 || c > kBandCols) c = kBandCols;
+      off[j][a] = int(c) * kBandStride + 8 * (lane >> 4);
+    }
+  }
+  f4_t acc[C][P];
+#pragma unroll
+  for (int j = 0; j < C; ++j)
+#pragma unroll
+    for (int u = 0; u < P; ++u) acc[j][u] = f4_t{0.f, 0.f, 0.f, 0.f};
+
+  // Loads: columns 2 tid and 2 tid + 1 of the band of all 32 rows of a chunk
+  // (8 bytes per lane, a 512-byte row segment per wave; n is even, so rows
+  // and pairs start 8-byte aligned).  A pair past the end of a row is moved
+  // back to the row's last pair; those columns lie outside the band's
+  // supernodes.
+  typedef float f2_t __attribute__((ext_vector_type(2)));
+  const int64_t cp = c0 + 2 * tid;
+  const uint32_t oc = uint32_t(cp + 1 < n ? cp : n - 2) * 4u;
+  f2_t rv[kBandRows];
+  auto load = [&](int64_t f0) {
+#pragma unroll
+    for (int r = 0; r < kBandRows; ++r) {
+      const int64_t f = f0 + r < m ? f0 + r : m - 1;
+      const char *row = reinterpret_cast<const char *>(M + f * n);
+      rv[r] = *reinterpret_cast<const f2_t *>(row + oc);
+    }
+  };
+  // rows past the part (the next part's, or past m) enter as zeros
+  auto store = [&](int64_t f0) {
+#pragma unroll
+    for (int q = 0; q < kBandRows / 4; ++q) {
+      f4_t va, vb;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const bool live = f0 + 4 * q + i < r1;
+        va[i] = live ? rv[4 * q + i].x : 0.f;
+        vb[i] = live ? rv[4 * q + i].y : 0.f;
+      }
+      *reinterpret_cast<f4_t *>(chunk + (2 * tid) * kBandStride + 4 * q) = va;
+      *reinterpret_cast<f4_t *>(chunk + (2 * tid + 1) * kBandStride + 4 * q) = vb;
+    }
+  };
+
+  const int64_t nch = r1 > r0 ? (r1 - r0 + kBandRows - 1) / kBandRows : 0;
+  if (nch > 0) load(r0);
+  for (int64_t k = 0; k < nch; ++k) {
+    __syncthreads();  // every wave is done with the previous chunk
+    store(r0 + k * kBandRows);
+    __syncthreads();
+    if (k + 1 < nch) load(r0 + (k + 1) * kBandRows);
+#pragma unroll
+    for (int j0 = 0; j0 < C; j0 += GS) {
+      if (j0 >= W.count) break;
+      bf8_t hi[GS][T], lo[GS][T];
+#pragma unroll
+      for (int g = 0; g < GS; ++g)
+#pragma unroll
+        for (int a = 0; a < T; ++a) {
+          const f4_t x0 = *reinterpret_cast<const f4_t *>(chunk + off[j0 + g][a]);
+          const f4_t x1 = *reinterpret_cast<const f4_t *>(chunk + off[j0 + g][a] + 4);
+          split_bf16(x0, x1, hi[g][a], lo[g][a]);
+        }
+#pragma unroll
+      for (int g = 0; g < GS; ++g)
+#pragma unroll
+        for (int u = 0; u < P; ++u)
+          acc[j0 + g][u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(hi[g][pair_a(u)], hi[g][pair_b(u)],
+                                                                   acc[j0 + g][u], 0, 0, 0);
+#pragma unroll
+      for (int g = 0; g < GS; ++g)
+#pragma unroll
+        for (int u = 0; u < P; ++u)
+          acc[j0 + g][u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(hi[g][pair_a(u)], lo[g][pair_b(u)],
+                                                                   acc[j0 + g][u], 0, 0, 0);
+#pragma unroll
+      for (int g = 0; g < GS; ++g)
+#pragma unroll
+        for (int u = 0; u < P; ++u)
+          acc[j0 + g][u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(lo[g][pair_a(u)], hi[g][pair_b(u)],
+                                                                   acc[j0 + g][u], 0, 0, 0);
+    }
+  }
+
+  // Partial Grams: lane l holds rows 4 (l >> 4) + v, column l & 15 of each
+  // 16 x 16 tile product.
+  constexpr int NS = 16 * T;
+#pragma unroll
+  for (int j = 0; j < C; ++j) {
+    if (j >= W.count) break;
+    float *g = gp + goff[items[W.first + j]];
+#pragma unroll
+    for (int u = 0; u < P; ++u) {
+      const int a = pair_a(u), b = pair_b(u);
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int p = 16 * a + 4 * (lane >> 4) + v, q = 16 * b + (lane & 15);
+        g[p * NS + q] = acc[j][u][v];
+        if (a != b) g[q * NS + p] = acc[j][u][v];
+      }
+    }
+  }
+}
+
+__global__ __launch_bounds__(kBandThreads, 1) void k_band_gram(
+    const float *__restrict__ M, int64_t m, int64_t n, const BandBlock *__restrict__ blocks,
+    const int32_t *__restrict__ items, const SuperDesc *__restrict__ sd,
+    const int32_t *__restrict__ sind, const int32_t *__restrict__ Pinv,
+    const int64_t *__restrict__ goff, float *__restrict__ gram, int64_t gram_part, int64_t part_rows) {
+  __shared__ __align__(16) float chunk[kBandLds];
+  const BandBlock &B = blocks[blockIdx.x];
+  const int32_t c0 = B.c0;
+  if (c0 < 0) return;  // padding block (uniform)
+  const int wave = __builtin_amdgcn_readfirstlane(int(threadIdx.x) >> 6);
+  const BandWave W = B.w[wave];
+  if (threadIdx.x < kBandStride) chunk[kBandCols * kBandStride + threadIdx.x] = 0.f;
+  const int64_t r0 = int64_t(B.part) * part_rows;
+  const int64_t r1 = r0 + part_rows < m ? r0 + part_rows : m;
+  float *gp = gram + int64_t(B.part) * gram_part;
+  // Every wave runs the same chunk loop (loads, stores, barriers); idle waves
+  // (count 0) take the T = 1 path and compute nothing.
+  switch (W.tiles) {
+    case 2: band_wave<2>(M, m, n, c0, r0, r1, chunk, W, items, sd, sind, Pinv, goff, gp); break;
+    case 3: band_wave<3>(M, m, n, c0, r0, r1, chunk, W, items, sd, sind, Pinv, goff, gp); break;
+    case 4: band_wave<4>(M, m, n, c0, r0, r1, chunk, W, items, sd, sind, Pinv, goff, gp); break;
+    case 6: band_wave<6>(M, m, n, c0, r0, r1, chunk, W, items, sd, sind, Pinv, goff, gp); break;
+    default: band_wave<1>(M, m, n, c0, r0, r1, chunk, W, items, sd, sind, Pinv, goff, gp); break;
+  }
+}
+
 // ------------------------------------------------------------------ fused supernode factor
 // PTS: the points form -- the Gram's operands are evaluated from the packed
 // point records (tgt: m x 12, srcp: n x 8) instead of read from M^T.  Each
 // lane evaluates exactly the MFMA operand entries it supplies, A[f][p] with
 // p = 16 a + (lane & 15) for every tile a and f = f0 + (lane >> 4), keeping
 // its T source points in registers: no LDS staging and no barriers.
-template <int T, typename TC, typename TV, bool PTS, bool GB = false>
+// GIN: the Gram comes summed from the band partials (FS.gram); only the
+// Cholesky and the KL columns run.
+template <int T, typename TC, typename TV, bool PTS, bool GB = false, bool GIN = false>
 __global__ __launch_bounds__(kThreads) void k_super_factor(const FactorScene *__restrict__ scenes,
                                                            const FactorItem *__restrict__ items,
                                                            TC *__restrict__ th_scratch,
@@ -242,7 +459,15 @@ __global__ __launch_bounds__(kThreads) void k_super_factor(const FactorScene *__
   const SuperDesc D = sd[s];
   const int ns = D.ns;
 
-  for (int p = tid; p < NS; p += kThreads) src[p] = (p < ns) ? Pinv[sind[D.rows_off + p]] : -1;
+  const int32_t *__restrict__ colmap = FS.colmap;
+  for (int p = tid; p < NS; p += kThreads) {
+    int32_t c = -1;
+    if (p < ns) {
+      c = Pinv[sind[D.rows_off + p]];
+      if (colmap != nullptr) c = colmap[c];  // row of the compact M^T
+    }
+    src[p] = c;
+  }
   __syncthreads();
 
   // ---- 1. Gram on MFMA ------------------------------------------------------
@@ -552,7 +777,21 @@ __global__ __launch_bounds__(kThreads) void k_super_factor(const FactorScene *__
   }
   __syncthreads();
   };
-  static_for<NPASS>(gram_pass);
+  if constexpr (GIN) {
+    // TH (reversed order) = the sum of the band partials, in part order
+    const float *__restrict__ g = FS.gram + FS.goff[s];
+    const int32_t parts = FS.parts;
+    const int64_t gpart = FS.gram_part;
+    for (int idx = tid; idx < ns * ns; idx += kThreads) {
+      const int p = idx / ns, q = idx - p * ns;
+      TC sum = TC(0);
+      for (int r = 0; r < parts; ++r) sum += TC(g[r * gpart + p * NS + q]);
+      TH[(ns - 1 - p) * RS + (ns - 1 - q)] = sum;
+    }
+    __syncthreads();
+  } else {
+    static_for<NPASS>(gram_pass);
+  }
 
   // Row splits: every block publishes its partial Gram (LARGE: its TH in
   // global scratch already); the last of the S blocks to arrive sums all S
@@ -689,6 +928,7 @@ struct FactorLaunch {
   void *th_scratch;
   FactorSplit sp;
   bool split_bf16;
+  bool gram_in;
 };
 
 template <int T, typename TC, typename TV>
@@ -699,6 +939,13 @@ hipError_t launch_t(hipStream_t st, const FactorLaunch &L) {
     k_super_factor<T, TC, TV, true><<<grid, kThreads, 0, st>>>(
         L.scenes, L.items, static_cast<TC *>(L.th_scratch), L.sp);
   else if constexpr (sizeof(TC) == 4 && sizeof(TV) == 4) {
+    if constexpr (T <= 6) {
+      if (L.gram_in) {
+        k_super_factor<T, TC, TV, false, false, true><<<grid, kThreads, 0, st>>>(
+            L.scenes, L.items, static_cast<TC *>(L.th_scratch), L.sp);
+        return hipGetLastError();
+      }
+    }
     if (L.split_bf16)
       k_super_factor<T, TC, TV, false, true><<<grid, kThreads, 0, st>>>(
           L.scenes, L.items, static_cast<TC *>(L.th_scratch), L.sp);
@@ -741,14 +988,35 @@ hipError_t launch_transpose(hipStream_t st, const float *M, int64_t m, int64_t n
 
 hipError_t launch_super_factor(hipStream_t st, int tiles, int precision, bool points,
                                const FactorScene *scenes, const FactorItem *items, int32_t count,
-                               void *th_scratch, const FactorSplit &sp, bool split_bf16) {
+                               void *th_scratch, const FactorSplit &sp, bool split_bf16,
+                               bool gram_in) {
   if (sp.splits > 1 && ((sp.partial == nullptr && tiles <= 8) || sp.counters == nullptr))
     return hipErrorInvalidValue;
   if (split_bf16 && (precision != 1 || points)) return hipErrorInvalidValue;
-  const FactorLaunch L{scenes, items, count, points, th_scratch, sp, split_bf16};
+  if (gram_in && (precision != 1 || points || sp.splits > 1 || tiles > 6)) return hipErrorInvalidValue;
+  const FactorLaunch L{scenes, items, count, points, th_scratch, sp, split_bf16, gram_in};
   if (precision == 0) return launch_prec<double, double>(st, tiles, L);
   if (precision == 1) return launch_prec<float, float>(st, tiles, L);
   return launch_prec<double, float>(st, tiles, L);
+}
+
+hipError_t launch_band_gram(hipStream_t st, const float *M, int64_t m, int64_t n,
+                            const BandBlock *blocks, int32_t count, const int32_t *items,
+                            const SuperDesc *sd, const int32_t *sind, const int32_t *Pinv,
+                            const int64_t *goff, float *gram, int64_t gram_part,
+                            int64_t part_rows) {
+  if (count <= 0) return hipSuccess;
+  k_band_gram<<<unsigned(count), kBandThreads, 0, st>>>(M, m, n, blocks, items, sd, sind, Pinv, goff,
+                                                        gram, gram_part, part_rows);
+  return hipGetLastError();
+}
+
+hipError_t launch_gather_transpose(hipStream_t st, const float *M, int64_t m, int64_t n,
+                                   const int32_t *cols, int32_t ncols, float *Mt, int64_t ldmt) {
+  if (ncols <= 0) return hipSuccess;
+  dim3 grid(unsigned((ncols + 63) / 64), unsigned(ldmt / 64));
+  k_gather_transpose<<<grid, kThreads, 0, st>>>(M, m, n, cols, ncols, Mt, ldmt);
+  return hipGetLastError();
 }
 
 hipError_t launch_gather_values(hipStream_t st, int precision, int32_t nnz, const int32_t *tmap,

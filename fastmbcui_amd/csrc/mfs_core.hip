@@ -192,6 +192,22 @@ struct mfs_ctx {
   // 20.4 ms against 23.0 ms for the factor at 65,536^2, with column errors
   // against the FP64 factor no larger (max 9.9e-5 against 1.4e-4).
   bool gram_split = true;
+  // Band form of the dense FP32 factor with split products (k_band_gram):
+  // the Grams of the supernodes inside a band of kBandCols columns come
+  // straight from row-major M, the rest from a compact M^T of their columns.
+  // Packed at plan time (plan_bands); MFS_PARAM_FACTOR_BANDS switches it off.
+  bool band_mode = true;
+  std::vector<BandBlock> band_groups;  // blocks of 8 waves (part 0)
+  std::vector<int8_t> band_xcd;        // XCD slot of each block (its band's index mod 8)
+  std::vector<int32_t> band_items;     // supernode ids of the waves
+  std::vector<int64_t> goff;           // per supernode: its Gram's offset in a part (-1: no band)
+  int64_t gram_elems = 0;              // one part: sum over band supernodes of (16 T)^2
+  int32_t band_supernodes = 0;
+  std::vector<FactorItem> gin_items, rest_items;
+  std::vector<Bucket> gin_buckets, rest_buckets;
+  std::vector<int32_t> rest_cols;      // original columns the rest read (ascending)
+  size_t rest_scratch = 8;             // th_scratch bytes of the rest's large tiles
+  DevBuf dbitems, dgoff, dgram, dblocks, dginlist, drestlist, drestcols, dcolmap, dMtc;
   int track_mask = 0;
   struct Track {
     std::vector<std::pair<hipEvent_t, hipEvent_t>> pairs;
@@ -296,6 +312,140 @@ double phase_end(mfs_ctx *ctx) {
   float ms = 0.f;
   (void)hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
   return ms;
+}
+
+// The band form's packing.  A supernode goes into a band when its tile count
+// has a band capacity and its columns (original order) lie within kBandCols
+// of its first column rounded down to even; bands are runs of kBandCols
+// columns, each opened at the (even-rounded) first column of the supernodes
+// not yet placed.  A band's supernodes are
+// packed into waves of one tile count (band_capacity(T) each) and the waves
+// into blocks of 8, heaviest first onto the least loaded block, the two waves
+// sharing a SIMD (w and w + 4) one heavy, one light.  The other supernodes
+// (spanning bands, or larger tiles) read a compact M^T of their columns.
+int plan_bands(mfs_ctx *ctx, int32_t n) {
+  const int32_t n_super = int32_t(ctx->sdesc.size());
+  ctx->band_groups.clear();
+  ctx->band_xcd.clear();
+  ctx->band_items.clear();
+  ctx->gin_items.clear();
+  ctx->rest_items.clear();
+  ctx->gin_buckets.clear();
+  ctx->rest_buckets.clear();
+  ctx->rest_cols.clear();
+  ctx->goff.assign(size_t(n_super), -1);
+  ctx->gram_elems = 0;
+  ctx->band_supernodes = 0;
+  ctx->rest_scratch = 8;
+  std::vector<int32_t> cmin(n_super), cmax(n_super), tt(n_super), elig;
+  for (int32_t s = 0; s < n_super; ++s) {
+    const SuperDesc &d = ctx->sdesc[s];
+    int32_t lo = n, hi = -1;
+    for (int32_t p = 0; p < d.ns; ++p) {
+      const int32_t c = ctx->Pinv[ctx->sind[d.rows_off + p]];
+      lo = std::min(lo, c);
+      hi = std::max(hi, c);
+    }
+    cmin[s] = lo;
+    cmax[s] = hi;
+    tt[s] = tiles_for(d.ns, ctx->plan_mask);
+    // bands start at even columns (the band Gram loads column pairs, 8-byte aligned)
+    if (band_capacity(tt[s]) > 0 && hi - (lo & ~1) < kBandCols) elig.push_back(s);
+  }
+  std::stable_sort(elig.begin(), elig.end(), [&](int32_t a, int32_t b) { return cmin[a] < cmin[b]; });
+  std::vector<std::pair<int32_t, std::vector<int32_t>>> bands;
+  for (int32_t s : elig) {
+    if (bands.empty() || cmax[s] >= bands.back().first + kBandCols) bands.push_back({cmin[s] & ~1, {}});
+    bands.back().second.push_back(s);
+  }
+  static constexpr int kBandTiles[] = {6, 4, 3, 2, 1};
+  std::vector<std::vector<FactorItem>> gin_by_t(17);
+  for (size_t bi = 0; bi < bands.size(); ++bi) {
+    struct Wv {
+      BandWave w;
+      double work;
+    };
+    std::vector<Wv> waves;
+    for (int T : kBandTiles) {
+      std::vector<int32_t> v;
+      for (int32_t s : bands[bi].second)
+        if (tt[s] == T) v.push_back(s);
+      const int C = band_capacity(T), P = T * (T + 1) / 2;
+      for (size_t k = 0; k < v.size(); k += size_t(C)) {
+        const int32_t cnt = int32_t(std::min(v.size() - k, size_t(C)));
+        Wv wv{{T, cnt, int32_t(ctx->band_items.size())}, 0.0};
+        for (int32_t q = 0; q < cnt; ++q) {
+          const int32_t sid = v[k + size_t(q)];
+          ctx->band_items.push_back(sid);
+          ctx->goff[sid] = ctx->gram_elems;
+          ctx->gram_elems += int64_t(16 * T) * (16 * T);
+          gin_by_t[T].push_back({0, sid});
+        }
+        // MFMA cycles (3 products of 16 per tile pair) + operand split per tile
+        wv.work = double(cnt) * (48.0 * P + 32.0 * T);
+        waves.push_back(wv);
+      }
+    }
+    std::stable_sort(waves.begin(), waves.end(), [](const Wv &a, const Wv &b) { return a.work > b.work; });
+    const size_t nb = (waves.size() + kBandWaves - 1) / kBandWaves;
+    std::vector<std::vector<Wv>> blk(nb);
+    std::vector<double> load(nb, 0.0);
+    for (const Wv &w : waves) {
+      size_t best = nb;
+      for (size_t b = 0; b < nb; ++b)
+        if (blk[b].size() < size_t(kBandWaves) && (best == nb || load[b] < load[best])) best = b;
+      blk[best].push_back(w);
+      load[best] += w.work;
+    }
+    for (size_t b = 0; b < nb; ++b) {
+      BandBlock B{};
+      B.c0 = bands[bi].first;
+      B.part = 0;
+      for (int i = 0; i < kBandWaves; ++i) B.w[i] = BandWave{0, 0, 0};
+      // heaviest first; SIMD i holds the i-th heaviest and the i-th lightest
+      const int nw = int(blk[b].size());
+      for (int i = 0; i < nw; ++i) {
+        const int pos = i < kBandWaves / 2 ? i : kBandWaves - 1 - (i - kBandWaves / 2);
+        B.w[pos] = blk[b][size_t(i)].w;
+      }
+      ctx->band_groups.push_back(B);
+      ctx->band_xcd.push_back(int8_t(bi % 8));
+    }
+  }
+  for (int T : kBandTiles) {
+    if (gin_by_t[T].empty()) continue;
+    ctx->gin_buckets.push_back({T, int32_t(ctx->gin_items.size()), int32_t(gin_by_t[T].size()), {}});
+    ctx->gin_items.insert(ctx->gin_items.end(), gin_by_t[T].begin(), gin_by_t[T].end());
+  }
+  ctx->band_supernodes = int32_t(ctx->gin_items.size());
+  // the rest, bucketed by tile count, and the columns they read
+  std::vector<std::vector<FactorItem>> rest_by_t(17);
+  std::vector<char> used(size_t(n), 0);
+  for (int32_t s = 0; s < n_super; ++s) {
+    if (ctx->goff[s] >= 0) continue;
+    rest_by_t[tt[s]].push_back({0, s});
+    const SuperDesc &d = ctx->sdesc[s];
+    for (int32_t p = 0; p < d.ns; ++p) used[size_t(ctx->Pinv[ctx->sind[d.rows_off + p]])] = 1;
+  }
+  for (int t : kTileSizes) {
+    if (rest_by_t[t].empty()) continue;
+    ctx->rest_buckets.push_back({t, int32_t(ctx->rest_items.size()), int32_t(rest_by_t[t].size()), {}});
+    ctx->rest_items.insert(ctx->rest_items.end(), rest_by_t[t].begin(), rest_by_t[t].end());
+    ctx->rest_scratch = std::max(ctx->rest_scratch, rest_by_t[t].size() * size_t(factor_scratch_elems(t)) * 4);
+  }
+  std::vector<int32_t> colmap(size_t(n), -1);
+  for (int32_t c = 0; c < n; ++c)
+    if (used[size_t(c)]) {
+      colmap[size_t(c)] = int32_t(ctx->rest_cols.size());
+      ctx->rest_cols.push_back(c);
+    }
+  HIPCHK(upload(ctx, ctx->dbitems, ctx->band_items), "upload");
+  HIPCHK(upload(ctx, ctx->dgoff, ctx->goff), "upload");
+  HIPCHK(upload(ctx, ctx->dginlist, ctx->gin_items), "upload");
+  HIPCHK(upload(ctx, ctx->drestlist, ctx->rest_items), "upload");
+  HIPCHK(upload(ctx, ctx->drestcols, ctx->rest_cols), "upload");
+  HIPCHK(upload(ctx, ctx->dcolmap, colmap), "upload");
+  return MFS_OK;
 }
 
 int plan_impl(mfs_ctx *ctx, std::vector<double> &&xyz, int64_t n64, double rho, double lambda,
@@ -512,6 +662,10 @@ int plan_impl(mfs_ctx *ctx, std::vector<double> &&xyz, int64_t n64, double rho, 
   for (size_t i = 0; i < ctx->items.size(); ++i) ctx->items[i] = {0, ctx->list[i]};
   HIPCHK(upload(ctx, ctx->dlist, ctx->items), "upload");
   HIPCHK(ctx->dstatus.ensure(size_t(n_super) * 4), "alloc");
+  {
+    const int rc = plan_bands(ctx, n);
+    if (rc != MFS_OK) return rc;
+  }
 
   ctx->n = n;
   ctx->has_plan = true;
@@ -843,18 +997,29 @@ void set_pass_work(mfs_ctx *ctx, const PointsOp *pts, int64_t m, int64_t n, int 
 // statuses, and the scene record the factor kernels read (ctx->fscene,
 // ctx->dfscene).
 int factor_prepare(mfs_ctx *ctx, const float *M, const PointsOp *pts, int64_t m, int64_t n,
-                   int precision) {
+                   int precision, int parts = 0) {
   if (precision != MFS_FP64 && precision != MFS_FP32)
     return ctx->fail(MFS_ERR_ARG, "factor: precision must be MFS_FP64 or MFS_FP32");
   HIPCHK(hipSetDevice(ctx->device), "hipSetDevice");
   ctx->has_factor = false;
   ctx->alt_prec = -1;
   const int64_t ldmt = (m + kGramChunk - 1) / kGramChunk * kGramChunk;
-  if (pts == nullptr) {  // filled by factor_impl, range by range; row n stays zero
-    HIPCHK(ctx->dMt.ensure(size_t(n + 1) * size_t(ldmt) * sizeof(float)), "alloc M^T");
-    HIPCHK(hipMemsetAsync(ctx->dMt.as<float>() + size_t(n) * size_t(ldmt), 0,
-                          size_t(ldmt) * sizeof(float), ctx->stream),
+  // Dense form: M^T of every column (filled by factor_impl, range by range),
+  // or in the band form (parts > 0) a compact M^T of the rest's columns, and
+  // the band partial Grams.  The row after the last stays zero.
+  float *mt = nullptr;
+  int64_t mt_rows = 0;
+  if (pts == nullptr) {
+    DevBuf &buf = parts > 0 ? ctx->dMtc : ctx->dMt;
+    mt_rows = parts > 0 ? int64_t(ctx->rest_cols.size()) : n;
+    HIPCHK(buf.ensure(size_t(mt_rows + 1) * size_t(ldmt) * sizeof(float)), "alloc M^T");
+    mt = buf.as<float>();
+    HIPCHK(hipMemsetAsync(mt + size_t(mt_rows) * size_t(ldmt), 0, size_t(ldmt) * sizeof(float),
+                          ctx->stream),
            "clear zero row");
+    if (parts > 0)
+      HIPCHK(ctx->dgram.ensure(size_t(std::max<int64_t>(ctx->gram_elems, 1)) * size_t(parts) * sizeof(float)),
+             "alloc band Grams");
   }
   const int64_t nnz = ctx->sptr[ctx->n];
   const size_t w = (precision == MFS_FP64) ? 8 : 4;
@@ -863,9 +1028,14 @@ int factor_prepare(mfs_ctx *ctx, const float *M, const PointsOp *pts, int64_t m,
   const int32_t n_super = int32_t(ctx->sdesc.size());
   HIPCHK(hipMemsetAsync(ctx->dstatus.p, 0, size_t(n_super) * 4, ctx->stream), "memset");
   FactorScene &F = ctx->fscene;
-  F.Mt = pts ? nullptr : ctx->dMt.as<float>();
-  F.Mz = pts ? nullptr : ctx->dMt.as<float>() + size_t(n) * size_t(ldmt);
+  F.Mt = mt;
+  F.Mz = pts ? nullptr : mt + size_t(mt_rows) * size_t(ldmt);
   F.ldmt = ldmt;
+  F.colmap = parts > 0 ? ctx->dcolmap.as<int32_t>() : nullptr;
+  F.gram = parts > 0 ? ctx->dgram.as<float>() : nullptr;
+  F.goff = parts > 0 ? ctx->dgoff.as<int64_t>() : nullptr;
+  F.gram_part = parts > 0 ? ctx->gram_elems : 0;
+  F.parts = parts;
   F.tgt = pts ? pts->tgt32 : nullptr;
   F.src = pts ? pts->src32 : nullptr;
   F.m = m;
@@ -889,7 +1059,8 @@ int factor_prepare(mfs_ctx *ctx, const float *M, const PointsOp *pts, int64_t m,
 
 // Factor phase 3 (after the bucket launches): statuses, FP64 retry of failed
 // FP32 pivots (values stay FP32), transposed copy of the values.
-int factor_finish(mfs_ctx *ctx, bool points, int precision) {
+int factor_finish(mfs_ctx *ctx, bool points, int precision, const float *M = nullptr,
+                  int64_t m = 0, int64_t n = 0) {
   const int32_t n_super = int32_t(ctx->sdesc.size());
   const int64_t nnz = ctx->sptr[ctx->n];
   std::vector<int32_t> status(n_super);
@@ -903,6 +1074,26 @@ int factor_finish(mfs_ctx *ctx, bool points, int precision) {
   ctx->stats.retried_supernodes = 0;
   if (!failed.empty() && precision == MFS_FP32) {
     ctx->stats.retried_supernodes = int32_t(failed.size());
+    if (!points && ctx->fscene.parts > 0) {
+      // the band form built no M^T of the band columns: the FP64 retry reads
+      // M^T of every column
+      FactorScene &F = ctx->fscene;
+      const int64_t ldmt = F.ldmt;
+      HIPCHK(ctx->dMt.ensure(size_t(n + 1) * size_t(ldmt) * sizeof(float)), "alloc M^T");
+      HIPCHK(hipMemsetAsync(ctx->dMt.as<float>() + size_t(n) * size_t(ldmt), 0,
+                            size_t(ldmt) * sizeof(float), ctx->stream),
+             "clear zero row");
+      HIPCHK(launch_transpose(ctx->stream, M, m, n, ctx->dMt.as<float>(), ldmt, 0, n), "transpose");
+      F.Mt = ctx->dMt.as<float>();
+      F.Mz = F.Mt + size_t(n) * size_t(ldmt);
+      F.colmap = nullptr;
+      F.gram = nullptr;
+      F.goff = nullptr;
+      F.gram_part = 0;
+      F.parts = 0;
+      HIPCHK(hipMemcpyAsync(ctx->dfscene.p, &F, sizeof(FactorScene), hipMemcpyHostToDevice, ctx->stream),
+             "upload scene");
+    }
     std::vector<std::vector<FactorItem>> by_t(17);
     for (int32_t s : failed) by_t[tiles_for(ctx->sdesc[s].ns, ctx->plan_mask)].push_back({0, s});
     std::vector<FactorItem> flist;
@@ -942,6 +1133,85 @@ int factor_finish(mfs_ctx *ctx, bool points, int precision) {
   return MFS_OK;
 }
 
+// Row parts of the band Gram: the fewest (up to 8, at least 512 rows each)
+// that keep the last round of blocks about as full as the others.
+int band_parts(const mfs_ctx *ctx, int64_t m) {
+  const int64_t G = int64_t(ctx->band_groups.size());
+  const int64_t cus = ctx->cus > 0 ? ctx->cus : 256;
+  int R = 1;
+  double best = 1e30;
+  for (int r = 1; r <= 8; ++r) {
+    if (r > 1 && m / r < 512) break;
+    const double rounds = double((G * r + cus - 1) / cus) / r;
+    if (rounds < best * (1.0 - 1e-3)) {
+      best = rounds;
+      R = r;
+    }
+  }
+  return R;
+}
+
+// Dense FP32 factor in the band form: one band-Gram launch over all bands
+// (straight from M), then on the side streams the rest (compact M^T of their
+// columns, then their buckets) and the Cholesky and KL columns of the band
+// supernodes from the summed partials, bucket by bucket.
+int factor_bands(mfs_ctx *ctx, const float *M, int64_t m, int64_t n, int parts) {
+  const int64_t part_rows = ((m + parts - 1) / parts + 31) / 32 * 32;
+  // Launch order: blocks are dealt to the XCDs round-robin (block i to XCD
+  // i mod 8), so XCD x gets the blocks of the bands with index x mod 8, part
+  // by part, a band's blocks side by side: blocks that share a band's rows
+  // run together on one L2.
+  std::vector<std::vector<BandBlock>> seq(8);
+  for (int r = 0; r < parts; ++r)
+    for (size_t g = 0; g < ctx->band_groups.size(); ++g) {
+      BandBlock b = ctx->band_groups[g];
+      b.part = r;
+      seq[size_t(ctx->band_xcd[g])].push_back(b);
+    }
+  size_t len = 0;
+  for (auto &q : seq) len = std::max(len, q.size());
+  BandBlock empty{};
+  empty.c0 = -1;
+  std::vector<BandBlock> blocks(8 * len, empty);
+  for (size_t x = 0; x < 8; ++x)
+    for (size_t i = 0; i < seq[x].size(); ++i) blocks[8 * i + x] = seq[x][i];
+  HIPCHK(upload(ctx, ctx->dblocks, blocks), "upload band blocks");
+  HIPCHK(ctx->dthg.ensure(ctx->rest_scratch), "alloc factor scratch");
+  const int nside = std::max(1, std::min<int>(ctx->side_streams, mfs_ctx::kSide));
+  HIPCHK(launch_band_gram(ctx->stream, M, m, n, ctx->dblocks.as<BandBlock>(), int32_t(blocks.size()),
+                          ctx->dbitems.as<int32_t>(), ctx->dsd.as<SuperDesc>(), ctx->dsind.as<int32_t>(),
+                          ctx->dPinv.as<int32_t>(), ctx->dgoff.as<int64_t>(), ctx->dgram.as<float>(),
+                          ctx->gram_elems, part_rows),
+         "band Gram");
+  HIPCHK(hipEventRecord(ctx->fork_ev, ctx->stream), "fork");
+  for (int i = 0; i < nside; ++i) HIPCHK(hipStreamWaitEvent(ctx->side[i], ctx->fork_ev, 0), "fork");
+  // the rest on side stream 0
+  if (!ctx->rest_items.empty()) {
+    HIPCHK(launch_gather_transpose(ctx->side[0], M, m, n, ctx->drestcols.as<int32_t>(),
+                                   int32_t(ctx->rest_cols.size()), ctx->dMtc.as<float>(), ctx->fscene.ldmt),
+           "gather M^T");
+    for (const Bucket &b : ctx->rest_buckets)
+      HIPCHK(launch_super_factor(ctx->side[0], b.tiles, 1, false, ctx->dfscene.as<FactorScene>(),
+                                 ctx->drestlist.as<FactorItem>() + b.offset, b.count, ctx->dthg.p,
+                                 FactorSplit{}, true),
+             "supernode factor");
+  }
+  // Cholesky and KL columns of the band supernodes
+  for (size_t i = 0; i < ctx->gin_buckets.size(); ++i) {
+    const Bucket &b = ctx->gin_buckets[i];
+    hipStream_t st = nside > 1 ? ctx->side[1 + i % size_t(nside - 1)] : ctx->side[0];
+    HIPCHK(launch_super_factor(st, b.tiles, 1, false, ctx->dfscene.as<FactorScene>(),
+                               ctx->dginlist.as<FactorItem>() + b.offset, b.count, nullptr,
+                               FactorSplit{}, false, true),
+           "supernode factor");
+  }
+  for (int i = 0; i < nside; ++i) {
+    HIPCHK(hipEventRecord(ctx->join_ev[i], ctx->side[i]), "join");
+    HIPCHK(hipStreamWaitEvent(ctx->stream, ctx->join_ev[i], 0), "join");
+  }
+  return MFS_OK;
+}
+
 // Supernodal factor from M (pts == nullptr) or from the packed points.
 int factor_impl(mfs_ctx *ctx, const float *M, const PointsOp *pts, int64_t m, int64_t n,
                 int precision) {
@@ -949,8 +1219,25 @@ int factor_impl(mfs_ctx *ctx, const float *M, const PointsOp *pts, int64_t m, in
     return ctx->fail(MFS_ERR_ARG, "factor: precision must be MFS_FP64 or MFS_FP32");
   HIPCHK(hipSetDevice(ctx->device), "hipSetDevice");
   phase_begin(ctx);
-  int rc = factor_prepare(ctx, M, pts, m, n, precision);
+  const bool bands = pts == nullptr && precision == MFS_FP32 && ctx->gram_split && ctx->band_mode &&
+                     !ctx->band_groups.empty() && n % 2 == 0;
+  const int parts = bands ? band_parts(ctx, m) : 0;
+  int rc = factor_prepare(ctx, M, pts, m, n, precision, parts);
   if (rc != MFS_OK) return rc;
+  ctx->stats.band_supernodes = bands ? ctx->band_supernodes : 0;
+  if (bands) {
+    rc = ensure_side_streams(ctx);
+    if (rc != MFS_OK) return rc;
+    track_cb(ctx, MFS_K_GRAM, 0);
+    rc = factor_bands(ctx, M, m, n, parts);
+    if (rc != MFS_OK) return rc;
+    track_cb(ctx, MFS_K_GRAM, 1);
+    rc = factor_finish(ctx, false, precision, M, m, n);
+    if (rc != MFS_OK) return rc;
+    ctx->stats.factor_ms = phase_end(ctx);
+    track_fold(ctx);
+    return MFS_OK;
+  }
   track_cb(ctx, MFS_K_GRAM, 0);
   // Buckets (largest tiles first) run on side streams at once, so that the
   // few large supernodes and each launch's tail overlap the bulk of the
@@ -1043,7 +1330,7 @@ int factor_impl(mfs_ctx *ctx, const float *M, const PointsOp *pts, int64_t m, in
     HIPCHK(hipStreamWaitEvent(ctx->stream, ctx->join_ev[i], 0), "join");
   }
   track_cb(ctx, MFS_K_GRAM, 1);
-  rc = factor_finish(ctx, pts != nullptr, precision);
+  rc = factor_finish(ctx, pts != nullptr, precision, M, m, n);
   if (rc != MFS_OK) return rc;
   ctx->stats.factor_ms = phase_end(ctx);
   track_fold(ctx);
@@ -1967,6 +2254,10 @@ int mfs_ctx_set_param(mfs_ctx *ctx, int param, int64_t value) {
     case MFS_PARAM_GRAM_SPLIT:
       if (value > 2) return ctx->fail(MFS_ERR_ARG, "set_param: Gram products are 0, 1 or 2");
       ctx->gram_split = value != 1;
+      return MFS_OK;
+    case MFS_PARAM_FACTOR_BANDS:
+      if (value > 1) return ctx->fail(MFS_ERR_ARG, "set_param: factor bands are 0 (on) or 1 (off)");
+      ctx->band_mode = value == 0;
       return MFS_OK;
     case MFS_PARAM_FACTOR_STREAMS:
       if (value > mfs_ctx::kSide) return ctx->fail(MFS_ERR_ARG, "set_param: 1 to 4 factor streams");

@@ -87,6 +87,13 @@ hipError_t launch_transpose(hipStream_t st, const float *M, int64_t m, int64_t n
 // One scene's inputs and outputs of the factor (device pointers).  Dense form:
 // Mt (n x ldmt) and a row of ldmt zeros Mz (read by the split-BF16 Gram for
 // padding columns); points form: tgt (m x 12) and src (n x 8) records.
+//
+// Band form of the dense FP32 factor (k_band_gram): `gram` holds `parts`
+// partial Grams per supernode (part r at gram + r * gram_part), supernode s's
+// at offset goff[s] (16 T x 16 T, row-major, both triangles; -1: not in a
+// band), which the Cholesky launches sum in part order.  The other supernodes
+// read their columns from a compact M^T whose row for original column c is
+// colmap[c] (nullptr: Mt holds every column, row c).
 struct FactorScene {
   const float *Mt;
   const float *Mz;
@@ -97,7 +104,43 @@ struct FactorScene {
   const int32_t *sind, *Pinv, *sup_ind, *sptr;
   void *val;        // factor values, TV
   int32_t *status;  // per supernode: 0 done, 1 non-positive pivot
+  const int32_t *colmap = nullptr;
+  const float *gram = nullptr;
+  const int64_t *goff = nullptr;
+  int64_t gram_part = 0;
+  int32_t parts = 0;
 };
+
+// ---- band Gram (dense FP32 factor, BF16 split products) -------------------
+// A band is kBandCols consecutive columns of M; its supernodes (all columns
+// inside it) are packed into waves of one tile count T each, at most
+// band_capacity(T) supernodes per wave, 8 waves per block.  Block `blk`
+// sums its waves' Grams over the rows of part `part` of the row range.
+constexpr int kBandCols = 1024;
+constexpr int kBandWaves = 8;
+__host__ __device__ constexpr int band_capacity(int T) {
+  return T == 1 ? 18 : T == 2 ? 8 : T == 3 ? 4 : T == 4 ? 2 : T == 6 ? 1 : 0;
+}
+struct BandWave {
+  int32_t tiles;  // T (0: idle wave)
+  int32_t count;  // supernodes, at most band_capacity(T)
+  int32_t first;  // their ids are items[first .. first + count)
+};
+struct BandBlock {
+  int32_t c0;    // first column of the band, even (-1: empty block)
+  int32_t part;  // row part
+  BandWave w[kBandWaves];
+};
+// One launch over `count` blocks (n even: 8-byte loads of column pairs);
+// rows of part r: [r * part_rows, min(m, (r + 1) * part_rows)).
+hipError_t launch_band_gram(hipStream_t st, const float *M, int64_t m, int64_t n,
+                            const BandBlock *blocks, int32_t count, const int32_t *items,
+                            const SuperDesc *sd, const int32_t *sind, const int32_t *Pinv,
+                            const int64_t *goff, float *gram, int64_t gram_part,
+                            int64_t part_rows);
+// Mt[i][f] = M[f][cols[i]] for f < m, 0 for m <= f < ldmt, i < ncols.
+hipError_t launch_gather_transpose(hipStream_t st, const float *M, int64_t m, int64_t n,
+                                   const int32_t *cols, int32_t ncols, float *Mt, int64_t ldmt);
 
 // A supernode of a scene: one block of a factor launch.
 struct FactorItem {
@@ -124,10 +167,12 @@ struct FactorSplit {
 // factor_scratch_elems(tiles) elements of the compute type (8 bytes for
 // precision 0 and 2, 4 for 1).  split_bf16 (precision 1, dense form only):
 // the Gram's FP32 products as three BF16 products of a two-part split.
+// gram_in (precision 1, dense form): the Grams come summed from the band
+// partials (FactorScene::gram); only the Cholesky and the KL columns run.
 hipError_t launch_super_factor(hipStream_t st, int tiles, int precision, bool points,
                                const FactorScene *scenes, const FactorItem *items, int32_t count,
                                void *th_scratch, const FactorSplit &sp = FactorSplit{},
-                               bool split_bf16 = false);
+                               bool split_bf16 = false, bool gram_in = false);
 inline int64_t factor_scratch_elems(int tiles) {
   return tiles > 8 ? int64_t(16 * tiles) * (16 * tiles + 1) : 0;
 }

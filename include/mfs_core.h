@@ -46,7 +46,9 @@ enum mfs_rhs_space {
 /* Solve statistics of the last mfs_solve_* call and phase timings (HIP events
  * on the context stream, milliseconds) of the last call of each phase. */
 typedef struct mfs_stats {
-    int32_t n, n_super, max_local, pad0;
+    int32_t n, n_super, max_local;
+    int32_t band_supernodes;  /* last dense FP32 factor: supernodes whose Gram came from the
+                                 band kernel (0: none, or another form) */
     int64_t nnz_pattern, nnz_factor, theta_entries;
     double sum_ns2, sum_ns3;
     double plan_ms, factor_ms, solve_ms;
@@ -269,11 +271,15 @@ enum mfs_param {
                                      fewer supernodes than 4 x CUs spread each Gram's rows
                                      over several blocks and sum their partial Grams (0: 1,
                                      no splits) */
-    MFS_PARAM_GRAM_SPLIT = 8      /* products of the dense FP32 factor's Gram (changes its
+    MFS_PARAM_GRAM_SPLIT = 8,     /* products of the dense FP32 factor's Gram (changes its
                                      rounding, not only the summation order): 1 FP32 MFMA,
                                      2 three BF16 MFMA products of each value split into
                                      hi + lo (relative error per product about 2^-16, as
                                      FP32 sums over 65,536 rows have); 0: the default, 2 */
+    MFS_PARAM_FACTOR_BANDS = 9    /* dense FP32 factor with split products: 0 (default) the
+                                     Grams of supernodes inside a band of 1024 columns come
+                                     from row-major M in one launch, without M^T of those
+                                     columns; 1 every Gram from M^T.  From the next factor */
 };
 int mfs_ctx_set_param(mfs_ctx *ctx, int param, int64_t value);
 
