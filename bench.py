@@ -781,7 +781,7 @@ def main():
                                         round(mv_bytes / (c_ms / c_l * 1e-3) / 1e9, 1)]}
     g_l, g_ms, g_bytes = tracked[_lib.MFS_K_GRAM]
     fused_launches = tracked[_lib.MFS_K_MATVEC_FUSED][0]
-    kernel_ms = {names.get(c, "supernode factor (M^T ranges + Gram + Cholesky + L, overlapped)"): round(tracked[c][1] / args.steps, 3)
+    kernel_ms = {names.get(c, "supernode factor (Grams + Cholesky + L, side streams overlapped)"): round(tracked[c][1] / args.steps, 3)
                  for c in classes}
 
     traffic = None
@@ -940,8 +940,11 @@ def main():
         "factor_ms": round(float(np.mean(factor_ms)), 3),
         "factor_retried_supernodes": retried,
         "factor_tuning": ({"best": {"tile_set": hex(ftune["best"][0]), "column_ranges": ftune["best"][1],
-                                    "streams": ftune["best"][2], "row_splits": ftune["splits"]},
-                           "ms": {f"{hex(c[0])}/{c[1]}/{c[2]}": round(t, 3) for c, t in ftune["ms"].items()},
+                                    "streams": ftune["best"][2],
+                                    "gram_from": "bands of M" if ftune["best"][3] == 0 else "M^T",
+                                    "row_splits": ftune["splits"]},
+                           "ms": {f"{hex(c[0])}/{c[1]}/{c[2]}/{'bands' if c[3] == 0 else 'Mt'}": round(t, 3)
+                                  for c, t in ftune["ms"].items()},
                            "row_split_ms": {str(k): round(v, 3) for k, v in ftune["split_ms"].items()}}
                           if ftune else None),
         "pcg_inner_iters": iters,

@@ -10,10 +10,12 @@ hand rather than fixed:
 * the batch's PCG chunk -- iterations per round: longer chunks cost fewer
   polls, shorter ones launch fewer no-op kernels after a scene converged;
 * the supernode factor's launch shape (MFS_PARAM_TILE_SET, _FACTOR_PHASES,
-  _FACTOR_STREAMS) -- which tile counts form buckets (a supernode of ns rows
-  runs in the smallest enabled 16 T >= ns: fewer buckets mean fewer, fuller
-  launches but more zero padding), how many column ranges of the dense
-  transpose the Gram overlaps, and on how many streams the buckets run.
+  _FACTOR_STREAMS, _FACTOR_BANDS) -- which tile counts form buckets (a
+  supernode of ns rows runs in the smallest enabled 16 T >= ns: fewer buckets
+  mean fewer, fuller launches but more zero padding), whether the dense FP32
+  Grams come from row-major M by bands or from M^T, how many column ranges of
+  the dense transpose the Gram overlaps (the M^T form only), and on how many
+  streams the buckets run.
 
 Each candidate is timed with HIP events on the work it controls; the fastest
 is set and returned.  Results are kept per shape key for the process.
@@ -93,28 +95,38 @@ def tune_batch_chunk(batch: _lib.Batch, Bs, candidates=(4, 8), graphs=(0, 2), ma
 FACTOR_TILE_SETS = (0xff, 0xd5, 0x5f)
 
 
-def factor_candidates(tile_sets=FACTOR_TILE_SETS, phases=(1, 2, 4), streams=(2, 4)):
-    return [(t, p, s) for t in tile_sets for p in phases for s in streams]
+def factor_candidates(tile_sets=FACTOR_TILE_SETS, phases=(1, 2, 4), streams=(2, 4), bands=(0, 1)):
+    """(tile set, column ranges, streams, bands) tuples; bands 0 = the band
+    form (column ranges do not apply: one value), 1 = the Gram from M^T."""
+    out = []
+    for b in bands:
+        for t in tile_sets:
+            for p in (phases[:1] if b == 0 else phases):
+                for s in streams:
+                    out.append((t, p, s, b))
+    return out
 
 
 def tune_factor(ctx: _lib.Context, M, xyz, rho: float, precision=_lib.MFS_FP32, candidates=None,
                 reps: int = 2, points=None, splits=(1, 2, 4, 8)) -> dict:
     """Time the numeric factor of M (dense form; or from `points` = (x, nx, y)
-    device tensors) for each (tile set, column ranges, streams) candidate --
+    device tensors) for each (tile set, column ranges, streams, bands) candidate --
     re-planning for each, since the plan's buckets depend on the first two --
     and keep the fastest (the context is left planned with it); then time the
     row splits of small launches (MFS_PARAM_FACTOR_SPLITS) on that shape.
-    Returns {"best": (tile_set, phases, streams), "ms": {candidate: factor
+    Returns {"best": (tile_set, phases, streams, bands), "ms": {candidate: factor
     ms}, "splits": best split count, "split_ms": {splits: factor ms}}, each the
     best of `reps` timed factorisations (the context's own HIP event timing
     of the factor)."""
     key = ("factor", tuple(M.shape) if M is not None else points[0].shape[0], ctx.n, float(rho), precision)
     cands = candidates or factor_candidates()
     times = {}
-    for (tset, ph, st) in cands:
+    for cand in cands:
+        tset, ph, st, bd = cand
         ctx.set_param(_lib.MFS_PARAM_TILE_SET, tset)
         ctx.set_param(_lib.MFS_PARAM_FACTOR_PHASES, ph)
         ctx.set_param(_lib.MFS_PARAM_FACTOR_STREAMS, st)
+        ctx.set_param(_lib.MFS_PARAM_FACTOR_BANDS, bd)
         ctx.plan(xyz, rho)
         best = float("inf")
         for r in range(reps + 1):   # the first factorisation warms buffers
@@ -124,11 +136,12 @@ def tune_factor(ctx: _lib.Context, M, xyz, rho: float, precision=_lib.MFS_FP32, 
                 ctx.factor_points(*points, precision)
             if r:
                 best = min(best, ctx.stats().factor_ms)
-        times[(tset, ph, st)] = best
+        times[cand] = best
     win = min(times, key=times.get)
     ctx.set_param(_lib.MFS_PARAM_TILE_SET, win[0])
     ctx.set_param(_lib.MFS_PARAM_FACTOR_PHASES, win[1])
     ctx.set_param(_lib.MFS_PARAM_FACTOR_STREAMS, win[2])
+    ctx.set_param(_lib.MFS_PARAM_FACTOR_BANDS, win[3])
     ctx.plan(xyz, rho)
     # then the row splits of the small launches, on the winning shape
     split_ms = {}

@@ -257,8 +257,7 @@ enum mfs_param {
     MFS_PARAM_POINTS_BLOCKS = 1,  /* blocks per points pass, owners x splits (4096) */
     MFS_PARAM_COLS_BLOCKS = 2,    /* blocks of the dense M^T pass (0: one round, CUs x resident) */
     MFS_PARAM_FUSED = 3,          /* one-pass M^T M p for dense FP32 batches of 8 where
-                                     the shape qualifies (n % 4 == 0, m >= 2,
-                                     n <= 65,536): 0 default (when m n >= 2^27, else
+                                     the shape qualifies (n even, n <= 65,536): 0 default (when m n >= 2^27, else
                                      two passes), 1 off (two passes), 2 always */
     MFS_PARAM_TILE_SET = 4,       /* factor buckets: bit i enables tile count
                                      {16, 12, 8, 6, 4, 3, 2, 1}[i] (16 always on); a

@@ -13,7 +13,7 @@
   source-space right-hand sides, and the batch (which runs until its slowest
   column converges) takes no more iterations than with two passes, up to one
   (per column the counts scatter both ways with the FP32 rounding);
-* shapes the kernel does not take (n % 4 != 0, batches wider than 8 or of 4
+* shapes the kernel does not take (odd n, batches wider than 8 or of 4
   and fewer, n above 65,536) use two passes, and so does the default mode
   below 2^27 entries of M.
 """
@@ -141,7 +141,7 @@ def test_normal_apply_exact_tiny_shapes(m, n, k):
     ref = M.astype(np.int64).T @ (M.astype(np.int64) @ X.astype(np.int64))
     ctx = _lib.Context(0)
     Md, Xd = torch.from_numpy(M).to(DEV), torch.from_numpy(X).to(DEV)
-    eligible = k == 8 and n % 4 == 0 and m >= 2
+    eligible = k == 8 and n % 2 == 0
     for mode in (OFF, FORCE, DEFAULT):      # the default keeps two passes at these sizes
         Q, launches = _apply(ctx, Md, Xd, mode)
         assert (launches > 0) == (mode == FORCE and eligible), (mode, launches)
@@ -279,7 +279,7 @@ def test_fused_source_space_right_hand_sides():
     assert all(s.relres[c] <= 1e-6 for c in range(8))
 
 
-def test_fused_not_used_when_columns_are_not_float4_aligned():
+def test_fused_not_used_for_an_odd_column_count():
     sc = scenes.coins(seed=3, n_bodies=3, pts_per_body=161, k=8, box=6.0)  # n = 483
     ctx, Md, B = _setup(sc)
     _compare(ctx, Md, B, want_fused=False)
