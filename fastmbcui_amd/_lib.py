@@ -115,6 +115,7 @@ def lib():
             "mfs_batch_stats_get": (ci, [vp, ctypes.POINTER(BatchStats)]),
             "mfs_batch_plan": (ci, [vp, vp, vp, f64, f64, i32, i32]),
             "mfs_batch_factor_points": (ci, [vp, vp, vp, vp, vp, ci]),
+            "mfs_batch_set_factor_mode": (ci, [vp, i32]),
             "mfs_batch_solve_points": (ci, [vp, vp, vp, vp, i32, f64, f64, i32, ci]),
             "mfs_ctx_set_param": (ci, [vp, ci, i64]),
             "mfs_mlp_create": (ci, [ctypes.POINTER(vp), ci, vp, ctypes.c_uint64]),
@@ -640,6 +641,12 @@ class Batch:
                                          int(max_super), int(threads)), "mfs_batch_plan")
         self.n = [int(v) for v in n]
 
+    def set_factor_mode(self, mode):
+        """2: every scene's supernodes in one launch over a device queue;
+        1: one launch per tile size and row split count; 0 (default): the
+        queue launch for up to 32,768 units, else a launch per bucket."""
+        self._check(lib().mfs_batch_set_factor_mode(self._h, int(mode)), "mfs_batch_set_factor_mode")
+
     def factor_points(self, xs, nxs, ys, precision=MFS_FP32):
         """Device FP64 (m_i, 3) targets/normals and (n_i, 3) planned sources per scene."""
         import torch
@@ -686,3 +693,12 @@ class Batch:
         if code != MFS_OK:
             raise MfsError(code, "mfs_ctx_stats")
         return s
+
+    def scene_factor_export(self, i):
+        """The factor values of scene i (FP64 host copy, mfs_factor_export)."""
+        s = self.scene_stats(i)
+        val = np.empty(max(s.nnz_factor, 1), np.float64)
+        code = lib().mfs_factor_export(lib().mfs_batch_scene(self._h, int(i)), _ptr(val))
+        if code != MFS_OK:
+            raise MfsError(code, "mfs_factor_export")
+        return val[: s.nnz_factor]

@@ -400,16 +400,39 @@ __global__ __launch_bounds__(kBandThreads, 1) void k_band_gram(
 // its T source points in registers: no LDS staging and no barriers.
 // GIN: the Gram comes summed from the band partials (FS.gram); only the
 // Cholesky and the KL columns run.
-template <int T, typename TC, typename TV, bool PTS, bool GB = false, bool GIN = false>
-__global__ __launch_bounds__(kThreads) void k_super_factor(const FactorScene *__restrict__ scenes,
-                                                           const FactorItem *__restrict__ items,
-                                                           TC *__restrict__ th_scratch,
-                                                           FactorSplit sp) {
-  // This block's supernode and its scene (one launch may mix scenes); with
-  // row splits, blocks item * S .. item * S + S - 1 share one supernode.
+//
+// The LDS of one supernode factor, carved from one buffer (a launch per
+// bucket declares it per tile count; the queue launch sizes it for its
+// largest tile count): TH (NS x RS, none for the large tiles: global scratch),
+// the local-to-original column map, the pivots and their inverses, the
+// member right-hand sides W, the member batch's bookkeeping, the split flag,
+// the FP32 diagonals.
+template <int T, typename TC>
+struct FactorLds {
+  static constexpr int NS = 16 * T, RS = NS + 1, MB = 16;
+  static constexpr size_t al(size_t x) { return (x + 15) / 16 * 16; }
+  static constexpr size_t o_src = al(T > 8 ? 16 : size_t(NS) * RS * sizeof(TC));
+  static constexpr size_t o_dg = al(o_src + size_t(NS) * 4);
+  static constexpr size_t o_dinv = al(o_dg + size_t(NS) * sizeof(TC));
+  static constexpr size_t o_w = al(o_dinv + size_t(NS) * sizeof(TC));
+  static constexpr size_t o_mcol = al(o_w + size_t(NS) * MB * sizeof(TC));
+  static constexpr size_t o_mc = o_mcol + MB * 4, o_moff = o_mc + MB * 4, o_last = o_moff + MB * 4;
+  static constexpr size_t o_d0 = al(o_last + 4);
+  static constexpr size_t bytes = al(o_d0 + (sizeof(TC) == 4 ? size_t(NS) : 1) * sizeof(TC));
+};
+
+// One unit of a factor launch: a supernode, or one row split of it.  `unit`
+// numbers the units of the launch's items (item * S + split); it also picks
+// the unit's regions of th_scratch and of the split partials.
+template <int T, typename TC, typename TV, bool PTS, bool GB, bool GIN>
+__device__ inline void factor_unit(const FactorScene *__restrict__ scenes,
+                                   const FactorItem *__restrict__ items, TC *__restrict__ th_scratch,
+                                   FactorSplit sp, int32_t unit, unsigned char *lds) {
+  // This unit's supernode and its scene (one launch may mix scenes); with
+  // row splits, units item * S .. item * S + S - 1 share one supernode.
   const int32_t S = sp.splits;
-  const int32_t bi = S > 1 ? int32_t(blockIdx.x) / S : int32_t(blockIdx.x);
-  const int32_t split = S > 1 ? int32_t(blockIdx.x) % S : 0;
+  const int32_t bi = S > 1 ? unit / S : unit;
+  const int32_t split = S > 1 ? unit % S : 0;
   const FactorItem item = items[bi];
   const FactorScene &FS = scenes[item.scene];
   const float *__restrict__ Mt = FS.Mt;
@@ -445,12 +468,17 @@ __global__ __launch_bounds__(kThreads) void k_super_factor(const FactorScene *__
   constexpr int PPW = (PAIRS + NG * NPASS - 1) / (NG * NPASS);  // tile pairs per wave and pass
   typedef typename Mfma<TC>::v4 acc_t;
 
-  __shared__ __align__(16) unsigned char big_raw[LARGE ? 16 : NS * RS * sizeof(TC)];
-  __shared__ int32_t src[NS];
-  __shared__ TC dg[NS], dinv[NS];
-  __shared__ TC W[NS * MB];
-  __shared__ int32_t mcol[MB], mc[MB], moff[MB];
-  __shared__ int last_split;
+  typedef FactorLds<T, TC> LY;
+  static_assert(LY::MB == MB, "member batch");
+  unsigned char *big_raw = lds;
+  int32_t *src = reinterpret_cast<int32_t *>(lds + LY::o_src);
+  TC *dg = reinterpret_cast<TC *>(lds + LY::o_dg);
+  TC *dinv = reinterpret_cast<TC *>(lds + LY::o_dinv);
+  TC *W = reinterpret_cast<TC *>(lds + LY::o_w);
+  int32_t *mcol = reinterpret_cast<int32_t *>(lds + LY::o_mcol);
+  int32_t *mc = reinterpret_cast<int32_t *>(lds + LY::o_mc);
+  int32_t *moff = reinterpret_cast<int32_t *>(lds + LY::o_moff);
+  int &last_split = *reinterpret_cast<int *>(lds + LY::o_last);
 
   // wave index in a scalar register: loop bounds derived from it stay
   // uniform branches (and exact wait counts) instead of exec-masked regions
@@ -478,7 +506,7 @@ __global__ __launch_bounds__(kThreads) void k_super_factor(const FactorScene *__
   const int64_t r0 = int64_t(split) * chunk;
   const int64_t r1 = (S > 1 && r0 + chunk < rows_total) ? r0 + chunk : rows_total;
   const int grp = wave % NG, kph = wave / NG;
-  TC *TH = LARGE ? th_scratch + size_t(blockIdx.x) * NS * RS : reinterpret_cast<TC *>(big_raw);
+  TC *TH = LARGE ? th_scratch + size_t(unit) * NS * RS : reinterpret_cast<TC *>(big_raw);
   // Pass P accumulates the pairs grp + NG (P PPW + u), u < PPW, over all rows.
   auto gram_pass = [&](auto pass_c) {
   constexpr int U0 = decltype(pass_c)::value * PPW;
@@ -802,7 +830,7 @@ __global__ __launch_bounds__(kThreads) void k_super_factor(const FactorScene *__
     if constexpr (!LARGE) {
       for (int idx = tid; idx < ns * ns; idx += kThreads) {
         const int r = idx / ns, c = idx % ns;
-        part[size_t(blockIdx.x) * NS * NS + r * NS + c] = TH[r * RS + c];
+        part[size_t(unit) * NS * NS + r * NS + c] = TH[r * RS + c];
       }
     }
     __threadfence();
@@ -832,7 +860,7 @@ __global__ __launch_bounds__(kThreads) void k_super_factor(const FactorScene *__
   // diagonal (8 sqrt(m) eps, kept within [2^-20, 2^-10]) has no correct digits
   // left and is marked like a non-positive one, for the FP64 retry.  At rho = 3
   // on surface scenes every pivot keeps over a fifth of its diagonal.
-  __shared__ TC d0[sizeof(TC) == 4 ? NS : 1];
+  TC *d0 = reinterpret_cast<TC *>(lds + LY::o_d0);
   TC floor_rel = TC(0);
   if constexpr (sizeof(TC) == 4) {
     for (int p = tid; p < ns; p += kThreads) d0[p] = TH[p * RS + p];
@@ -909,6 +937,56 @@ __global__ __launch_bounds__(kThreads) void k_super_factor(const FactorScene *__
     }
   }
   if (tid == 0) status[s] = 0;
+}
+
+template <int T, typename TC, typename TV, bool PTS, bool GB = false, bool GIN = false>
+__global__ __launch_bounds__(kThreads) void k_super_factor(const FactorScene *__restrict__ scenes,
+                                                           const FactorItem *__restrict__ items,
+                                                           TC *__restrict__ th_scratch,
+                                                           FactorSplit sp) {
+  __shared__ __align__(16) unsigned char lds[FactorLds<T, TC>::bytes];
+  factor_unit<T, TC, TV, PTS, GB, GIN>(scenes, items, th_scratch, sp, int32_t(blockIdx.x), lds);
+}
+
+// The points form's buckets of a batch in one launch: resident blocks take
+// the units of all buckets in turn from a device counter (bucket by bucket,
+// largest tiles first, as the host lists them), so the large supernodes start
+// first and the small ones fill in behind them.  The LDS is sized for the
+// largest tile count present.  Each unit runs exactly the per-bucket launch's
+// code on the same regions of the scratch and split partials, so the factor
+// is the same bit for bit.
+template <typename TC, typename TV>
+__global__ __launch_bounds__(kThreads) void k_factor_queue(const FactorScene *__restrict__ scenes,
+                                                           const FactorItem *__restrict__ items,
+                                                           const FactorQueueBucket *__restrict__ qb,
+                                                           int32_t nb, int32_t total,
+                                                           int32_t *__restrict__ next) {
+  extern __shared__ __align__(16) unsigned char dyn[];
+  __shared__ int32_t unit_s;
+  for (;;) {
+    if (threadIdx.x == 0) unit_s = atomicAdd(next, 1);
+    __syncthreads();
+    const int32_t u = unit_s;
+    __syncthreads();  // every thread has read the unit before it is overwritten
+    if (u >= total) break;
+    int32_t b = 0;
+    while (b + 1 < nb && u >= qb[b + 1].first_unit) ++b;
+    const FactorQueueBucket Q = qb[b];
+    const int32_t local = u - Q.first_unit;
+    const FactorItem *it = items + Q.offset;
+    TC *ths = static_cast<TC *>(Q.th_scratch);
+    switch (Q.tiles) {
+      case 1: factor_unit<1, TC, TV, true, false, false>(scenes, it, ths, Q.sp, local, dyn); break;
+      case 2: factor_unit<2, TC, TV, true, false, false>(scenes, it, ths, Q.sp, local, dyn); break;
+      case 3: factor_unit<3, TC, TV, true, false, false>(scenes, it, ths, Q.sp, local, dyn); break;
+      case 4: factor_unit<4, TC, TV, true, false, false>(scenes, it, ths, Q.sp, local, dyn); break;
+      case 6: factor_unit<6, TC, TV, true, false, false>(scenes, it, ths, Q.sp, local, dyn); break;
+      case 8: factor_unit<8, TC, TV, true, false, false>(scenes, it, ths, Q.sp, local, dyn); break;
+      case 12: factor_unit<12, TC, TV, true, false, false>(scenes, it, ths, Q.sp, local, dyn); break;
+      default: factor_unit<16, TC, TV, true, false, false>(scenes, it, ths, Q.sp, local, dyn); break;
+    }
+    __syncthreads();  // the unit's LDS is free for the next one
+  }
 }
 
 template <typename TV>
@@ -998,6 +1076,48 @@ hipError_t launch_super_factor(hipStream_t st, int tiles, int precision, bool po
   if (precision == 0) return launch_prec<double, double>(st, tiles, L);
   if (precision == 1) return launch_prec<float, float>(st, tiles, L);
   return launch_prec<double, float>(st, tiles, L);
+}
+
+size_t factor_unit_lds(int tiles, int precision) {
+  auto pick = [&](auto tc) -> size_t {
+    typedef decltype(tc) TC;
+    switch (tiles) {
+      case 1: return FactorLds<1, TC>::bytes;
+      case 2: return FactorLds<2, TC>::bytes;
+      case 3: return FactorLds<3, TC>::bytes;
+      case 4: return FactorLds<4, TC>::bytes;
+      case 6: return FactorLds<6, TC>::bytes;
+      case 8: return FactorLds<8, TC>::bytes;
+      case 12: return FactorLds<12, TC>::bytes;
+      case 16: return FactorLds<16, TC>::bytes;
+      default: return 0;
+    }
+  };
+  return precision == 1 ? pick(float{}) : pick(double{});
+}
+
+int factor_queue_blocks(int precision, size_t lds_bytes, int cus) {
+  int per_cu = 0;
+  const hipError_t e =
+      precision == 1
+          ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_factor_queue<float, float>, kThreads, lds_bytes)
+          : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_factor_queue<double, double>, kThreads,
+                                                         lds_bytes);
+  if (e != hipSuccess || per_cu < 1) return 0;
+  return per_cu * cus;
+}
+
+hipError_t launch_factor_queue(hipStream_t st, int precision, const FactorScene *scenes,
+                               const FactorItem *items, const FactorQueueBucket *qb, int32_t nb,
+                               int32_t total, int32_t *next, size_t lds_bytes, int blocks) {
+  if (total <= 0) return hipSuccess;
+  if (blocks < 1 || (precision != 0 && precision != 1)) return hipErrorInvalidValue;
+  if (precision == 1)
+    k_factor_queue<float, float><<<unsigned(blocks), kThreads, lds_bytes, st>>>(scenes, items, qb, nb, total, next);
+  else
+    k_factor_queue<double, double><<<unsigned(blocks), kThreads, lds_bytes, st>>>(scenes, items, qb, nb, total,
+                                                                                  next);
+  return hipGetLastError();
 }
 
 hipError_t launch_band_gram(hipStream_t st, const float *M, int64_t m, int64_t n,

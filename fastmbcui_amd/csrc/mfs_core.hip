@@ -2329,7 +2329,10 @@ struct mfs_batch {
   hipEvent_t root_ev = nullptr;
   hipStream_t side[mfs_ctx::kSide] = {};
   hipEvent_t side_ev[mfs_ctx::kSide] = {};
-  DevBuf dtable, ditems, dthg, dctr, dstates, dactive;
+  DevBuf dtable, ditems, dthg, dctr, dstates, dactive, dqb, dqnext;
+  // 0: automatic (the queue launch for up to kQueueUnits units); 1: a launch
+  // per bucket; 2: one launch over a device queue of all buckets' units
+  int32_t factor_mode = 0;
   std::vector<FactorScene> table;
   std::vector<FactorItem> items;
   std::vector<PcgState *> states;
@@ -2473,6 +2476,14 @@ int mfs_batch_plan(mfs_batch *b, const double *const *xyz, const int64_t *n, dou
   return MFS_OK;
 }
 
+int mfs_batch_set_factor_mode(mfs_batch *b, int32_t mode) {
+  if (b == nullptr) return MFS_ERR_ARG;
+  if (mode < 0 || mode > 2)
+    return b->fail(MFS_ERR_ARG, "factor mode is 0 (automatic), 1 (a launch per bucket) or 2 (one launch)");
+  b->factor_mode = mode;
+  return MFS_OK;
+}
+
 int mfs_batch_factor_points(mfs_batch *b, const double *const *x, const double *const *nx,
                             const int64_t *m, const double *const *y, int precision) {
   if (b == nullptr || x == nullptr || nx == nullptr || m == nullptr || y == nullptr)
@@ -2557,23 +2568,59 @@ int mfs_batch_factor_points(mfs_batch *b, const double *const *x, const double *
     BHIPCHK(hipMemcpyAsync(b->ditems.p, b->items.data(), b->items.size() * sizeof(FactorItem),
                            hipMemcpyHostToDevice, b->root),
             "upload");
-  BHIPCHK(hipEventRecord(b->root_ev, b->root), "fork");
-  const int nside = std::min<int>(mfs_ctx::kSide, int(buckets.size()));
-  for (int i = 0; i < nside; ++i) BHIPCHK(hipStreamWaitEvent(b->side[i], b->root_ev, 0), "fork");
-  for (size_t bi = 0; bi < buckets.size(); ++bi) {
-    BLaunch &L = buckets[bi];
+  for (BLaunch &L : buckets)
     if (L.sp.splits > 1) {
       L.sp.counters = b->dctr.as<int32_t>() + L.ctr;
       if (L.tiles <= 8) L.sp.partial = b->dthg.as<char>() + L.off;
     }
-    BHIPCHK(launch_super_factor(b->side[bi % nside], L.tiles, precision == MFS_FP64 ? 0 : 1, true,
-                                b->dtable.as<FactorScene>(), b->ditems.as<FactorItem>() + L.offset,
-                                L.count, b->dthg.as<char>() + L.off, L.sp),
-            "supernode factor");
+  const int prec_code = precision == MFS_FP64 ? 0 : 1;
+  // One launch: every bucket's units (largest tiles first) from a device
+  // queue, as many blocks as stay resident with the LDS of the largest tile
+  // count present.  Its kernel carries every tile count's code, so it has the
+  // registers of the widest (248 against 35-118 for the small tiles' own
+  // kernels) and fewer blocks per CU: measured on the uneven batch's scenes,
+  // 14.5 ms against 15.2 ms for 24 scenes (17,416 supernodes) but 1,345 ms
+  // against 620 ms for 256 scenes (697,733).  Automatic mode takes it up to
+  // kQueueUnits units; a launch per bucket otherwise, or when no block fits.
+  constexpr int64_t kQueueUnits = 32768;
+  int queue_blocks = 0;
+  size_t queue_lds = 0;
+  std::vector<FactorQueueBucket> qb;
+  int64_t units = 0;
+  for (const BLaunch &L : buckets) {
+    qb.push_back({L.tiles, L.offset, L.count, int32_t(units), b->dthg.as<char>() + L.off, L.sp});
+    units += int64_t(L.count) * L.sp.splits;
+    queue_lds = std::max(queue_lds, factor_unit_lds(L.tiles, prec_code));
   }
-  for (int i = 0; i < nside; ++i) {
-    BHIPCHK(hipEventRecord(b->side_ev[i], b->side[i]), "join");
-    BHIPCHK(hipStreamWaitEvent(b->root, b->side_ev[i], 0), "join");
+  const bool use_queue = !buckets.empty() && units < INT32_MAX &&
+                         (b->factor_mode == 2 || (b->factor_mode == 0 && units <= kQueueUnits));
+  if (use_queue) queue_blocks = factor_queue_blocks(prec_code, queue_lds, b->scenes[0]->cus);
+  if (queue_blocks > 0) {
+    BHIPCHK(b->dqb.ensure(qb.size() * sizeof(FactorQueueBucket)), "alloc");
+    BHIPCHK(b->dqnext.ensure(4), "alloc");
+    BHIPCHK(hipMemcpyAsync(b->dqb.p, qb.data(), qb.size() * sizeof(FactorQueueBucket),
+                           hipMemcpyHostToDevice, b->root),
+            "upload");
+    BHIPCHK(hipMemsetAsync(b->dqnext.p, 0, 4, b->root), "clear queue");
+    BHIPCHK(launch_factor_queue(b->root, prec_code, b->dtable.as<FactorScene>(), b->ditems.as<FactorItem>(),
+                                b->dqb.as<FactorQueueBucket>(), int32_t(qb.size()), int32_t(units),
+                                b->dqnext.as<int32_t>(), queue_lds, queue_blocks),
+            "supernode factor queue");
+  } else {
+    BHIPCHK(hipEventRecord(b->root_ev, b->root), "fork");
+    const int nside = std::min<int>(mfs_ctx::kSide, int(buckets.size()));
+    for (int i = 0; i < nside; ++i) BHIPCHK(hipStreamWaitEvent(b->side[i], b->root_ev, 0), "fork");
+    for (size_t bi = 0; bi < buckets.size(); ++bi) {
+      const BLaunch &L = buckets[bi];
+      BHIPCHK(launch_super_factor(b->side[bi % nside], L.tiles, prec_code, true, b->dtable.as<FactorScene>(),
+                                  b->ditems.as<FactorItem>() + L.offset, L.count, b->dthg.as<char>() + L.off,
+                                  L.sp),
+              "supernode factor");
+    }
+    for (int i = 0; i < nside; ++i) {
+      BHIPCHK(hipEventRecord(b->side_ev[i], b->side[i]), "join");
+      BHIPCHK(hipStreamWaitEvent(b->root, b->side_ev[i], 0), "join");
+    }
   }
   if (fork_scenes(b) != MFS_OK) return MFS_ERR_HIP;
   // 3. per scene: statuses, FP64 retries, transposed values
@@ -2582,7 +2629,7 @@ int mfs_batch_factor_points(mfs_batch *b, const double *const *x, const double *
     if (rc != MFS_OK) return scene_error(b, i, rc);
   }
   b->stats.factor_ms = ms_since(t0);
-  b->stats.factor_launches = int32_t(buckets.size());
+  b->stats.factor_launches = queue_blocks > 0 ? 1 : int32_t(buckets.size());
   b->stats.factor_items = int32_t(b->items.size());
   return MFS_OK;
 }

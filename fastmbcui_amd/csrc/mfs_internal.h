@@ -173,6 +173,23 @@ hipError_t launch_super_factor(hipStream_t st, int tiles, int precision, bool po
                                const FactorScene *scenes, const FactorItem *items, int32_t count,
                                void *th_scratch, const FactorSplit &sp = FactorSplit{},
                                bool split_bf16 = false, bool gram_in = false);
+// The points form's buckets of a batch in one launch (k_factor_queue): bucket
+// b's units (count * sp.splits) are units first_unit .. of the queue, its
+// items items[offset ..], its scratch and split partials as in its own launch.
+struct FactorQueueBucket {
+  int32_t tiles, offset, count, first_unit;
+  void *th_scratch;
+  FactorSplit sp;
+};
+// LDS bytes of one unit of `tiles` tiles (precision 0: FP64, 1: FP32).
+size_t factor_unit_lds(int tiles, int precision);
+// Resident blocks of the queue launch with `lds_bytes` of LDS (0: none fit).
+int factor_queue_blocks(int precision, size_t lds_bytes, int cus);
+// `next`: the unit counter, zero before the launch.  `blocks`: at most
+// factor_queue_blocks (all resident; none waits for another).
+hipError_t launch_factor_queue(hipStream_t st, int precision, const FactorScene *scenes,
+                               const FactorItem *items, const FactorQueueBucket *qb, int32_t nb,
+                               int32_t total, int32_t *next, size_t lds_bytes, int blocks);
 inline int64_t factor_scratch_elems(int tiles) {
   return tiles > 8 ? int64_t(16 * tiles) * (16 * tiles + 1) : 0;
 }

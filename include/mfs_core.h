@@ -328,7 +328,15 @@ int mfs_batch_stats_get(const mfs_batch *b, mfs_batch_stats *out);
 int mfs_batch_plan(mfs_batch *b, const double *const *xyz, const int64_t *n, double rho,
                    double lambda, int32_t max_super, int32_t threads);
 /* mfs_factor_points for every scene (device FP64 x[i], nx[i]: m[i] x 3, y[i]:
- * the planned sources), all scenes' supernodes in one launch per tile size. */
+ * the planned sources).  mfs_batch_set_factor_mode: 2 -- all scenes'
+ * supernodes in one launch whose resident blocks take them from a device
+ * queue, largest tile counts first; 1 -- one launch per tile size (and row
+ * split count) on four streams; 0 (default) -- the queue launch for up to
+ * 32,768 units (supernodes times row splits), else a launch per bucket.  All
+ * give the same factor bit for bit.  Replaces the per-scene factor of
+ * mfs_torch_ops.cu:128-131 (one scene per call there); the queue launch has
+ * no counterpart in the reference. */
+int mfs_batch_set_factor_mode(mfs_batch *b, int32_t mode);
 int mfs_batch_factor_points(mfs_batch *b, const double *const *x, const double *const *nx,
                             const int64_t *m, const double *const *y, int precision);
 /* mfs_solve_points for every scene (B[i]: m[i] x k[i], X[i]: n[i] x k[i], FP32

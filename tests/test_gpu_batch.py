@@ -142,3 +142,37 @@ def test_batch_with_row_splits_equals_single_scene_solves():
         ctx.factor_points(_dev(sc.x), _dev(sc.nx), _dev(sc.y), _lib.MFS_FP32)
         X = ctx.solve_points(_dev(sc.b.astype(np.float32)), tol=1e-6)
         assert torch.equal(X, Xs[i]), f"scene {i}"
+
+
+@pytest.mark.parametrize("precision", [_lib.MFS_FP32, _lib.MFS_FP64])
+def test_batch_factor_in_one_launch_equals_launch_per_bucket(precision):
+    """Every scene's supernodes in one launch over a device queue (mode 2)
+    against one launch per tile size and split count (mode 1): the same units
+    run the same code on the same scratch, so the factors are equal bit for
+    bit; 24 scenes drawn from the uneven batch's ranges (2-32 bodies, 256-2048
+    points per body, capped at 16 bodies x 512 points here) with row splits
+    on.  The automatic mode (0) gives the same factor again."""
+    scs = [scenes.batch_scene(s, bodies=(2, 16), points=(256, 512)) for s in range(24)]
+    vals = {}
+    for mode in (2, 1, 0):
+        b = _lib.Batch(len(scs))
+        b.set_factor_mode(mode)
+        b.set_param(_lib.MFS_PARAM_FACTOR_SPLITS, 4)
+        b.plan([sc.y for sc in scs], 3.0)
+        b.factor_points([_dev(sc.x) for sc in scs], [_dev(sc.nx) for sc in scs], [_dev(sc.y) for sc in scs],
+                        precision)
+        st = b.stats()
+        print(f"factor mode {mode}: launches {st.factor_launches}, items {st.factor_items}, "
+              f"{st.factor_ms:.1f} ms", flush=True)
+        if mode != 0:
+            assert (st.factor_launches == 1) == (mode == 2)
+        vals[mode] = [b.scene_factor_export(i) for i in range(len(scs))]
+    for i in range(len(scs)):
+        assert np.array_equal(vals[2][i], vals[1][i]), f"scene {i}"
+        assert np.array_equal(vals[0][i], vals[1][i]), f"scene {i}"
+
+
+def test_batch_factor_mode_is_checked():
+    b = _lib.Batch(1)
+    with pytest.raises(_lib.MfsError):
+        b.set_factor_mode(3)
