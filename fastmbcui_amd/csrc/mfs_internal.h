@@ -114,12 +114,13 @@ struct FactorScene {
 // ---- band Gram (dense FP32 factor, BF16 split products) -------------------
 // A band is kBandCols consecutive columns of M; its supernodes (all columns
 // inside it) are packed into waves of one tile count T each, at most
-// band_capacity(T) supernodes per wave, 8 waves per block.  Block `blk`
+// band_capacity(T) supernodes per wave (what the registers hold with no spill
+// next to the staging: 251 of 256), 8 waves per block.  Block `blk`
 // sums its waves' Grams over the rows of part `part` of the row range.
 constexpr int kBandCols = 1024;
 constexpr int kBandWaves = 8;
 __host__ __device__ constexpr int band_capacity(int T) {
-  return T == 1 ? 18 : T == 2 ? 8 : T == 3 ? 4 : T == 4 ? 2 : T == 6 ? 1 : 0;
+  return T == 1 ? 18 : T == 2 ? 8 : T == 3 ? 5 : T == 4 ? 3 : T == 6 ? 1 : 0;
 }
 struct BandWave {
   int32_t tiles;  // T (0: idle wave)
