@@ -396,7 +396,8 @@ __global__ __launch_bounds__(kBandThreads, 1) void k_band_gram(
 // PTS: the points form -- the Gram's operands are evaluated from the packed
 // point records (tgt: m x 12, srcp: n x 8) instead of read from M^T.  Each
 // lane evaluates exactly the MFMA operand entries it supplies, A[f][p] with
-// p = 16 a + (lane & 15) for every tile a and f = f0 + (lane >> 4), keeping
+// p = 16 a + (lane & 15) for every tile a and f = f0 + (lane >> 4) (FP32
+// products; f = f0 + 8 (lane >> 4) + i, i < 8, with split products), keeping
 // its T source points in registers: no LDS staging and no barriers.
 // GIN: the Gram comes summed from the band partials (FS.gram); only the
 // Cholesky and the KL columns run.
@@ -423,9 +424,12 @@ struct FactorLds {
 
 // One unit of a factor launch: a supernode, or one row split of it.  `unit`
 // numbers the units of the launch's items (item * S + split); it also picks
-// the unit's regions of th_scratch and of the split partials.
+// the unit's regions of th_scratch and of the split partials.  Forced inline:
+// with two callers (a bucket's kernel and the batch queue) the compiler would
+// keep it out of line, a call per unit with its LDS behind flat pointers and
+// its registers spilled.
 template <int T, typename TC, typename TV, bool PTS, bool GB, bool GIN>
-__device__ inline void factor_unit(const FactorScene *__restrict__ scenes,
+__device__ __forceinline__ void factor_unit(const FactorScene *__restrict__ scenes,
                                    const FactorItem *__restrict__ items, TC *__restrict__ th_scratch,
                                    FactorSplit sp, int32_t unit, unsigned char *lds) {
   // This unit's supernode and its scene (one launch may mix scenes); with
@@ -587,25 +591,64 @@ __device__ inline void factor_unit(const FactorScene *__restrict__ scenes,
         t[2] = make_float4(0.f, 0.f, 0.f, 0.f);
       }
     };
-    constexpr int FSTEP = 4 * KS;  // rows per k-step over the waves sharing a pair group
     const float scale = float(-kInv4Pi);
-    float4 cur[3], nxt[3];
-    int64_t f = r0 + kph * 4 + (lane >> 4);
-    load_target(f, cur);
-    for (int64_t f0 = r0 + kph * 4; f0 < r1; f0 += FSTEP, f += FSTEP) {
-      load_target(f + FSTEP, nxt);
-      const float sn0 = scale * cur[1].z, sn1 = scale * cur[1].w, sn2 = scale * cur[2].x;
-      TC fr[T];
+    auto eval = [&](const float4 (&t)[3], int a) {
+      return dlp32((t[0].x - yh[a][0]) + (t[0].w - yl[a][0]), (t[0].y - yh[a][1]) + (t[1].x - yl[a][1]),
+                   (t[0].z - yh[a][2]) + (t[1].y - yl[a][2]), scale * t[1].z, scale * t[1].w,
+                   scale * t[2].x);
+    };
+    if constexpr (GB) {
+      // Split products (FP32 values, FP32 sums, as for the Gram from M^T):
+      // lane l supplies the k-slots 8 (l >> 4) .. + 7 of both operands, rows
+      // f0 + 8 (l >> 4) + i, i < 8, of each 32-row step.  It evaluates them a
+      // row at a time, the next row's record in flight (the next step's first
+      // row after the eighth), and splits each tile's eight values once they
+      // are complete: the same evaluations as the FP32 products, 3/16 of their
+      // MFMA time.  FSTEP <= 128 divides the row splits' chunks.
+      constexpr int FSTEP = 32 * KS;
+      const int64_t fs = r0 + kph * 32 + 8 * (lane >> 4);  // this lane's first row
+      float4 cur[3], nxt[3];
+      load_target(fs, cur);
+      for (int64_t f0 = r0 + kph * 32, fb = fs; f0 < r1; f0 += FSTEP, fb += FSTEP) {
+        float v[T][8];
 #pragma unroll
-      for (int a = 0; a < T; ++a) {
-        const float v = dlp32((cur[0].x - yh[a][0]) + (cur[0].w - yl[a][0]),
-                              (cur[0].y - yh[a][1]) + (cur[1].x - yl[a][1]),
-                              (cur[0].z - yh[a][2]) + (cur[1].y - yl[a][2]), sn0, sn1, sn2);
-        fr[a] = ok[a] ? TC(v) : TC(0);
+        for (int i = 0; i < 8; ++i) {
+          load_target(i < 7 ? fb + i + 1 : fb + FSTEP, nxt);
+#pragma unroll
+          for (int a = 0; a < T; ++a) {
+            const float e = eval(cur, a);
+            v[a][i] = ok[a] ? e : 0.f;
+          }
+#pragma unroll
+          for (int d = 0; d < 3; ++d) cur[d] = nxt[d];
+          // a row at a time: the scheduler would otherwise interleave the
+          // eight rows' loads and evaluations, past the register file
+          __builtin_amdgcn_sched_barrier(0);
+        }
+        bf8_t hi[T], lo[T];
+#pragma unroll
+        for (int a = 0; a < T; ++a)
+          split_bf16(f4_t{v[a][0], v[a][1], v[a][2], v[a][3]}, f4_t{v[a][4], v[a][5], v[a][6], v[a][7]},
+                     hi[a], lo[a]);
+        step3(hi, lo);
       }
-      step(fr);
+    } else {
+      constexpr int FSTEP = 4 * KS;  // rows per k-step over the waves sharing a pair group
+      float4 cur[3], nxt[3];
+      int64_t f = r0 + kph * 4 + (lane >> 4);
+      load_target(f, cur);
+      for (int64_t f0 = r0 + kph * 4; f0 < r1; f0 += FSTEP, f += FSTEP) {
+        load_target(f + FSTEP, nxt);
+        TC fr[T];
 #pragma unroll
-      for (int d = 0; d < 3; ++d) cur[d] = nxt[d];
+        for (int a = 0; a < T; ++a) {
+          const float v = eval(cur, a);
+          fr[a] = ok[a] ? TC(v) : TC(0);
+        }
+        step(fr);
+#pragma unroll
+        for (int d = 0; d < 3; ++d) cur[d] = nxt[d];
+      }
     }
   } else if constexpr (GB) {
     // Split products (FP32 values, FP32 sums): lane l loads, for every tile a
@@ -955,7 +998,7 @@ __global__ __launch_bounds__(kThreads) void k_super_factor(const FactorScene *__
 // largest tile count present.  Each unit runs exactly the per-bucket launch's
 // code on the same regions of the scratch and split partials, so the factor
 // is the same bit for bit.
-template <typename TC, typename TV>
+template <typename TC, typename TV, bool GB>
 __global__ __launch_bounds__(kThreads) void k_factor_queue(const FactorScene *__restrict__ scenes,
                                                            const FactorItem *__restrict__ items,
                                                            const FactorQueueBucket *__restrict__ qb,
@@ -976,14 +1019,14 @@ __global__ __launch_bounds__(kThreads) void k_factor_queue(const FactorScene *__
     const FactorItem *it = items + Q.offset;
     TC *ths = static_cast<TC *>(Q.th_scratch);
     switch (Q.tiles) {
-      case 1: factor_unit<1, TC, TV, true, false, false>(scenes, it, ths, Q.sp, local, dyn); break;
-      case 2: factor_unit<2, TC, TV, true, false, false>(scenes, it, ths, Q.sp, local, dyn); break;
-      case 3: factor_unit<3, TC, TV, true, false, false>(scenes, it, ths, Q.sp, local, dyn); break;
-      case 4: factor_unit<4, TC, TV, true, false, false>(scenes, it, ths, Q.sp, local, dyn); break;
-      case 6: factor_unit<6, TC, TV, true, false, false>(scenes, it, ths, Q.sp, local, dyn); break;
-      case 8: factor_unit<8, TC, TV, true, false, false>(scenes, it, ths, Q.sp, local, dyn); break;
-      case 12: factor_unit<12, TC, TV, true, false, false>(scenes, it, ths, Q.sp, local, dyn); break;
-      default: factor_unit<16, TC, TV, true, false, false>(scenes, it, ths, Q.sp, local, dyn); break;
+      case 1: factor_unit<1, TC, TV, true, GB, false>(scenes, it, ths, Q.sp, local, dyn); break;
+      case 2: factor_unit<2, TC, TV, true, GB, false>(scenes, it, ths, Q.sp, local, dyn); break;
+      case 3: factor_unit<3, TC, TV, true, GB, false>(scenes, it, ths, Q.sp, local, dyn); break;
+      case 4: factor_unit<4, TC, TV, true, GB, false>(scenes, it, ths, Q.sp, local, dyn); break;
+      case 6: factor_unit<6, TC, TV, true, GB, false>(scenes, it, ths, Q.sp, local, dyn); break;
+      case 8: factor_unit<8, TC, TV, true, GB, false>(scenes, it, ths, Q.sp, local, dyn); break;
+      case 12: factor_unit<12, TC, TV, true, GB, false>(scenes, it, ths, Q.sp, local, dyn); break;
+      default: factor_unit<16, TC, TV, true, GB, false>(scenes, it, ths, Q.sp, local, dyn); break;
     }
     __syncthreads();  // the unit's LDS is free for the next one
   }
@@ -1013,10 +1056,17 @@ template <int T, typename TC, typename TV>
 hipError_t launch_t(hipStream_t st, const FactorLaunch &L) {
   if (L.count <= 0) return hipSuccess;
   const unsigned grid = unsigned(int64_t(L.count) * (L.sp.splits > 1 ? L.sp.splits : 1));
-  if (L.points)
+  if (L.points) {
+    if constexpr (sizeof(TC) == 4 && sizeof(TV) == 4) {
+      if (L.split_bf16) {
+        k_super_factor<T, TC, TV, true, true><<<grid, kThreads, 0, st>>>(
+            L.scenes, L.items, static_cast<TC *>(L.th_scratch), L.sp);
+        return hipGetLastError();
+      }
+    }
     k_super_factor<T, TC, TV, true><<<grid, kThreads, 0, st>>>(
         L.scenes, L.items, static_cast<TC *>(L.th_scratch), L.sp);
-  else if constexpr (sizeof(TC) == 4 && sizeof(TV) == 4) {
+  } else if constexpr (sizeof(TC) == 4 && sizeof(TV) == 4) {
     if constexpr (T <= 6) {
       if (L.gram_in) {
         k_super_factor<T, TC, TV, false, false, true><<<grid, kThreads, 0, st>>>(
@@ -1070,7 +1120,7 @@ hipError_t launch_super_factor(hipStream_t st, int tiles, int precision, bool po
                                bool gram_in) {
   if (sp.splits > 1 && ((sp.partial == nullptr && tiles <= 8) || sp.counters == nullptr))
     return hipErrorInvalidValue;
-  if (split_bf16 && (precision != 1 || points)) return hipErrorInvalidValue;
+  if (split_bf16 && precision != 1) return hipErrorInvalidValue;
   if (gram_in && (precision != 1 || points || sp.splits > 1 || tiles > 6)) return hipErrorInvalidValue;
   const FactorLaunch L{scenes, items, count, points, th_scratch, sp, split_bf16, gram_in};
   if (precision == 0) return launch_prec<double, double>(st, tiles, L);
@@ -1096,27 +1146,34 @@ size_t factor_unit_lds(int tiles, int precision) {
   return precision == 1 ? pick(float{}) : pick(double{});
 }
 
-int factor_queue_blocks(int precision, size_t lds_bytes, int cus) {
+int factor_queue_blocks(int precision, bool split_bf16, size_t lds_bytes, int cus) {
   int per_cu = 0;
   const hipError_t e =
-      precision == 1
-          ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_factor_queue<float, float>, kThreads, lds_bytes)
-          : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_factor_queue<double, double>, kThreads,
-                                                         lds_bytes);
+      precision != 1 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_factor_queue<double, double, false>,
+                                                                    kThreads, lds_bytes)
+      : split_bf16   ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_factor_queue<float, float, true>,
+                                                                    kThreads, lds_bytes)
+                     : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_factor_queue<float, float, false>,
+                                                                    kThreads, lds_bytes);
   if (e != hipSuccess || per_cu < 1) return 0;
   return per_cu * cus;
 }
 
 hipError_t launch_factor_queue(hipStream_t st, int precision, const FactorScene *scenes,
                                const FactorItem *items, const FactorQueueBucket *qb, int32_t nb,
-                               int32_t total, int32_t *next, size_t lds_bytes, int blocks) {
+                               int32_t total, int32_t *next, size_t lds_bytes, int blocks, bool split_bf16) {
   if (total <= 0) return hipSuccess;
-  if (blocks < 1 || (precision != 0 && precision != 1)) return hipErrorInvalidValue;
-  if (precision == 1)
-    k_factor_queue<float, float><<<unsigned(blocks), kThreads, lds_bytes, st>>>(scenes, items, qb, nb, total, next);
+  if (blocks < 1 || (precision != 0 && precision != 1) || (split_bf16 && precision != 1))
+    return hipErrorInvalidValue;
+  if (precision != 1)
+    k_factor_queue<double, double, false><<<unsigned(blocks), kThreads, lds_bytes, st>>>(scenes, items, qb, nb,
+                                                                                         total, next);
+  else if (split_bf16)
+    k_factor_queue<float, float, true><<<unsigned(blocks), kThreads, lds_bytes, st>>>(scenes, items, qb, nb, total,
+                                                                                      next);
   else
-    k_factor_queue<double, double><<<unsigned(blocks), kThreads, lds_bytes, st>>>(scenes, items, qb, nb, total,
-                                                                                  next);
+    k_factor_queue<float, float, false><<<unsigned(blocks), kThreads, lds_bytes, st>>>(scenes, items, qb, nb,
+                                                                                       total, next);
   return hipGetLastError();
 }
 

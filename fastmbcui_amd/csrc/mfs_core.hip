@@ -1294,7 +1294,7 @@ int factor_impl(mfs_ctx *ctx, const float *M, const PointsOp *pts, int64_t m, in
                                ctx->dfscene.as<FactorScene>(),
                                ctx->dlist.as<FactorItem>() + b.offset + L.lo, L.cnt,
                                ctx->dthg.as<char>() + L.off, L.sp,
-                               ctx->gram_split && prec_code == 1 && pts == nullptr),
+                               ctx->gram_split && prec_code == 1),
            "supernode factor");
     return MFS_OK;
   };
@@ -2571,6 +2571,10 @@ int mfs_batch_factor_points(mfs_batch *b, const double *const *x, const double *
       if (L.tiles <= 8) L.sp.partial = b->dthg.as<char>() + L.off;
     }
   const int prec_code = precision == MFS_FP64 ? 0 : 1;
+  // Split products in FP32 when every scene asks for them (one launch serves
+  // them all).
+  bool split = prec_code == 1;
+  for (size_t i = 0; i < count; ++i) split = split && b->scenes[i]->gram_split;
   // One launch: every bucket's units (largest tiles first) from a device
   // queue, as many blocks as stay resident with the LDS of the largest tile
   // count present.  Its kernel carries every tile count's code, so it has the
@@ -2591,7 +2595,7 @@ int mfs_batch_factor_points(mfs_batch *b, const double *const *x, const double *
   }
   const bool use_queue = !buckets.empty() && units < INT32_MAX &&
                          (b->factor_mode == 2 || (b->factor_mode == 0 && units <= kQueueUnits));
-  if (use_queue) queue_blocks = factor_queue_blocks(prec_code, queue_lds, b->scenes[0]->cus);
+  if (use_queue) queue_blocks = factor_queue_blocks(prec_code, split, queue_lds, b->scenes[0]->cus);
   if (queue_blocks > 0) {
     BHIPCHK(b->dqb.ensure(qb.size() * sizeof(FactorQueueBucket)), "alloc");
     BHIPCHK(b->dqnext.ensure(4), "alloc");
@@ -2601,7 +2605,7 @@ int mfs_batch_factor_points(mfs_batch *b, const double *const *x, const double *
     BHIPCHK(hipMemsetAsync(b->dqnext.p, 0, 4, b->root), "clear queue");
     BHIPCHK(launch_factor_queue(b->root, prec_code, b->dtable.as<FactorScene>(), b->ditems.as<FactorItem>(),
                                 b->dqb.as<FactorQueueBucket>(), int32_t(qb.size()), int32_t(units),
-                                b->dqnext.as<int32_t>(), queue_lds, queue_blocks),
+                                b->dqnext.as<int32_t>(), queue_lds, queue_blocks, split),
             "supernode factor queue");
   } else {
     BHIPCHK(hipEventRecord(b->root_ev, b->root), "fork");
@@ -2611,7 +2615,7 @@ int mfs_batch_factor_points(mfs_batch *b, const double *const *x, const double *
       const BLaunch &L = buckets[bi];
       BHIPCHK(launch_super_factor(b->side[bi % nside], L.tiles, prec_code, true, b->dtable.as<FactorScene>(),
                                   b->ditems.as<FactorItem>() + L.offset, L.count, b->dthg.as<char>() + L.off,
-                                  L.sp),
+                                  L.sp, split),
               "supernode factor");
     }
     for (int i = 0; i < nside; ++i) {

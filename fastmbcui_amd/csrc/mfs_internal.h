@@ -166,8 +166,8 @@ struct FactorSplit {
 // Gram's operands are evaluated from the records).  tiles in {1, 2, 3, 4, 6,
 // 8, 12, 16}; for 12 and 16 th_scratch holds count * splits *
 // factor_scratch_elems(tiles) elements of the compute type (8 bytes for
-// precision 0 and 2, 4 for 1).  split_bf16 (precision 1, dense form only):
-// the Gram's FP32 products as three BF16 products of a two-part split.
+// precision 0 and 2, 4 for 1).  split_bf16 (precision 1, either form): the
+// Gram's FP32 products as three BF16 products of a two-part split.
 // gram_in (precision 1, dense form): the Grams come summed from the band
 // partials (FactorScene::gram); only the Cholesky and the KL columns run.
 hipError_t launch_super_factor(hipStream_t st, int tiles, int precision, bool points,
@@ -185,12 +185,14 @@ struct FactorQueueBucket {
 // LDS bytes of one unit of `tiles` tiles (precision 0: FP64, 1: FP32).
 size_t factor_unit_lds(int tiles, int precision);
 // Resident blocks of the queue launch with `lds_bytes` of LDS (0: none fit).
-int factor_queue_blocks(int precision, size_t lds_bytes, int cus);
+int factor_queue_blocks(int precision, bool split_bf16, size_t lds_bytes, int cus);
 // `next`: the unit counter, zero before the launch.  `blocks`: at most
-// factor_queue_blocks (all resident; none waits for another).
+// factor_queue_blocks (all resident; none waits for another).  split_bf16 as
+// for launch_super_factor.
 hipError_t launch_factor_queue(hipStream_t st, int precision, const FactorScene *scenes,
                                const FactorItem *items, const FactorQueueBucket *qb, int32_t nb,
-                               int32_t total, int32_t *next, size_t lds_bytes, int blocks);
+                               int32_t total, int32_t *next, size_t lds_bytes, int blocks,
+                               bool split_bf16);
 inline int64_t factor_scratch_elems(int tiles) {
   return tiles > 8 ? int64_t(16 * tiles) * (16 * tiles + 1) : 0;
 }

@@ -270,8 +270,9 @@ enum mfs_param {
                                      fewer supernodes than 4 x CUs spread each Gram's rows
                                      over several blocks and sum their partial Grams (0: 1,
                                      no splits) */
-    MFS_PARAM_GRAM_SPLIT = 8,     /* products of the dense FP32 factor's Gram (changes its
-                                     rounding, not only the summation order): 1 FP32 MFMA,
+    MFS_PARAM_GRAM_SPLIT = 8,     /* products of the FP32 factor's Gram, dense or points form
+                                     (changes its rounding, not only the summation order;
+                                     a batch splits when all its scenes do): 1 FP32 MFMA,
                                      2 three BF16 MFMA products of each value split into
                                      hi + lo (relative error per product about 2^-16, as
                                      FP32 sums over 65,536 rows have); 0: the default, 2 */

@@ -144,20 +144,22 @@ def test_batch_with_row_splits_equals_single_scene_solves():
         assert torch.equal(X, Xs[i]), f"scene {i}"
 
 
-@pytest.mark.parametrize("precision", [_lib.MFS_FP32, _lib.MFS_FP64])
-def test_batch_factor_in_one_launch_equals_launch_per_bucket(precision):
+@pytest.mark.parametrize("precision,products", [(_lib.MFS_FP32, 2), (_lib.MFS_FP32, 1), (_lib.MFS_FP64, 0)])
+def test_batch_factor_in_one_launch_equals_launch_per_bucket(precision, products):
     """Every scene's supernodes in one launch over a device queue (mode 2)
     against one launch per tile size and split count (mode 1): the same units
     run the same code on the same scratch, so the factors are equal bit for
     bit; 24 scenes drawn from the uneven batch's ranges (2-32 bodies, 256-2048
     points per body, capped at 16 bodies x 512 points here) with row splits
-    on.  The automatic mode (0) gives the same factor again."""
+    on.  The automatic mode (0) gives the same factor again.  FP32 with the
+    Gram's products split into BF16 (2, the default) and on FP32 MFMA (1)."""
     scs = [scenes.batch_scene(s, bodies=(2, 16), points=(256, 512)) for s in range(24)]
     vals = {}
     for mode in (2, 1, 0):
         b = _lib.Batch(len(scs))
         b.set_factor_mode(mode)
         b.set_param(_lib.MFS_PARAM_FACTOR_SPLITS, 4)
+        b.set_param(_lib.MFS_PARAM_GRAM_SPLIT, products)
         b.plan([sc.y for sc in scs], 3.0)
         b.factor_points([_dev(sc.x) for sc in scs], [_dev(sc.nx) for sc in scs], [_dev(sc.y) for sc in scs],
                         precision)

@@ -3,7 +3,8 @@
 * mfs_points_apply: M v and M^T v evaluated from the packed points against the
   FP64 kernel matrix (FP64 evaluation to 1e-12, FP32 hi/lo evaluation to 2e-6);
 * mfs_factor_points: the KL factor with the Gram evaluated from the points,
-  against the oracle's factor of the same scene;
+  against the oracle's factor of the same scene, with the FP32 Gram's products
+  on FP32 MFMA and as split BF16 products;
 * mfs_solve_points: relative normal-equation residual <= tol, recomputed by the
   oracle on the FP64 kernel matrix; fixed-iteration trajectories against the
   oracle's PCG with the same factor values; warm start; error paths.
@@ -245,6 +246,43 @@ def test_points_large_supernodes(precision):
           f"residual {res:.2e}", flush=True)
     assert ctx.stats().max_local > 192
     assert res <= 2e-6
+
+
+def _coins_large():
+    return scenes.coins(seed=3, n_bodies=4, pts_per_body=512, k=2, box=6.0)
+
+
+@pytest.mark.parametrize("maker,rho,tol", [(_ellips, 3.0, 1e-5), (_coins_large, 7.0, None)])
+def test_factor_points_gram_products(maker, rho, tol):
+    """The FP32 points factor with its Gram's products on FP32 MFMA (1) and as
+    three BF16 products of a hi + lo split of each evaluated value (2, the
+    default; 8 rows per lane per 32-row step), against the oracle's FP64
+    factor; rho 7 gives supernodes of 12 and 16 tiles.  The split leaves a
+    relative error of about 2^-16 per product, so its factor may be a few
+    times further from the FP64 one than the FP32 products' but within the
+    FP32 factor bound; the solve on it still reaches its tolerance."""
+    sc = maker()
+    M = scenes.kernel_matrix(sc.x, sc.nx, sc.y)
+    pl = oracle.plan(sc.y, rho)
+    ref = oracle.factor(oracle.permuted_colmajor(M, pl.Pinv), M.shape[0], pl)
+    errs = {}
+    for products in (1, 2):
+        ctx = _lib.Context(0)
+        ctx.set_param(_lib.MFS_PARAM_GRAM_SPLIT, products)
+        ctx.plan(sc.y, rho)
+        ctx.factor_points(_dev(sc.x), _dev(sc.nx), _dev(sc.y), _lib.MFS_FP32)
+        errs[products] = _col_rel_err(pl, ctx.factor_export(), ref)
+        st = ctx.stats()
+        print(f"{sc.name} rho {rho}: Gram products {products}: max column rel err {errs[products]:.2e}, "
+              f"max local {st.max_local}, retried {st.retried_supernodes}", flush=True)
+        if products == 2:
+            B = _dev(sc.b.astype(np.float32))
+            X = ctx.solve_points(B, maxit=500, tol=1e-6)
+            res = oracle.normal_residual(M, sc.b.astype(np.float32), X.cpu().numpy())
+            assert res <= 2e-6
+    if tol is not None:  # the bound of the FP32 points factor on this scene
+        assert errs[1] < tol and errs[2] < tol
+    assert errs[2] <= 20 * errs[1] + 1e-7
 
 
 def test_time_stepper_warm_start_meets_tolerance_each_step():
