@@ -580,22 +580,29 @@ __device__ __forceinline__ void factor_unit(const FactorScene *__restrict__ scen
         yl[a][d] = y[3 + d];
       }
     }
-    auto load_target = [&](int64_t f, float4 (&t)[3]) {
-      if (f < m) {
-        t[0] = *reinterpret_cast<const float4 *>(tgt + f * 12);
-        t[1] = *reinterpret_cast<const float4 *>(tgt + f * 12 + 4);
-        t[2] = *reinterpret_cast<const float4 *>(tgt + f * 12 + 8);
-      } else {  // padding row: zero normal gives zero values
-        t[0] = make_float4(1e15f, 0.f, 0.f, 0.f);
-        t[1] = make_float4(0.f, 0.f, 0.f, 0.f);
-        t[2] = make_float4(0.f, 0.f, 0.f, 0.f);
-      }
-    };
+    // Row records, branch-free: a row past the end reads the last row with a
+    // zero scale, so its values are exactly zero (a target never coincides
+    // with a source, so the distance stays finite).  A branch around the
+    // loads would end in a wait for them, exposing every row's latency.
+    typedef const __attribute__((address_space(1))) float gfloat;
+    typedef const __attribute__((address_space(1))) f4_t gf4;
+    gfloat *const tg = (gfloat *)tgt;
     const float scale = float(-kInv4Pi);
-    auto eval = [&](const float4 (&t)[3], int a) {
-      return dlp32((t[0].x - yh[a][0]) + (t[0].w - yl[a][0]), (t[0].y - yh[a][1]) + (t[1].x - yl[a][1]),
-                   (t[0].z - yh[a][2]) + (t[1].y - yl[a][2]), scale * t[1].z, scale * t[1].w,
-                   scale * t[2].x);
+    struct Rec {
+      f4_t a, b;   // x_hi[3] x_lo[0] | x_lo[1..2] n[0..1]
+      float c, s;  // n[2], the scale (0 past the end)
+    };
+    auto load_rec = [&](int64_t f, Rec &r) {
+      const bool in = f < m;
+      gfloat *p = tg + (in ? f : m - 1) * 12;
+      r.a = *(gf4 *)p;
+      r.b = *(gf4 *)(p + 4);
+      r.c = p[8];
+      r.s = in ? scale : 0.f;
+    };
+    auto eval = [&](const Rec &t, int a) {
+      return dlp32((t.a.x - yh[a][0]) + (t.a.w - yl[a][0]), (t.a.y - yh[a][1]) + (t.b.x - yl[a][1]),
+                   (t.a.z - yh[a][2]) + (t.b.y - yl[a][2]), t.s * t.b.z, t.s * t.b.w, t.s * t.c);
     };
     if constexpr (GB) {
       // Split products (FP32 values, FP32 sums, as for the Gram from M^T):
@@ -607,23 +614,19 @@ __device__ __forceinline__ void factor_unit(const FactorScene *__restrict__ scen
       // MFMA time.  FSTEP <= 128 divides the row splits' chunks.
       constexpr int FSTEP = 32 * KS;
       const int64_t fs = r0 + kph * 32 + 8 * (lane >> 4);  // this lane's first row
-      float4 cur[3], nxt[3];
-      load_target(fs, cur);
+      Rec cur, nxt;
+      load_rec(fs, cur);
       for (int64_t f0 = r0 + kph * 32, fb = fs; f0 < r1; f0 += FSTEP, fb += FSTEP) {
         float v[T][8];
 #pragma unroll
         for (int i = 0; i < 8; ++i) {
-          load_target(i < 7 ? fb + i + 1 : fb + FSTEP, nxt);
+          load_rec(i < 7 ? fb + i + 1 : fb + FSTEP, nxt);
 #pragma unroll
           for (int a = 0; a < T; ++a) {
             const float e = eval(cur, a);
             v[a][i] = ok[a] ? e : 0.f;
           }
-#pragma unroll
-          for (int d = 0; d < 3; ++d) cur[d] = nxt[d];
-          // a row at a time: the scheduler would otherwise interleave the
-          // eight rows' loads and evaluations, past the register file
-          __builtin_amdgcn_sched_barrier(0);
+          cur = nxt;
         }
         bf8_t hi[T], lo[T];
 #pragma unroll
@@ -633,16 +636,34 @@ __device__ __forceinline__ void factor_unit(const FactorScene *__restrict__ scen
         step3(hi, lo);
       }
     } else {
+      // FP32 products: one row per lane and k-step.  This loop keeps the
+      // branch around its loads (padding rows far away with a zero normal):
+      // at 1-6 tiles it measured faster than the branch-free records above
+      // (30.8 against 34.3 ms for the factor of 64 ellipsoids x 1024 points).
+      auto load_target = [&](int64_t f, float4 (&t)[3]) {
+        if (f < m) {
+          t[0] = *reinterpret_cast<const float4 *>(tgt + f * 12);
+          t[1] = *reinterpret_cast<const float4 *>(tgt + f * 12 + 4);
+          t[2] = *reinterpret_cast<const float4 *>(tgt + f * 12 + 8);
+        } else {
+          t[0] = make_float4(1e15f, 0.f, 0.f, 0.f);
+          t[1] = make_float4(0.f, 0.f, 0.f, 0.f);
+          t[2] = make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+      };
       constexpr int FSTEP = 4 * KS;  // rows per k-step over the waves sharing a pair group
       float4 cur[3], nxt[3];
       int64_t f = r0 + kph * 4 + (lane >> 4);
       load_target(f, cur);
       for (int64_t f0 = r0 + kph * 4; f0 < r1; f0 += FSTEP, f += FSTEP) {
         load_target(f + FSTEP, nxt);
+        const float sn0 = scale * cur[1].z, sn1 = scale * cur[1].w, sn2 = scale * cur[2].x;
         TC fr[T];
 #pragma unroll
         for (int a = 0; a < T; ++a) {
-          const float v = eval(cur, a);
+          const float v = dlp32((cur[0].x - yh[a][0]) + (cur[0].w - yl[a][0]),
+                                (cur[0].y - yh[a][1]) + (cur[1].x - yl[a][1]),
+                                (cur[0].z - yh[a][2]) + (cur[1].y - yl[a][2]), sn0, sn1, sn2);
           fr[a] = ok[a] ? TC(v) : TC(0);
         }
         step(fr);

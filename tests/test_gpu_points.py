@@ -252,12 +252,19 @@ def _coins_large():
     return scenes.coins(seed=3, n_bodies=4, pts_per_body=512, k=2, box=6.0)
 
 
-@pytest.mark.parametrize("maker,rho,tol", [(_ellips, 3.0, 1e-5), (_coins_large, 7.0, None)])
+def _coins_ragged():
+    return scenes.coins(seed=5, n_bodies=3, pts_per_body=150, k=2, box=6.0)   # m = 450: a partial 32-row step
+
+
+@pytest.mark.parametrize("maker,rho,tol", [(_ellips, 3.0, 1e-5), (_coins_large, 7.0, None),
+                                           (_coins_ragged, 3.0, None)])
 def test_factor_points_gram_products(maker, rho, tol):
     """The FP32 points factor with its Gram's products on FP32 MFMA (1) and as
     three BF16 products of a hi + lo split of each evaluated value (2, the
     default; 8 rows per lane per 32-row step), against the oracle's FP64
-    factor; rho 7 gives supernodes of 12 and 16 tiles.  The split leaves a
+    factor; rho 7 gives supernodes of 12 and 16 tiles; 450 targets end in a
+    partial 32-row step (rows past the end read the last row with a zero
+    scale: their values must be exactly zero).  The split leaves a
     relative error of about 2^-16 per product, so its factor may be a few
     times further from the FP64 one than the FP32 products' but within the
     FP32 factor bound; the solve on it still reaches its tolerance."""
