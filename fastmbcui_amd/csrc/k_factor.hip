@@ -119,6 +119,15 @@ __device__ inline void split_bf16(f4_t x0, f4_t x1, bf8_t &hi, bf8_t &lo) {
   lo = __builtin_convertvector(x - hf, bf8_t);
 }
 
+// split_bf16 for two values (x0 the lower half of each word), as its words.
+typedef __bf16 bf2_t __attribute__((ext_vector_type(2)));
+typedef float f2_t __attribute__((ext_vector_type(2)));
+__device__ inline void split_pair(float x0, float x1, uint32_t &hw, uint32_t &lw) {
+  hw = __builtin_bit_cast(uint32_t, __builtin_convertvector(f2_t{x0, x1}, bf2_t));
+  const float h0 = __uint_as_float(hw << 16), h1 = __uint_as_float(hw & 0xffff0000u);
+  lw = __builtin_bit_cast(uint32_t, __builtin_convertvector(f2_t{x0 - h0, x1 - h1}, bf2_t));
+}
+
 // One 32-row k-step of the split Gram for the tile pairs GRP + NG (U0 + u):
 // the large products of all pairs first, then the two small ones, so that no
 // MFMA waits for the one before it on the same accumulator.
@@ -617,22 +626,34 @@ __device__ __forceinline__ void factor_unit(const FactorScene *__restrict__ scen
       Rec cur, nxt;
       load_rec(fs, cur);
       for (int64_t f0 = r0 + kph * 32, fb = fs; f0 < r1; f0 += FSTEP, fb += FSTEP) {
-        float v[T][8];
+        // Rows are split in pairs as soon as both are evaluated (hi and lo
+        // words of two BF16 each), so a step holds the words and one row's
+        // values, not all eight rows' values beside their split.
+        u4_t hw[T], lw[T];
+        float pend[T];
 #pragma unroll
         for (int i = 0; i < 8; ++i) {
           load_rec(i < 7 ? fb + i + 1 : fb + FSTEP, nxt);
 #pragma unroll
           for (int a = 0; a < T; ++a) {
-            const float e = eval(cur, a);
-            v[a][i] = ok[a] ? e : 0.f;
+            const float e = ok[a] ? eval(cur, a) : 0.f;
+            if (i % 2 == 0) {
+              pend[a] = e;
+            } else {
+              uint32_t h, l;
+              split_pair(pend[a], e, h, l);
+              hw[a][i / 2] = h;
+              lw[a][i / 2] = l;
+            }
           }
           cur = nxt;
         }
         bf8_t hi[T], lo[T];
 #pragma unroll
-        for (int a = 0; a < T; ++a)
-          split_bf16(f4_t{v[a][0], v[a][1], v[a][2], v[a][3]}, f4_t{v[a][4], v[a][5], v[a][6], v[a][7]},
-                     hi[a], lo[a]);
+        for (int a = 0; a < T; ++a) {
+          hi[a] = __builtin_bit_cast(bf8_t, hw[a]);
+          lo[a] = __builtin_bit_cast(bf8_t, lw[a]);
+        }
         step3(hi, lo);
       }
     } else {
