@@ -245,7 +245,7 @@ static_assert(2 * kBandThreads == kBandCols, "the loads cover a band in two colu
 static_assert(kBandLds * 4 <= 160 * 1024, "LDS");
 
 template <int T>
-__device__ inline void band_wave(const float *__restrict__ M, int64_t m, int64_t n, int64_t c0,
+__device__ inline void band_wave(const float *__restrict__ M, int64_t m, int64_t n, int64_t c0, int cols,
                                  int64_t r0, int64_t r1, float *chunk, const BandWave W,
                                  const int32_t *__restrict__ items, const SuperDesc *__restrict__ sd,
                                  const int32_t *__restrict__ sind, const int32_t *__restrict__ Pinv,
@@ -282,16 +282,19 @@ __device__ inline void band_wave(const float *__restrict__ M, int64_t m, int64_t
 #pragma unroll
     for (int u = 0; u < P; ++u) acc[j][u] = f4_t{0.f, 0.f, 0.f, 0.f};
 
-  // Loads: columns 2 tid and 2 tid + 1 of the band of all 32 rows of a chunk
-  // (8 bytes per lane, a 512-byte row segment per wave; n is even, so rows
-  // and pairs start 8-byte aligned).  A pair past the end of a row is moved
-  // back to the row's last pair; those columns lie outside the band's
-  // supernodes.
+  // Loads: columns 2 tid and 2 tid + 1 of the window of all 32 rows of a
+  // chunk (8 bytes per lane, a 512-byte row segment per wave; n is even, so
+  // rows and pairs start 8-byte aligned).  A pair past the end of a row is
+  // moved back to the row's last pair; those columns lie outside the block's
+  // supernodes.  Lanes past the window's width neither load nor store: no
+  // supernode of the block reads those columns.
   typedef float f2_t __attribute__((ext_vector_type(2)));
+  const bool stage = 2 * tid < cols;
   const int64_t cp = c0 + 2 * tid;
   const uint32_t oc = uint32_t(cp + 1 < n ? cp : n - 2) * 4u;
   f2_t rv[kBandRows];
   auto load = [&](int64_t f0) {
+    if (!stage) return;
 #pragma unroll
     for (int r = 0; r < kBandRows; ++r) {
       const int64_t f = f0 + r < m ? f0 + r : m - 1;
@@ -301,6 +304,7 @@ __device__ inline void band_wave(const float *__restrict__ M, int64_t m, int64_t
   };
   // rows past the part (the next part's, or past m) enter as zeros
   auto store = [&](int64_t f0) {
+    if (!stage) return;
 #pragma unroll
     for (int q = 0; q < kBandRows / 4; ++q) {
       f4_t va, vb;
@@ -393,11 +397,11 @@ __global__ __launch_bounds__(kBandThreads, 1) void k_band_gram(
   // Every wave runs the same chunk loop (loads, stores, barriers); idle waves
   // (count 0) take the T = 1 path and compute nothing.
   switch (W.tiles) {
-    case 2: band_wave<2>(M, m, n, c0, r0, r1, chunk, W, items, sd, sind, Pinv, goff, gp); break;
-    case 3: band_wave<3>(M, m, n, c0, r0, r1, chunk, W, items, sd, sind, Pinv, goff, gp); break;
-    case 4: band_wave<4>(M, m, n, c0, r0, r1, chunk, W, items, sd, sind, Pinv, goff, gp); break;
-    case 6: band_wave<6>(M, m, n, c0, r0, r1, chunk, W, items, sd, sind, Pinv, goff, gp); break;
-    default: band_wave<1>(M, m, n, c0, r0, r1, chunk, W, items, sd, sind, Pinv, goff, gp); break;
+    case 2: band_wave<2>(M, m, n, c0, B.cols, r0, r1, chunk, W, items, sd, sind, Pinv, goff, gp); break;
+    case 3: band_wave<3>(M, m, n, c0, B.cols, r0, r1, chunk, W, items, sd, sind, Pinv, goff, gp); break;
+    case 4: band_wave<4>(M, m, n, c0, B.cols, r0, r1, chunk, W, items, sd, sind, Pinv, goff, gp); break;
+    case 6: band_wave<6>(M, m, n, c0, B.cols, r0, r1, chunk, W, items, sd, sind, Pinv, goff, gp); break;
+    default: band_wave<1>(M, m, n, c0, B.cols, r0, r1, chunk, W, items, sd, sind, Pinv, goff, gp); break;
   }
 }
 

@@ -128,8 +128,9 @@ struct BandWave {
   int32_t first;  // their ids are items[first .. first + count)
 };
 struct BandBlock {
-  int32_t c0;    // first column of the band, even (-1: empty block)
+  int32_t c0;    // first column of the block's window, even (-1: empty block)
   int32_t part;  // row part
+  int32_t cols;  // window width, even, at most kBandCols
   BandWave w[kBandWaves];
 };
 // One launch over `count` blocks (n even: 8-byte loads of column pairs);
