@@ -2587,11 +2587,12 @@ int mfs_batch_factor_points(mfs_batch *b, const double *const *x, const double *
   // One launch: every bucket's units (largest tiles first) from a device
   // queue, as many blocks as stay resident with the LDS of the largest tile
   // count present.  Its kernel carries every tile count's code, so it has the
-  // registers of the widest (248 against 35-118 for the small tiles' own
-  // kernels) and fewer blocks per CU: measured on the uneven batch's scenes,
-  // 14.5 ms against 15.2 ms for 24 scenes (17,416 supernodes) but 1,345 ms
-  // against 620 ms for 256 scenes (697,733).  Automatic mode takes it up to
-  // kQueueUnits units; a launch per bucket otherwise, or when no block fits.
+  // registers of the widest (all 256 and accumulators besides, against
+  // 32-215 for the tiles' own kernels) and fewer blocks per CU: measured on
+  // the uneven batch's scenes, 12.2 ms against 15.2 ms for 24 scenes (17,416
+  // supernodes, 4 row splits) but 817 ms against 428 ms for 256 scenes
+  // (697,733).  Automatic mode takes it up to kQueueUnits units; a launch per
+  // bucket otherwise, or when no block fits.
   constexpr int64_t kQueueUnits = 32768;
   int queue_blocks = 0;
   size_t queue_lds = 0;
